@@ -1,0 +1,178 @@
+#!/usr/bin/env python3
+"""Benchmark: DFSPH dam break, 1,000,000 FP32 fluid particles, one MI355X.
+
+A step is one DFSPH solver step (src/DFSPHSolver.cpp:38-112 of the
+reference) of the whole scene, inputs resident in HBM.
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+
+Prints one JSON line.  value = fluid particle-steps/s over the K timed steps
+(host wall clock around the step loop, bracketed by device synchronizes).
+The roofline object is measured in a second pass of K steps with a HIP event
+pair around every kernel launch on the library's stream; `achieved` is the
+dominant kernel's algorithmic bytes per launch (per-particle figure x N plus
+boundary and cell terms) divided by its average launch duration.
+cpu_baseline times the FP64 OpenMP restatement of the reference
+(oracle/liboracle_fast.so) on the host cores, stepping from the GPU's
+warmed-up state.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+# Algorithmic bytes per fluid particle (FP32) of each kernel, and whether the
+# launch also counts one boundary sweep (16 B per boundary particle) and one
+# cell-table sweep (4 B per cell).
+BALG = {
+    "df_vel_density": (52, 1, 1),
+    "df_vel_divergence": (52, 1, 1),
+    "df_err_density": (36, 1, 1),
+    "df_err_divergence": (36, 1, 1),
+    "df_vel_warm": (44, 1, 1),
+    "build": (20, 1, 1),
+    "dt": (12, 0, 0),
+    "df_positions": (40, 0, 0),
+    "reorder": (56, 0, 0),
+}
+
+
+def balg_step(n, nb, ncells, it_d, it_v):
+    """Algorithmic HBM bytes of one DFSPH step: 264 B per particle fixed,
+    88 B per particle per density and per divergence iteration, 16 B per
+    boundary particle for each of the 4 + 2 it_d + 2 it_v boundary sweeps,
+    4 B per cell per neighbor sweep and 12 B per cell per grid build."""
+    sweeps = 4 + 2 * it_d + 2 * it_v
+    return (264 + 88 * (it_d + it_v)) * n + 16 * nb * sweeps + 4 * ncells * sweeps + 12 * ncells
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--n-side", type=int, default=100)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--no-roofline", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus})",
+              file=sys.stderr)
+        sys.exit(2)
+
+    from sph_amd import configs, sphg
+
+    spec = configs.dfsph_dam_1m(n_side=args.n_side)
+    print(f"[bench] scene {spec.name}: {spec.n} fluid, {spec.nb} boundary", file=sys.stderr, flush=True)
+    ctx = sphg.Context(0)
+    scene = sphg.Scene(ctx, spec.params, spec.fluid, spec.boundary)
+    info = scene.info()
+    if args.warmup:
+        scene.step(args.warmup)
+    ctx.synchronize()
+    print(f"[bench] warmup {args.warmup} done", file=sys.stderr, flush=True)
+
+    s0 = scene.scalars()
+    t0 = time.perf_counter()
+    st = scene.step(args.steps)
+    ctx.synchronize()
+    t1 = time.perf_counter()
+    s1 = scene.scalars()
+    wall = t1 - t0
+    n = spec.n
+    value = n * args.steps / wall
+    it_d = (s1.sum_iterations - s0.sum_iterations) / args.steps
+    it_v = (s1.sum_iterations_v - s0.sum_iterations_v) / args.steps
+    mean_dt = (s1.time - s0.time) / args.steps
+    print(f"[bench] timed {args.steps} steps: {wall * 1e3 / args.steps:.3f} ms/step, it_d {it_d:.2f} it_v {it_v:.2f}",
+          file=sys.stderr, flush=True)
+    step_bytes = balg_step(n, spec.nb, info["ncells"], it_d, it_v)
+
+    roof = None
+    kernels = {}
+    if not args.no_roofline:
+        ctx.kernel_timing(True)
+        sa = scene.scalars()
+        scene.step(args.steps)
+        ctx.synchronize()
+        sb = scene.scalars()
+        ctx.kernel_timing(False)
+        names = ["keys", "scan", "scatter", "cell_sort", "reorder", "build", "normals", "forces", "dt",
+                 "df_predict", "df_vel_warm", "df_vel_density", "df_vel_divergence", "df_err_density",
+                 "df_err_divergence", "df_positions"]
+        for nm in names:
+            cnt, ms = ctx.kernel_time(nm)
+            if cnt:
+                kernels[nm] = {"launches": cnt, "total_ms": round(ms, 4), "avg_us": round(1e3 * ms / cnt, 3)}
+        dom = max((k for k in kernels if k in BALG), key=lambda k: kernels[k]["total_ms"])
+        per, bsw, csw = BALG[dom]
+        bytes_launch = per * n + 16 * spec.nb * bsw + 4 * info["ncells"] * csw
+        avg_s = kernels[dom]["avg_us"] * 1e-6
+        achieved = bytes_launch / avg_s / 1e9
+        roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
+                "bytes_per_launch": bytes_launch, "avg_launch_us": kernels[dom]["avg_us"]}
+        print(f"[bench] kernels: {json.dumps(kernels)}", file=sys.stderr, flush=True)
+
+    cpu = None
+    if not args.no_cpu:
+        import oracle.pyoracle as po
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
+        x = scene.get(sphg.FIELD_X)
+        v = scene.get(sphg.FIELD_V)
+        k = scene.get(sphg.FIELD_K)
+        kv = scene.get(sphg.FIELD_KV)
+        sc = scene.scalars()
+        o = po.Oracle(spec.params, spec.fluid, spec.boundary, fast=True, threads=threads)
+        o.set_state(x, v, k, kv, sc.time_step, sc.iterations)
+        c0 = time.perf_counter()
+        o.step(args.cpu_steps)
+        c1 = time.perf_counter()
+        cpu = {"value": round(n * args.cpu_steps / (c1 - c0), 1), "unit": "particle-steps/s", "cores": threads,
+               "kind": "port",
+               "sample": f"{args.cpu_steps} DFSPH steps of the same 1M-particle scene from the GPU's state after "
+                         f"{args.warmup + 2 * args.steps} steps, FP64 OpenMP restatement (-O3 -ffast-math -mavx2)"}
+        o.close()
+
+    scene.close()
+    ctx.close()
+    out = {
+        "metric": "particle-steps/s (DFSPH 1M)",
+        "value": round(value, 1),
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall * 1e3 / args.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"DFSPH dam break, {n} fluid + {spec.nb} Akinci boundary particles, spacing 0.02, "
+                        "box 4x2.03x2.03, CFL 0.4, eta 0.01%, eta_V 0.1%",
+            "n_fluid": n, "n_boundary": spec.nb, "cells": info["ncells"],
+            "mean_density_iterations": round(it_d, 3), "mean_divergence_iterations": round(it_v, 3),
+            "mean_dt": mean_dt, "host_syncs_per_step": st.host_syncs / max(1, args.steps),
+            "balg_bytes_per_step": int(step_bytes),
+            "step_roofline_frac": round(step_bytes / (wall / args.steps) / (HBM_PEAK_GBS * 1e9), 4),
+        },
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    if kernels:
+        out["kernels"] = kernels
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,190 @@
+/*
+ * sphg.h -- C ABI of libsphg, the gfx950 (MI355X) SPH solver hot path.
+ *
+ * The path behind this ABI is the reference's SPHSolver::step for WCSPH,
+ * PCISPH and DFSPH with Akinci boundaries, XSPH viscosity, Akinci surface
+ * tension and Akinci adhesion (josemanuel28v/SPH src/SPHSolver.h:6-50,
+ * src/WCSPHSolver.cpp, src/PCISPHSolver.cpp, src/DFSPHSolver.cpp).  The C++
+ * host layer (sph_amd/host: Simulation, GpuSolver, FluidModel, ...) calls
+ * these functions where the reference calls its OpenMP solver.
+ *
+ * Conventions: every function returns 0 (SPHG_OK) or a negative SPHG_E*
+ * status and never throws; sphg_last_error() gives the message of the
+ * calling thread's last failure.  Arrays are plain host pointers, borrowed
+ * for the duration of the call and copied before it returns; xyz arrays hold
+ * n*3 floats in particle-id order.  A context is used by one host thread at
+ * a time.  All device memory is owned by the library.
+ */
+#ifndef SPHG_H
+#define SPHG_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    SPHG_OK = 0,
+    SPHG_EINVAL = -1,    /* bad argument or scene description */
+    SPHG_EHIP = -2,      /* HIP runtime error */
+    SPHG_ENOMEM = -3,    /* device allocation failed */
+    SPHG_ESTATE = -4,    /* call out of order (e.g. step before init) */
+    SPHG_EOVERFLOW = -5  /* a particle had more neighbors than max_neighbors */
+};
+
+/* Solver and boundary method numbers of the reference scene format
+ * (src/Simulation.cpp:21-33). */
+enum { SPHG_WCSPH = 0, SPHG_PCISPH = 1, SPHG_DFSPH = 2 };
+enum { SPHG_BOUNDARY_AKINCI = 2 };
+enum { SPHG_VISCOSITY_XSPH = 2, SPHG_SURFACE_TENSION_AKINCI = 3, SPHG_ADHESION_AKINCI = 4 };
+
+typedef struct sphg_ctx sphg_ctx;
+typedef struct sphg_scene sphg_scene;
+
+/* One scene: the reference's SceneInfo/FluidInfo fields
+ * (src/Simulation.h:47-68, read at src/SceneLoader.cpp:38-141) plus the
+ * device-side sizing.  Replaces the solver set-up done by
+ * Simulation::importScene (src/Simulation.cpp:342-424). */
+typedef struct sphg_scene_desc {
+    int32_t method;                 /* SPHG_WCSPH / SPHG_PCISPH / SPHG_DFSPH */
+    int32_t boundary_method;        /* must be SPHG_BOUNDARY_AKINCI */
+    double particle_radius;         /* r; support radius h = 4r (src/Simulation.cpp:197) */
+    double density0;                /* rest density rho0 */
+    double gravity[3];
+    double time_step;               /* initial (DFSPH) or fixed (WCSPH, PCISPH) dt */
+    double min_time_step, max_time_step, fps;   /* dt clamps; min = min(1/fps, min) */
+    double cfl_factor;              /* DFSPH */
+    double stiffness, gamma;        /* WCSPH Tait EOS */
+    double eta, eta_v;              /* PCISPH/DFSPH density, DFSPH divergence tolerance */
+    uint32_t min_iterations, max_iterations, min_iterations_v, max_iterations_v;
+    int32_t viscosity_method;       /* -1 or SPHG_VISCOSITY_XSPH */
+    double viscosity, boundary_viscosity;
+    int32_t surface_tension_method; /* -1 or SPHG_SURFACE_TENSION_AKINCI */
+    double surface_tension;
+    int32_t adhesion_method;        /* -1 or SPHG_ADHESION_AKINCI */
+    double adhesion;
+    uint32_t max_neighbors;         /* neighbor-list capacity; 0 = 96 */
+    double domain_min[3], domain_max[3];  /* grid bounds; all zero = fit the particles */
+} sphg_scene_desc;
+
+/* Device-side per-scene state, as the reference's solver members
+ * (src/DFSPHSolver.h:26-31, src/PCISPHSolver.h:17, src/TimeManager.h:30-35). */
+typedef struct sphg_scene_scalars {
+    double time, time_step, max_vel, cfl_time_step;
+    double avg_density_error;       /* last density-solver iteration */
+    double avg_density_error0;      /* before the density loop */
+    double avg_divergence_error, avg_divergence_error0;
+    double max_density_error;       /* PCISPH */
+    int32_t iterations, iterations_v;
+    uint32_t steps, sum_iterations, sum_iterations_v;
+    uint32_t neighbor_overflow;     /* largest neighbor count seen above capacity, or 0 */
+} sphg_scene_scalars;
+
+typedef struct sphg_step_stats {
+    double gpu_ms;                  /* device time of the stepped region (HIP events) */
+    uint32_t steps;
+    uint32_t iterations;            /* density/pressure iterations summed over steps */
+    uint32_t iterations_v;          /* DFSPH divergence iterations summed over steps */
+    uint32_t host_syncs;
+} sphg_step_stats;
+
+/* Fields for sphg_scene_get, returned in particle-id order. */
+enum {
+    SPHG_FIELD_X = 0,        /* float[n*3] positions */
+    SPHG_FIELD_V = 1,        /* float[n*3] velocities */
+    SPHG_FIELD_A = 2,        /* float[n*3] acceleration of the last step before gravity and pressure
+                                correction (WCSPH: pressure + non-pressure forces) */
+    SPHG_FIELD_RHO = 3,      /* float[n] density (PCISPH: last predicted density) */
+    SPHG_FIELD_P = 4,        /* float[n] pressure (WCSPH, PCISPH) */
+    SPHG_FIELD_ALPHA = 5,    /* float[n] DFSPH alpha */
+    SPHG_FIELD_RHOSTAR = 6,  /* float[n] DFSPH predicted density of the last density iteration */
+    SPHG_FIELD_DIVERR = 7,   /* float[n] DFSPH divergence error of the last divergence iteration */
+    SPHG_FIELD_K = 8,        /* float[n] DFSPH density warm-start accumulator */
+    SPHG_FIELD_KV = 9,       /* float[n] DFSPH divergence warm-start accumulator */
+    SPHG_FIELD_BVOL = 10,    /* float[nb] Akinci boundary volumes, boundary input order */
+    SPHG_FIELD_NORMAL = 11,  /* float[n*3] surface-tension normals */
+    SPHG_FIELD_PACC = 12,    /* float[n*3] PCISPH pressure acceleration */
+    SPHG_FIELD_KEYS = 13,    /* int32[n*3] cell keys floor(x/h) */
+    SPHG_FIELD_NNB = 14,     /* int32[n] fluid neighbors with |r| <= h, self excluded */
+    SPHG_FIELD_NGRAD = 15    /* int32[n] of which with nonzero kernel gradient */
+};
+
+const char* sphg_version(void);
+const char* sphg_last_error(void);
+
+/* Contexts: one device, one stream. */
+int sphg_ctx_create(int32_t device, sphg_ctx** out);
+int sphg_ctx_destroy(sphg_ctx* ctx);
+int sphg_ctx_synchronize(sphg_ctx* ctx);
+
+/* Scenes.  create -> set_boundary / set_fluid -> init -> step ...
+ * sphg_scene_create replaces Simulation::setSimulationMethod and the solver
+ * setters (src/Simulation.cpp:207-231, 342-380). */
+int sphg_scene_create(sphg_ctx* ctx, const sphg_scene_desc* desc, sphg_scene** out);
+int sphg_scene_destroy(sphg_scene* scene);
+/* FluidModel::init(points, velocities) (src/FluidModel.cpp:37-49); vel may be NULL. */
+int sphg_scene_set_fluid(sphg_scene* scene, const float* xyz, const float* vel, uint32_t n);
+/* BoundaryModel::init / AkinciBoundaryModel::addCube... (src/BoundaryModel.cpp:3-9). */
+int sphg_scene_set_boundary(sphg_scene* scene, const float* xyz, uint32_t nb);
+/* Simulation::init -> solver->init(): boundary volumes, and for DFSPH the
+ * first grid, densities and alpha (src/Simulation.cpp:102-110,
+ * src/SPHSolver.cpp:22-39, src/DFSPHSolver.cpp:24-36). */
+int sphg_scene_init(sphg_scene* scene);
+/* Replace the particle state (id order) and rebuild what derives from the
+ * positions, as after a DFSPH step's grid rebuild (src/DFSPHSolver.cpp:
+ * 87-97).  k, kv may be NULL (zero).  last_iterations feeds the DFSPH dt
+ * rule (src/DFSPHSolver.cpp:857-860). */
+int sphg_scene_set_state(sphg_scene* scene, const float* xyz, const float* vel, const float* k, const float* kv,
+                         double time_step, uint32_t last_iterations);
+/* nsteps x SPHSolver::step() (src/SPHSolver.h:45; WCSPH src/WCSPHSolver.cpp:
+ * 14-78, PCISPH src/PCISPHSolver.cpp:21-81, DFSPH src/DFSPHSolver.cpp:38-112).
+ * stats may be NULL. */
+int sphg_scene_step(sphg_scene* scene, uint32_t nsteps, sphg_step_stats* stats);
+/* Field read-back in id order (FluidModel getters, src/FluidModel.h:56-66;
+ * DFSPHSolver getters src/DFSPHSolver.h:60-62). */
+int sphg_scene_get(sphg_scene* scene, int32_t field, void* out);
+int sphg_scene_get_scalars(sphg_scene* scene, sphg_scene_scalars* out);
+/* Grid parity read-back: ids in sorted (cell, id) order and their linear
+ * cells; both arrays of n.  Buckets of the reference's HashTable
+ * (src/HashTable.h:45-53) are the runs of equal cell. */
+int sphg_scene_get_sorted(sphg_scene* scene, uint32_t* ids, int32_t* cells);
+/* Neighbor lists in id order: counts[n], ids[n*cap] (row i lists the ids of
+ * i's neighbors, unordered).  cap >= max_neighbors. */
+int sphg_scene_get_neighbors(sphg_scene* scene, uint32_t* counts, uint32_t* ids, uint32_t cap);
+/* Sizes: fluid, boundary, grid cells, device bytes held. */
+int sphg_scene_info(sphg_scene* scene, uint32_t* n, uint32_t* nb, uint32_t* ncells, uint64_t* device_bytes);
+
+/* Step several scenes of one method together: one launch per solver phase
+ * covers every scene (no reference counterpart: the reference runs one
+ * scene per process through its Simulation singleton,
+ * src/Simulation.cpp:84-90). */
+int sphg_batch_step(sphg_ctx* ctx, sphg_scene* const* scenes, uint32_t nscenes, uint32_t nsteps,
+                    sphg_step_stats* stats);
+
+/* Per-kernel device timing with HIP events on the library's stream.
+ * enable != 0 starts recording every launch; read returns, for kernel
+ * `name`, the launch count and the summed milliseconds since enabling. */
+int sphg_ctx_kernel_timing(sphg_ctx* ctx, int32_t enable);
+int sphg_ctx_kernel_time(sphg_ctx* ctx, const char* name, uint64_t* launches, double* total_ms);
+
+/* Scene geometry (set-up path, FP64 then rounded to float):
+ * AkinciBoundaryModel::sampleCube (src/AkinciBoundayModel.cpp:100-134);
+ * returns the count, writes min(count, cap) points. */
+int64_t sphg_sample_box(const double* mn, const double* mx, double support_radius, float* out, int64_t cap);
+/* AkinciBoundaryModel::sampleSphere (src/AkinciBoundayModel.cpp:136-164). */
+int64_t sphg_sample_sphere(const double* center, double radius, double particle_radius, float* out, int64_t cap);
+/* Simulation::buildFluidBlock (src/Simulation.cpp:512-539) with an optional
+ * seeded jitter of +-jitter*2r per coordinate (SplitMix64; 0 = none). */
+int64_t sphg_fluid_block(const double* mn, const double* mx, double particle_radius, double jitter, uint64_t seed,
+                         float* out, int64_t cap);
+/* Lattice sites (block lattice of box [mn, mx]) inside any of the spheres,
+ * in lattice order (scene extension; no reference counterpart). */
+int64_t sphg_fluid_spheres(const double* mn, const double* mx, double particle_radius, const double* centers,
+                           const double* radii, int32_t nspheres, float* out, int64_t cap);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SPHG_H */

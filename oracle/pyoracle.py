@@ -1,0 +1,253 @@
+"""ctypes wrapper of the FP64 CPU oracle (oracle/sph_oracle.cpp).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's CPU-baseline leg, never by the product path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Optional
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+
+OF_X, OF_V, OF_A, OF_RHO, OF_P, OF_ALPHA, OF_RHOSTAR = 0, 1, 2, 3, 4, 5, 6
+OF_DIVERR, OF_K, OF_KV, OF_BVOL, OF_NORMAL, OF_PACC, OF_ACC_STEP, OF_BX = 7, 8, 9, 10, 11, 12, 13, 14
+_VEC3 = {OF_X, OF_V, OF_A, OF_NORMAL, OF_PACC, OF_ACC_STEP}
+
+
+class OracleDesc(C.Structure):
+    _fields_ = [
+        ("method", C.c_int),
+        ("particle_radius", C.c_double),
+        ("density0", C.c_double),
+        ("gravity", C.c_double * 3),
+        ("time_step", C.c_double),
+        ("min_time_step", C.c_double),
+        ("max_time_step", C.c_double),
+        ("fps", C.c_double),
+        ("cfl_factor", C.c_double),
+        ("stiffness", C.c_double),
+        ("gamma", C.c_double),
+        ("eta", C.c_double),
+        ("eta_v", C.c_double),
+        ("min_iterations", C.c_uint),
+        ("max_iterations", C.c_uint),
+        ("min_iterations_v", C.c_uint),
+        ("max_iterations_v", C.c_uint),
+        ("viscosity_method", C.c_int),
+        ("viscosity", C.c_double),
+        ("boundary_viscosity", C.c_double),
+        ("surface_tension_method", C.c_int),
+        ("surface_tension", C.c_double),
+        ("adhesion_method", C.c_int),
+        ("adhesion", C.c_double),
+    ]
+
+
+_libs = {}
+
+
+def load(fast: bool = False) -> C.CDLL:
+    name = "liboracle_fast.so" if fast else "liboracle.so"
+    if name in _libs:
+        return _libs[name]
+    path = os.path.join(_HERE, name)
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} missing: run `make -C oracle`")
+    lib = C.CDLL(path)
+    P, U, D = C.c_void_p, C.c_uint, C.c_double
+    fp = C.POINTER(C.c_float)
+    dp = C.POINTER(C.c_double)
+    ip = C.POINTER(C.c_int)
+    up = C.POINTER(C.c_uint)
+    sig = {
+        "oracle_create": (P, [C.POINTER(OracleDesc)]),
+        "oracle_destroy": (None, [P]),
+        "oracle_set_threads": (C.c_int, [C.c_int]),
+        "oracle_set_fluid": (C.c_int, [P, fp, fp, U]),
+        "oracle_set_boundary": (C.c_int, [P, fp, U]),
+        "oracle_init": (C.c_int, [P]),
+        "oracle_set_state": (C.c_int, [P, fp, fp, fp, fp, D, U]),
+        "oracle_step": (C.c_int, [P, U]),
+        "oracle_num_fluid": (U, [P]),
+        "oracle_num_boundary": (U, [P]),
+        "oracle_get": (C.c_int, [P, C.c_int, dp]),
+        "oracle_scalars": (C.c_int, [P, dp]),
+        "oracle_fluid_keys": (C.c_int, [P, ip]),
+        "oracle_cell_points": (C.c_int, [P, U, up, U]),
+        "oracle_candidates": (C.c_int, [P, U, up, U, C.c_int]),
+        "oracle_sample_cube": (C.c_long, [dp, dp, D, dp, C.c_long]),
+        "oracle_sample_sphere": (C.c_long, [dp, D, D, dp, C.c_long]),
+        "oracle_fluid_block": (C.c_long, [dp, dp, D, dp, C.c_long]),
+        "oracle_kernel_W": (D, [D, D]),
+        "oracle_kernel_gradW": (None, [D, dp, dp]),
+        "oracle_kernel_cohesion": (D, [D, D]),
+        "oracle_kernel_adhesion": (D, [D, D]),
+        "oracle_kernel_spiky_grad": (None, [D, dp, dp]),
+        "oracle_kernel_poly6_grad": (None, [D, dp, dp]),
+    }
+    for n, (res, args) in sig.items():
+        f = getattr(lib, n)
+        f.restype = res
+        f.argtypes = args
+    _libs[name] = lib
+    return lib
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _fp(a):
+    return None if a is None else a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def make_desc(params) -> OracleDesc:
+    """From sph_amd.sphg.SceneParams (same field names)."""
+    d = OracleDesc()
+    for name, _ in OracleDesc._fields_:
+        v = getattr(params, name)
+        if name == "gravity":
+            d.gravity[:] = [float(x) for x in v]
+        else:
+            setattr(d, name, v)
+    return d
+
+
+class Oracle:
+    """FP64 restatement of the reference solver for one scene."""
+
+    def __init__(self, params, fluid, boundary, velocities=None, init: bool = True, fast: bool = False,
+                 threads: Optional[int] = None):
+        self.lib = load(fast)
+        if threads:
+            self.lib.oracle_set_threads(int(threads))
+        self.params = params
+        d = make_desc(params)
+        self.h = self.lib.oracle_create(C.byref(d))
+        x = np.ascontiguousarray(np.asarray(fluid, dtype=np.float32).reshape(-1, 3))
+        v = None if velocities is None else np.ascontiguousarray(np.asarray(velocities, dtype=np.float32).reshape(-1, 3))
+        b = np.ascontiguousarray(np.asarray(boundary, dtype=np.float32).reshape(-1, 3))
+        self.n, self.nb = x.shape[0], b.shape[0]
+        self.lib.oracle_set_fluid(self.h, _fp(x), _fp(v), self.n)
+        self.lib.oracle_set_boundary(self.h, _fp(b), self.nb)
+        if init:
+            self.lib.oracle_init(self.h)
+
+    def close(self):
+        if self.h:
+            self.lib.oracle_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def step(self, n: int = 1):
+        self.lib.oracle_step(self.h, int(n))
+
+    def set_state(self, x, v, k=None, kv=None, time_step=None, last_iterations=0):
+        x = np.ascontiguousarray(np.asarray(x, dtype=np.float32).reshape(-1, 3))
+        v = np.ascontiguousarray(np.asarray(v, dtype=np.float32).reshape(-1, 3))
+        k = None if k is None else np.ascontiguousarray(np.asarray(k, dtype=np.float32))
+        kv = None if kv is None else np.ascontiguousarray(np.asarray(kv, dtype=np.float32))
+        dt = self.params.time_step if time_step is None else float(time_step)
+        self.lib.oracle_set_state(self.h, _fp(x), _fp(v), _fp(k), _fp(kv), dt, int(last_iterations))
+
+    def get(self, field: int) -> np.ndarray:
+        if field in (OF_BVOL,):
+            out = np.zeros(max(self.nb, 1))
+            self.lib.oracle_get(self.h, field, _dp(out))
+            return out[: self.nb]
+        if field == OF_BX:
+            out = np.zeros((max(self.nb, 1), 3))
+            self.lib.oracle_get(self.h, field, _dp(out))
+            return out[: self.nb]
+        out = np.zeros((self.n, 3)) if field in _VEC3 else np.zeros(self.n)
+        self.lib.oracle_get(self.h, field, _dp(out))
+        return out
+
+    def scalars(self) -> dict:
+        out = np.zeros(16)
+        self.lib.oracle_scalars(self.h, _dp(out))
+        keys = ["time", "time_step", "iterations", "iterations_v", "avg_density_error", "avg_divergence_error",
+                "max_density_error", "steps", "sum_iterations", "sum_iterations_v", "max_vel", "pcisph_delta",
+                "cfl_time_step", "avg_density_error0", "avg_divergence_error0"]
+        return {k: float(out[i]) for i, k in enumerate(keys)}
+
+    def keys(self) -> np.ndarray:
+        out = np.zeros((self.n, 3), dtype=np.int32)
+        self.lib.oracle_fluid_keys(self.h, out.ctypes.data_as(C.POINTER(C.c_int)))
+        return out
+
+    def cell_points(self, i: int) -> np.ndarray:
+        cap = 4096
+        out = np.zeros(cap, dtype=np.uint32)
+        n = self.lib.oracle_cell_points(self.h, int(i), out.ctypes.data_as(C.POINTER(C.c_uint)), cap)
+        return out[:n].copy()
+
+    def candidates(self, i: int, boundary: bool = False) -> np.ndarray:
+        cap = 1 << 16
+        out = np.zeros(cap, dtype=np.uint32)
+        n = self.lib.oracle_candidates(self.h, int(i), out.ctypes.data_as(C.POINTER(C.c_uint)), cap,
+                                       1 if boundary else 0)
+        return out[:n].copy()
+
+
+def _sampler(fn, *args):
+    n = fn(*args, None, 0)
+    out = np.zeros((max(n, 1), 3))
+    fn(*args, _dp(out), n)
+    return out[:n]
+
+
+def sample_cube(mn, mx, support_radius):
+    lib = load()
+    return _sampler(lib.oracle_sample_cube, _dp(np.asarray(mn, float)), _dp(np.asarray(mx, float)),
+                    float(support_radius))
+
+
+def sample_sphere(center, radius, particle_radius):
+    lib = load()
+    return _sampler(lib.oracle_sample_sphere, _dp(np.asarray(center, float)), float(radius), float(particle_radius))
+
+
+def fluid_block(mn, mx, particle_radius):
+    lib = load()
+    return _sampler(lib.oracle_fluid_block, _dp(np.asarray(mn, float)), _dp(np.asarray(mx, float)),
+                    float(particle_radius))
+
+
+def kernel_W(h, r):
+    return load().oracle_kernel_W(float(h), float(r))
+
+
+def kernel_gradW(h, r):
+    out = np.zeros(3)
+    load().oracle_kernel_gradW(float(h), _dp(np.asarray(r, float)), _dp(out))
+    return out
+
+
+def kernel_spiky_grad(h, r):
+    out = np.zeros(3)
+    load().oracle_kernel_spiky_grad(float(h), _dp(np.asarray(r, float)), _dp(out))
+    return out
+
+
+def kernel_poly6_grad(h, r):
+    out = np.zeros(3)
+    load().oracle_kernel_poly6_grad(float(h), _dp(np.asarray(r, float)), _dp(out))
+    return out
+
+
+def kernel_cohesion(h, r):
+    return load().oracle_kernel_cohesion(float(h), float(r))
+
+
+def kernel_adhesion(h, r):
+    return load().oracle_kernel_adhesion(float(h), float(r))

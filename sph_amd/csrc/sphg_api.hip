@@ -1,0 +1,1133 @@
+// sphg_api.hip -- C ABI of libsphg: contexts, scenes, batched steps.
+//
+// The step sequences restate the reference solvers' phase order:
+//   WCSPH  src/WCSPHSolver.cpp:14-78
+//   PCISPH src/PCISPHSolver.cpp:21-81 (loop :360-387)
+//   DFSPH  src/DFSPHSolver.cpp:38-112 (loops :311-406, :494-590)
+// The iteration loops are driven from the host here, reading one int (the
+// number of scenes still iterating) per iteration.
+
+#include "../../include/sphg.h"
+#include "sphg_internal.h"
+#include "sphg_kernels.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+using namespace sphg;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                          \
+    do {                                                                                       \
+        hipError_t e_ = (expr);                                                                \
+        if (e_ != hipSuccess)                                                                  \
+            return fail(SPHG_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_));          \
+    } while (0)
+
+constexpr int kDefaultMaxNbr = 96;
+
+// ----------------------------------------------------------- profiling
+
+enum Kind {
+    K_KEYS, K_SCAN, K_SCATTER, K_CELLSORT, K_REORDER, K_BUILD, K_NORMALS, K_FORCES, K_STEPBEGIN, K_DT,
+    K_DFPREDICT, K_DFVEL_WARM, K_DFVEL_DENS, K_DFVEL_DIV, K_DFERR_DENS, K_DFERR_DIV, K_DFPOS, K_PCIDENS,
+    K_PCIPACC, K_PCIINT, K_TIME, K_NKINDS
+};
+const char* kKindNames[K_NKINDS] = {
+    "keys", "scan", "scatter", "cell_sort", "reorder", "build", "normals", "forces", "step_begin", "dt",
+    "df_predict", "df_vel_warm", "df_vel_density", "df_vel_divergence", "df_err_density", "df_err_divergence",
+    "df_positions", "pci_density", "pci_pacc", "pci_integrate", "time_advance"};
+
+struct Profiler {
+    bool on = false;
+    struct Rec {
+        int kind;
+        hipEvent_t a, b;
+    };
+    std::vector<Rec> pending;
+    std::vector<hipEvent_t> pool;
+    uint64_t count[K_NKINDS] = {};
+    double ms[K_NKINDS] = {};
+    hipEvent_t get() {
+        if (!pool.empty()) {
+            hipEvent_t e = pool.back();
+            pool.pop_back();
+            return e;
+        }
+        hipEvent_t e;
+        (void)hipEventCreate(&e);
+        return e;
+    }
+    void collect() {
+        if (pending.empty()) return;
+        for (auto& r : pending) {
+            (void)hipEventSynchronize(r.b);
+            float t = 0.f;
+            (void)hipEventElapsedTime(&t, r.a, r.b);
+            count[r.kind] += 1;
+            ms[r.kind] += t;
+            pool.push_back(r.a);
+            pool.push_back(r.b);
+        }
+        pending.clear();
+    }
+    ~Profiler() {
+        for (auto& r : pending) {
+            (void)hipEventDestroy(r.a);
+            (void)hipEventDestroy(r.b);
+        }
+        for (auto e : pool) (void)hipEventDestroy(e);
+    }
+};
+
+struct Scope {
+    Profiler* p;
+    hipStream_t st;
+    int kind;
+    hipEvent_t a{}, b{};
+    Scope(Profiler* p_, hipStream_t s, int k) : p(p_), st(s), kind(k) {
+        if (p->on) {
+            a = p->get();
+            (void)hipEventRecord(a, st);
+        }
+    }
+    ~Scope() {
+        if (p->on) {
+            b = p->get();
+            (void)hipEventRecord(b, st);
+            p->pending.push_back({kind, a, b});
+            if (p->pending.size() > 4096) p->collect();
+        }
+    }
+};
+
+}  // namespace
+
+struct sphg_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int* dActive = nullptr;
+    int* hActive = nullptr;   // pinned
+    Profiler prof;
+    std::vector<sphg_scene*> scenes;
+};
+
+namespace {
+
+struct Arena {
+    char* base = nullptr;
+    size_t size = 0, used = 0;
+    size_t reserve(size_t bytes) {
+        const size_t off = (used + 255) & ~size_t(255);
+        used = off + bytes;
+        return off;
+    }
+};
+
+struct Batch {
+    std::vector<sphg_scene*> scenes;
+    DevScene* dT[2] = {nullptr, nullptr};
+    Chunk* dP = nullptr;
+    Chunk* dC = nullptr;
+    int nP = 0, nC = 0;
+    int maxIt = 0;
+    std::vector<int> parity;   // parity of each scene when the tables were built
+    ~Batch() {
+        if (dT[0]) (void)hipFree(dT[0]);
+        if (dT[1]) (void)hipFree(dT[1]);
+        if (dP) (void)hipFree(dP);
+        if (dC) (void)hipFree(dC);
+    }
+};
+
+}  // namespace
+
+struct sphg_scene {
+    sphg_ctx* ctx = nullptr;
+    sphg_scene_desc desc{};
+    DevScene view[2];          // view[p]: buffers of parity p current
+    int par = 0;
+    bool ready = false;
+    std::vector<float> fxyz, fvel, bxyz;
+    std::vector<uint32_t> border;   // sorted boundary slot -> input index
+    uint32_t n = 0, nb = 0;
+    double dt0 = 0, dtMin = 0, dtMax = 0;
+    char* mem = nullptr;
+    size_t memBytes = 0;
+    float* tmp = nullptr;      // 4 floats per particle of staging
+    Scalars* ss = nullptr;
+    Batch* self = nullptr;
+    ~sphg_scene();
+};
+
+sphg_scene::~sphg_scene() {
+    delete self;
+    if (mem) (void)hipFree(mem);
+}
+
+namespace {
+
+// PCISPH scaling factor delta (src/PCISPHSolver.cpp:389-421): Poly6
+// gradients over a 3x3x3 prototype at spacing 2r.
+double pcisph_delta(double r, double h, double m, double rho0, double dt) {
+    const double h2 = h * h;
+    const double h9 = h * h * h * h * h * h * h * h * h;
+    const double gc = 945.0 / (32.0 * M_PI * h9);
+    const double dist = 2.0 * r;
+    double sx = 0, sy = 0, sz = 0, sq = 0;
+    for (int i = -1; i <= 1; i++)
+        for (int j = -1; j <= 1; j++)
+            for (int k = -1; k <= 1; k++) {
+                const double rx = -i * dist, ry = -j * dist, rz = -k * dist;
+                const double ml = std::sqrt(rx * rx + ry * ry + rz * rz);
+                const double ml2 = ml * ml;
+                double gx = 0, gy = 0, gz = 0;
+                if (ml2 <= h2) {
+                    const double d = h2 - ml2;
+                    const double f = -d * d * gc;
+                    gx = f * rx;
+                    gy = f * ry;
+                    gz = f * rz;
+                }
+                sx += gx;
+                sy += gy;
+                sz += gz;
+                sq += gx * gx + gy * gy + gz * gz;
+            }
+    const double beta = 2 * std::pow(m * dt / rho0, 2);
+    const double dot = sx * sx + sy * sy + sz * sz;
+    return -1.0 / (beta * (-dot - sq));
+}
+
+int check_desc(const sphg_scene_desc& d) {
+    if (d.method < 0 || d.method > 2) return fail(SPHG_EINVAL, "simulationMethod must be 0 (WCSPH), 1 (PCISPH) or 2 (DFSPH)");
+    if (d.boundary_method != SPHG_BOUNDARY_AKINCI) return fail(SPHG_EINVAL, "only boundaryMethod 2 (Akinci) runs on the GPU solver");
+    if (!(d.particle_radius > 0)) return fail(SPHG_EINVAL, "particleRadius must be > 0");
+    if (!(d.density0 > 0)) return fail(SPHG_EINVAL, "density0 must be > 0");
+    if (!(d.time_step > 0)) return fail(SPHG_EINVAL, "timeStep must be > 0");
+    if (!(d.fps > 0)) return fail(SPHG_EINVAL, "fps must be > 0");
+    if (d.method == 0 && !(d.gamma != 0)) return fail(SPHG_EINVAL, "WCSPH gamma must be nonzero");
+    if (d.viscosity_method >= 0 && d.viscosity_method != SPHG_VISCOSITY_XSPH && d.viscosity != 0.0)
+        return fail(SPHG_EINVAL, "only viscosityMethod 2 (XSPH) runs on the GPU solver");
+    if (d.max_neighbors > 65535) return fail(SPHG_EINVAL, "max_neighbors must be <= 65535");
+    return SPHG_OK;
+}
+
+void fill_coef(KCoef& k, double h) {
+    const double pi = M_PI;
+    const double h3 = h * h * h;
+    const double cl = 48.0 / (pi * h3);
+    const double h6 = h3 * h3;
+    const double h9 = h6 * h3;
+    k.h = (float)h;
+    k.invh = (float)(1.0 / h);
+    k.h2 = (float)(h * h);
+    k.ck = (float)(8.0 / (pi * h3));
+    k.clh2 = (float)(cl / (h * h));
+    k.clh = (float)(cl / h);
+    k.spg = (float)(45.0 / (pi * h6));
+    k.co1 = (float)(32.0 / (pi * h9));
+    k.co2 = (float)(h6 / 64.0);
+    k.hlo2 = (float)(h * h * (1.0 - 1e-4));
+    k.hhi2 = (float)(h * h * (1.0 + 1e-4));
+}
+
+int upload_batch_tables(sphg_ctx* ctx, Batch& B) {
+    const int ns = (int)B.scenes.size();
+    std::vector<DevScene> t0(ns), t1(ns);
+    std::vector<Chunk> P, Cc;
+    B.parity.resize(ns);
+    B.maxIt = 0;
+    for (int s = 0; s < ns; ++s) {
+        sphg_scene* sc = B.scenes[s];
+        t0[s] = sc->view[sc->par];
+        t1[s] = sc->view[1 - sc->par];
+        B.parity[s] = sc->par;
+        for (int c = 0; c < t0[s].nChunks; ++c) P.push_back(Chunk{s, c * kBlock});
+        for (int c = 0; c < t0[s].nCellChunks; ++c) Cc.push_back(Chunk{s, c * kCellChunk});
+        B.maxIt = std::max(B.maxIt, std::max(t0[s].maxIt, std::max(t0[s].maxItV, 0)));
+    }
+    if (!B.dT[0]) {
+        HIP_TRY(hipMalloc(&B.dT[0], sizeof(DevScene) * ns));
+        HIP_TRY(hipMalloc(&B.dT[1], sizeof(DevScene) * ns));
+    }
+    if (!B.dP) {
+        HIP_TRY(hipMalloc(&B.dP, sizeof(Chunk) * std::max<size_t>(P.size(), 1)));
+        HIP_TRY(hipMalloc(&B.dC, sizeof(Chunk) * std::max<size_t>(Cc.size(), 1)));
+    }
+    HIP_TRY(hipMemcpyAsync(B.dT[0], t0.data(), sizeof(DevScene) * ns, hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(B.dT[1], t1.data(), sizeof(DevScene) * ns, hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(B.dP, P.data(), sizeof(Chunk) * P.size(), hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(B.dC, Cc.data(), sizeof(Chunk) * Cc.size(), hipMemcpyHostToDevice, ctx->stream));
+    B.nP = (int)P.size();
+    B.nC = (int)Cc.size();
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return SPHG_OK;
+}
+
+// Counting sort of the fluid into (cell, id) order; tables T (before) and
+// T2 (after: the other half of each double buffer is current).  Keys and
+// histogram are formed by the caller (k_keys or k_df_positions).
+void grid_sort(sphg_ctx* ctx, Batch& B, const DevScene* T) {
+    Profiler* P = &ctx->prof;
+    hipStream_t st = ctx->stream;
+    const int ns = (int)B.scenes.size();
+    { Scope s(P, st, K_SCAN); launch_scan(st, T, B.dC, B.nC, ns); }
+    { Scope s(P, st, K_SCATTER); launch_scatter(st, T, B.dP, B.nP); }
+    { Scope s(P, st, K_CELLSORT); launch_cell_sort(st, T, B.dC, B.nC); }
+    { Scope s(P, st, K_REORDER); launch_reorder(st, T, B.dP, B.nP); }
+}
+
+int read_active(sphg_ctx* ctx, int& v, uint32_t& syncs) {
+    HIP_TRY(hipMemcpyAsync(ctx->hActive, ctx->dActive, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    v = *ctx->hActive;
+    ++syncs;
+    return SPHG_OK;
+}
+
+void flip(Batch& B) {
+    std::swap(B.dT[0], B.dT[1]);
+    for (size_t i = 0; i < B.scenes.size(); ++i) {
+        B.scenes[i]->par ^= 1;
+        B.parity[i] = B.scenes[i]->par;
+    }
+}
+
+// The one-scene tables of s, refreshed if another batch moved its parity.
+int self_tables(sphg_scene* s) {
+    Batch& B = *s->self;
+    if (B.parity[0] != s->par) return upload_batch_tables(s->ctx, B);
+    return SPHG_OK;
+}
+
+// One step of every scene in B (all of one method).
+int batch_one_step(sphg_ctx* ctx, Batch& B, uint32_t& syncs) {
+    Profiler* P = &ctx->prof;
+    hipStream_t st = ctx->stream;
+    const int method = B.scenes[0]->desc.method;
+    const int ns = (int)B.scenes.size();
+    int anyFlags = 0, anySt = 0;
+    for (auto* s : B.scenes) {
+        anyFlags |= s->view[0].flags;
+        anySt |= (s->view[0].flags & F_ST);
+    }
+    int rc;
+    if (method == DFSPH) {
+        const DevScene* T = B.dT[0];
+        if (anyFlags) {
+            if (anySt) { Scope s(P, st, K_NORMALS); launch_normals(st, T, B.dP, B.nP); }
+            Scope s(P, st, K_FORCES);
+            launch_forces(st, DFSPH, T, B.dP, B.nP);
+        }
+        { Scope s(P, st, K_DT); launch_dt(st, T, B.dP, B.nP); }
+        { Scope s(P, st, K_DFPREDICT); launch_df_predict(st, T, B.dP, B.nP); }
+        { Scope s(P, st, K_DFVEL_WARM); launch_df_vel(st, 0, T, B.dP, B.nP); }
+        launch_fill_int(st, ctx->dActive, 0, 1);
+        { Scope s(P, st, K_DFERR_DENS); launch_df_err(st, 0, T, B.dP, B.nP, ctx->dActive); }
+        for (int it = 0; it < B.maxIt; ++it) {
+            int act = 0;
+            if ((rc = read_active(ctx, act, syncs))) return rc;
+            if (act <= 0) break;
+            { Scope s(P, st, K_DFVEL_DENS); launch_df_vel(st, 1, T, B.dP, B.nP); }
+            { Scope s(P, st, K_DFERR_DENS); launch_df_err(st, 1, T, B.dP, B.nP, ctx->dActive); }
+        }
+        { Scope s(P, st, K_DFPOS); launch_df_positions(st, T, B.dP, B.nP); }
+        grid_sort(ctx, B, T);
+        const DevScene* T2 = B.dT[1];
+        { Scope s(P, st, K_BUILD); launch_build(st, DFSPH, T2, B.dP, B.nP); }
+        launch_fill_int(st, ctx->dActive, 0, 1);
+        { Scope s(P, st, K_DFERR_DIV); launch_df_err(st, 2, T2, B.dP, B.nP, ctx->dActive); }
+        for (int it = 0; it < B.maxIt; ++it) {
+            int act = 0;
+            if ((rc = read_active(ctx, act, syncs))) return rc;
+            if (act <= 0) break;
+            { Scope s(P, st, K_DFVEL_DIV); launch_df_vel(st, 2, T2, B.dP, B.nP); }
+            { Scope s(P, st, K_DFERR_DIV); launch_df_err(st, 3, T2, B.dP, B.nP, ctx->dActive); }
+        }
+        flip(B);
+    } else if (method == WCSPH) {
+        const DevScene* T = B.dT[0];
+        launch_fill_int(st, ctx->dActive, 0, 1);
+        { Scope s(P, st, K_STEPBEGIN); launch_step_begin(st, T, B.dP, B.nP, ctx->dActive); }
+        { Scope s(P, st, K_KEYS); launch_keys(st, T, B.dP, B.nP); }
+        grid_sort(ctx, B, T);
+        const DevScene* T2 = B.dT[1];
+        { Scope s(P, st, K_BUILD); launch_build(st, WCSPH, T2, B.dP, B.nP); }
+        if (anySt) { Scope s(P, st, K_NORMALS); launch_normals(st, T2, B.dP, B.nP); }
+        { Scope s(P, st, K_FORCES); launch_forces(st, WCSPH, T2, B.dP, B.nP); }
+        { Scope s(P, st, K_TIME); launch_time_advance(st, T2, ns); }
+        flip(B);
+    } else {
+        const DevScene* T = B.dT[0];
+        launch_fill_int(st, ctx->dActive, 0, 1);
+        { Scope s(P, st, K_STEPBEGIN); launch_step_begin(st, T, B.dP, B.nP, ctx->dActive); }
+        { Scope s(P, st, K_KEYS); launch_keys(st, T, B.dP, B.nP); }
+        grid_sort(ctx, B, T);
+        const DevScene* T2 = B.dT[1];
+        { Scope s(P, st, K_BUILD); launch_build(st, PCISPH, T2, B.dP, B.nP); }
+        if (anySt) { Scope s(P, st, K_NORMALS); launch_normals(st, T2, B.dP, B.nP); }
+        { Scope s(P, st, K_FORCES); launch_forces(st, PCISPH, T2, B.dP, B.nP); }
+        for (int it = 0; it < B.maxIt; ++it) {
+            int act = 0;
+            if ((rc = read_active(ctx, act, syncs))) return rc;
+            if (act <= 0) break;
+            { Scope s(P, st, K_PCIDENS); launch_pci_density(st, T2, B.dP, B.nP, ctx->dActive); }
+            { Scope s(P, st, K_PCIPACC); launch_pci_pacc(st, T2, B.dP, B.nP, it); }
+        }
+        { Scope s(P, st, K_PCIINT); launch_pci_integrate(st, T2, B.dP, B.nP); }
+        flip(B);
+    }
+    return SPHG_OK;
+}
+
+int run_batch(sphg_ctx* ctx, Batch& B, uint32_t nsteps, sphg_step_stats* stats) {
+    hipStream_t st = ctx->stream;
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    std::vector<Scalars> before(B.scenes.size());
+    if (stats) {
+        for (size_t s = 0; s < B.scenes.size(); ++s)
+            HIP_TRY(hipMemcpyAsync(&before[s], B.scenes[s]->ss, sizeof(Scalars), hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+    }
+    uint32_t syncs = 0;
+    HIP_TRY(hipEventRecord(e0, st));
+    for (uint32_t k = 0; k < nsteps; ++k) {
+        int rc = batch_one_step(ctx, B, syncs);
+        if (rc) {
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            return rc;
+        }
+    }
+    HIP_TRY(hipEventRecord(e1, st));
+    HIP_TRY(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    unsigned overflow = 0;
+    if (stats) {
+        std::memset(stats, 0, sizeof(*stats));
+        stats->gpu_ms = ms;
+        stats->steps = nsteps;
+        stats->host_syncs = syncs;
+    }
+    for (size_t s = 0; s < B.scenes.size(); ++s) {
+        Scalars now;
+        HIP_TRY(hipMemcpy(&now, B.scenes[s]->ss, sizeof(Scalars), hipMemcpyDeviceToHost));
+        if (stats) {
+            stats->iterations += now.sumIt - before[s].sumIt;
+            stats->iterations_v += now.sumItV - before[s].sumItV;
+        }
+        overflow = std::max(overflow, now.overflow);
+    }
+    if (overflow)
+        return fail(SPHG_EOVERFLOW, "a particle had " + std::to_string(overflow) +
+                                        " neighbors, above max_neighbors; results of that step are invalid");
+    return SPHG_OK;
+}
+
+}  // namespace
+
+// ================================================================ C API
+
+extern "C" {
+
+const char* sphg_version(void) { return "sphg 0.1 (gfx950)"; }
+const char* sphg_last_error(void) { return g_err.c_str(); }
+
+int sphg_ctx_create(int32_t device, sphg_ctx** out) {
+    if (!out) return fail(SPHG_EINVAL, "out is NULL");
+    int n = 0;
+    HIP_TRY(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n) return fail(SPHG_EINVAL, "no HIP device " + std::to_string(device));
+    HIP_TRY(hipSetDevice(device));
+    sphg_ctx* c = new sphg_ctx();
+    c->device = device;
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipMalloc(&c->dActive, sizeof(int));
+    if (e == hipSuccess) e = hipHostMalloc(&c->hActive, sizeof(int), hipHostMallocDefault);
+    if (e != hipSuccess) {
+        delete c;
+        return fail(SPHG_EHIP, std::string("context set-up: ") + hipGetErrorString(e));
+    }
+    *out = c;
+    return SPHG_OK;
+}
+
+int sphg_ctx_destroy(sphg_ctx* ctx) {
+    if (!ctx) return SPHG_OK;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    for (auto* s : ctx->scenes) {
+        s->ctx = nullptr;
+        delete s;
+    }
+    ctx->scenes.clear();
+    ctx->prof.collect();
+    if (ctx->dActive) (void)hipFree(ctx->dActive);
+    if (ctx->hActive) (void)hipHostFree(ctx->hActive);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return SPHG_OK;
+}
+
+int sphg_ctx_synchronize(sphg_ctx* ctx) {
+    if (!ctx) return fail(SPHG_EINVAL, "ctx is NULL");
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return SPHG_OK;
+}
+
+int sphg_scene_create(sphg_ctx* ctx, const sphg_scene_desc* desc, sphg_scene** out) {
+    if (!ctx || !desc || !out) return fail(SPHG_EINVAL, "NULL argument");
+    int rc = check_desc(*desc);
+    if (rc) return rc;
+    sphg_scene* s = new sphg_scene();
+    s->ctx = ctx;
+    s->desc = *desc;
+    ctx->scenes.push_back(s);
+    *out = s;
+    return SPHG_OK;
+}
+
+int sphg_scene_destroy(sphg_scene* s) {
+    if (!s) return SPHG_OK;
+    if (s->ctx) {
+        (void)hipSetDevice(s->ctx->device);
+        (void)hipStreamSynchronize(s->ctx->stream);
+        auto& v = s->ctx->scenes;
+        v.erase(std::remove(v.begin(), v.end(), s), v.end());
+    }
+    delete s;
+    return SPHG_OK;
+}
+
+int sphg_scene_set_fluid(sphg_scene* s, const float* xyz, const float* vel, uint32_t n) {
+    if (!s || (!xyz && n)) return fail(SPHG_EINVAL, "NULL argument");
+    if (s->ready) return fail(SPHG_ESTATE, "set_fluid after init: use sphg_scene_set_state");
+    if (n == 0) return fail(SPHG_EINVAL, "a scene needs at least one fluid particle");
+    s->fxyz.assign(xyz, xyz + 3 * (size_t)n);
+    if (vel) s->fvel.assign(vel, vel + 3 * (size_t)n);
+    else s->fvel.assign(3 * (size_t)n, 0.f);
+    s->n = n;
+    return SPHG_OK;
+}
+
+int sphg_scene_set_boundary(sphg_scene* s, const float* xyz, uint32_t nb) {
+    if (!s || (!xyz && nb)) return fail(SPHG_EINVAL, "NULL argument");
+    if (s->ready) return fail(SPHG_ESTATE, "set_boundary after init");
+    s->bxyz.assign(xyz, xyz + 3 * (size_t)nb);
+    s->nb = nb;
+    return SPHG_OK;
+}
+
+int sphg_scene_init(sphg_scene* s) {
+    if (!s || !s->ctx) return fail(SPHG_EINVAL, "NULL scene");
+    if (s->ready) return fail(SPHG_ESTATE, "scene already initialised");
+    if (s->n == 0) return fail(SPHG_ESTATE, "no fluid particles");
+    sphg_ctx* ctx = s->ctx;
+    HIP_TRY(hipSetDevice(ctx->device));
+    const sphg_scene_desc& d = s->desc;
+    const double r = d.particle_radius;
+    const double h = 4.0 * r;
+    const double rho0 = d.density0;
+    const double m = rho0 * std::pow(2.0 * r, 3.0);
+    const int method = d.method;
+
+    // grid bounds in key space
+    double lo[3], hi[3];
+    bool given = false;
+    for (int a = 0; a < 3; ++a) given |= (d.domain_min[a] != 0.0 || d.domain_max[a] != 0.0);
+    if (given) {
+        for (int a = 0; a < 3; ++a) { lo[a] = d.domain_min[a]; hi[a] = d.domain_max[a]; }
+    } else {
+        for (int a = 0; a < 3; ++a) { lo[a] = 1e300; hi[a] = -1e300; }
+        auto acc = [&](const std::vector<float>& p) {
+            for (size_t i = 0; i + 2 < p.size(); i += 3)
+                for (int a = 0; a < 3; ++a) {
+                    const double v = p[i + a];
+                    if (std::isfinite(v)) { lo[a] = std::min(lo[a], v); hi[a] = std::max(hi[a], v); }
+                }
+        };
+        acc(s->fxyz);
+        acc(s->bxyz);
+    }
+    int k0[3], dims[3];
+    long ncells = 1;
+    for (int a = 0; a < 3; ++a) {
+        const long kl = (long)std::floor(lo[a] / h) - 1;
+        const long kh = (long)std::floor(hi[a] / h) + 1;
+        if (kh - kl + 3 > (1L << 20)) return fail(SPHG_EINVAL, "grid too large along one axis");
+        k0[a] = (int)kl;
+        dims[a] = (int)(kh - kl + 1) + 2;
+        ncells *= dims[a];
+    }
+    if (ncells > (1L << 28)) return fail(SPHG_EINVAL, "grid has more than 2^28 cells; set domain_min/max");
+
+    const uint32_t n = s->n, nb = s->nb;
+    const int nChunks = (int)((n + kBlock - 1) / kBlock);
+    const size_t npad = (size_t)nChunks * kBlock;
+    const int nCellChunks = (int)((ncells + 1 + kCellChunk - 1) / kCellChunk);
+    const int maxNbr = d.max_neighbors ? (int)d.max_neighbors : kDefaultMaxNbr;
+
+    int flags = 0;
+    if (d.viscosity_method == SPHG_VISCOSITY_XSPH && d.viscosity != 0.0) flags |= F_XSPH;
+    if (d.surface_tension_method == SPHG_SURFACE_TENSION_AKINCI && d.surface_tension != 0.0) flags |= F_ST;
+    if (d.adhesion_method == SPHG_ADHESION_AKINCI && d.adhesion != 0.0) flags |= F_ADH;
+
+    // memory plan
+    Arena A;
+    const size_t f4 = sizeof(float4) * npad, f1 = sizeof(float) * npad, u1 = sizeof(uint32_t) * npad;
+    size_t oX[2], oV[2], oK[2] = {0, 0}, oKV[2] = {0, 0}, oID[2];
+    for (int p = 0; p < 2; ++p) { oX[p] = A.reserve(f4); oV[p] = A.reserve(f4); oID[p] = A.reserve(u1); }
+    if (method == DFSPH)
+        for (int p = 0; p < 2; ++p) { oK[p] = A.reserve(f1); oKV[p] = A.reserve(f1); }
+    const size_t oA = A.reserve(f4), oXK = A.reserve(f4), oGB = A.reserve(f4);
+    const size_t oAUX = (method != DFSPH) ? A.reserve(f4) : 0;
+    const size_t oXS = (method == PCISPH) ? A.reserve(f4) : 0;
+    const size_t oPACC = (method == PCISPH) ? A.reserve(f4) : 0;
+    const size_t oGBS = (method == PCISPH) ? A.reserve(f4) : 0;
+    const size_t oNRM = (flags & F_ST) ? A.reserve(f4) : 0;
+    const size_t oADH = (flags & F_ADH) ? A.reserve(f4) : 0;
+    const size_t oRHO = A.reserve(f1);
+    const size_t oALPHA = (method == DFSPH) ? A.reserve(f1) : 0;
+    const size_t oRHOS = (method == DFSPH) ? A.reserve(f1) : 0;
+    const size_t oDIV = (method == DFSPH) ? A.reserve(f1) : 0;
+    const size_t oP = (method != DFSPH) ? A.reserve(f1) : 0;
+    const size_t oCELLU = A.reserve(u1), oCELL = A.reserve(u1), oPERM = A.reserve(u1);
+    const size_t oCC = A.reserve(sizeof(int) * (size_t)ncells);
+    const size_t oCS = A.reserve(sizeof(int) * (size_t)(ncells + 1));
+    const size_t oCCS = A.reserve(sizeof(int) * (size_t)nCellChunks);
+    const size_t oNBR = A.reserve(sizeof(uint32_t) * npad * (size_t)maxNbr);
+    const size_t oNNB = A.reserve(sizeof(uint16_t) * npad);
+    const size_t oNG = A.reserve(npad);
+    const size_t oXB = A.reserve(sizeof(float4) * std::max<size_t>(nb, 1));
+    const size_t oBCS = A.reserve(sizeof(int) * (size_t)(ncells + 1));
+    const size_t oPART = A.reserve(sizeof(double) * (size_t)std::max(nChunks, 1));
+    const size_t oSS = A.reserve(sizeof(Scalars));
+    const size_t oTMP = A.reserve(std::max(f4, sizeof(float4) * std::max<size_t>(nb, 1)));
+    hipError_t he = hipMalloc(&s->mem, A.used);
+    if (he != hipSuccess) return fail(SPHG_ENOMEM, "device allocation of " + std::to_string(A.used) + " bytes failed");
+    s->memBytes = A.used;
+    char* b = s->mem;
+    HIP_TRY(hipMemsetAsync(b, 0, A.used, ctx->stream));
+
+    DevScene D{};
+    fill_coef(D.kc, h);
+    D.hD = h;
+    D.r = r;
+    D.cfl = d.cfl_factor;
+    const double minFps = 1.0 / d.fps;
+    s->dtMin = (minFps < d.min_time_step) ? minFps : d.min_time_step;   // src/TimeManager.h:55
+    s->dtMax = d.max_time_step;
+    D.dtMin = s->dtMin;
+    D.dtMax = s->dtMax;
+    D.eta = d.eta;
+    D.etaV = d.eta_v;
+    D.m = (float)m;
+    D.rho0 = (float)rho0;
+    D.W0 = D.kc.ck;
+    D.pciDelta = (method == PCISPH) ? (float)pcisph_delta(r, h, m, rho0, d.time_step) : 0.f;
+    D.gx = (float)d.gravity[0];
+    D.gy = (float)d.gravity[1];
+    D.gz = (float)d.gravity[2];
+    D.nu = (float)d.viscosity;
+    D.nub = (float)d.boundary_viscosity;
+    D.stCoef = (float)d.surface_tension;
+    D.adhBeta = (float)d.adhesion;
+    D.stiffness = (float)d.stiffness;
+    D.gammaEos = (float)d.gamma;
+    D.method = method;
+    D.flags = flags;
+    D.minIt = (int)d.min_iterations;
+    D.maxIt = (int)d.max_iterations;
+    D.minItV = (int)d.min_iterations_v;
+    D.maxItV = (int)d.max_iterations_v;
+    D.kx0 = k0[0]; D.ky0 = k0[1]; D.kz0 = k0[2];
+    D.nx = dims[0]; D.ny = dims[1]; D.nz = dims[2];
+    D.ncells = (int)ncells;
+    D.n = (int)n;
+    D.nb = (int)nb;
+    D.maxNbr = maxNbr;
+    D.nChunks = nChunks;
+    D.nCellChunks = nCellChunks;
+    auto F4 = [&](size_t o) { return (float4*)(b + o); };
+    auto F1 = [&](size_t o) { return (float*)(b + o); };
+    D.a = F4(oA);
+    D.xk = F4(oXK);
+    D.gb = F4(oGB);
+    D.aux = oAUX ? F4(oAUX) : nullptr;
+    D.xs = oXS ? F4(oXS) : nullptr;
+    D.pacc = oPACC ? F4(oPACC) : nullptr;
+    D.gbs = oGBS ? F4(oGBS) : nullptr;
+    D.nrm = oNRM ? F4(oNRM) : nullptr;
+    D.adh = oADH ? F4(oADH) : nullptr;
+    D.rho = F1(oRHO);
+    D.alpha = oALPHA ? F1(oALPHA) : nullptr;
+    D.rhoS = oRHOS ? F1(oRHOS) : nullptr;
+    D.divErr = oDIV ? F1(oDIV) : nullptr;
+    D.p = oP ? F1(oP) : nullptr;
+    D.cellU = (int*)(b + oCELLU);
+    D.cell = (int*)(b + oCELL);
+    D.perm = (uint32_t*)(b + oPERM);
+    D.cellCount = (int*)(b + oCC);
+    D.cellStart = (int*)(b + oCS);
+    D.cellChunkSum = (int*)(b + oCCS);
+    D.nbr = (uint32_t*)(b + oNBR);
+    D.nnb = (uint16_t*)(b + oNNB);
+    D.ngrad = (uint8_t*)(b + oNG);
+    D.xb = F4(oXB);
+    D.bCellStart = (int*)(b + oBCS);
+    D.partial = (double*)(b + oPART);
+    D.ss = (Scalars*)(b + oSS);
+    s->ss = D.ss;
+    s->tmp = F1(oTMP);
+    for (int p = 0; p < 2; ++p) {
+        DevScene v = D;
+        v.x = F4(oX[p]); v.x2 = F4(oX[1 - p]);
+        v.v = F4(oV[p]); v.v2 = F4(oV[1 - p]);
+        v.id = (uint32_t*)(b + oID[p]); v.id2 = (uint32_t*)(b + oID[1 - p]);
+        v.k = oK[p] ? F1(oK[p]) : nullptr; v.k2 = oK[1 - p] ? F1(oK[1 - p]) : nullptr;
+        v.kv = oKV[p] ? F1(oKV[p]) : nullptr; v.kv2 = oKV[1 - p] ? F1(oKV[1 - p]) : nullptr;
+        s->view[p] = v;
+    }
+    s->par = 0;
+
+    // boundary: stable sort by linear cell on the host, cell table, upload
+    {
+        std::vector<std::pair<long, uint32_t>> keyed(nb);
+        for (uint32_t i = 0; i < nb; ++i) {
+            long lin = 0, mul = 1;
+            for (int a = 0; a < 3; ++a) {
+                double f = std::floor((double)s->bxyz[3 * i + a] / h);
+                if (!(f == f)) f = k0[a];
+                long ix = (long)std::min(std::max(f - k0[a] + 1.0, 1.0), (double)(dims[a] - 2));
+                lin += ix * mul;
+                mul *= dims[a];
+            }
+            keyed[i] = {lin, i};
+        }
+        std::stable_sort(keyed.begin(), keyed.end(), [](auto& x, auto& y) { return x.first < y.first; });
+        std::vector<float4> xb(std::max<size_t>(nb, 1));
+        std::vector<int> bcs((size_t)ncells + 1, 0);
+        s->border.resize(nb);
+        for (uint32_t i = 0; i < nb; ++i) {
+            const uint32_t o = keyed[i].second;
+            s->border[i] = o;
+            xb[i] = make_float4(s->bxyz[3 * o], s->bxyz[3 * o + 1], s->bxyz[3 * o + 2], 0.f);
+            bcs[keyed[i].first + 1] += 1;
+        }
+        for (long c = 0; c < ncells; ++c) bcs[c + 1] += bcs[c];
+        HIP_TRY(hipMemcpyAsync(D.xb, xb.data(), sizeof(float4) * nb, hipMemcpyHostToDevice, ctx->stream));
+        HIP_TRY(hipMemcpyAsync(D.bCellStart, bcs.data(), sizeof(int) * bcs.size(), hipMemcpyHostToDevice, ctx->stream));
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+    }
+
+    // scalars
+    Scalars ss0{};
+    ss0.dt = d.time_step;
+    HIP_TRY(hipMemcpyAsync(D.ss, &ss0, sizeof(Scalars), hipMemcpyHostToDevice, ctx->stream));
+
+    // tables of a one-scene batch
+    s->self = new Batch();
+    s->self->scenes = {s};
+    int rc = upload_batch_tables(ctx, *s->self);
+    if (rc) return rc;
+
+    // boundary volumes (src/SPHSolver.cpp:22-39)
+    if (nb) launch_bvolume(ctx->stream, s->self->dT[0], 0, (int)nb, s->tmp);
+
+    // fluid upload (staging in tmp is too small for xyz+vel: copy through a scratch allocation)
+    float* dstage = nullptr;
+    HIP_TRY(hipMalloc(&dstage, sizeof(float) * 6 * (size_t)n));
+    HIP_TRY(hipMemcpyAsync(dstage, s->fxyz.data(), sizeof(float) * 3 * n, hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(dstage + 3 * (size_t)n, s->fvel.data(), sizeof(float) * 3 * n, hipMemcpyHostToDevice, ctx->stream));
+    launch_load_state(ctx->stream, s->self->dT[0], 0, (int)n, dstage, dstage + 3 * (size_t)n, nullptr, nullptr);
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    (void)hipFree(dstage);
+
+    if (method == DFSPH) {
+        // DFSPHSolver::init: insert, neighborhoodSearch, densities, alpha
+        Batch& B = *s->self;
+        launch_keys(ctx->stream, B.dT[0], B.dP, B.nP);
+        grid_sort(ctx, B, B.dT[0]);
+        launch_build(ctx->stream, DFSPH, B.dT[1], B.dP, B.nP);
+        flip(B);
+    }
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    HIP_TRY(hipGetLastError());
+    s->ready = true;
+    s->fxyz.clear();
+    s->fxyz.shrink_to_fit();
+    s->fvel.clear();
+    s->fvel.shrink_to_fit();
+    return SPHG_OK;
+}
+
+int sphg_scene_set_state(sphg_scene* s, const float* xyz, const float* vel, const float* k, const float* kv,
+                         double time_step, uint32_t last_iterations) {
+    if (!s || !xyz || !vel) return fail(SPHG_EINVAL, "NULL argument");
+    if (!s->ready) return fail(SPHG_ESTATE, "set_state before init");
+    sphg_ctx* ctx = s->ctx;
+    HIP_TRY(hipSetDevice(ctx->device));
+    const size_t n = s->n;
+    float* dstage = nullptr;
+    HIP_TRY(hipMalloc(&dstage, sizeof(float) * 8 * n));
+    HIP_TRY(hipMemcpyAsync(dstage, xyz, sizeof(float) * 3 * n, hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(dstage + 3 * n, vel, sizeof(float) * 3 * n, hipMemcpyHostToDevice, ctx->stream));
+    if (k) HIP_TRY(hipMemcpyAsync(dstage + 6 * n, k, sizeof(float) * n, hipMemcpyHostToDevice, ctx->stream));
+    if (kv) HIP_TRY(hipMemcpyAsync(dstage + 7 * n, kv, sizeof(float) * n, hipMemcpyHostToDevice, ctx->stream));
+    Batch& B = *s->self;
+    int rc = self_tables(s);
+    if (rc) return rc;
+    launch_load_state(ctx->stream, B.dT[0], 0, (int)n, dstage, dstage + 3 * n, k ? dstage + 6 * n : nullptr,
+                      kv ? dstage + 7 * n : nullptr);
+    Scalars ss;
+    HIP_TRY(hipMemcpyAsync(&ss, s->ss, sizeof(Scalars), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    ss.dt = time_step;
+    ss.it = (int)last_iterations;
+    HIP_TRY(hipMemcpyAsync(s->ss, &ss, sizeof(Scalars), hipMemcpyHostToDevice, ctx->stream));
+    if (s->desc.method == DFSPH) {
+        launch_keys(ctx->stream, B.dT[0], B.dP, B.nP);
+        grid_sort(ctx, B, B.dT[0]);
+        launch_build(ctx->stream, DFSPH, B.dT[1], B.dP, B.nP);
+        flip(B);
+    }
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    (void)hipFree(dstage);
+    HIP_TRY(hipGetLastError());
+    return SPHG_OK;
+}
+
+int sphg_scene_step(sphg_scene* s, uint32_t nsteps, sphg_step_stats* stats) {
+    if (!s || !s->ctx) return fail(SPHG_EINVAL, "NULL scene");
+    if (!s->ready) return fail(SPHG_ESTATE, "step before init");
+    HIP_TRY(hipSetDevice(s->ctx->device));
+    Batch& B = *s->self;
+    int rc = self_tables(s);
+    if (rc) return rc;
+    rc = run_batch(s->ctx, B, nsteps, stats);
+    if (rc) return rc;
+    HIP_TRY(hipGetLastError());
+    return SPHG_OK;
+}
+
+int sphg_batch_step(sphg_ctx* ctx, sphg_scene* const* scenes, uint32_t nscenes, uint32_t nsteps,
+                    sphg_step_stats* stats) {
+    if (!ctx || (!scenes && nscenes)) return fail(SPHG_EINVAL, "NULL argument");
+    if (nscenes == 0) return SPHG_OK;
+    HIP_TRY(hipSetDevice(ctx->device));
+    const int method = scenes[0]->desc.method;
+    for (uint32_t i = 0; i < nscenes; ++i) {
+        if (!scenes[i] || scenes[i]->ctx != ctx) return fail(SPHG_EINVAL, "scene of another context");
+        if (!scenes[i]->ready) return fail(SPHG_ESTATE, "scene not initialised");
+        if (scenes[i]->desc.method != method) return fail(SPHG_EINVAL, "a batch steps scenes of one method");
+        for (uint32_t j = 0; j < i; ++j)
+            if (scenes[j] == scenes[i]) return fail(SPHG_EINVAL, "scene listed twice");
+    }
+    Batch B;
+    B.scenes.assign(scenes, scenes + nscenes);
+    int rc = upload_batch_tables(ctx, B);
+    if (rc) return rc;
+    rc = run_batch(ctx, B, nsteps, stats);
+    if (rc) return rc;
+    HIP_TRY(hipGetLastError());
+    return SPHG_OK;
+}
+
+int sphg_scene_get(sphg_scene* s, int32_t field, void* out) {
+    if (!s || !out) return fail(SPHG_EINVAL, "NULL argument");
+    if (!s->ready) return fail(SPHG_ESTATE, "get before init");
+    sphg_ctx* ctx = s->ctx;
+    HIP_TRY(hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    const DevScene& V = s->view[s->par];
+    const int n = (int)s->n;
+    size_t bytes = 0;
+    switch (field) {
+        case SPHG_FIELD_X: launch_unsort_f4(st, V.x, V.id, s->tmp, n, 3); bytes = 12ull * n; break;
+        case SPHG_FIELD_V: launch_unsort_f4(st, V.v, V.id, s->tmp, n, 3); bytes = 12ull * n; break;
+        case SPHG_FIELD_A: launch_unsort_f4(st, V.a, V.id, s->tmp, n, 3); bytes = 12ull * n; break;
+        case SPHG_FIELD_RHO: launch_unsort_f1(st, V.rho, V.id, s->tmp, n); bytes = 4ull * n; break;
+        case SPHG_FIELD_P:
+            if (!V.p) return fail(SPHG_EINVAL, "no pressure field for this method");
+            launch_unsort_f1(st, V.p, V.id, s->tmp, n); bytes = 4ull * n; break;
+        case SPHG_FIELD_ALPHA:
+        case SPHG_FIELD_RHOSTAR:
+        case SPHG_FIELD_DIVERR:
+        case SPHG_FIELD_K:
+        case SPHG_FIELD_KV: {
+            if (V.method != DFSPH) return fail(SPHG_EINVAL, "DFSPH field on another method");
+            const float* src = field == SPHG_FIELD_ALPHA ? V.alpha : field == SPHG_FIELD_RHOSTAR ? V.rhoS
+                             : field == SPHG_FIELD_DIVERR ? V.divErr : field == SPHG_FIELD_K ? V.k : V.kv;
+            launch_unsort_f1(st, src, V.id, s->tmp, n);
+            bytes = 4ull * n;
+            break;
+        }
+        case SPHG_FIELD_NORMAL:
+            if (!V.nrm) return fail(SPHG_EINVAL, "no surface tension in this scene");
+            launch_unsort_f4(st, V.nrm, V.id, s->tmp, n, 3); bytes = 12ull * n; break;
+        case SPHG_FIELD_PACC:
+            if (!V.pacc) return fail(SPHG_EINVAL, "PCISPH field on another method");
+            launch_unsort_f4(st, V.pacc, V.id, s->tmp, n, 3); bytes = 12ull * n; break;
+        case SPHG_FIELD_KEYS: {
+            int rc = self_tables(s);
+            if (rc) return rc;
+            launch_keys3(st, s->self->dT[0], 0, n, (int*)s->tmp);
+            bytes = 12ull * n;
+            break;
+        }
+        case SPHG_FIELD_NNB:
+        case SPHG_FIELD_NGRAD: {
+            std::vector<uint16_t> c16;
+            std::vector<uint8_t> c8;
+            std::vector<uint32_t> ids(n);
+            HIP_TRY(hipMemcpyAsync(ids.data(), V.id, 4ull * n, hipMemcpyDeviceToHost, st));
+            if (field == SPHG_FIELD_NNB) {
+                c16.resize(n);
+                HIP_TRY(hipMemcpyAsync(c16.data(), V.nnb, 2ull * n, hipMemcpyDeviceToHost, st));
+            } else {
+                c8.resize(n);
+                HIP_TRY(hipMemcpyAsync(c8.data(), V.ngrad, 1ull * n, hipMemcpyDeviceToHost, st));
+            }
+            HIP_TRY(hipStreamSynchronize(st));
+            int32_t* o = (int32_t*)out;
+            for (int i = 0; i < n; ++i) o[ids[i]] = field == SPHG_FIELD_NNB ? (int32_t)c16[i] : (int32_t)c8[i];
+            return SPHG_OK;
+        }
+        case SPHG_FIELD_BVOL: {
+            std::vector<float4> xb(std::max<uint32_t>(s->nb, 1));
+            HIP_TRY(hipMemcpyAsync(xb.data(), V.xb, sizeof(float4) * s->nb, hipMemcpyDeviceToHost, st));
+            HIP_TRY(hipStreamSynchronize(st));
+            float* o = (float*)out;
+            for (uint32_t i = 0; i < s->nb; ++i) o[s->border[i]] = xb[i].w;
+            return SPHG_OK;
+        }
+        default: return fail(SPHG_EINVAL, "unknown field " + std::to_string(field));
+    }
+    HIP_TRY(hipMemcpyAsync(out, s->tmp, bytes, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    HIP_TRY(hipGetLastError());
+    return SPHG_OK;
+}
+
+int sphg_scene_get_scalars(sphg_scene* s, sphg_scene_scalars* out) {
+    if (!s || !out) return fail(SPHG_EINVAL, "NULL argument");
+    if (!s->ready) return fail(SPHG_ESTATE, "get before init");
+    HIP_TRY(hipSetDevice(s->ctx->device));
+    Scalars v;
+    HIP_TRY(hipMemcpyAsync(&v, s->ss, sizeof(Scalars), hipMemcpyDeviceToHost, s->ctx->stream));
+    HIP_TRY(hipStreamSynchronize(s->ctx->stream));
+    out->time = v.time;
+    out->time_step = v.dt;
+    out->max_vel = v.maxVel;
+    out->cfl_time_step = v.cflTs;
+    out->avg_density_error = v.avgErr;
+    out->avg_density_error0 = v.avgErr0;
+    out->avg_divergence_error = v.avgErrV;
+    out->avg_divergence_error0 = v.avgErrV0;
+    out->max_density_error = v.maxDensErr;
+    out->iterations = v.it;
+    out->iterations_v = v.itV;
+    out->steps = v.steps;
+    out->sum_iterations = v.sumIt;
+    out->sum_iterations_v = v.sumItV;
+    out->neighbor_overflow = v.overflow;
+    return SPHG_OK;
+}
+
+int sphg_scene_get_sorted(sphg_scene* s, uint32_t* ids, int32_t* cells) {
+    if (!s || !ids || !cells) return fail(SPHG_EINVAL, "NULL argument");
+    if (!s->ready) return fail(SPHG_ESTATE, "get before init");
+    const DevScene& V = s->view[s->par];
+    hipStream_t st = s->ctx->stream;
+    HIP_TRY(hipMemcpyAsync(ids, V.id, 4ull * s->n, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(cells, V.cell, 4ull * s->n, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    return SPHG_OK;
+}
+
+int sphg_scene_get_neighbors(sphg_scene* s, uint32_t* counts, uint32_t* ids, uint32_t cap) {
+    if (!s || !counts || !ids) return fail(SPHG_EINVAL, "NULL argument");
+    if (!s->ready) return fail(SPHG_ESTATE, "get before init");
+    const DevScene& V = s->view[s->par];
+    if ((int)cap < V.maxNbr) return fail(SPHG_EINVAL, "cap below max_neighbors");
+    const size_t n = s->n;
+    const size_t npad = (size_t)V.nChunks * kBlock;
+    std::vector<uint32_t> sid(n), lst(npad * (size_t)V.maxNbr);
+    std::vector<uint16_t> c16(n);
+    hipStream_t st = s->ctx->stream;
+    HIP_TRY(hipMemcpyAsync(sid.data(), V.id, 4 * n, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(c16.data(), V.nnb, 2 * n, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(lst.data(), V.nbr, 4 * lst.size(), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    for (size_t i = 0; i < n; ++i) {
+        const uint32_t me = sid[i];
+        counts[me] = c16[i];
+        const size_t base = (i >> 6) * (size_t)V.maxNbr * kGroup + (i & 63);
+        for (uint32_t q = 0; q < c16[i]; ++q) ids[(size_t)me * cap + q] = sid[lst[base + (size_t)q * kGroup]];
+    }
+    return SPHG_OK;
+}
+
+int sphg_scene_info(sphg_scene* s, uint32_t* n, uint32_t* nb, uint32_t* ncells, uint64_t* bytes) {
+    if (!s) return fail(SPHG_EINVAL, "NULL scene");
+    if (n) *n = s->n;
+    if (nb) *nb = s->nb;
+    if (ncells) *ncells = s->ready ? (uint32_t)s->view[0].ncells : 0;
+    if (bytes) *bytes = s->memBytes;
+    return SPHG_OK;
+}
+
+int sphg_ctx_kernel_timing(sphg_ctx* ctx, int32_t enable) {
+    if (!ctx) return fail(SPHG_EINVAL, "ctx is NULL");
+    HIP_TRY(hipSetDevice(ctx->device));
+    ctx->prof.collect();
+    ctx->prof.on = enable != 0;
+    if (enable) {
+        for (int k = 0; k < K_NKINDS; ++k) {
+            ctx->prof.count[k] = 0;
+            ctx->prof.ms[k] = 0.0;
+        }
+    }
+    return SPHG_OK;
+}
+
+int sphg_ctx_kernel_time(sphg_ctx* ctx, const char* name, uint64_t* launches, double* total_ms) {
+    if (!ctx || !name) return fail(SPHG_EINVAL, "NULL argument");
+    ctx->prof.collect();
+    for (int k = 0; k < K_NKINDS; ++k)
+        if (std::strcmp(name, kKindNames[k]) == 0) {
+            if (launches) *launches = ctx->prof.count[k];
+            if (total_ms) *total_ms = ctx->prof.ms[k];
+            return SPHG_OK;
+        }
+    return fail(SPHG_EINVAL, std::string("unknown kernel name ") + name);
+}
+
+// ------------------------------------------------------------- samplers
+
+int64_t sphg_sample_box(const double* mn, const double* mx, double supportRadius, float* out, int64_t cap) {
+    if (!mn || !mx || !(supportRadius > 0)) return -1;
+    const double lx = std::fabs(mn[0] - mx[0]), ly = std::fabs(mn[1] - mx[1]), lz = std::fabs(mn[2] - mx[2]);
+    const unsigned nx = (unsigned)std::ceil(lx / (0.5 * supportRadius));
+    const unsigned ny = (unsigned)std::ceil(ly / (0.5 * supportRadius));
+    const unsigned nz = (unsigned)std::ceil(lz / (0.5 * supportRadius));
+    const double dx = lx / nx, dy = ly / ny, dz = lz / nz;
+    int64_t c = 0;
+    for (unsigned i = 0; i <= nx; i++)
+        for (unsigned j = 0; j <= ny; j++)
+            for (unsigned k = 0; k <= nz; k++)
+                if (i == 0 || i == nx || j == 0 || j == ny || k == 0 || k == nz) {
+                    if (out && c < cap) {
+                        out[3 * c] = (float)(i * dx + mn[0]);
+                        out[3 * c + 1] = (float)(j * dy + mn[1]);
+                        out[3 * c + 2] = (float)(k * dz + mn[2]);
+                    }
+                    ++c;
+                }
+    return c;
+}
+
+int64_t sphg_sample_sphere(const double* o, double radius, double pr, float* out, int64_t cap) {
+    if (!o || !(radius > 0) || !(pr > 0)) return -1;
+    const double sep = 1.5;
+    const unsigned num = (unsigned)std::floor(4.0 * M_PI * radius * radius / (M_PI * pr * pr) * sep);
+    const double a = 4.0 * M_PI / num;
+    const double dd = std::sqrt(a);
+    const int mt = int(std::round(M_PI / dd));
+    const double dth = M_PI / mt;
+    const double dph = a / dth;
+    int64_t c = 0;
+    for (int m = 0; m < mt; ++m) {
+        const double th = M_PI * (m + 0.5) / mt;
+        const int mp = int(std::round(2.0 * M_PI * std::sin(th) / dph));
+        for (int q = 0; q < mp; ++q) {
+            const double ph = 2.0 * M_PI * q / mp;
+            if (out && c < cap) {
+                out[3 * c] = (float)(radius * (std::sin(th) * std::cos(ph)) + o[0]);
+                out[3 * c + 1] = (float)(radius * (std::sin(th) * std::sin(ph)) + o[1]);
+                out[3 * c + 2] = (float)(radius * std::cos(th) + o[2]);
+            }
+            ++c;
+        }
+    }
+    return c;
+}
+
+static inline uint64_t splitmix64(uint64_t& s) {
+    uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+static inline double unit01(uint64_t& s) { return (double)(splitmix64(s) >> 11) * (1.0 / 9007199254740992.0); }
+
+int64_t sphg_fluid_block(const double* mn, const double* mx, double pr, double jitter, uint64_t seed, float* out,
+                         int64_t cap) {
+    if (!mn || !mx || !(pr > 0)) return -1;
+    const double dist = 2.0 * pr;
+    const int px = (int)std::floor((mx[0] - mn[0]) / dist + 1e-5);
+    const int py = (int)std::floor((mx[1] - mn[1]) / dist + 1e-5);
+    const int pz = (int)std::floor((mx[2] - mn[2]) / dist + 1e-5);
+    if (px <= 0 || py <= 0 || pz <= 0) return 0;
+    uint64_t st = seed;
+    int64_t c = 0;
+    for (int i = 0; i < px; ++i)
+        for (int j = 0; j < py; ++j)
+            for (int k = 0; k < pz; ++k) {
+                double p[3] = {(double)i * dist + (mn[0] + pr), (double)j * dist + (mn[1] + pr),
+                               (double)k * dist + (mn[2] + pr)};
+                if (jitter != 0.0)
+                    for (int a = 0; a < 3; ++a) p[a] += (2.0 * unit01(st) - 1.0) * jitter * dist;
+                if (out && c < cap)
+                    for (int a = 0; a < 3; ++a) out[3 * c + a] = (float)p[a];
+                ++c;
+            }
+    return c;
+}
+
+int64_t sphg_fluid_spheres(const double* mn, const double* mx, double pr, const double* centers, const double* radii,
+                           int32_t ns, float* out, int64_t cap) {
+    if (!mn || !mx || !(pr > 0) || (ns > 0 && (!centers || !radii))) return -1;
+    const double dist = 2.0 * pr;
+    const int px = (int)std::floor((mx[0] - mn[0]) / dist + 1e-5);
+    const int py = (int)std::floor((mx[1] - mn[1]) / dist + 1e-5);
+    const int pz = (int)std::floor((mx[2] - mn[2]) / dist + 1e-5);
+    int64_t c = 0;
+    for (int i = 0; i < px; ++i)
+        for (int j = 0; j < py; ++j)
+            for (int k = 0; k < pz; ++k) {
+                const double p[3] = {(double)i * dist + (mn[0] + pr), (double)j * dist + (mn[1] + pr),
+                                     (double)k * dist + (mn[2] + pr)};
+                bool in = false;
+                for (int s = 0; s < ns && !in; ++s) {
+                    const double dx = p[0] - centers[3 * s], dy = p[1] - centers[3 * s + 1], dz = p[2] - centers[3 * s + 2];
+                    in = dx * dx + dy * dy + dz * dz <= radii[s] * radii[s];
+                }
+                if (!in) continue;
+                if (out && c < cap)
+                    for (int a = 0; a < 3; ++a) out[3 * c + a] = (float)p[a];
+                ++c;
+            }
+    return c;
+}
+
+}  // extern "C"

@@ -1,0 +1,98 @@
+// sphg_internal.h -- device-side data layout shared by the HIP kernels and
+// the C-ABI layer of libsphg.
+//
+// One scene = one fluid model + one Akinci boundary model (the reference's
+// own constraint, src/Simulation.h:24-29 and src/AkinciBoundayModel.cpp:32).
+// Every kernel walks a chunk table: each 256-thread block owns one chunk of
+// one scene, so a launch covers any number of scenes of uneven size
+// (batching) and a single scene is a batch of one.
+//
+// HBM layout per scene (FP32, struct-of-arrays, particles sorted by
+// (cell, id) every step):
+//   x, v          float4 (w unused)              -- double-buffered for the sort
+//   xk            float4 (x, y, z, s)             -- s = the scalar the next
+//                 neighbor sweep gathers (warm-start k, kappa, kappa_v, p/rho^2)
+//   gb            float4 (sum_b rho0 V_b gradW_ib, sum_b rho0 V_b W_ib)
+//   rho, alpha, rhoS, divErr, p  float
+//   k, kv         float (DFSPH warm-start accumulators, double-buffered)
+//   id            uint32 original particle id (double-buffered)
+//   cell          int    linear cell of each sorted particle
+//   nbr           uint32 neighbor list, neighbor-major inside 64-particle groups
+//   nnb, ngrad    uint16/uint8 neighbor count and count of nonzero-gradient
+//                 neighbors (the DFSPH 20-neighbor cutoff)
+// Boundary: xb float4 (x, y, z, V_b) sorted by cell, bCellStart table.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace sphg {
+
+constexpr int kBlock = 256;          // threads per block = particles per chunk
+constexpr int kGroup = 64;           // particles per neighbor-list group (one wavefront)
+constexpr int kCellChunk = 4096;     // cells per scan chunk (256 threads x 16)
+constexpr int kCellPerThread = kCellChunk / kBlock;
+
+enum Method { WCSPH = 0, PCISPH = 1, DFSPH = 2 };
+enum Flags { F_XSPH = 1, F_ST = 2, F_ADH = 4, F_XSPH_B = 8 };
+
+// FP32 kernel coefficients (computed in FP64 on the host, then rounded).
+struct KCoef {
+    float h, invh, h2;
+    float ck;      // cubic spline W:    8 / (pi h^3)        src/CubicSpline.h:25
+    float clh2;    // cubic grad, q<=1/2: 48/(pi h^3) / h^2  src/CubicSpline.h:26,64
+    float clh;     // cubic grad, q>1/2:  48/(pi h^3) / h
+    float spg;     // spiky grad:        45 / (pi h^6)      src/Spiky.h:26
+    float co1;     // cohesion:          32 / (pi h^9)      src/Cohesion.h:26
+    float co2;     // cohesion offset:   h^6 / 64           src/Cohesion.h:27
+    float hlo2, hhi2;   // h^2 (1 -/+ 1e-4): FP32 pre-filter band of the exact test
+};
+
+// Device-resident control state of one scene (one writer at a time: the
+// last block of a reduction, or the first block of a scene).
+struct Scalars {
+    double dt, time, maxVel, cflTs;
+    double avgErr, avgErr0, avgErrV, avgErrV0, maxDensErr;
+    int it, itV, done, doneV;
+    unsigned steps, sumIt, sumItV, ctr, overflow, pad;
+};
+
+struct DevScene {
+    KCoef kc;
+    double hD;                  // support radius in FP64 (cell keys, exact neighbor test)
+    double r, cfl, dtMin, dtMax, eta, etaV;
+    float m, rho0, W0, pciDelta;
+    float gx, gy, gz, pad0;
+    float nu, nub, stCoef, adhBeta;
+    float stiffness, gammaEos, pad1, pad2;
+    int method, flags;
+    int minIt, maxIt, minItV, maxItV;
+    int kx0, ky0, kz0;          // key of the first unpadded cell
+    int nx, ny, nz, ncells;     // padded grid (one empty layer on every side)
+    int n, nb, maxNbr, nChunks;
+    int nCellChunks, pad3;
+    // particle arrays; the *2 members are the other half of each double buffer
+    float4 *x, *x2, *v, *v2;
+    float *k, *k2, *kv, *kv2;
+    uint32_t *id, *id2;
+    float4 *a, *xk, *gb, *aux, *xs, *nrm, *pacc, *gbs, *adh;
+    float *rho, *alpha, *rhoS, *divErr, *p;
+    int *cellU;                 // cell of each particle before the sort
+    int *cell;                  // cell of each particle after the sort
+    uint32_t *perm;
+    int *cellCount, *cellStart, *cellChunkSum;
+    uint32_t *nbr;
+    uint16_t *nnb;
+    uint8_t *ngrad;
+    float4 *xb;
+    int *bCellStart;
+    double *partial;            // one per particle chunk
+    Scalars *ss;
+};
+
+struct Chunk {
+    int scene;   // index into the launch's scene table
+    int begin;   // first particle (or first cell) of the chunk inside the scene
+};
+
+}  // namespace sphg

@@ -1,0 +1,36 @@
+// sphg_kernels.h -- host-side launchers of the kernels in sphg_kernels.hip.
+// Each takes the scene table and chunk table of one batch launch.
+#pragma once
+
+#include "sphg_internal.h"
+
+namespace sphg {
+
+void launch_keys(hipStream_t st, const DevScene* T, const Chunk* C, int nchunks);
+void launch_scan(hipStream_t st, const DevScene* T, const Chunk* CC, int ncellchunks, int nscenes);
+void launch_scatter(hipStream_t st, const DevScene* T, const Chunk* C, int nchunks);
+void launch_cell_sort(hipStream_t st, const DevScene* T, const Chunk* CC, int ncellchunks);
+void launch_reorder(hipStream_t st, const DevScene* T, const Chunk* C, int nchunks);
+void launch_bvolume(hipStream_t st, const DevScene* T, int scene, int nb, float* tmp);
+void launch_build(hipStream_t st, int method, const DevScene* T, const Chunk* C, int nchunks);
+void launch_normals(hipStream_t st, const DevScene* T, const Chunk* C, int nchunks);
+void launch_forces(hipStream_t st, int method, const DevScene* T, const Chunk* C, int nchunks);
+void launch_step_begin(hipStream_t st, const DevScene* T, const Chunk* C, int nchunks, int* active);
+void launch_dt(hipStream_t st, const DevScene* T, const Chunk* C, int nchunks);
+void launch_df_predict(hipStream_t st, const DevScene* T, const Chunk* C, int nchunks);
+void launch_df_vel(hipStream_t st, int mode, const DevScene* T, const Chunk* C, int nchunks);
+void launch_df_err(hipStream_t st, int mode, const DevScene* T, const Chunk* C, int nchunks, int* active);
+void launch_df_positions(hipStream_t st, const DevScene* T, const Chunk* C, int nchunks);
+void launch_pci_density(hipStream_t st, const DevScene* T, const Chunk* C, int nchunks, int* active);
+void launch_pci_pacc(hipStream_t st, const DevScene* T, const Chunk* C, int nchunks, int iter);
+void launch_pci_integrate(hipStream_t st, const DevScene* T, const Chunk* C, int nchunks);
+void launch_time_advance(hipStream_t st, const DevScene* T, int nscenes);
+void launch_fill_int(hipStream_t st, int* p, int v, long n);
+void launch_unsort_f4(hipStream_t st, const float4* src, const uint32_t* id, float* dst, int n, int comps);
+void launch_unsort_f1(hipStream_t st, const float* src, const uint32_t* id, float* dst, int n);
+void launch_unsort_i1(hipStream_t st, const int* src, const uint32_t* id, int* dst, int n);
+void launch_keys3(hipStream_t st, const DevScene* T, int scene, int n, int* dst);
+void launch_load_state(hipStream_t st, const DevScene* T, int scene, int n, const float* xyz, const float* vel,
+                       const float* k, const float* kv);
+
+}  // namespace sphg

@@ -1,0 +1,395 @@
+"""ctypes binding of libsphg (include/sphg.h), the gfx950 SPH solver.
+
+The library is loaded from this package directory; a missing or unloadable
+library raises immediately -- there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass, field, fields
+from typing import Optional, Sequence
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libsphg.so")
+
+WCSPH, PCISPH, DFSPH = 0, 1, 2
+BOUNDARY_AKINCI = 2
+VISCOSITY_XSPH, SURFACE_TENSION_AKINCI, ADHESION_AKINCI = 2, 3, 4
+
+FIELD_X, FIELD_V, FIELD_A, FIELD_RHO, FIELD_P = 0, 1, 2, 3, 4
+FIELD_ALPHA, FIELD_RHOSTAR, FIELD_DIVERR, FIELD_K, FIELD_KV = 5, 6, 7, 8, 9
+FIELD_BVOL, FIELD_NORMAL, FIELD_PACC, FIELD_KEYS, FIELD_NNB, FIELD_NGRAD = 10, 11, 12, 13, 14, 15
+
+_VEC3 = {FIELD_X, FIELD_V, FIELD_A, FIELD_NORMAL, FIELD_PACC}
+_INT = {FIELD_KEYS, FIELD_NNB, FIELD_NGRAD}
+
+
+class SphgError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"sphg error {code}: {msg}")
+        self.code = code
+
+
+class SceneDesc(C.Structure):
+    _fields_ = [
+        ("method", C.c_int32),
+        ("boundary_method", C.c_int32),
+        ("particle_radius", C.c_double),
+        ("density0", C.c_double),
+        ("gravity", C.c_double * 3),
+        ("time_step", C.c_double),
+        ("min_time_step", C.c_double),
+        ("max_time_step", C.c_double),
+        ("fps", C.c_double),
+        ("cfl_factor", C.c_double),
+        ("stiffness", C.c_double),
+        ("gamma", C.c_double),
+        ("eta", C.c_double),
+        ("eta_v", C.c_double),
+        ("min_iterations", C.c_uint32),
+        ("max_iterations", C.c_uint32),
+        ("min_iterations_v", C.c_uint32),
+        ("max_iterations_v", C.c_uint32),
+        ("viscosity_method", C.c_int32),
+        ("viscosity", C.c_double),
+        ("boundary_viscosity", C.c_double),
+        ("surface_tension_method", C.c_int32),
+        ("surface_tension", C.c_double),
+        ("adhesion_method", C.c_int32),
+        ("adhesion", C.c_double),
+        ("max_neighbors", C.c_uint32),
+        ("domain_min", C.c_double * 3),
+        ("domain_max", C.c_double * 3),
+    ]
+
+
+class SceneScalars(C.Structure):
+    _fields_ = [
+        ("time", C.c_double),
+        ("time_step", C.c_double),
+        ("max_vel", C.c_double),
+        ("cfl_time_step", C.c_double),
+        ("avg_density_error", C.c_double),
+        ("avg_density_error0", C.c_double),
+        ("avg_divergence_error", C.c_double),
+        ("avg_divergence_error0", C.c_double),
+        ("max_density_error", C.c_double),
+        ("iterations", C.c_int32),
+        ("iterations_v", C.c_int32),
+        ("steps", C.c_uint32),
+        ("sum_iterations", C.c_uint32),
+        ("sum_iterations_v", C.c_uint32),
+        ("neighbor_overflow", C.c_uint32),
+    ]
+
+
+class StepStats(C.Structure):
+    _fields_ = [
+        ("gpu_ms", C.c_double),
+        ("steps", C.c_uint32),
+        ("iterations", C.c_uint32),
+        ("iterations_v", C.c_uint32),
+        ("host_syncs", C.c_uint32),
+    ]
+
+
+@dataclass
+class SceneParams:
+    """Scene parameters with the reference scene-JSON meaning
+    (src/SceneLoader.cpp:38-141 of the reference)."""
+
+    method: int = DFSPH
+    particle_radius: float = 0.025
+    density0: float = 1000.0
+    gravity: Sequence[float] = (0.0, 0.0, -9.81)
+    time_step: float = 1e-3
+    min_time_step: float = 1e-6
+    max_time_step: float = 5e-3
+    fps: float = 25.0
+    cfl_factor: float = 0.4
+    stiffness: float = 1000.0
+    gamma: float = 7.0
+    eta: float = 1e-4
+    eta_v: float = 1e-3
+    min_iterations: int = 2
+    max_iterations: int = 100
+    min_iterations_v: int = 1
+    max_iterations_v: int = 100
+    viscosity_method: int = -1
+    viscosity: float = 0.0
+    boundary_viscosity: float = 0.0
+    surface_tension_method: int = -1
+    surface_tension: float = 0.0
+    adhesion_method: int = -1
+    adhesion: float = 0.0
+    max_neighbors: int = 0
+    domain_min: Sequence[float] = (0.0, 0.0, 0.0)
+    domain_max: Sequence[float] = (0.0, 0.0, 0.0)
+
+    @property
+    def support_radius(self) -> float:
+        return 4.0 * self.particle_radius
+
+    def to_desc(self) -> SceneDesc:
+        d = SceneDesc()
+        for f in fields(self):
+            v = getattr(self, f.name)
+            if f.name in ("gravity", "domain_min", "domain_max"):
+                getattr(d, f.name)[:] = [float(x) for x in v]
+            else:
+                setattr(d, f.name, v)
+        d.boundary_method = BOUNDARY_AKINCI
+        return d
+
+
+_lib: Optional[C.CDLL] = None
+
+
+def load(path: str = LIB_PATH) -> C.CDLL:
+    """Load libsphg (once).  Raises if it is missing: the solver has no CPU path."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is missing: build it with `make -C sph_amd/csrc` or __graft_entry__.build()")
+    lib = C.CDLL(path)
+    P, I32, U32, U64, I64, D = C.c_void_p, C.c_int32, C.c_uint32, C.c_uint64, C.c_int64, C.c_double
+    fp = C.POINTER(C.c_float)
+    dp = C.POINTER(C.c_double)
+    sig = {
+        "sphg_version": (C.c_char_p, []),
+        "sphg_last_error": (C.c_char_p, []),
+        "sphg_ctx_create": (C.c_int, [I32, C.POINTER(P)]),
+        "sphg_ctx_destroy": (C.c_int, [P]),
+        "sphg_ctx_synchronize": (C.c_int, [P]),
+        "sphg_scene_create": (C.c_int, [P, C.POINTER(SceneDesc), C.POINTER(P)]),
+        "sphg_scene_destroy": (C.c_int, [P]),
+        "sphg_scene_set_fluid": (C.c_int, [P, fp, fp, U32]),
+        "sphg_scene_set_boundary": (C.c_int, [P, fp, U32]),
+        "sphg_scene_init": (C.c_int, [P]),
+        "sphg_scene_set_state": (C.c_int, [P, fp, fp, fp, fp, D, U32]),
+        "sphg_scene_step": (C.c_int, [P, U32, C.POINTER(StepStats)]),
+        "sphg_scene_get": (C.c_int, [P, I32, P]),
+        "sphg_scene_get_scalars": (C.c_int, [P, C.POINTER(SceneScalars)]),
+        "sphg_scene_get_sorted": (C.c_int, [P, C.POINTER(C.c_uint32), C.POINTER(C.c_int32)]),
+        "sphg_scene_get_neighbors": (C.c_int, [P, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), U32]),
+        "sphg_scene_info": (C.c_int, [P, C.POINTER(U32), C.POINTER(U32), C.POINTER(U32), C.POINTER(U64)]),
+        "sphg_batch_step": (C.c_int, [P, C.POINTER(P), U32, U32, C.POINTER(StepStats)]),
+        "sphg_ctx_kernel_timing": (C.c_int, [P, I32]),
+        "sphg_ctx_kernel_time": (C.c_int, [P, C.c_char_p, C.POINTER(U64), dp]),
+        "sphg_sample_box": (I64, [dp, dp, D, fp, I64]),
+        "sphg_sample_sphere": (I64, [dp, D, D, fp, I64]),
+        "sphg_fluid_block": (I64, [dp, dp, D, D, U64, fp, I64]),
+        "sphg_fluid_spheres": (I64, [dp, dp, D, dp, dp, I32, fp, I64]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _check(rc: int) -> None:
+    if rc != 0:
+        raise SphgError(rc, load().sphg_last_error().decode())
+
+
+def _fptr(a: Optional[np.ndarray]):
+    if a is None:
+        return None
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def _dptr(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _f32(a, shape_last=None) -> np.ndarray:
+    a = np.ascontiguousarray(np.asarray(a, dtype=np.float32))
+    return a
+
+
+# ----------------------------------------------------------------- geometry
+
+
+def _sample(fn, *args) -> np.ndarray:
+    n = fn(*args, None, 0)
+    if n < 0:
+        raise ValueError("invalid sampler arguments")
+    out = np.zeros((max(n, 1), 3), dtype=np.float32)
+    fn(*args, _fptr(out), n)
+    return out[:n]
+
+
+def sample_box(mn, mx, support_radius: float) -> np.ndarray:
+    """AkinciBoundaryModel::sampleCube (src/AkinciBoundayModel.cpp:100-134)."""
+    lib = load()
+    a = np.asarray(mn, dtype=np.float64)
+    b = np.asarray(mx, dtype=np.float64)
+    return _sample(lib.sphg_sample_box, _dptr(a), _dptr(b), float(support_radius))
+
+
+def sample_sphere(center, radius: float, particle_radius: float) -> np.ndarray:
+    """AkinciBoundaryModel::sampleSphere (src/AkinciBoundayModel.cpp:136-164)."""
+    lib = load()
+    c = np.asarray(center, dtype=np.float64)
+    return _sample(lib.sphg_sample_sphere, _dptr(c), float(radius), float(particle_radius))
+
+
+def fluid_block(mn, mx, particle_radius: float, jitter: float = 0.0, seed: int = 0) -> np.ndarray:
+    """Simulation::buildFluidBlock (src/Simulation.cpp:512-539) + seeded jitter."""
+    lib = load()
+    a = np.asarray(mn, dtype=np.float64)
+    b = np.asarray(mx, dtype=np.float64)
+    return _sample(lib.sphg_fluid_block, _dptr(a), _dptr(b), float(particle_radius), float(jitter), int(seed))
+
+
+def fluid_spheres(mn, mx, particle_radius: float, centers, radii) -> np.ndarray:
+    lib = load()
+    a = np.asarray(mn, dtype=np.float64)
+    b = np.asarray(mx, dtype=np.float64)
+    c = np.ascontiguousarray(np.asarray(centers, dtype=np.float64).reshape(-1, 3))
+    r = np.ascontiguousarray(np.asarray(radii, dtype=np.float64).reshape(-1))
+    return _sample(lib.sphg_fluid_spheres, _dptr(a), _dptr(b), float(particle_radius), _dptr(c), _dptr(r),
+                   int(len(r)))
+
+
+# ------------------------------------------------------------- contexts
+
+
+class Context:
+    """One device, one stream (sphg_ctx)."""
+
+    def __init__(self, device: int = 0):
+        self.lib = load()
+        h = C.c_void_p()
+        _check(self.lib.sphg_ctx_create(int(device), C.byref(h)))
+        self.h = h
+        self._scenes = []
+
+    def close(self) -> None:
+        if self.h:
+            for s in self._scenes:
+                s.h = None
+            self.lib.sphg_ctx_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def synchronize(self) -> None:
+        _check(self.lib.sphg_ctx_synchronize(self.h))
+
+    def scene(self, params: SceneParams, fluid, boundary, velocities=None) -> "Scene":
+        return Scene(self, params, fluid, boundary, velocities)
+
+    def batch_step(self, scenes: Sequence["Scene"], nsteps: int = 1) -> StepStats:
+        arr = (C.c_void_p * len(scenes))(*[s.h for s in scenes])
+        st = StepStats()
+        _check(self.lib.sphg_batch_step(self.h, arr, len(scenes), int(nsteps), C.byref(st)))
+        return st
+
+    def kernel_timing(self, enable: bool) -> None:
+        _check(self.lib.sphg_ctx_kernel_timing(self.h, 1 if enable else 0))
+
+    def kernel_time(self, name: str):
+        n = C.c_uint64()
+        ms = C.c_double()
+        _check(self.lib.sphg_ctx_kernel_time(self.h, name.encode(), C.byref(n), C.byref(ms)))
+        return int(n.value), float(ms.value)
+
+
+class Scene:
+    """One fluid + Akinci boundary scene resident on the device (sphg_scene)."""
+
+    def __init__(self, ctx: Context, params: SceneParams, fluid, boundary, velocities=None, init: bool = True):
+        self.ctx = ctx
+        self.lib = ctx.lib
+        self.params = params
+        h = C.c_void_p()
+        desc = params.to_desc()
+        _check(self.lib.sphg_scene_create(ctx.h, C.byref(desc), C.byref(h)))
+        self.h = h
+        ctx._scenes.append(self)
+        x = _f32(fluid).reshape(-1, 3)
+        v = None if velocities is None else _f32(velocities).reshape(-1, 3)
+        b = _f32(boundary).reshape(-1, 3)
+        self.n = x.shape[0]
+        self.nb = b.shape[0]
+        _check(self.lib.sphg_scene_set_fluid(self.h, _fptr(x), _fptr(v), self.n))
+        _check(self.lib.sphg_scene_set_boundary(self.h, _fptr(b), self.nb))
+        if init:
+            _check(self.lib.sphg_scene_init(self.h))
+
+    def close(self) -> None:
+        if self.h:
+            self.lib.sphg_scene_destroy(self.h)
+            self.h = None
+            if self in self.ctx._scenes:
+                self.ctx._scenes.remove(self)
+
+    def step(self, nsteps: int = 1) -> StepStats:
+        st = StepStats()
+        _check(self.lib.sphg_scene_step(self.h, int(nsteps), C.byref(st)))
+        return st
+
+    def set_state(self, x, v, k=None, kv=None, time_step: float = None, last_iterations: int = 0) -> None:
+        x = _f32(x).reshape(-1, 3)
+        v = _f32(v).reshape(-1, 3)
+        k = None if k is None else _f32(k).reshape(-1)
+        kv = None if kv is None else _f32(kv).reshape(-1)
+        dt = self.params.time_step if time_step is None else float(time_step)
+        _check(self.lib.sphg_scene_set_state(self.h, _fptr(x), _fptr(v), _fptr(k), _fptr(kv), dt,
+                                             int(last_iterations)))
+
+    def get(self, fld: int) -> np.ndarray:
+        if fld == FIELD_BVOL:
+            out = np.zeros(max(self.nb, 1), dtype=np.float32)
+            _check(self.lib.sphg_scene_get(self.h, fld, out.ctypes.data_as(C.c_void_p)))
+            return out[: self.nb]
+        if fld in _VEC3 or fld == FIELD_KEYS:
+            dt = np.int32 if fld == FIELD_KEYS else np.float32
+            out = np.zeros((self.n, 3), dtype=dt)
+        else:
+            out = np.zeros(self.n, dtype=np.int32 if fld in _INT else np.float32)
+        _check(self.lib.sphg_scene_get(self.h, fld, out.ctypes.data_as(C.c_void_p)))
+        return out
+
+    def scalars(self) -> SceneScalars:
+        s = SceneScalars()
+        _check(self.lib.sphg_scene_get_scalars(self.h, C.byref(s)))
+        return s
+
+    def sorted_ids(self):
+        ids = np.zeros(self.n, dtype=np.uint32)
+        cells = np.zeros(self.n, dtype=np.int32)
+        _check(self.lib.sphg_scene_get_sorted(self.h, ids.ctypes.data_as(C.POINTER(C.c_uint32)),
+                                              cells.ctypes.data_as(C.POINTER(C.c_int32))))
+        return ids, cells
+
+    def neighbors(self, cap: int = None):
+        cap = cap or (self.params.max_neighbors or 96)
+        counts = np.zeros(self.n, dtype=np.uint32)
+        ids = np.zeros((self.n, cap), dtype=np.uint32)
+        _check(self.lib.sphg_scene_get_neighbors(self.h, counts.ctypes.data_as(C.POINTER(C.c_uint32)),
+                                                 ids.ctypes.data_as(C.POINTER(C.c_uint32)), cap))
+        return counts, ids
+
+    def info(self):
+        n, nb, nc = C.c_uint32(), C.c_uint32(), C.c_uint32()
+        by = C.c_uint64()
+        _check(self.lib.sphg_scene_info(self.h, C.byref(n), C.byref(nb), C.byref(nc), C.byref(by)))
+        return {"n": n.value, "nb": nb.value, "ncells": nc.value, "device_bytes": by.value}

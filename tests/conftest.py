@@ -1,0 +1,45 @@
+import os
+import sys
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an AMD GPU (runs through libsphg's HIP kernels)")
+
+
+def have_gpu() -> bool:
+    return os.path.exists("/dev/kfd")
+
+
+@pytest.fixture(scope="session")
+def ctx():
+    """One libsphg context on device 0.  On a machine with a GPU a failure
+    here is an error, not a skip: the solver has no CPU path."""
+    if not have_gpu():
+        pytest.skip("no GPU in this container")
+    from sph_amd import sphg
+    c = sphg.Context(0)
+    yield c
+    c.close()
+
+
+def rel_norm(a, b) -> float:
+    """max |a - b| / max |b| over a field (normwise relative error)."""
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    den = np.abs(b).max() if b.size else 0.0
+    num = np.abs(a - b).max() if b.size else 0.0
+    return float(num / den) if den > 0 else float(num)
+
+
+def rel_elem(a, b) -> float:
+    """max |a - b| / |b| elementwise (for fields bounded away from zero)."""
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    return float(np.max(np.abs(a - b) / np.abs(b))) if b.size else 0.0

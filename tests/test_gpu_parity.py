@@ -1,0 +1,217 @@
+"""HIP solver vs the FP64 oracle on identical states.
+
+Bars:
+  * cell keys, sort order, bucket membership and neighbor sets: bit-exact;
+  * per-step density, pressure / kappa, acceleration and velocity: within
+    1e-5 relative in FP32 (density and boundary volume elementwise; signed
+    fields normwise, max|gpu - ref| / max|ref|);
+  * solver iteration counts: within +-1 (fields are compared on the steps
+    whose counts agree, the others are reported).
+"""
+import numpy as np
+import pytest
+
+from conftest import rel_elem, rel_norm
+from sph_amd import configs, sphg
+import oracle.pyoracle as po
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5
+
+
+def _pair(ctx, spec):
+    g = sphg.Scene(ctx, spec.params, spec.fluid, spec.boundary)
+    o = po.Oracle(spec.params, spec.fluid, spec.boundary)
+    return g, o
+
+
+def test_grid_keys_sort_order_buckets(ctx):
+    spec = configs.small_scene(sphg.DFSPH, n_side=12)
+    g, o = _pair(ctx, spec)
+    assert np.array_equal(g.get(sphg.FIELD_KEYS), o.keys())
+    ids, cells = g.sorted_ids()
+    assert np.all(np.diff(cells) >= 0)
+    same = np.diff(cells) == 0
+    assert np.all(np.diff(ids.astype(np.int64))[same] > 0), "within-cell order is not ascending id"
+    starts = np.flatnonzero(np.r_[True, np.diff(cells) != 0])
+    ends = np.r_[starts[1:], len(ids)]
+    okeys = o.keys()
+    for s, e in zip(starts, ends):
+        members = ids[s:e]
+        assert np.array_equal(members, o.cell_points(int(members[0])))
+        assert len({tuple(okeys[m]) for m in members}) == 1
+    g.close()
+
+
+def _exact_neighbors(o, x64, i, h):
+    cand = o.candidates(i)
+    d = x64[cand] - x64[i]
+    rl = np.sqrt((d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1]) + d[:, 2] * d[:, 2])
+    q = rl / h
+    keep = (q <= 1.0) & (cand != i)
+    return cand[keep], rl[keep], q[keep]
+
+
+def test_neighbor_sets_exact(ctx):
+    spec = configs.small_scene(sphg.DFSPH, n_side=10, jitter=0.02)
+    g, o = _pair(ctx, spec)
+    counts, nb = g.neighbors()
+    ngrad = g.get(sphg.FIELD_NGRAD)
+    x64 = spec.fluid.astype(np.float64)
+    h = spec.params.support_radius
+    for i in range(spec.n):
+        exp, rl, q = _exact_neighbors(o, x64, i, h)
+        got = nb[i, : counts[i]]
+        assert set(got.tolist()) == set(exp.tolist()), f"particle {i}"
+        # reference neighbor count for the divergence cutoff: |gradW| > 0
+        nz = (rl > 1e-5) & np.where(q <= 0.5, q * (3.0 * q - 2.0) != 0.0, (1.0 - q) * (1.0 - q) != 0.0)
+        assert ngrad[i] == int(nz.sum()), f"particle {i}"
+    g.close()
+
+
+def test_init_density_alpha_volume(ctx):
+    spec = configs.small_scene(sphg.DFSPH, n_side=12)
+    g, o = _pair(ctx, spec)
+    assert rel_elem(g.get(sphg.FIELD_BVOL), o.get(po.OF_BVOL)) < TOL
+    assert rel_elem(g.get(sphg.FIELD_RHO), o.get(po.OF_RHO)) < TOL
+    a_g, a_o = g.get(sphg.FIELD_ALPHA), o.get(po.OF_ALPHA)
+    assert np.array_equal(a_g == 0, a_o == 0)
+    nz = a_o != 0
+    assert rel_elem(a_g[nz], a_o[nz]) < TOL
+    g.close()
+
+
+def _dfsph_compare(g, o, report):
+    sg, so = g.scalars(), o.scalars()
+    its = (sg.iterations, int(so["iterations"]), sg.iterations_v, int(so["iterations_v"]))
+    assert abs(its[0] - its[1]) <= 1 and abs(its[2] - its[3]) <= 1, its
+    assert abs(sg.time_step - so["time_step"]) <= 1e-6 * so["time_step"]
+    if its[0] != its[1] or its[2] != its[3]:
+        report.append(("iteration mismatch", its))
+        return
+    errs = {
+        "v": rel_norm(g.get(sphg.FIELD_V), o.get(po.OF_V)),
+        "x": rel_norm(g.get(sphg.FIELD_X), o.get(po.OF_X)),
+        "rho": rel_elem(g.get(sphg.FIELD_RHO), o.get(po.OF_RHO)),
+        "rhostar": rel_elem(g.get(sphg.FIELD_RHOSTAR), o.get(po.OF_RHOSTAR)),
+        "alpha": rel_norm(g.get(sphg.FIELD_ALPHA), o.get(po.OF_ALPHA)),
+        "k": rel_norm(g.get(sphg.FIELD_K), o.get(po.OF_K)),
+        "kv": rel_norm(g.get(sphg.FIELD_KV), o.get(po.OF_KV)),
+        "diverr": rel_norm(g.get(sphg.FIELD_DIVERR), o.get(po.OF_DIVERR)),
+    }
+    report.append(errs)
+    bad = {k: v for k, v in errs.items() if not v < TOL}
+    assert not bad, bad
+
+
+@pytest.mark.parametrize("forces", [False, True])
+def test_dfsph_steps_identical_state(ctx, forces):
+    kw = {}
+    if forces:
+        kw = dict(viscosity_method=sphg.VISCOSITY_XSPH, viscosity=0.01, boundary_viscosity=0.01,
+                  surface_tension_method=sphg.SURFACE_TENSION_AKINCI, surface_tension=0.05,
+                  adhesion_method=sphg.ADHESION_AKINCI, adhesion=0.1)
+    spec = configs.small_scene(sphg.DFSPH, n_side=10, **kw)
+    g, o = _pair(ctx, spec)
+    report = []
+    for step in range(8):
+        sc = g.scalars()
+        o.set_state(g.get(sphg.FIELD_X), g.get(sphg.FIELD_V), g.get(sphg.FIELD_K), g.get(sphg.FIELD_KV),
+                    sc.time_step, sc.iterations)
+        g.step(1)
+        o.step(1)
+        if forces:
+            assert rel_norm(g.get(sphg.FIELD_A), o.get(po.OF_ACC_STEP)) < TOL
+        _dfsph_compare(g, o, report)
+    assert sum(1 for r in report if isinstance(r, dict)) >= 6, report
+    g.close()
+
+
+def test_dfsph_from_init_matches(ctx):
+    """First step straight from init (no injected state)."""
+    spec = configs.small_scene(sphg.DFSPH, n_side=12)
+    g, o = _pair(ctx, spec)
+    g.step(1)
+    o.step(1)
+    report = []
+    _dfsph_compare(g, o, report)
+    g.close()
+
+
+def test_wcsph_steps_identical_state(ctx):
+    spec = configs.small_scene(sphg.WCSPH, n_side=12, viscosity_method=sphg.VISCOSITY_XSPH, viscosity=0.01,
+                               boundary_viscosity=0.01)
+    g, o = _pair(ctx, spec)
+    for step in range(8):
+        o.set_state(g.get(sphg.FIELD_X), g.get(sphg.FIELD_V), None, None, spec.params.time_step, 0)
+        g.step(1)
+        o.step(1)
+        errs = {
+            "rho": rel_elem(g.get(sphg.FIELD_RHO), o.get(po.OF_RHO)),
+            "p": rel_norm(g.get(sphg.FIELD_P), o.get(po.OF_P)),
+            "a": rel_norm(g.get(sphg.FIELD_A), o.get(po.OF_ACC_STEP)),
+            "v": rel_norm(g.get(sphg.FIELD_V), o.get(po.OF_V)),
+            "x": rel_norm(g.get(sphg.FIELD_X), o.get(po.OF_X)),
+        }
+        bad = {k: v for k, v in errs.items() if not v < TOL}
+        assert not bad, (step, errs)
+    assert abs(g.scalars().time - o.scalars()["time"]) < 1e-12
+    g.close()
+
+
+def test_pcisph_surface_tension_adhesion(ctx):
+    spec = configs.small_scene(sphg.PCISPH, n_side=10, surface_tension_method=sphg.SURFACE_TENSION_AKINCI,
+                               surface_tension=0.05, adhesion_method=sphg.ADHESION_AKINCI, adhesion=0.1,
+                               viscosity_method=sphg.VISCOSITY_XSPH, viscosity=0.01)
+    g, o = _pair(ctx, spec)
+    matched = 0
+    for step in range(8):
+        o.set_state(g.get(sphg.FIELD_X), g.get(sphg.FIELD_V), None, None, spec.params.time_step, 0)
+        g.step(1)
+        o.step(1)
+        sg, so = g.scalars(), o.scalars()
+        assert abs(sg.iterations - int(so["iterations"])) <= 1
+        assert rel_norm(g.get(sphg.FIELD_A), o.get(po.OF_ACC_STEP)) < TOL
+        assert rel_norm(g.get(sphg.FIELD_NORMAL), o.get(po.OF_NORMAL)) < TOL
+        if sg.iterations != int(so["iterations"]):
+            continue
+        matched += 1
+        errs = {
+            "rho": rel_elem(g.get(sphg.FIELD_RHO), o.get(po.OF_RHO)),
+            "p": rel_norm(g.get(sphg.FIELD_P), o.get(po.OF_P)),
+            "pacc": rel_norm(g.get(sphg.FIELD_PACC), o.get(po.OF_PACC)),
+            "v": rel_norm(g.get(sphg.FIELD_V), o.get(po.OF_V)),
+            "x": rel_norm(g.get(sphg.FIELD_X), o.get(po.OF_X)),
+        }
+        bad = {k: v for k, v in errs.items() if not v < TOL}
+        assert not bad, (step, errs)
+    assert matched >= 6
+    g.close()
+
+
+def test_batch_alone_vs_mix(ctx):
+    """Scenes stepped together produce exactly what they produce alone."""
+    specs = [configs.small_scene(sphg.DFSPH, n_side=n, seed=s) for n, s in ((6, 1), (11, 2), (8, 3))]
+    alone = [sphg.Scene(ctx, s.params, s.fluid, s.boundary) for s in specs]
+    for a in alone:
+        a.step(5)
+    mixed = [sphg.Scene(ctx, s.params, s.fluid, s.boundary) for s in specs]
+    ctx.batch_step(mixed, 5)
+    for a, m in zip(alone, mixed):
+        assert np.array_equal(a.get(sphg.FIELD_X), m.get(sphg.FIELD_X))
+        assert np.array_equal(a.get(sphg.FIELD_V), m.get(sphg.FIELD_V))
+        assert a.scalars().sum_iterations == m.scalars().sum_iterations
+    for s in alone + mixed:
+        s.close()
+
+
+def test_step_is_deterministic(ctx):
+    spec = configs.small_scene(sphg.DFSPH, n_side=12)
+    runs = []
+    for _ in range(2):
+        g = sphg.Scene(ctx, spec.params, spec.fluid, spec.boundary)
+        g.step(4)
+        runs.append((g.get(sphg.FIELD_X), g.get(sphg.FIELD_V)))
+        g.close()
+    assert np.array_equal(runs[0][0], runs[1][0]) and np.array_equal(runs[0][1], runs[1][1])
