@@ -39,7 +39,7 @@ BALG = {
     "build": (20, 1, 1),
     "dt": (12, 0, 0),
     "df_positions": (40, 0, 0),
-    "reorder": (56, 0, 0),
+    "rank_reorder": (56, 0, 0),
 }
 
 
@@ -105,7 +105,7 @@ def main():
         ctx.synchronize()
         sb = scene.scalars()
         ctx.kernel_timing(False)
-        names = ["keys", "scan", "scatter", "cell_sort", "reorder", "build", "normals", "forces", "dt",
+        names = ["keys", "scan", "scatter", "rank_reorder", "finalize", "build", "normals", "forces", "dt",
                  "df_predict", "df_vel_warm", "df_vel_density", "df_vel_divergence", "df_err_density",
                  "df_err_divergence", "df_positions"]
         for nm in names:

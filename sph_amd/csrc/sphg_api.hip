@@ -42,12 +42,12 @@ constexpr int kDefaultMaxNbr = 96;
 // ----------------------------------------------------------- profiling
 
 enum Kind {
-    K_KEYS, K_SCAN, K_SCATTER, K_CELLSORT, K_REORDER, K_BUILD, K_NORMALS, K_FORCES, K_STEPBEGIN, K_DT,
+    K_KEYS, K_SCAN, K_SCATTER, K_FINALIZE, K_REORDER, K_BUILD, K_NORMALS, K_FORCES, K_STEPBEGIN, K_DT,
     K_DFPREDICT, K_DFVEL_WARM, K_DFVEL_DENS, K_DFVEL_DIV, K_DFERR_DENS, K_DFERR_DIV, K_DFPOS, K_PCIDENS,
     K_PCIPACC, K_PCIINT, K_TIME, K_NKINDS
 };
 const char* kKindNames[K_NKINDS] = {
-    "keys", "scan", "scatter", "cell_sort", "reorder", "build", "normals", "forces", "step_begin", "dt",
+    "keys", "scan", "scatter", "finalize", "rank_reorder", "build", "normals", "forces", "step_begin", "dt",
     "df_predict", "df_vel_warm", "df_vel_density", "df_vel_divergence", "df_err_density", "df_err_divergence",
     "df_positions", "pci_density", "pci_pacc", "pci_integrate", "time_advance"};
 
@@ -144,6 +144,7 @@ struct Batch {
     Chunk* dC = nullptr;
     int nP = 0, nC = 0;
     int maxIt = 0;
+    int minIt = 0, minItV = 0;   // smallest minimum iteration counts in the batch
     std::vector<int> parity;   // parity of each scene when the tables were built
     ~Batch() {
         if (dT[0]) (void)hipFree(dT[0]);
@@ -226,6 +227,25 @@ int check_desc(const sphg_scene_desc& d) {
     return SPHG_OK;
 }
 
+void fill_coef_d(KCoefD& k, double h) {
+    const double pi = M_PI;
+    const double h3 = h * h * h;
+    const double cl = 48.0 / (pi * h3);
+    const double h6 = h3 * h3;
+    const double h9 = h6 * h3;
+    k.h = h;
+    k.invh = 1.0 / h;
+    k.h2 = h * h;
+    k.ck = 8.0 / (pi * h3);
+    k.clh2 = cl / (h * h);
+    k.clh = cl / h;
+    k.spg = 45.0 / (pi * h6);
+    k.co1 = 32.0 / (pi * h9);
+    k.co2 = h6 / 64.0;
+    k.hlo2 = h * h * (1.0 - 1e-9);
+    k.hhi2 = h * h * (1.0 + 1e-9);
+}
+
 void fill_coef(KCoef& k, double h) {
     const double pi = M_PI;
     const double h3 = h * h * h;
@@ -251,6 +271,8 @@ int upload_batch_tables(sphg_ctx* ctx, Batch& B) {
     std::vector<Chunk> P, Cc;
     B.parity.resize(ns);
     B.maxIt = 0;
+    B.minIt = 1 << 30;
+    B.minItV = 1 << 30;
     for (int s = 0; s < ns; ++s) {
         sphg_scene* sc = B.scenes[s];
         t0[s] = sc->view[sc->par];
@@ -259,6 +281,8 @@ int upload_batch_tables(sphg_ctx* ctx, Batch& B) {
         for (int c = 0; c < t0[s].nChunks; ++c) P.push_back(Chunk{s, c * kBlock});
         for (int c = 0; c < t0[s].nCellChunks; ++c) Cc.push_back(Chunk{s, c * kCellChunk});
         B.maxIt = std::max(B.maxIt, std::max(t0[s].maxIt, std::max(t0[s].maxItV, 0)));
+        B.minIt = std::min(B.minIt, std::min(t0[s].minIt, t0[s].maxIt));
+        B.minItV = std::min(B.minItV, std::min(t0[s].minItV, t0[s].maxIt));
     }
     if (!B.dT[0]) {
         HIP_TRY(hipMalloc(&B.dT[0], sizeof(DevScene) * ns));
@@ -287,8 +311,7 @@ void grid_sort(sphg_ctx* ctx, Batch& B, const DevScene* T) {
     const int ns = (int)B.scenes.size();
     { Scope s(P, st, K_SCAN); launch_scan(st, T, B.dC, B.nC, ns); }
     { Scope s(P, st, K_SCATTER); launch_scatter(st, T, B.dP, B.nP); }
-    { Scope s(P, st, K_CELLSORT); launch_cell_sort(st, T, B.dC, B.nC); }
-    { Scope s(P, st, K_REORDER); launch_reorder(st, T, B.dP, B.nP); }
+    { Scope s(P, st, K_REORDER); launch_rank_reorder(st, T, B.dP, B.nP); }
 }
 
 int read_active(sphg_ctx* ctx, int& v, uint32_t& syncs) {
@@ -314,6 +337,27 @@ int self_tables(sphg_scene* s) {
     return SPHG_OK;
 }
 
+// Finalize kernel of a reduction (one block per scene).
+void finalize(sphg_ctx* ctx, Batch& B, const DevScene* T, int kind) {
+    Scope s(&ctx->prof, ctx->stream, K_FINALIZE);
+    launch_finalize(ctx->stream, kind, T, (int)B.scenes.size(), ctx->dActive);
+}
+
+// Whether another iteration of a loop is needed.  Below the smallest
+// minimum iteration count of the batch every scene iterates, so no read
+// is needed; past it, one int (the number of scenes still iterating) is
+// read back.
+int loop_more(sphg_ctx* ctx, int it, int minIt, bool& more, uint32_t& syncs) {
+    if (it < minIt) {
+        more = true;
+        return SPHG_OK;
+    }
+    int act = 0;
+    int rc = read_active(ctx, act, syncs);
+    more = act > 0;
+    return rc;
+}
+
 // One step of every scene in B (all of one method).
 int batch_one_step(sphg_ctx* ctx, Batch& B, uint32_t& syncs) {
     Profiler* P = &ctx->prof;
@@ -326,6 +370,7 @@ int batch_one_step(sphg_ctx* ctx, Batch& B, uint32_t& syncs) {
         anySt |= (s->view[0].flags & F_ST);
     }
     int rc;
+    bool more = false;
     if (method == DFSPH) {
         const DevScene* T = B.dT[0];
         if (anyFlags) {
@@ -334,29 +379,32 @@ int batch_one_step(sphg_ctx* ctx, Batch& B, uint32_t& syncs) {
             launch_forces(st, DFSPH, T, B.dP, B.nP);
         }
         { Scope s(P, st, K_DT); launch_dt(st, T, B.dP, B.nP); }
+        finalize(ctx, B, T, 0);
         { Scope s(P, st, K_DFPREDICT); launch_df_predict(st, T, B.dP, B.nP); }
         { Scope s(P, st, K_DFVEL_WARM); launch_df_vel(st, 0, T, B.dP, B.nP); }
         launch_fill_int(st, ctx->dActive, 0, 1);
-        { Scope s(P, st, K_DFERR_DENS); launch_df_err(st, 0, T, B.dP, B.nP, ctx->dActive); }
+        { Scope s(P, st, K_DFERR_DENS); launch_df_err(st, 0, T, B.dP, B.nP); }
+        finalize(ctx, B, T, 1);
         for (int it = 0; it < B.maxIt; ++it) {
-            int act = 0;
-            if ((rc = read_active(ctx, act, syncs))) return rc;
-            if (act <= 0) break;
+            if ((rc = loop_more(ctx, it, B.minIt, more, syncs))) return rc;
+            if (!more) break;
             { Scope s(P, st, K_DFVEL_DENS); launch_df_vel(st, 1, T, B.dP, B.nP); }
-            { Scope s(P, st, K_DFERR_DENS); launch_df_err(st, 1, T, B.dP, B.nP, ctx->dActive); }
+            { Scope s(P, st, K_DFERR_DENS); launch_df_err(st, 1, T, B.dP, B.nP); }
+            finalize(ctx, B, T, 2);
         }
         { Scope s(P, st, K_DFPOS); launch_df_positions(st, T, B.dP, B.nP); }
         grid_sort(ctx, B, T);
         const DevScene* T2 = B.dT[1];
         { Scope s(P, st, K_BUILD); launch_build(st, DFSPH, T2, B.dP, B.nP); }
         launch_fill_int(st, ctx->dActive, 0, 1);
-        { Scope s(P, st, K_DFERR_DIV); launch_df_err(st, 2, T2, B.dP, B.nP, ctx->dActive); }
+        { Scope s(P, st, K_DFERR_DIV); launch_df_err(st, 2, T2, B.dP, B.nP); }
+        finalize(ctx, B, T2, 3);
         for (int it = 0; it < B.maxIt; ++it) {
-            int act = 0;
-            if ((rc = read_active(ctx, act, syncs))) return rc;
-            if (act <= 0) break;
+            if ((rc = loop_more(ctx, it, B.minItV, more, syncs))) return rc;
+            if (!more) break;
             { Scope s(P, st, K_DFVEL_DIV); launch_df_vel(st, 2, T2, B.dP, B.nP); }
-            { Scope s(P, st, K_DFERR_DIV); launch_df_err(st, 3, T2, B.dP, B.nP, ctx->dActive); }
+            { Scope s(P, st, K_DFERR_DIV); launch_df_err(st, 3, T2, B.dP, B.nP); }
+            finalize(ctx, B, T2, 4);
         }
         flip(B);
     } else if (method == WCSPH) {
@@ -382,10 +430,10 @@ int batch_one_step(sphg_ctx* ctx, Batch& B, uint32_t& syncs) {
         if (anySt) { Scope s(P, st, K_NORMALS); launch_normals(st, T2, B.dP, B.nP); }
         { Scope s(P, st, K_FORCES); launch_forces(st, PCISPH, T2, B.dP, B.nP); }
         for (int it = 0; it < B.maxIt; ++it) {
-            int act = 0;
-            if ((rc = read_active(ctx, act, syncs))) return rc;
-            if (act <= 0) break;
-            { Scope s(P, st, K_PCIDENS); launch_pci_density(st, T2, B.dP, B.nP, ctx->dActive); }
+            if ((rc = loop_more(ctx, it, B.minIt, more, syncs))) return rc;
+            if (!more) break;
+            { Scope s(P, st, K_PCIDENS); launch_pci_density(st, T2, B.dP, B.nP); }
+            finalize(ctx, B, T2, 5);
             { Scope s(P, st, K_PCIPACC); launch_pci_pacc(st, T2, B.dP, B.nP, it); }
         }
         { Scope s(P, st, K_PCIINT); launch_pci_integrate(st, T2, B.dP, B.nP); }
@@ -607,7 +655,7 @@ int sphg_scene_init(sphg_scene* s) {
     const size_t oADH = (flags & F_ADH) ? A.reserve(f4) : 0;
     const size_t oRHO = A.reserve(f1);
     const size_t oALPHA = (method == DFSPH) ? A.reserve(f1) : 0;
-    const size_t oRHOS = (method == DFSPH) ? A.reserve(f1) : 0;
+    const size_t oDRHOS = (method == DFSPH) ? A.reserve(f1) : 0;
     const size_t oDIV = (method == DFSPH) ? A.reserve(f1) : 0;
     const size_t oP = (method != DFSPH) ? A.reserve(f1) : 0;
     const size_t oCELLU = A.reserve(u1), oCELL = A.reserve(u1), oPERM = A.reserve(u1);
@@ -630,6 +678,7 @@ int sphg_scene_init(sphg_scene* s) {
 
     DevScene D{};
     fill_coef(D.kc, h);
+    fill_coef_d(D.kd, h);
     D.hD = h;
     D.r = r;
     D.cfl = d.cfl_factor;
@@ -642,8 +691,11 @@ int sphg_scene_init(sphg_scene* s) {
     D.etaV = d.eta_v;
     D.m = (float)m;
     D.rho0 = (float)rho0;
+    D.mD = m;
+    D.rho0D = rho0;
     D.W0 = D.kc.ck;
-    D.pciDelta = (method == PCISPH) ? (float)pcisph_delta(r, h, m, rho0, d.time_step) : 0.f;
+    D.pciDeltaD = (method == PCISPH) ? pcisph_delta(r, h, m, rho0, d.time_step) : 0.0;
+    D.pciDelta = (float)D.pciDeltaD;
     D.gx = (float)d.gravity[0];
     D.gy = (float)d.gravity[1];
     D.gz = (float)d.gravity[2];
@@ -678,9 +730,9 @@ int sphg_scene_init(sphg_scene* s) {
     D.gbs = oGBS ? F4(oGBS) : nullptr;
     D.nrm = oNRM ? F4(oNRM) : nullptr;
     D.adh = oADH ? F4(oADH) : nullptr;
-    D.rho = F1(oRHO);
+    D.drho = F1(oRHO);
     D.alpha = oALPHA ? F1(oALPHA) : nullptr;
-    D.rhoS = oRHOS ? F1(oRHOS) : nullptr;
+    D.drhoS = oDRHOS ? F1(oDRHOS) : nullptr;
     D.divErr = oDIV ? F1(oDIV) : nullptr;
     D.p = oP ? F1(oP) : nullptr;
     D.cellU = (int*)(b + oCELLU);
@@ -865,7 +917,7 @@ int sphg_scene_get(sphg_scene* s, int32_t field, void* out) {
         case SPHG_FIELD_X: launch_unsort_f4(st, V.x, V.id, s->tmp, n, 3); bytes = 12ull * n; break;
         case SPHG_FIELD_V: launch_unsort_f4(st, V.v, V.id, s->tmp, n, 3); bytes = 12ull * n; break;
         case SPHG_FIELD_A: launch_unsort_f4(st, V.a, V.id, s->tmp, n, 3); bytes = 12ull * n; break;
-        case SPHG_FIELD_RHO: launch_unsort_f1(st, V.rho, V.id, s->tmp, n); bytes = 4ull * n; break;
+        case SPHG_FIELD_RHO: launch_unsort_f1(st, V.drho, V.id, s->tmp, n, V.rho0D); bytes = 4ull * n; break;
         case SPHG_FIELD_P:
             if (!V.p) return fail(SPHG_EINVAL, "no pressure field for this method");
             launch_unsort_f1(st, V.p, V.id, s->tmp, n); bytes = 4ull * n; break;
@@ -875,9 +927,9 @@ int sphg_scene_get(sphg_scene* s, int32_t field, void* out) {
         case SPHG_FIELD_K:
         case SPHG_FIELD_KV: {
             if (V.method != DFSPH) return fail(SPHG_EINVAL, "DFSPH field on another method");
-            const float* src = field == SPHG_FIELD_ALPHA ? V.alpha : field == SPHG_FIELD_RHOSTAR ? V.rhoS
+            const float* src = field == SPHG_FIELD_ALPHA ? V.alpha : field == SPHG_FIELD_RHOSTAR ? V.drhoS
                              : field == SPHG_FIELD_DIVERR ? V.divErr : field == SPHG_FIELD_K ? V.k : V.kv;
-            launch_unsort_f1(st, src, V.id, s->tmp, n);
+            launch_unsort_f1(st, src, V.id, s->tmp, n, field == SPHG_FIELD_RHOSTAR ? V.rho0D : 0.0);
             bytes = 4ull * n;
             break;
         }

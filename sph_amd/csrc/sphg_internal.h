@@ -13,7 +13,7 @@
 //   xk            float4 (x, y, z, s)             -- s = the scalar the next
 //                 neighbor sweep gathers (warm-start k, kappa, kappa_v, p/rho^2)
 //   gb            float4 (sum_b rho0 V_b gradW_ib, sum_b rho0 V_b W_ib)
-//   rho, alpha, rhoS, divErr, p  float
+//   drho = rho - rho0, alpha, drhoS = rho* - rho0, divErr, p  float
 //   k, kv         float (DFSPH warm-start accumulators, double-buffered)
 //   id            uint32 original particle id (double-buffered)
 //   cell          int    linear cell of each sorted particle
@@ -48,6 +48,13 @@ struct KCoef {
     float hlo2, hhi2;   // h^2 (1 -/+ 1e-4): FP32 pre-filter band of the exact test
 };
 
+// FP64 copies used by the neighbor sums (FP32 state, FP64 arithmetic).
+struct KCoefD {
+    double h, invh, h2;
+    double ck, clh2, clh, spg, co1, co2;
+    double hlo2, hhi2;   // h^2 (1 -/+ 1e-9): band of the operation-exact neighbor test
+};
+
 // Device-resident control state of one scene (one writer at a time: the
 // last block of a reduction, or the first block of a scene).
 struct Scalars {
@@ -59,9 +66,11 @@ struct Scalars {
 
 struct DevScene {
     KCoef kc;
+    KCoefD kd;
     double hD;                  // support radius in FP64 (cell keys, exact neighbor test)
     double r, cfl, dtMin, dtMax, eta, etaV;
     float m, rho0, W0, pciDelta;
+    double mD, rho0D, pciDeltaD;
     float gx, gy, gz, pad0;
     float nu, nub, stCoef, adhBeta;
     float stiffness, gammaEos, pad1, pad2;
@@ -76,7 +85,8 @@ struct DevScene {
     float *k, *k2, *kv, *kv2;
     uint32_t *id, *id2;
     float4 *a, *xk, *gb, *aux, *xs, *nrm, *pacc, *gbs, *adh;
-    float *rho, *alpha, *rhoS, *divErr, *p;
+    float *drho;                // rho - rho0 (keeps the deviation's precision in FP32)
+    float *alpha, *drhoS, *divErr, *p;   // drhoS = rho* - rho0
     int *cellU;                 // cell of each particle before the sort
     int *cell;                  // cell of each particle after the sort
     uint32_t *perm;

@@ -9,8 +9,7 @@ namespace sphg {
 void launch_keys(hipStream_t st, const DevScene* T, const Chunk* C, int nchunks);
 void launch_scan(hipStream_t st, const DevScene* T, const Chunk* CC, int ncellchunks, int nscenes);
 void launch_scatter(hipStream_t st, const DevScene* T, const Chunk* C, int nchunks);
-void launch_cell_sort(hipStream_t st, const DevScene* T, const Chunk* CC, int ncellchunks);
-void launch_reorder(hipStream_t st, const DevScene* T, const Chunk* C, int nchunks);
+void launch_rank_reorder(hipStream_t st, const DevScene* T, const Chunk* C, int nchunks);
 void launch_bvolume(hipStream_t st, const DevScene* T, int scene, int nb, float* tmp);
 void launch_build(hipStream_t st, int method, const DevScene* T, const Chunk* C, int nchunks);
 void launch_normals(hipStream_t st, const DevScene* T, const Chunk* C, int nchunks);
@@ -19,15 +18,17 @@ void launch_step_begin(hipStream_t st, const DevScene* T, const Chunk* C, int nc
 void launch_dt(hipStream_t st, const DevScene* T, const Chunk* C, int nchunks);
 void launch_df_predict(hipStream_t st, const DevScene* T, const Chunk* C, int nchunks);
 void launch_df_vel(hipStream_t st, int mode, const DevScene* T, const Chunk* C, int nchunks);
-void launch_df_err(hipStream_t st, int mode, const DevScene* T, const Chunk* C, int nchunks, int* active);
+void launch_df_err(hipStream_t st, int mode, const DevScene* T, const Chunk* C, int nchunks);
+// kind: 0 dt, 1 density (before loop), 2 density, 3 divergence (before loop), 4 divergence, 5 PCISPH
+void launch_finalize(hipStream_t st, int kind, const DevScene* T, int nscenes, int* active);
 void launch_df_positions(hipStream_t st, const DevScene* T, const Chunk* C, int nchunks);
-void launch_pci_density(hipStream_t st, const DevScene* T, const Chunk* C, int nchunks, int* active);
+void launch_pci_density(hipStream_t st, const DevScene* T, const Chunk* C, int nchunks);
 void launch_pci_pacc(hipStream_t st, const DevScene* T, const Chunk* C, int nchunks, int iter);
 void launch_pci_integrate(hipStream_t st, const DevScene* T, const Chunk* C, int nchunks);
 void launch_time_advance(hipStream_t st, const DevScene* T, int nscenes);
 void launch_fill_int(hipStream_t st, int* p, int v, long n);
 void launch_unsort_f4(hipStream_t st, const float4* src, const uint32_t* id, float* dst, int n, int comps);
-void launch_unsort_f1(hipStream_t st, const float* src, const uint32_t* id, float* dst, int n);
+void launch_unsort_f1(hipStream_t st, const float* src, const uint32_t* id, float* dst, int n, double offset = 0.0);
 void launch_unsort_i1(hipStream_t st, const int* src, const uint32_t* id, int* dst, int n);
 void launch_keys3(hipStream_t st, const DevScene* T, int scene, int n, int* dst);
 void launch_load_state(hipStream_t st, const DevScene* T, int scene, int n, const float* xyz, const float* vel,

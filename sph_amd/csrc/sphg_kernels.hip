@@ -48,21 +48,6 @@ __device__ inline double block_max(double v, double* sh) {
     return r;
 }
 
-// Arrival counter: returns true in every thread of the last block of the
-// scene to arrive.  The caller has written its partial from thread 0.
-__device__ inline bool arrive_last(unsigned* ctr, unsigned total) {
-    __shared__ int s_last;
-    if (threadIdx.x == 0) {
-        __threadfence();
-        const unsigned prev = atomicAdd(ctr, 1u);
-        s_last = (prev == total - 1u);
-    }
-    __syncthreads();
-    const bool last = s_last != 0;
-    if (last) __threadfence();
-    return last;
-}
-
 // Sum (or max) of one scene's chunk partials in chunk order.
 __device__ inline double scene_partial_sum(const double* partial, int n, double* sh) {
     double acc = 0.0;
@@ -75,55 +60,73 @@ __device__ inline double scene_partial_max(const double* partial, int n, double*
     return block_max(acc, sh);
 }
 
-// ------------------------------------------------------- SPH kernels, FP32
+// ------------------------------------------------ SPH kernels (FP64 math)
+//
+// State is FP32 in HBM; every neighbor sum is formed in FP64.  On CDNA3/4
+// an FP64 FMA issues at the rate of an unpacked FP32 FMA, and the FP32 sums
+// lose the small differences the solvers run on (rho* - rho0 of a density
+// near 1000).  1/|r| is an FP32 rsqrt refined by one FP64 Newton step
+// (relative error ~1e-14), so no FP64 transcendental is issued.
 
-// Cubic spline W(r) (src/CubicSpline.h:29-48).
-__device__ inline float W_cubic(float r2, const KCoef& k) {
-    const float rl = sqrtf(r2);
-    const float q = rl * k.invh;
-    const float om = 1.0f - q;
-    const float w = (q <= 0.5f) ? k.ck * ((6.0f * q - 6.0f) * q * q + 1.0f) : 2.0f * k.ck * om * om * om;
-    return (q <= 1.0f) ? w : 0.0f;
+__device__ inline double rsqrt_d(double r2) {
+    double y = (double)rsqrtf((float)r2);
+    return y * (1.5 - 0.5 * r2 * y * y);
 }
-// Cubic spline gradient as a factor of r: gradW(r) = g * r, zero for
-// |r| <= 1e-5 or q > 1 (src/CubicSpline.h:55-73).  For q <= 1/2 the
-// reference's l q (3q-2) / (|r| h) equals l (3q-2) / h^2.
-__device__ inline float G_cubic(float r2, const KCoef& k) {
-    const float inv = rsqrtf(r2);
-    const float rl = r2 * inv;
-    const float q = rl * k.invh;
-    const float om = 1.0f - q;
-    const float g = (q <= 0.5f) ? k.clh2 * (3.0f * q - 2.0f) : -k.clh * om * om * inv;
-    return (rl > 1e-5f && q <= 1.0f) ? g : 0.0f;
+
+struct Pair {
+    double dx, dy, dz, r2, rl, inv, q;
+};
+__device__ inline Pair pair_of(double xi, double yi, double zi, float4 b, const KCoefD& k) {
+    Pair p;
+    p.dx = xi - (double)b.x;
+    p.dy = yi - (double)b.y;
+    p.dz = zi - (double)b.z;
+    p.r2 = p.dx * p.dx + p.dy * p.dy + p.dz * p.dz;
+    p.inv = rsqrt_d(p.r2);
+    p.rl = (p.r2 > 1e-30) ? p.r2 * p.inv : 0.0;
+    p.q = p.rl * k.invh;
+    return p;
+}
+// Cubic spline W (src/CubicSpline.h:29-48).
+__device__ inline double W_d(double q, const KCoefD& k) {
+    const double om = 1.0 - q;
+    const double w = (q <= 0.5) ? k.ck * ((6.0 * q - 6.0) * q * q + 1.0) : 2.0 * k.ck * om * om * om;
+    return (q <= 1.0) ? w : 0.0;
+}
+// Cubic spline gradient factor: gradW(r) = g r, zero for |r| <= 1e-5 or
+// q > 1 (src/CubicSpline.h:55-73).  For q <= 1/2 the reference's
+// l q (3q-2) / (|r| h) equals l (3q-2) / h^2.
+__device__ inline double G_d(const Pair& p, const KCoefD& k) {
+    const double om = 1.0 - p.q;
+    const double g = (p.q <= 0.5) ? k.clh2 * (3.0 * p.q - 2.0) : -k.clh * om * om * p.inv;
+    return (p.rl > 1e-5 && p.q <= 1.0) ? g : 0.0;
 }
 // Spiky gradient factor (src/Spiky.h:44-55): -45/(pi h^6) (h-r)^2 / r.
-__device__ inline float G_spiky(float r2, const KCoef& k) {
-    const float inv = rsqrtf(r2);
-    const float rl = r2 * inv;
-    const float d = k.h - rl;
-    const float g = -k.spg * d * d * inv;
-    return (r2 > 0.0f && rl <= k.h) ? g : 0.0f;
+__device__ inline double Gs_d(const Pair& p, const KCoefD& k) {
+    const double d = k.h - p.rl;
+    const double g = -k.spg * d * d * p.inv;
+    return (p.r2 > 0.0 && p.rl <= k.h) ? g : 0.0;
 }
 // Cohesion C(r) (src/Cohesion.h:29-40) for 0 < r <= h; `outer` is the
 // branch test 2r > h.  As written there the h^6/64 offset is not scaled by
 // 32/(pi h^9), so C jumps at r = h/2 and the branch is decided exactly by
 // the caller near that point.
-__device__ inline float C_cohesion(float rl, bool outer, const KCoef& k) {
-    const float d = k.h - rl;
-    const float t = d * d * d * rl * rl * rl;
-    return outer ? k.co1 * t : k.co1 * 2.0f * t - k.co2;
+__device__ inline double C_d(double rl, bool outer, const KCoefD& k) {
+    const double d = k.h - rl;
+    const double t = d * d * d * rl * rl * rl;
+    return outer ? k.co1 * t : k.co1 * 2.0 * t - k.co2;
 }
 // Adhesion A(r) (src/Adhesion.h:24-35): (-4r^2/h + 6r - 2h)^(1/4) on
 // (h/2, h], evaluated as ((2/h)(2r-h)(h-r))^(1/4), the same polynomial in a
 // form that cannot round below zero.
-__device__ inline float A_adhesion(float rl, const KCoef& k) {
-    if (!(2.0f * rl > k.h && rl <= k.h)) return 0.0f;
-    const float v = (2.0f * k.invh) * (2.0f * rl - k.h) * (k.h - rl);
-    return sqrtf(sqrtf(fmaxf(v, 0.0f)));
+__device__ inline double A_d(double rl, const KCoefD& k) {
+    if (!(2.0 * rl > k.h && rl <= k.h)) return 0.0;
+    const double v = (2.0 * k.invh) * (2.0 * rl - k.h) * (k.h - rl);
+    return sqrt(sqrt(fmax(v, 0.0)));
 }
 
 // Exact FP64 distance test, operation for operation the reference's
-// length(ri - rj) / h <= 1 on FP32 positions promoted to FP64
+// length(ri - rj) / h on FP32 positions promoted to FP64
 // (src/CubicSpline.h:55-60, glm dot summed left to right).
 struct Exact {
     double rl, q;
@@ -273,7 +276,7 @@ __global__ __launch_bounds__(kBlock) void k_scan_final(const DevScene* __restric
     if (c0 <= S.ncells && S.ncells < c0 + kCellPerThread) S.cellStart[S.ncells] = S.n;
 }
 
-// Scatter into cell slots (order inside a cell fixed afterwards).
+// Scatter the particle ids into their cells' slots (arrival order).
 __global__ __launch_bounds__(kBlock) void k_scatter(const DevScene* __restrict__ T, const Chunk* __restrict__ C) {
     const Chunk c = C[blockIdx.x];
     const DevScene& S = T[c.scene];
@@ -281,49 +284,35 @@ __global__ __launch_bounds__(kBlock) void k_scatter(const DevScene* __restrict__
     if (i >= S.n) return;
     const int cl = S.cellU[i];
     const int slot = S.cellStart[cl] + atomicAdd(&S.cellCount[cl], 1);
-    S.perm[slot] = (uint32_t)i;
+    S.perm[slot] = S.id[i];
 }
 
-// Order each cell by particle id: the reference inserts in ascending id
-// (src/SPHSolver.cpp:188-191), so a bucket lists its particles by id.
-__global__ __launch_bounds__(kBlock) void k_cell_sort(const DevScene* __restrict__ T, const Chunk* __restrict__ C) {
+// Final slot of each particle = cell start + number of particles of its
+// cell with a smaller id, so every bucket lists its particles in ascending
+// id, the reference's insertion order (src/SPHSolver.cpp:188-191,
+// src/HashTable.h:79-86).  The state is written straight to that slot (the
+// other half of each double buffer); the cell counters are reset for the
+// next step.
+__global__ __launch_bounds__(kBlock) void k_rank_reorder(const DevScene* __restrict__ T, const Chunk* __restrict__ C) {
     const Chunk c = C[blockIdx.x];
     const DevScene& S = T[c.scene];
-    const int c0 = c.begin + threadIdx.x * kCellPerThread;
-    for (int u = 0; u < kCellPerThread; ++u) {
-        const int cc = c0 + u;
-        if (cc >= S.ncells) break;
-        const int s = S.cellStart[cc], e = S.cellStart[cc + 1];
-        S.cellCount[cc] = 0;
-        for (int a = s + 1; a < e; ++a) {
-            const uint32_t pa = S.perm[a];
-            const uint32_t ida = S.id[pa];
-            int b = a - 1;
-            while (b >= s && S.id[S.perm[b]] > ida) {
-                S.perm[b + 1] = S.perm[b];
-                --b;
-            }
-            S.perm[b + 1] = pa;
-        }
-    }
-}
-
-// Gather every persistent per-particle array into sorted order (writes the
-// other half of each double buffer).
-__global__ __launch_bounds__(kBlock) void k_reorder(const DevScene* __restrict__ T, const Chunk* __restrict__ C) {
-    const Chunk c = C[blockIdx.x];
-    const DevScene& S = T[c.scene];
-    const int s = c.begin + threadIdx.x;
-    if (s >= S.n) return;
-    const uint32_t o = S.perm[s];
-    S.x2[s] = S.x[o];
-    S.v2[s] = S.v[o];
-    S.id2[s] = S.id[o];
-    S.cell[s] = S.cellU[o];
+    const int i = c.begin + threadIdx.x;
+    if (i >= S.n) return;
+    const int cl = S.cellU[i];
+    const int s = S.cellStart[cl], e = S.cellStart[cl + 1];
+    const uint32_t me = S.id[i];
+    int rank = 0;
+    for (int q = s; q < e; ++q) rank += (S.perm[q] < me) ? 1 : 0;
+    const int d = s + rank;
+    S.x2[d] = S.x[i];
+    S.v2[d] = S.v[i];
+    S.id2[d] = me;
+    S.cell[d] = cl;
     if (S.method == DFSPH) {
-        S.k2[s] = S.k[o];
-        S.kv2[s] = S.kv[o];
+        S.k2[d] = S.k[i];
+        S.kv2[d] = S.kv[i];
     }
+    S.cellCount[cl] = 0;
 }
 
 // =========================================================== boundary
@@ -336,13 +325,14 @@ __global__ __launch_bounds__(kBlock) void k_bvolume(const DevScene* __restrict__
     if (b >= S.nb) return;
     const float4 xb = S.xb[b];
     const int cb = cell_of(S, xb);
-    float sum = 0.0f;
+    const KCoefD k = S.kd;
+    const double bx = xb.x, by = xb.y, bz = xb.z;
+    double sum = 0.0;
     walk27(S.bCellStart, cb, S.nx, S.nx * S.ny, [&](int j) {
-        const float4 y = S.xb[j];
-        const float dx = xb.x - y.x, dy = xb.y - y.y, dz = xb.z - y.z;
-        sum += W_cubic(dx * dx + dy * dy + dz * dz, S.kc);
+        const Pair p = pair_of(bx, by, bz, S.xb[j], k);
+        sum += W_d(p.q, k);
     });
-    bvol[b] = 1.0f / sum;
+    bvol[b] = (float)(1.0 / sum);
 }
 __global__ __launch_bounds__(kBlock) void k_bvolume_pack(const DevScene* __restrict__ T, int scene, const float* __restrict__ bvol) {
     const DevScene& S = T[scene];
@@ -370,24 +360,26 @@ __global__ __launch_bounds__(kBlock) void k_build(const DevScene* __restrict__ T
     const DevScene& S = T[c.scene];
     const int i = c.begin + threadIdx.x;
     if (i >= S.n) return;
-    const KCoef k = S.kc;
+    const KCoefD k = S.kd;
     const float4 xi = S.x[i];
+    const double xx = xi.x, xy = xi.y, xz = xi.z;
     const int ci = S.cell[i];
     const int nx = S.nx, nxny = S.nx * S.ny;
     uint32_t* L = list_of_w(S, i);
     const int maxN = S.maxNbr;
     int cnt = 0, ngr = 0;
-    float ws = 0.f, sgx = 0.f, sgy = 0.f, sgz = 0.f, ssq = 0.f;
+    double ws = 0.0, sgx = 0.0, sgy = 0.0, sgz = 0.0, ssq = 0.0;
     walk27(S.cellStart, ci, nx, nxny, [&](int j) {
         if (j == i) return;
         const float4 xj = S.x[j];
-        const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
-        const float r2 = dx * dx + dy * dy + dz * dz;
-        if (!(r2 <= k.hhi2)) return;
-        const float g = G_cubic(r2, k);
-        bool gnz = (g != 0.0f);
-        const float qf = sqrtf(r2) * k.invh;
-        if (r2 >= k.hlo2 || fabsf(qf - 0.6666667f) < 1e-5f || r2 < 1e-9f) {
+        {
+            const double ex = xx - (double)xj.x, ey = xy - (double)xj.y, ez = xz - (double)xj.z;
+            if (!(ex * ex + ey * ey + ez * ez <= k.hhi2)) return;
+        }
+        const Pair p = pair_of(xx, xy, xz, xj, k);
+        const double g = G_d(p, k);
+        bool gnz = (g != 0.0);
+        if (p.r2 >= k.hlo2 || fabs(p.q - 0.6666666666666666) < 1e-9 || p.r2 < 1e-9) {
             const Exact e = exact_q(xi, xj, S.hD);
             if (!(e.q <= 1.0)) return;
             gnz = grad_nonzero64(e);
@@ -395,92 +387,97 @@ __global__ __launch_bounds__(kBlock) void k_build(const DevScene* __restrict__ T
         if (cnt < maxN) L[(size_t)cnt * kGroup] = (uint32_t)j;
         ++cnt;
         ngr += gnz ? 1 : 0;
-        ws += W_cubic(r2, k);
-        sgx += g * dx;
-        sgy += g * dy;
-        sgz += g * dz;
-        ssq += g * g * r2;
+        ws += W_d(p.q, k);
+        sgx += g * p.dx;
+        sgy += g * p.dy;
+        sgz += g * p.dz;
+        ssq += g * g * p.r2;
     });
-    float bw = 0.f, bgx = 0.f, bgy = 0.f, bgz = 0.f;
-    float sx = 0.f, sy = 0.f, sz = 0.f;   // spiky boundary sum (PCISPH)
-    float ax = 0.f, ay = 0.f, az = 0.f;   // adhesion sum
+    double bw = 0.0, bgx = 0.0, bgy = 0.0, bgz = 0.0;
+    double sx = 0.0, sy = 0.0, sz = 0.0;   // spiky boundary sum (PCISPH)
+    double ax = 0.0, ay = 0.0, az = 0.0;   // adhesion sum
     const bool wantAdh = (S.flags & F_ADH) != 0;
+    const double rho0 = S.rho0D;
     walk27(S.bCellStart, ci, nx, nxny, [&](int b) {
         const float4 y = S.xb[b];
-        const float dx = xi.x - y.x, dy = xi.y - y.y, dz = xi.z - y.z;
-        const float r2 = dx * dx + dy * dy + dz * dz;
-        if (!(r2 <= k.hhi2)) return;
-        const float cb = S.rho0 * y.w;
-        bw += cb * W_cubic(r2, k);
-        const float g = G_cubic(r2, k);
-        bgx += cb * g * dx;
-        bgy += cb * g * dy;
-        bgz += cb * g * dz;
-        if (M == PCISPH) {
-            const float gs = G_spiky(r2, k);
-            sx += cb * gs * dx;
-            sy += cb * gs * dy;
-            sz += cb * gs * dz;
+        {
+            const double ex = xx - (double)y.x, ey = xy - (double)y.y, ez = xz - (double)y.z;
+            if (!(ex * ex + ey * ey + ez * ez <= k.hhi2)) return;
         }
-        if (wantAdh && r2 > 0.0f) {
-            const float rl = sqrtf(r2);
-            const float s = cb * A_adhesion(rl, k) / rl;
-            ax += s * dx;
-            ay += s * dy;
-            az += s * dz;
+        const Pair p = pair_of(xx, xy, xz, y, k);
+        if (!(p.q <= 1.0)) return;
+        const double cb = rho0 * (double)y.w;
+        bw += cb * W_d(p.q, k);
+        const double g = cb * G_d(p, k);
+        bgx += g * p.dx;
+        bgy += g * p.dy;
+        bgz += g * p.dz;
+        if (M == PCISPH) {
+            const double gs = cb * Gs_d(p, k);
+            sx += gs * p.dx;
+            sy += gs * p.dy;
+            sz += gs * p.dz;
+        }
+        if (wantAdh && p.r2 > 0.0) {
+            const double s = cb * A_d(p.rl, k) * p.inv;
+            ax += s * p.dx;
+            ay += s * p.dy;
+            az += s * p.dz;
         }
     });
     if (cnt > maxN) atomicMax(&S.ss->overflow, (unsigned)cnt);
     S.nnb[i] = (uint16_t)min(cnt, maxN);
     S.ngrad[i] = (uint8_t)min(ngr, 255);
-    const float rho = S.m * (S.W0 + ws) + bw;
-    S.rho[i] = rho;
-    S.gb[i] = make_float4(bgx, bgy, bgz, bw);
-    if (wantAdh) S.adh[i] = make_float4(ax, ay, az, 0.f);
+    const double m = S.mD;
+    const double drho = (m * (k.ck + ws) + bw) - rho0;
+    const double rho = rho0 + drho;
+    S.drho[i] = (float)drho;
+    S.gb[i] = make_float4((float)bgx, (float)bgy, (float)bgz, (float)bw);
+    if (wantAdh) S.adh[i] = make_float4((float)ax, (float)ay, (float)az, 0.f);
     if (M == DFSPH) {
-        const float tx = S.m * sgx + bgx, ty = S.m * sgy + bgy, tz = S.m * sgz + bgz;
-        const float den = tx * tx + ty * ty + tz * tz + S.m * S.m * ssq;
-        S.alpha[i] = (fabsf(den) > 1e-5f) ? rho / den : 0.0f;
+        const double tx = m * sgx + bgx, ty = m * sgy + bgy, tz = m * sgz + bgz;
+        const double den = tx * tx + ty * ty + tz * tz + m * m * ssq;
+        S.alpha[i] = (fabs(den) > 1e-5) ? (float)(rho / den) : 0.0f;
     } else if (M == WCSPH) {
-        // p = max(0, kappa rho0/gamma ((rho/rho0)^gamma - 1)) with
-        // (1+x)^g - 1 = expm1(g log1p(x)) and x = (rho - rho0)/rho0 exact-ish.
-        const float x = (rho - S.rho0) / S.rho0;
-        const float pr = fmaxf(0.0f, S.stiffness * S.rho0 / S.gammaEos * expm1f(S.gammaEos * log1pf(x)));
-        S.p[i] = pr;
-        S.xk[i] = make_float4(xi.x, xi.y, xi.z, pr / (rho * rho));
+        // p = max(0, kappa rho0/gamma ((rho/rho0)^gamma - 1)), written
+        // (1+x)^g - 1 = expm1(g log1p(x)) with x = (rho - rho0)/rho0
+        const double x = drho / rho0;
+        const double pr = fmax(0.0, (double)S.stiffness * rho0 / (double)S.gammaEos * expm1((double)S.gammaEos * log1p(x)));
+        S.p[i] = (float)pr;
+        S.xk[i] = make_float4(xi.x, xi.y, xi.z, (float)(pr / (rho * rho)));
         const float4 vi = S.v[i];
-        S.aux[i] = make_float4(vi.x, vi.y, vi.z, S.m / rho);
+        S.aux[i] = make_float4(vi.x, vi.y, vi.z, (float)(m / rho));
     } else {
-        S.gbs[i] = make_float4(sx, sy, sz, 0.f);
-        const float4 vi = S.v[i];
-        S.aux[i] = make_float4(vi.x, vi.y, vi.z, S.m / rho);
+        S.gbs[i] = make_float4((float)sx, (float)sy, (float)sz, 0.f);
     }
 }
 
 // ======================================================= non-pressure
 
 // Akinci surface normals n_i = h sum_j (m/rho_j) gradW_ij
-// (src/SurfaceTensionAkinci.h:46-61).  nrm.w carries rho_i for pass 2.
+// (src/SurfaceTensionAkinci.h:46-61).  nrm.w carries rho_i - rho0 for pass 2.
 __global__ __launch_bounds__(kBlock) void k_normals(const DevScene* __restrict__ T, const Chunk* __restrict__ C) {
     const Chunk c = C[blockIdx.x];
     const DevScene& S = T[c.scene];
     const int i = c.begin + threadIdx.x;
     if (i >= S.n) return;
+    const KCoefD k = S.kd;
     const float4 xi = S.x[i];
+    const double xx = xi.x, xy = xi.y, xz = xi.z;
     const uint32_t* L = list_of(S, i);
     const int cnt = S.nnb[i];
-    float nx = 0.f, ny = 0.f, nz = 0.f;
+    const double m = S.mD, rho0 = S.rho0D;
+    double nx = 0.0, ny = 0.0, nz = 0.0;
     for (int q = 0; q < cnt; ++q) {
         const int j = (int)L[(size_t)q * kGroup];
-        const float4 xj = S.x[j];
-        const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
-        const float g = G_cubic(dx * dx + dy * dy + dz * dz, S.kc) * (S.m / S.rho[j]);
-        nx += g * dx;
-        ny += g * dy;
-        nz += g * dz;
+        const Pair p = pair_of(xx, xy, xz, S.x[j], k);
+        const double g = G_d(p, k) * (m / (rho0 + (double)S.drho[j]));
+        nx += g * p.dx;
+        ny += g * p.dy;
+        nz += g * p.dz;
     }
-    const float h = S.kc.h;
-    S.nrm[i] = make_float4(nx * h, ny * h, nz * h, S.rho[i]);
+    const double h = k.h;
+    S.nrm[i] = make_float4((float)(nx * h), (float)(ny * h), (float)(nz * h), S.drho[i]);
 }
 
 // Non-pressure accelerations (and, for WCSPH, the pressure force and the
@@ -500,106 +497,109 @@ __global__ __launch_bounds__(kBlock) void k_forces(const DevScene* __restrict__ 
     const DevScene& S = T[c.scene];
     const int i = c.begin + threadIdx.x;
     if (i >= S.n) return;
-    const KCoef k = S.kc;
-    const float dt = (float)S.ss->dt;
-    const float invdt = 1.0f / dt;
+    const KCoefD k = S.kd;
+    const double dt = S.ss->dt;
     const bool xs = (S.flags & F_XSPH) != 0, st = (S.flags & F_ST) != 0;
     const float4 xi = (M == WCSPH) ? S.xk[i] : S.x[i];
     const float4 vi = (M == WCSPH) ? S.aux[i] : S.v[i];
-    const float rhoi = S.rho[i];
+    const double xx = xi.x, xy = xi.y, xz = xi.z;
+    const double vx0 = vi.x, vy0 = vi.y, vz0 = vi.z;
+    const double m = S.mD, rho0 = S.rho0D;
+    const double rhoi = rho0 + (double)S.drho[i];
     const float4 ni = st ? S.nrm[i] : make_float4(0.f, 0.f, 0.f, 0.f);
     const uint32_t* L = list_of(S, i);
     const int cnt = S.nnb[i];
-    const float m = S.m;
-    float px = 0.f, py = 0.f, pz = 0.f;   // pressure part
-    float vx = 0.f, vy = 0.f, vz = 0.f;   // XSPH part (before nu/dt)
-    float sx = 0.f, sy = 0.f, sz = 0.f;   // surface tension part (before gamma)
+    const double pri = xi.w;
+    double px = 0.0, py = 0.0, pz = 0.0;   // pressure part
+    double vx = 0.0, vy = 0.0, vz = 0.0;   // XSPH part (before nu/dt)
+    double sx = 0.0, sy = 0.0, sz = 0.0;   // surface tension part (before gamma)
     for (int q = 0; q < cnt; ++q) {
         const int j = (int)L[(size_t)q * kGroup];
         const float4 xj = (M == WCSPH) ? S.xk[j] : S.x[j];
-        const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
-        const float r2 = dx * dx + dy * dy + dz * dz;
+        const Pair p = pair_of(xx, xy, xz, xj, k);
         if (M == WCSPH) {
-            const float g = G_cubic(r2, k) * m * (xi.w + xj.w);
-            px -= g * dx;
-            py -= g * dy;
-            pz -= g * dz;
+            const double g = G_d(p, k) * m * (pri + (double)xj.w);
+            px -= g * p.dx;
+            py -= g * p.dy;
+            pz -= g * p.dz;
         }
         if (xs) {
-            float4 vj;
-            float mj;
+            double wx, wy, wz, mj;
             if (M == WCSPH) {
-                vj = S.aux[j];
+                const float4 vj = S.aux[j];
+                wx = vj.x; wy = vj.y; wz = vj.z;
                 mj = vj.w;
             } else {
-                vj = S.v[j];
-                mj = m / S.rho[j];
+                const float4 vj = S.v[j];
+                wx = vj.x; wy = vj.y; wz = vj.z;
+                mj = m / (rho0 + (double)S.drho[j]);
             }
-            const float w = mj * W_cubic(r2, k);
-            vx += w * (vj.x - vi.x);
-            vy += w * (vj.y - vi.y);
-            vz += w * (vj.z - vi.z);
+            const double w = mj * W_d(p.q, k);
+            vx += w * (wx - vx0);
+            vy += w * (wy - vy0);
+            vz += w * (wz - vz0);
         }
-        if (st) {
-            // list members satisfy |r| <= h exactly (FP64), as the
-            // reference's test at src/SurfaceTensionAkinci.h:88
-            const float rl = sqrtf(r2);
-            if (r2 > 0.0f) {
-                const float4 nj = S.nrm[j];
-                const float kij = 2.0f * S.rho0 / (rhoi + nj.w);
-                bool outer = 2.0f * rl > k.h;
-                if (fabsf(2.0f * rl - k.h) < 1e-5f * k.h) outer = 2.0 * exact_q(xi, xj, S.hD).rl > S.hD;
-                const float cs = -m * C_cohesion(rl, outer, k) / rl;
-                sx += kij * (cs * dx - (ni.x - nj.x));
-                sy += kij * (cs * dy - (ni.y - nj.y));
-                sz += kij * (cs * dz - (ni.z - nj.z));
-            }
+        if (st && p.r2 > 0.0) {
+            // list members satisfy |r| <= h exactly (FP64), the reference's
+            // test at src/SurfaceTensionAkinci.h:88
+            const float4 nj = S.nrm[j];
+            const double kij = 2.0 * rho0 / (rhoi + (rho0 + (double)nj.w));
+            bool outer = 2.0 * p.rl > k.h;
+            if (fabs(2.0 * p.rl - k.h) < 1e-9 * k.h) outer = 2.0 * exact_q(xi, xj, S.hD).rl > S.hD;
+            const double cs = -m * C_d(p.rl, outer, k) * p.inv;
+            sx += kij * (cs * p.dx - ((double)ni.x - (double)nj.x));
+            sy += kij * (cs * p.dy - ((double)ni.y - (double)nj.y));
+            sz += kij * (cs * p.dz - ((double)ni.z - (double)nj.z));
         }
     }
-    float ax = 0.f, ay = 0.f, az = 0.f;
+    double ax = 0.0, ay = 0.0, az = 0.0;
     if (M == WCSPH) {
         const float4 gb = S.gb[i];
-        ax = px - xi.w * gb.x;
-        ay = py - xi.w * gb.y;
-        az = pz - xi.w * gb.z;
+        ax = px - pri * (double)gb.x;
+        ay = py - pri * (double)gb.y;
+        az = pz - pri * (double)gb.z;
     }
     if (xs) {
-        const float f = S.nu * invdt;
+        const double f = (double)S.nu / dt;
         ax += f * vx;
         ay += f * vy;
         az += f * vz;
         if (S.nub != 0.0f) {
-            const float fb = S.nub * (S.gb[i].w / rhoi) * invdt;
-            ax -= fb * vi.x;
-            ay -= fb * vi.y;
-            az -= fb * vi.z;
+            const double fb = (double)S.nub * ((double)S.gb[i].w / rhoi) / dt;
+            ax -= fb * vx0;
+            ay -= fb * vy0;
+            az -= fb * vz0;
         }
     }
     if (st) {
-        ax += S.stCoef * sx;
-        ay += S.stCoef * sy;
-        az += S.stCoef * sz;
+        ax += (double)S.stCoef * sx;
+        ay += (double)S.stCoef * sy;
+        az += (double)S.stCoef * sz;
     }
     if (S.flags & F_ADH) {
         const float4 ad = S.adh[i];
-        ax -= S.adhBeta * ad.x;
-        ay -= S.adhBeta * ad.y;
-        az -= S.adhBeta * ad.z;
+        ax -= (double)S.adhBeta * (double)ad.x;
+        ay -= (double)S.adhBeta * (double)ad.y;
+        az -= (double)S.adhBeta * (double)ad.z;
     }
-    S.a[i] = make_float4(ax, ay, az, 0.f);
+    S.a[i] = make_float4((float)ax, (float)ay, (float)az, 0.f);
     if (M == WCSPH) {
-        const float nvx = vi.x + (ax + S.gx) * dt, nvy = vi.y + (ay + S.gy) * dt, nvz = vi.z + (az + S.gz) * dt;
-        S.v[i] = make_float4(nvx, nvy, nvz, 0.f);
-        S.x[i] = make_float4(xi.x + nvx * dt, xi.y + nvy * dt, xi.z + nvz * dt, 0.f);
+        const double nvx = vx0 + (ax + (double)S.gx) * dt;
+        const double nvy = vy0 + (ay + (double)S.gy) * dt;
+        const double nvz = vz0 + (az + (double)S.gz) * dt;
+        S.v[i] = make_float4((float)nvx, (float)nvy, (float)nvz, 0.f);
+        S.x[i] = make_float4((float)(xx + nvx * dt), (float)(xy + nvy * dt), (float)(xz + nvz * dt), 0.f);
     } else if (M == PCISPH) {
-        const float wx = vi.x + (ax + S.gx) * dt, wy = vi.y + (ay + S.gy) * dt, wz = vi.z + (az + S.gz) * dt;
-        S.xs[i] = make_float4(xi.x + wx * dt, xi.y + wy * dt, xi.z + wz * dt, 0.f);
+        const double wx = vx0 + (ax + (double)S.gx) * dt;
+        const double wy = vy0 + (ay + (double)S.gy) * dt;
+        const double wz = vz0 + (az + (double)S.gz) * dt;
+        S.xs[i] = make_float4((float)(wx * dt), (float)(wy * dt), (float)(wz * dt), 0.f);
     }
 }
 
-// WCSPH/PCISPH step-time bookkeeping: time += dt, steps++ (one thread per
-// scene) and, for PCISPH, the entry test of the pressure loop
-// (src/PCISPHSolver.cpp:365) with p = pacc = 0 (:83-120).
+// WCSPH/PCISPH step-time bookkeeping: steps++ (one thread per scene) and,
+// for PCISPH, the entry test of the pressure loop (src/PCISPHSolver.cpp:365)
+// with p = pacc = 0 (:83-120).
 __global__ __launch_bounds__(kBlock) void k_step_begin(const DevScene* __restrict__ T, const Chunk* __restrict__ C, int* active) {
     const Chunk c = C[blockIdx.x];
     const DevScene& S = T[c.scene];
@@ -628,44 +628,25 @@ __global__ __launch_bounds__(kBlock) void k_step_begin(const DevScene* __restric
 __global__ __launch_bounds__(kBlock) void k_dt(const DevScene* __restrict__ T, const Chunk* __restrict__ C) {
     const Chunk c = C[blockIdx.x];
     const DevScene& S = T[c.scene];
-    Scalars* ss = S.ss;
+    const Scalars* ss = S.ss;
     __shared__ double sh[4];
     const int i = c.begin + threadIdx.x;
-    const float dt = (float)ss->dt;
+    const double dt = ss->dt;
     double val = 0.0;
     if (i < S.n) {
         const float4 v = S.v[i];
-        float ax = S.gx, ay = S.gy, az = S.gz;
+        double ax = S.gx, ay = S.gy, az = S.gz;
         if (S.flags & (F_XSPH | F_ST | F_ADH)) {
             const float4 a = S.a[i];
-            ax += a.x;
-            ay += a.y;
-            az += a.z;
+            ax = (double)a.x + ax;
+            ay = (double)a.y + ay;
+            az = (double)a.z + az;
         }
-        const float cx = v.x + ax * dt, cy = v.y + ay * dt, cz = v.z + az * dt;
-        val = (double)sqrtf(sqrtf(cx * cx + cy * cy + cz * cz));
+        const double cx = v.x + ax * dt, cy = v.y + ay * dt, cz = v.z + az * dt;
+        val = sqrt(sqrt(cx * cx + cy * cy + cz * cz));
     }
     const double bm = block_max(val, sh);
     if (threadIdx.x == 0) S.partial[c.begin / kBlock] = bm;
-    if (arrive_last(&ss->ctr, (unsigned)S.nChunks)) {
-        const double mv = scene_partial_max(S.partial, S.nChunks, sh);
-        if (threadIdx.x == 0) {
-            const double maxVel = fmax(0.1, mv);
-            const double ts = ss->dt;
-            const double cflTs = S.cfl * 0.4 * 2.0 * S.r / maxVel;
-            double nts;
-            if (cflTs < ts) nts = cflTs;
-            else if (ss->it > 3) nts = ts * 0.99;
-            else nts = ts * 1.001;
-            nts = fmax(nts, S.dtMin);
-            nts = fmin(nts, S.dtMax);
-            ss->dt = nts;
-            ss->maxVel = maxVel;
-            ss->cflTs = cflTs;
-            ss->ctr = 0u;
-            ss->steps += 1u;
-        }
-    }
 }
 
 // predictVelocities (src/DFSPHSolver.cpp:191-214) fused with the first loop
@@ -676,22 +657,23 @@ __global__ __launch_bounds__(kBlock) void k_df_predict(const DevScene* __restric
     const DevScene& S = T[c.scene];
     const int i = c.begin + threadIdx.x;
     if (i >= S.n) return;
-    const float dt = (float)S.ss->dt;
+    const double dt = S.ss->dt;
     float4 v = S.v[i];
-    float ax = S.gx, ay = S.gy, az = S.gz;
+    double ax = S.gx, ay = S.gy, az = S.gz;
     if (S.flags & (F_XSPH | F_ST | F_ADH)) {
         const float4 a = S.a[i];
-        ax = a.x + S.gx;
-        ay = a.y + S.gy;
-        az = a.z + S.gz;
+        ax = (double)a.x + ax;
+        ay = (double)a.y + ay;
+        az = (double)a.z + az;
     }
-    v.x += ax * dt;
-    v.y += ay * dt;
-    v.z += az * dt;
+    v.x = (float)((double)v.x + ax * dt);
+    v.y = (float)((double)v.y + ay * dt);
+    v.z = (float)((double)v.z + az * dt);
     S.v[i] = v;
-    const float kw = -fmaxf(-S.k[i] / S.rho[i], -1.0e-9f) / (dt * dt);
+    const double rho = S.rho0D + (double)S.drho[i];
+    const double kw = -fmax(-(double)S.k[i] / rho, -1.0e-9) / (dt * dt);
     const float4 x = S.x[i];
-    S.xk[i] = make_float4(x.x, x.y, x.z, kw);
+    S.xk[i] = make_float4(x.x, x.y, x.z, (float)kw);
 }
 
 // Velocity correction sweep shared by the warm start (src/DFSPHSolver.cpp:
@@ -710,41 +692,43 @@ __global__ __launch_bounds__(kBlock) void k_df_vel(const DevScene* __restrict__ 
     if (MODE == 2 && ss->doneV) return;
     const int i = c.begin + threadIdx.x;
     if (i >= S.n) return;
-    const KCoef k = S.kc;
-    const float eps = 1.0e-5f;
-    const float dt = (float)ss->dt;
+    const KCoefD k = S.kd;
+    const double eps = 1.0e-5;
+    const double dt = ss->dt;
     const float4 xi = S.xk[i];
-    const float si = xi.w;
+    const double xx = xi.x, xy = xi.y, xz = xi.z;
+    const double si = xi.w;
     const uint32_t* L = list_of(S, i);
     const int cnt = S.nnb[i];
-    float sx = 0.f, sy = 0.f, sz = 0.f;
+    const double m = S.mD;
+    double sx = 0.0, sy = 0.0, sz = 0.0;
     for (int q = 0; q < cnt; ++q) {
         const int j = (int)L[(size_t)q * kGroup];
         const float4 xj = S.xk[j];
-        const float ks = si + xj.w;
-        if (fabsf(ks) > eps) {
-            const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
-            const float g = S.m * ks * G_cubic(dx * dx + dy * dy + dz * dz, k);
-            sx += g * dx;
-            sy += g * dy;
-            sz += g * dz;
+        const double ks = si + (double)xj.w;
+        if (fabs(ks) > eps) {
+            const Pair p = pair_of(xx, xy, xz, xj, k);
+            const double g = m * ks * G_d(p, k);
+            sx += g * p.dx;
+            sy += g * p.dy;
+            sz += g * p.dz;
         }
     }
-    const float cb = (MODE == 0) ? 2.0f * si : si;
-    if (fabsf(cb) > eps) {
+    const double cb = (MODE == 0) ? 2.0 * si : si;
+    if (fabs(cb) > eps) {
         const float4 gb = S.gb[i];
-        sx += cb * gb.x;
-        sy += cb * gb.y;
-        sz += cb * gb.z;
+        sx += cb * (double)gb.x;
+        sy += cb * (double)gb.y;
+        sz += cb * (double)gb.z;
     }
     float4 v = S.v[i];
-    v.x -= dt * sx;
-    v.y -= dt * sy;
-    v.z -= dt * sz;
+    v.x = (float)((double)v.x - dt * sx);
+    v.y = (float)((double)v.y - dt * sy);
+    v.z = (float)((double)v.z - dt * sz);
     S.v[i] = v;
     if (MODE == 0) S.k[i] = 0.0f;
-    if (MODE == 1) S.k[i] += (S.rhoS[i] - S.rho0) * S.alpha[i];
-    if (MODE == 2) S.kv[i] += S.divErr[i] * S.alpha[i];
+    if (MODE == 1) S.k[i] = (float)((double)S.k[i] + (double)S.drhoS[i] * (double)S.alpha[i]);
+    if (MODE == 2) S.kv[i] = (float)((double)S.kv[i] + (double)S.divErr[i] * (double)S.alpha[i]);
 }
 
 // Density or divergence error sweep with the scene reduction and the loop
@@ -759,96 +743,55 @@ __global__ __launch_bounds__(kBlock) void k_df_vel(const DevScene* __restrict__ 
 // (:327, :512; the divergence loop is bounded by maxIterations):
 //   continue while (err > eta (eta_V/dt) or it < minIt) and it < maxIt.
 template <int MODE>
-__global__ __launch_bounds__(kBlock) void k_df_err(const DevScene* __restrict__ T, const Chunk* __restrict__ C, int* active) {
+__global__ __launch_bounds__(kBlock) void k_df_err(const DevScene* __restrict__ T, const Chunk* __restrict__ C) {
     const Chunk c = C[blockIdx.x];
     const DevScene& S = T[c.scene];
-    Scalars* ss = S.ss;
+    const Scalars* ss = S.ss;
     constexpr bool DENS = (MODE <= 1);
     if (MODE == 1 && ss->done) return;
     if (MODE == 3 && ss->doneV) return;
     __shared__ double sh[4];
     const int i = c.begin + threadIdx.x;
-    const KCoef k = S.kc;
-    const float dt = (float)ss->dt;
+    const KCoefD k = S.kd;
+    const double dt = ss->dt;
     double e = 0.0;
     if (i < S.n) {
         const float4 xi = S.x[i];
         const float4 vi = S.v[i];
+        const double xx = xi.x, xy = xi.y, xz = xi.z;
+        const double ux = vi.x, uy = vi.y, uz = vi.z;
         const uint32_t* L = list_of(S, i);
         const int cnt = S.nnb[i];
-        float dc = 0.f;
+        double dc = 0.0;
         for (int q = 0; q < cnt; ++q) {
             const int j = (int)L[(size_t)q * kGroup];
             const float4 xj = S.x[j];
             const float4 vj = S.v[j];
-            const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
-            const float g = G_cubic(dx * dx + dy * dy + dz * dz, k);
-            dc += g * ((vi.x - vj.x) * dx + (vi.y - vj.y) * dy + (vi.z - vj.z) * dz);
+            const Pair p = pair_of(xx, xy, xz, xj, k);
+            const double g = G_d(p, k);
+            dc += g * ((ux - (double)vj.x) * p.dx + (uy - (double)vj.y) * p.dy + (uz - (double)vj.z) * p.dz);
         }
         const float4 gb = S.gb[i];
-        dc = S.m * dc + (vi.x * gb.x + vi.y * gb.y + vi.z * gb.z);
-        const float rho = S.rho[i], al = S.alpha[i];
-        float s;
+        dc = S.mD * dc + (ux * (double)gb.x + uy * (double)gb.y + uz * (double)gb.z);
+        const double drho = (double)S.drho[i];
+        const double rho = S.rho0D + drho;
+        const double al = S.alpha[i];
+        double s;
         if (DENS) {
-            const float rs = fmaxf(S.rho[i] + dc * dt, S.rho0);
-            S.rhoS[i] = rs;
-            s = (rs - S.rho0) * al / (dt * dt * rho);
-            e = ((double)rs - (double)S.rho0) / (double)S.rho0;
+            const double drs = fmax(drho + dc * dt, 0.0);
+            S.drhoS[i] = (float)drs;
+            s = drs * al / (dt * dt * rho);
+            e = drs / S.rho0D;
         } else {
-            const float de = (S.ngrad[i] < 20) ? 0.0f : fmaxf(dc, 0.0f);
-            S.divErr[i] = de;
+            const double de = (S.ngrad[i] < 20) ? 0.0 : fmax(dc, 0.0);
+            S.divErr[i] = (float)de;
             s = de * al / (dt * rho);
-            e = (double)de / (double)S.rho0;
+            e = de / S.rho0D;
         }
-        S.xk[i] = make_float4(xi.x, xi.y, xi.z, s);
+        S.xk[i] = make_float4(xi.x, xi.y, xi.z, (float)s);
     }
     const double bs = block_sum(e, sh);
     if (threadIdx.x == 0) S.partial[c.begin / kBlock] = bs;
-    if (arrive_last(&ss->ctr, (unsigned)S.nChunks)) {
-        const double tot = scene_partial_sum(S.partial, S.nChunks, sh);
-        if (threadIdx.x == 0) {
-            const double avg = (S.n == 0) ? 0.0 : tot / (double)S.n;
-            ss->ctr = 0u;
-            if (DENS) {
-                int it;
-                double err;
-                if (MODE == 0) {
-                    ss->avgErr0 = avg;
-                    ss->it = 0;
-                    it = 0;
-                    err = 0.0;
-                } else {
-                    it = ++ss->it;
-                    ss->avgErr = avg;
-                    err = avg;
-                }
-                const bool more = ((err > S.eta) || (it < S.minIt)) && (it < S.maxIt);
-                ss->done = more ? 0 : 1;
-                if (MODE == 0 && more) atomicAdd(active, 1);
-                if (MODE == 1 && !more) atomicSub(active, 1);
-                if (!more) ss->sumIt += (unsigned)it;
-            } else {
-                int it;
-                double err;
-                if (MODE == 2) {
-                    ss->avgErrV0 = avg;
-                    ss->itV = 0;
-                    it = 0;
-                    err = 0.0;
-                    ss->time += ss->dt;
-                } else {
-                    it = ++ss->itV;
-                    ss->avgErrV = avg;
-                    err = avg;
-                }
-                const bool more = ((err > S.etaV / ss->dt) || (it < S.minItV)) && (it < S.maxIt);
-                ss->doneV = more ? 0 : 1;
-                if (MODE == 2 && more) atomicAdd(active, 1);
-                if (MODE == 3 && !more) atomicSub(active, 1);
-                if (!more) ss->sumItV += (unsigned)it;
-            }
-        }
-    }
 }
 
 // updatePositions (src/DFSPHSolver.cpp:216-236) fused with the cell key and
@@ -858,12 +801,12 @@ __global__ __launch_bounds__(kBlock) void k_df_positions(const DevScene* __restr
     const DevScene& S = T[c.scene];
     const int i = c.begin + threadIdx.x;
     if (i >= S.n) return;
-    const float dt = (float)S.ss->dt;
+    const double dt = S.ss->dt;
     float4 x = S.x[i];
     const float4 v = S.v[i];
-    x.x += v.x * dt;
-    x.y += v.y * dt;
-    x.z += v.z * dt;
+    x.x = (float)((double)x.x + (double)v.x * dt);
+    x.y = (float)((double)x.y + (double)v.y * dt);
+    x.z = (float)((double)x.z + (double)v.z * dt);
     S.x[i] = x;
     const int cl = cell_of(S, x);
     S.cellU[i] = cl;
@@ -877,56 +820,53 @@ __global__ __launch_bounds__(kBlock) void k_df_positions(const DevScene* __restr
 // included), then updatePressure (:249-295): p_i += delta max(0, rho_i -
 // rho0); the scene's max relative error decides the loop (:365), counting
 // this iteration as done (its pressure acceleration always follows).
-__global__ __launch_bounds__(kBlock) void k_pci_density(const DevScene* __restrict__ T, const Chunk* __restrict__ C, int* active) {
+__global__ __launch_bounds__(kBlock) void k_pci_density(const DevScene* __restrict__ T, const Chunk* __restrict__ C) {
     const Chunk c = C[blockIdx.x];
     const DevScene& S = T[c.scene];
-    Scalars* ss = S.ss;
+    const Scalars* ss = S.ss;
     if (ss->done) return;
     __shared__ double sh[4];
     const int i = c.begin + threadIdx.x;
-    const KCoef k = S.kc;
+    const KCoefD k = S.kd;
     double e = 0.0;
     if (i < S.n) {
-        const float4 pi = S.xs[i];
+        // x* = x + d with d = v* dt held separately: x*_i - x*_j =
+        // (x_i - x_j) + (d_i - d_j) keeps FP64 accuracy from FP32 storage
+        const float4 xi = S.x[i];
+        const float4 di = S.xs[i];
+        const double px = xi.x, py = xi.y, pz = xi.z;
+        const double ex = di.x, ey = di.y, ez = di.z;
         const int ci = S.cell[i];
         const int nx = S.nx, nxny = S.nx * S.ny;
-        float ws = 0.f;
+        double ws = 0.0;
         walk27(S.cellStart, ci, nx, nxny, [&](int j) {
-            const float4 pj = S.xs[j];
-            const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-            ws += W_cubic(dx * dx + dy * dy + dz * dz, k);
+            const float4 xj = S.x[j];
+            const float4 dj = S.xs[j];
+            const double dx = (px - (double)xj.x) + (ex - (double)dj.x);
+            const double dy = (py - (double)xj.y) + (ey - (double)dj.y);
+            const double dz = (pz - (double)xj.z) + (ez - (double)dj.z);
+            const double r2 = dx * dx + dy * dy + dz * dz;
+            if (r2 <= k.hhi2) ws += W_d(sqrt(r2) * k.invh, k);
         });
-        float bw = 0.f;
+        double bw = 0.0;
         walk27(S.bCellStart, ci, nx, nxny, [&](int b) {
             const float4 y = S.xb[b];
-            const float dx = pi.x - y.x, dy = pi.y - y.y, dz = pi.z - y.z;
-            bw += y.w * W_cubic(dx * dx + dy * dy + dz * dz, k);
+            const double dx = (px - (double)y.x) + ex, dy = (py - (double)y.y) + ey, dz = (pz - (double)y.z) + ez;
+            const double r2 = dx * dx + dy * dy + dz * dz;
+            if (r2 <= k.hhi2) bw += (double)y.w * W_d(sqrt(r2) * k.invh, k);
         });
-        const float rho = S.m * ws + S.rho0 * bw;
-        S.rho[i] = rho;
-        const float err = fmaxf(0.0f, rho - S.rho0);
-        const float pr = S.p[i] + S.pciDelta * err;
-        S.p[i] = pr;
-        const float4 xi = S.x[i];
-        S.xk[i] = make_float4(xi.x, xi.y, xi.z, pr / (rho * rho));
-        e = (double)err / (double)S.rho0;
+        const double rho0 = S.rho0D;
+        const double drho = (S.mD * ws + rho0 * bw) - rho0;
+        const double rho = rho0 + drho;
+        S.drho[i] = (float)drho;
+        const double err = fmax(0.0, drho);
+        const double pr = (double)S.p[i] + S.pciDeltaD * err;
+        S.p[i] = (float)pr;
+        S.xk[i] = make_float4(xi.x, xi.y, xi.z, (float)(pr / (rho * rho)));
+        e = err / rho0;
     }
     const double bm = block_max(e, sh);
     if (threadIdx.x == 0) S.partial[c.begin / kBlock] = bm;
-    if (arrive_last(&ss->ctr, (unsigned)S.nChunks)) {
-        const double mx = scene_partial_max(S.partial, S.nChunks, sh);
-        if (threadIdx.x == 0) {
-            ss->ctr = 0u;
-            ss->maxDensErr = mx;
-            const int it = ++ss->it;
-            const bool more = ((mx > S.eta) || (it < S.minIt)) && (it < S.maxIt);
-            if (!more) {
-                ss->done = 1;
-                ss->sumIt += (unsigned)it;
-                atomicSub(active, 1);
-            }
-        }
-    }
 }
 
 // Pressure acceleration with the Spiky gradient at the current positions and
@@ -940,32 +880,35 @@ __global__ __launch_bounds__(kBlock) void k_pci_pacc(const DevScene* __restrict_
     if (ss->it <= iter) return;   // this scene finished before this iteration
     const int i = c.begin + threadIdx.x;
     if (i >= S.n) return;
-    const KCoef k = S.kc;
-    const float dt = (float)ss->dt;
+    const KCoefD k = S.kd;
+    const double dt = ss->dt;
     const float4 xi = S.xk[i];
+    const double xx = xi.x, xy = xi.y, xz = xi.z;
+    const double pri = xi.w;
     const uint32_t* L = list_of(S, i);
     const int cnt = S.nnb[i];
-    float ax = 0.f, ay = 0.f, az = 0.f;
+    const double m = S.mD;
+    double ax = 0.0, ay = 0.0, az = 0.0;
     for (int q = 0; q < cnt; ++q) {
         const int j = (int)L[(size_t)q * kGroup];
         const float4 xj = S.xk[j];
-        const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
-        const float g = S.m * (xi.w + xj.w) * G_spiky(dx * dx + dy * dy + dz * dz, k);
-        ax -= g * dx;
-        ay -= g * dy;
-        az -= g * dz;
+        const Pair p = pair_of(xx, xy, xz, xj, k);
+        const double g = m * (pri + (double)xj.w) * Gs_d(p, k);
+        ax -= g * p.dx;
+        ay -= g * p.dy;
+        az -= g * p.dz;
     }
     const float4 gs = S.gbs[i];
-    ax -= xi.w * gs.x;
-    ay -= xi.w * gs.y;
-    az -= xi.w * gs.z;
-    S.pacc[i] = make_float4(ax, ay, az, 0.f);
+    ax -= pri * (double)gs.x;
+    ay -= pri * (double)gs.y;
+    az -= pri * (double)gs.z;
+    S.pacc[i] = make_float4((float)ax, (float)ay, (float)az, 0.f);
     const float4 a = S.a[i];
     const float4 v = S.v[i];
-    const float wx = v.x + ((a.x + ax) + S.gx) * dt;
-    const float wy = v.y + ((a.y + ay) + S.gy) * dt;
-    const float wz = v.z + ((a.z + az) + S.gz) * dt;
-    S.xs[i] = make_float4(xi.x + wx * dt, xi.y + wy * dt, xi.z + wz * dt, 0.f);
+    const double wx = (double)v.x + (((double)a.x + ax) + (double)S.gx) * dt;
+    const double wy = (double)v.y + (((double)a.y + ay) + (double)S.gy) * dt;
+    const double wz = (double)v.z + (((double)a.z + az) + (double)S.gz) * dt;
+    S.xs[i] = make_float4((float)(wx * dt), (float)(wy * dt), (float)(wz * dt), 0.f);
 }
 
 // PCISPH integrate (src/PCISPHSolver.cpp:444-474): a += g + pacc.
@@ -974,19 +917,110 @@ __global__ __launch_bounds__(kBlock) void k_pci_integrate(const DevScene* __rest
     const DevScene& S = T[c.scene];
     const int i = c.begin + threadIdx.x;
     if (i >= S.n) return;
-    const float dt = (float)S.ss->dt;
+    const double dt = S.ss->dt;
     const float4 a = S.a[i], pa = S.pacc[i];
-    float4 v = S.v[i];
-    float4 x = S.x[i];
-    v.x += (a.x + (S.gx + pa.x)) * dt;
-    v.y += (a.y + (S.gy + pa.y)) * dt;
-    v.z += (a.z + (S.gz + pa.z)) * dt;
-    x.x += v.x * dt;
-    x.y += v.y * dt;
-    x.z += v.z * dt;
-    S.v[i] = v;
-    S.x[i] = x;
+    const float4 v0 = S.v[i];
+    const float4 x0 = S.x[i];
+    const double vx = (double)v0.x + ((double)a.x + ((double)S.gx + (double)pa.x)) * dt;
+    const double vy = (double)v0.y + ((double)a.y + ((double)S.gy + (double)pa.y)) * dt;
+    const double vz = (double)v0.z + ((double)a.z + ((double)S.gz + (double)pa.z)) * dt;
+    S.v[i] = make_float4((float)vx, (float)vy, (float)vz, 0.f);
+    S.x[i] = make_float4((float)((double)x0.x + vx * dt), (float)((double)x0.y + vy * dt),
+                         (float)((double)x0.z + vz * dt), 0.f);
     if (c.begin == 0 && threadIdx.x == 0) S.ss->time += S.ss->dt;
+}
+
+// =========================================================== finalize
+
+// Scene reductions and loop control, one block per scene, after the sweep
+// that wrote the chunk partials: the partials are added (or maxed) in chunk
+// order, so the result does not depend on scheduling.
+//   FIN_DT    CFL time step (src/DFSPHSolver.cpp:827-870): maxVel =
+//             max(0.1, max_i sqrt(|v_i + (a_i + g) dt|)); dt' = cfl if
+//             cfl < dt, else 0.99 dt after more than 3 density iterations,
+//             else 1.001 dt; clamped to [min, max].
+//   FIN_DENS0 / FIN_DENS   density loop of src/DFSPHSolver.cpp:320-398
+//   FIN_DIV0 / FIN_DIV     divergence loop of src/DFSPHSolver.cpp:505-582
+//             (bounded by maxIterations, :512; time += dt at its start)
+//   FIN_PCI   PCISPH loop test after updatePressure (src/PCISPHSolver.cpp:365)
+// The *0 kinds follow the call before the loop, whose error only feeds the
+// log: the loop's error starts at 0 (:326, :511).  `active` counts the
+// scenes still iterating.
+enum FinKind { FIN_DT = 0, FIN_DENS0, FIN_DENS, FIN_DIV0, FIN_DIV, FIN_PCI };
+
+template <int KIND>
+__global__ __launch_bounds__(kBlock) void k_finalize(const DevScene* __restrict__ T, int nscenes, int* active) {
+    const int sc = blockIdx.x;
+    if (sc >= nscenes) return;
+    const DevScene& S = T[sc];
+    Scalars* ss = S.ss;
+    if ((KIND == FIN_DENS || KIND == FIN_PCI) && ss->done) return;
+    if (KIND == FIN_DIV && ss->doneV) return;
+    __shared__ double sh[4];
+    const double red = (KIND == FIN_DT || KIND == FIN_PCI) ? scene_partial_max(S.partial, S.nChunks, sh)
+                                                           : scene_partial_sum(S.partial, S.nChunks, sh);
+    if (threadIdx.x != 0) return;
+    if (KIND == FIN_DT) {
+        const double maxVel = fmax(0.1, red);
+        const double ts = ss->dt;
+        const double cflTs = S.cfl * 0.4 * 2.0 * S.r / maxVel;
+        double nts;
+        if (cflTs < ts) nts = cflTs;
+        else if (ss->it > 3) nts = ts * 0.99;
+        else nts = ts * 1.001;
+        nts = fmax(nts, S.dtMin);
+        nts = fmin(nts, S.dtMax);
+        ss->dt = nts;
+        ss->maxVel = maxVel;
+        ss->cflTs = cflTs;
+        ss->steps += 1u;
+    } else if (KIND == FIN_PCI) {
+        ss->maxDensErr = red;
+        const int it = ++ss->it;
+        const bool more = ((red > S.eta) || (it < S.minIt)) && (it < S.maxIt);
+        if (!more) {
+            ss->done = 1;
+            ss->sumIt += (unsigned)it;
+            atomicSub(active, 1);
+        }
+    } else {
+        const double avg = (S.n == 0) ? 0.0 : red / (double)S.n;
+        const bool dens = (KIND == FIN_DENS0 || KIND == FIN_DENS);
+        const bool first = (KIND == FIN_DENS0 || KIND == FIN_DIV0);
+        int it;
+        double err;
+        if (first) {
+            it = 0;
+            err = 0.0;
+            if (dens) {
+                ss->avgErr0 = avg;
+                ss->it = 0;
+            } else {
+                ss->avgErrV0 = avg;
+                ss->itV = 0;
+                ss->time += ss->dt;
+            }
+        } else {
+            err = avg;
+            if (dens) {
+                it = ++ss->it;
+                ss->avgErr = avg;
+            } else {
+                it = ++ss->itV;
+                ss->avgErrV = avg;
+            }
+        }
+        const bool more = dens ? (((err > S.eta) || (it < S.minIt)) && (it < S.maxIt))
+                               : (((err > S.etaV / ss->dt) || (it < S.minItV)) && (it < S.maxIt));
+        if (dens) ss->done = more ? 0 : 1;
+        else ss->doneV = more ? 0 : 1;
+        if (first && more) atomicAdd(active, 1);
+        if (!first && !more) atomicSub(active, 1);
+        if (!more) {
+            if (dens) ss->sumIt += (unsigned)it;
+            else ss->sumItV += (unsigned)it;
+        }
+    }
 }
 
 // WCSPH: time += dt after the integration (src/WCSPHSolver.cpp:77).
@@ -1013,10 +1047,11 @@ __global__ void k_unsort_f4(const float4* __restrict__ src, const uint32_t* __re
     if (comps > 2) d[2] = v.z;
     if (comps > 3) d[3] = v.w;
 }
-__global__ void k_unsort_f1(const float* __restrict__ src, const uint32_t* __restrict__ id, float* __restrict__ dst, int n) {
+__global__ void k_unsort_f1(const float* __restrict__ src, const uint32_t* __restrict__ id, float* __restrict__ dst, int n,
+                            double offset) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
-    dst[id[s]] = src[s];
+    dst[id[s]] = (float)(offset + (double)src[s]);
 }
 __global__ void k_unsort_i1(const int* __restrict__ src, const uint32_t* __restrict__ id, int* __restrict__ dst, int n) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1060,8 +1095,7 @@ void launch_scan(hipStream_t st, const DevScene* T, const Chunk* CC, int ncc, in
     LAUNCH(k_scan_final, ncc, st, T, CC);
 }
 void launch_scatter(hipStream_t st, const DevScene* T, const Chunk* C, int n) { if (n) LAUNCH(k_scatter, n, st, T, C); }
-void launch_cell_sort(hipStream_t st, const DevScene* T, const Chunk* CC, int ncc) { if (ncc) LAUNCH(k_cell_sort, ncc, st, T, CC); }
-void launch_reorder(hipStream_t st, const DevScene* T, const Chunk* C, int n) { if (n) LAUNCH(k_reorder, n, st, T, C); }
+void launch_rank_reorder(hipStream_t st, const DevScene* T, const Chunk* C, int n) { if (n) LAUNCH(k_rank_reorder, n, st, T, C); }
 void launch_bvolume(hipStream_t st, const DevScene* T, int scene, int nb, float* tmp) {
     if (!nb) return;
     const int g = (nb + kBlock - 1) / kBlock;
@@ -1090,15 +1124,26 @@ void launch_df_vel(hipStream_t st, int mode, const DevScene* T, const Chunk* C, 
     else if (mode == 1) LAUNCH(k_df_vel<1>, n, st, T, C);
     else LAUNCH(k_df_vel<2>, n, st, T, C);
 }
-void launch_df_err(hipStream_t st, int mode, const DevScene* T, const Chunk* C, int n, int* active) {
+void launch_df_err(hipStream_t st, int mode, const DevScene* T, const Chunk* C, int n) {
     if (!n) return;
-    if (mode == 0) LAUNCH(k_df_err<0>, n, st, T, C, active);
-    else if (mode == 1) LAUNCH(k_df_err<1>, n, st, T, C, active);
-    else if (mode == 2) LAUNCH(k_df_err<2>, n, st, T, C, active);
-    else LAUNCH(k_df_err<3>, n, st, T, C, active);
+    if (mode == 0) LAUNCH(k_df_err<0>, n, st, T, C);
+    else if (mode == 1) LAUNCH(k_df_err<1>, n, st, T, C);
+    else if (mode == 2) LAUNCH(k_df_err<2>, n, st, T, C);
+    else LAUNCH(k_df_err<3>, n, st, T, C);
+}
+void launch_finalize(hipStream_t st, int kind, const DevScene* T, int nscenes, int* active) {
+    if (!nscenes) return;
+    switch (kind) {
+        case FIN_DT: LAUNCH(k_finalize<FIN_DT>, nscenes, st, T, nscenes, active); break;
+        case FIN_DENS0: LAUNCH(k_finalize<FIN_DENS0>, nscenes, st, T, nscenes, active); break;
+        case FIN_DENS: LAUNCH(k_finalize<FIN_DENS>, nscenes, st, T, nscenes, active); break;
+        case FIN_DIV0: LAUNCH(k_finalize<FIN_DIV0>, nscenes, st, T, nscenes, active); break;
+        case FIN_DIV: LAUNCH(k_finalize<FIN_DIV>, nscenes, st, T, nscenes, active); break;
+        default: LAUNCH(k_finalize<FIN_PCI>, nscenes, st, T, nscenes, active); break;
+    }
 }
 void launch_df_positions(hipStream_t st, const DevScene* T, const Chunk* C, int n) { if (n) LAUNCH(k_df_positions, n, st, T, C); }
-void launch_pci_density(hipStream_t st, const DevScene* T, const Chunk* C, int n, int* active) { if (n) LAUNCH(k_pci_density, n, st, T, C, active); }
+void launch_pci_density(hipStream_t st, const DevScene* T, const Chunk* C, int n) { if (n) LAUNCH(k_pci_density, n, st, T, C); }
 void launch_pci_pacc(hipStream_t st, const DevScene* T, const Chunk* C, int n, int iter) { if (n) LAUNCH(k_pci_pacc, n, st, T, C, iter); }
 void launch_pci_integrate(hipStream_t st, const DevScene* T, const Chunk* C, int n) { if (n) LAUNCH(k_pci_integrate, n, st, T, C); }
 void launch_time_advance(hipStream_t st, const DevScene* T, int nscenes) {
@@ -1111,8 +1156,8 @@ void launch_fill_int(hipStream_t st, int* p, int v, long n) {
 void launch_unsort_f4(hipStream_t st, const float4* src, const uint32_t* id, float* dst, int n, int comps) {
     if (n) hipLaunchKernelGGL(k_unsort_f4, dim3((n + 255) / 256), dim3(256), 0, st, src, id, dst, n, comps);
 }
-void launch_unsort_f1(hipStream_t st, const float* src, const uint32_t* id, float* dst, int n) {
-    if (n) hipLaunchKernelGGL(k_unsort_f1, dim3((n + 255) / 256), dim3(256), 0, st, src, id, dst, n);
+void launch_unsort_f1(hipStream_t st, const float* src, const uint32_t* id, float* dst, int n, double offset) {
+    if (n) hipLaunchKernelGGL(k_unsort_f1, dim3((n + 255) / 256), dim3(256), 0, st, src, id, dst, n, offset);
 }
 void launch_unsort_i1(hipStream_t st, const int* src, const uint32_t* id, int* dst, int n) {
     if (n) hipLaunchKernelGGL(k_unsort_i1, dim3((n + 255) / 256), dim3(256), 0, st, src, id, dst, n);

@@ -1,0 +1,382 @@
+// SPH.cpp -- host layer of the reference class surface over libsphg.
+#include "SPH.h"
+
+#include <algorithm>
+#include <cstdio>
+#include <sstream>
+
+// ------------------------------------------------------------ helpers
+
+namespace {
+
+void check(int rc, const char* what) {
+    if (rc != SPHG_OK) throw SphError(std::string(what) + ": " + sphg_last_error());
+}
+
+std::vector<float> toFloat(const std::vector<Vector3r>& v) {
+    std::vector<float> out(3 * v.size());
+    for (size_t i = 0; i < v.size(); ++i) {
+        out[3 * i] = (float)v[i].x;
+        out[3 * i + 1] = (float)v[i].y;
+        out[3 * i + 2] = (float)v[i].z;
+    }
+    return out;
+}
+
+std::vector<Vector3r> fromFloat(const std::vector<float>& f) {
+    std::vector<Vector3r> out(f.size() / 3);
+    for (size_t i = 0; i < out.size(); ++i) out[i] = Vector3r(f[3 * i], f[3 * i + 1], f[3 * i + 2]);
+    return out;
+}
+
+sphg_ctx*& sharedCtx() {
+    static sphg_ctx* c = nullptr;
+    return c;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------- models
+
+void FluidModel::resizeFluid(unsigned int n) {
+    mass.resize(n);
+    density.resize(n);
+    pressure.resize(n);
+    r.resize(n);
+    v.resize(n);
+    a.resize(n);
+}
+
+void FluidModel::init(std::vector<Vector3r>& points, std::vector<Vector3r>& velocities) {
+    resizeFluid((unsigned int)points.size());
+    for (size_t i = 0; i < points.size(); ++i) {
+        r[i] = points[i];
+        v[i] = i < velocities.size() ? velocities[i] : Vector3r();
+    }
+    numActiveParticles = (unsigned int)points.size();
+}
+
+void FluidModel::setMasses(Real m) { std::fill(mass.begin(), mass.end(), m); }
+
+void FluidModel::setViscosityForce(Real viscosity, Real boundaryViscosity) {
+    Simulation* sim = Simulation::getCurrent();
+    const int m = sim->getViscosityMethod();
+    if (m == Simulation::XSPH_VISCOSITY_METHOD) {
+        auto* f = new XSPHViscosity(this);
+        f->init(viscosity, boundaryViscosity);
+        npForces.emplace_back(f);
+    } else if (m == Simulation::STANDARD_VISCOSITY_METHOD || m == Simulation::ARTIFICIAL_VISCOSITY_METHOD) {
+        throw SphError("viscosityMethod " + std::to_string(m) + " is not available on the GPU solver (XSPH = 2 is)");
+    }
+}
+
+void FluidModel::setSurfaceTensionForce(Real c) {
+    if (Simulation::getCurrent()->getSurfaceTensionMethod() == Simulation::AKINCI_SURFACE_TENSION_METHOD) {
+        auto* f = new SurfaceTensionAkinci(this);
+        f->init(c);
+        npForces.emplace_back(f);
+    }
+}
+
+void FluidModel::setAdhesionForce(Real beta) {
+    if (Simulation::getCurrent()->getAdhesionMethod() == Simulation::AKINCI_ADHESION_METHOD) {
+        auto* f = new AdhesionForce(this);
+        f->init(beta);
+        npForces.emplace_back(f);
+    }
+}
+
+void AkinciBoundaryModel::addCube(Vector3r mn, Vector3r mx) {
+    const double a[3] = {mn.x, mn.y, mn.z}, b[3] = {mx.x, mx.y, mx.z};
+    const double h = Simulation::getCurrent()->getSupportRadius();
+    const int64_t n = sphg_sample_box(a, b, h, nullptr, 0);
+    if (n < 0) throw SphError("bad box");
+    std::vector<float> pts(3 * (size_t)n);
+    sphg_sample_box(a, b, h, pts.data(), n);
+    auto add = fromFloat(pts);
+    const size_t base = r.size();
+    resizeBoundary((unsigned int)(base + add.size()));
+    std::copy(add.begin(), add.end(), r.begin() + base);
+}
+
+void AkinciBoundaryModel::addSphere(Vector3r o, Real radius) {
+    const double c[3] = {o.x, o.y, o.z};
+    const double pr = Simulation::getCurrent()->getParticleRadius();
+    const int64_t n = sphg_sample_sphere(c, radius, pr, nullptr, 0);
+    if (n < 0) throw SphError("bad sphere");
+    std::vector<float> pts(3 * (size_t)n);
+    sphg_sample_sphere(c, radius, pr, pts.data(), n);
+    auto add = fromFloat(pts);
+    const size_t base = r.size();
+    resizeBoundary((unsigned int)(base + add.size()));
+    std::copy(add.begin(), add.end(), r.begin() + base);
+}
+
+// ------------------------------------------------------------ solvers
+
+GpuSolver::~GpuSolver() {
+    if (scene) sphg_scene_destroy(scene);
+}
+
+void GpuSolver::fillDesc(sphg_scene_desc& d) const {
+    Simulation* sim = Simulation::getCurrent();
+    d.method = method;
+    d.boundary_method = sim->getBoundaryHandlingMethod();
+    d.particle_radius = sim->getParticleRadius();
+    d.density0 = sim->numberFluidModels() ? sim->getFluidModel(0)->getRefDensity() : 1000.0;
+    const Vector3r g = sim->getGravity();
+    d.gravity[0] = g.x;
+    d.gravity[1] = g.y;
+    d.gravity[2] = g.z;
+    d.time_step = sim->getTimeStep();
+    d.min_time_step = sim->getMinTimeStep();
+    d.max_time_step = sim->getMaxTimeStep();
+    d.fps = sim->getFPS();
+    d.eta = maxError;
+    d.min_iterations = minIterations;
+    d.max_iterations = maxIterations;
+    d.viscosity_method = -1;
+    d.surface_tension_method = -1;
+    d.adhesion_method = -1;
+    d.max_neighbors = sim->getMaxNeighbors();
+    if (sim->numberFluidModels()) {
+        FluidModel* fm = sim->getFluidModel(0);
+        for (unsigned int i = 0; i < fm->numberNonPressureForces(); ++i) {
+            NonPressureForce* f = fm->getNonPressureForce(i);
+            if (auto* x = dynamic_cast<XSPHViscosity*>(f)) {
+                d.viscosity_method = SPHG_VISCOSITY_XSPH;
+                d.viscosity = x->getViscosity();
+                d.boundary_viscosity = x->getBoundaryViscosity();
+            } else if (auto* s = dynamic_cast<SurfaceTensionAkinci*>(f)) {
+                d.surface_tension_method = SPHG_SURFACE_TENSION_AKINCI;
+                d.surface_tension = s->getSurfaceTension();
+            } else if (auto* a = dynamic_cast<AdhesionForce*>(f)) {
+                d.adhesion_method = SPHG_ADHESION_AKINCI;
+                d.adhesion = a->getAdhesion();
+            }
+        }
+    }
+}
+
+void WCSPHSolver::fillDesc(sphg_scene_desc& d) const {
+    GpuSolver::fillDesc(d);
+    d.stiffness = stiffness;
+    d.gamma = gamma;
+}
+
+void DFSPHSolver::fillDesc(sphg_scene_desc& d) const {
+    GpuSolver::fillDesc(d);
+    d.cfl_factor = cflFactor;
+    d.eta_v = maxErrorV;
+    d.min_iterations_v = minIterationsV;
+    d.max_iterations_v = maxIterationsV;
+}
+
+// Simulation::init -> solver->init(): one fluid model and one Akinci
+// boundary model are uploaded (the reference's own limits, its model-id
+// handling at src/Simulation.h:24-29 and src/AkinciBoundayModel.cpp:32).
+void GpuSolver::init() {
+    Simulation* sim = Simulation::getCurrent();
+    if (sim->getBoundaryHandlingMethod() != Simulation::AKINCI_BOUNDARY_METHOD)
+        throw SphError("the GPU solver needs boundaryMethod 2 (Akinci)");
+    if (sim->numberFluidModels() != 1) throw SphError("the GPU solver runs one fluid model per scene");
+    sphg_ctx*& c = sharedCtx();
+    if (!c) check(sphg_ctx_create(sim->getDevice(), &c), "sphg_ctx_create");
+    ctx = c;
+    sphg_scene_desc d{};
+    fillDesc(d);
+    check(sphg_scene_create(ctx, &d, &scene), "sphg_scene_create");
+    FluidModel* fm = sim->getFluidModel(0);
+    std::vector<float> x = toFloat(fm->getPositions()), v = toFloat(fm->getVelocitiesVec());
+    check(sphg_scene_set_fluid(scene, x.data(), v.data(), fm->getNumParticles()), "set_fluid");
+    std::vector<Vector3r> bpts;
+    for (unsigned int b = 0; b < sim->numberBoundaryModels(); ++b) {
+        auto& p = sim->getBoundaryModel(b)->getPosition();
+        bpts.insert(bpts.end(), p.begin(), p.end());
+    }
+    std::vector<float> bx = toFloat(bpts);
+    check(sphg_scene_set_boundary(scene, bx.data(), (uint32_t)bpts.size()), "set_boundary");
+    check(sphg_scene_init(scene), "sphg_scene_init");
+    // boundary volumes back into the Akinci models
+    std::vector<float> vol(std::max<size_t>(bpts.size(), 1));
+    check(sphg_scene_get(scene, SPHG_FIELD_BVOL, vol.data()), "get volumes");
+    size_t off = 0;
+    for (unsigned int b = 0; b < sim->numberBoundaryModels(); ++b) {
+        auto* abm = dynamic_cast<AkinciBoundaryModel*>(sim->getBoundaryModel(b));
+        const unsigned int nb = sim->getBoundaryModel(b)->getNumParticles();
+        if (abm)
+            for (unsigned int i = 0; i < nb; ++i) abm->getVolume(i) = vol[off + i];
+        off += nb;
+    }
+    check(sphg_scene_get_scalars(scene, &last), "scalars");
+    syncToHost();
+}
+
+void GpuSolver::step() {
+    if (!scene) throw SphError("step before init");
+    sphg_step_stats st{};
+    check(sphg_scene_step(scene, 1, &st), "sphg_scene_step");
+    gpuMs += st.gpu_ms;
+    check(sphg_scene_get_scalars(scene, &last), "scalars");
+    steps = last.steps;
+    iterations = (unsigned int)last.iterations;
+    sumIterations = last.sum_iterations;
+    maxVel = last.max_vel;
+    Simulation* sim = Simulation::getCurrent();
+    sim->setTimeStep(last.time_step);
+    sim->setTime(last.time);
+    hostFresh = false;
+}
+
+void GpuSolver::syncToHost() {
+    if (!scene || hostFresh) return;
+    Simulation* sim = Simulation::getCurrent();
+    FluidModel* fm = sim->getFluidModel(0);
+    const unsigned int n = fm->getNumParticles();
+    std::vector<float> buf(3 * (size_t)n);
+    check(sphg_scene_get(scene, SPHG_FIELD_X, buf.data()), "get x");
+    fm->getPositions() = fromFloat(buf);
+    check(sphg_scene_get(scene, SPHG_FIELD_V, buf.data()), "get v");
+    fm->getVelocitiesVec() = fromFloat(buf);
+    std::vector<float> s(n);
+    check(sphg_scene_get(scene, SPHG_FIELD_RHO, s.data()), "get rho");
+    for (unsigned int i = 0; i < n; ++i) fm->getDensity(i) = s[i];
+    if (method != SPHG_DFSPH) {
+        check(sphg_scene_get(scene, SPHG_FIELD_P, s.data()), "get p");
+        for (unsigned int i = 0; i < n; ++i) fm->getPressure(i) = s[i];
+    }
+    hostFresh = true;
+}
+
+// --------------------------------------------------------- simulation
+
+Simulation* Simulation::current = nullptr;
+const int Simulation::WCSPH_METHOD = 0;
+const int Simulation::PCISPH_METHOD = 1;
+const int Simulation::DFSPH_METHOD = 2;
+const int Simulation::CUBE_BOUNDARY_METHOD = 0;
+const int Simulation::PCISPH_BOUNDARY_METHOD = 1;
+const int Simulation::AKINCI_BOUNDARY_METHOD = 2;
+const int Simulation::STANDARD_VISCOSITY_METHOD = 0;
+const int Simulation::ARTIFICIAL_VISCOSITY_METHOD = 1;
+const int Simulation::XSPH_VISCOSITY_METHOD = 2;
+const int Simulation::AKINCI_SURFACE_TENSION_METHOD = 3;
+const int Simulation::AKINCI_ADHESION_METHOD = 4;
+
+Simulation* Simulation::getCurrent() {
+    if (!current) current = new Simulation();
+    return current;
+}
+void Simulation::setCurrent(Simulation* s) { current = s; }
+bool Simulation::hasCurrent() { return current != nullptr; }
+void Simulation::destroyCurrent() {
+    delete current;
+    current = nullptr;
+}
+
+void Simulation::setSimulationMethod(int method) {
+    if (method == current_method || method < 0) return;
+    if (method > 2) throw SphError("simulationMethod must be 0 (WCSPH), 1 (PCISPH) or 2 (DFSPH)");
+    current_method = method;
+    if (method == WCSPH_METHOD) solver.reset(new WCSPHSolver());
+    else if (method == PCISPH_METHOD) solver.reset(new PCISPHSolver());
+    else solver.reset(new DFSPHSolver());
+}
+
+FluidModel* Simulation::addFluidModel(std::vector<Vector3r>& p, std::vector<Vector3r>& v) {
+    fluidModels.emplace_back(new FluidModel());
+    fluidModels.back()->init(p, v);
+    return fluidModels.back().get();
+}
+FluidModel* Simulation::addFluidModel() {
+    fluidModels.emplace_back(new FluidModel());
+    return fluidModels.back().get();
+}
+
+void Simulation::initMasses() {
+    const Real d = 2.0 * particleRadius;
+    const Real vol = std::pow(d, 3.0);
+    for (auto& fm : fluidModels) fm->setMasses(fm->getRefDensity() * vol);
+}
+
+void Simulation::computeNonPressureForces() {
+    for (auto& fm : fluidModels)
+        for (unsigned int i = 0; i < fm->numberNonPressureForces(); ++i) fm->getNonPressureForce(i)->step();
+}
+
+void Simulation::init() {
+    if (!solver) throw SphError("no simulation method set");
+    initMasses();
+    solver->init();
+}
+
+bool Simulation::step() {
+    const Real time = getTime();
+    bool save = false;
+    if (time >= getStartTime() && time <= getEndTime()) {
+        solver->step();
+        save = tm.hasToSave();
+        if (save) {
+            if (activeSave) SceneLoader::writeFluid();
+            tm.setFrame(tm.getFrame() + 1);
+        }
+    }
+    return save;
+}
+
+void Simulation::run(unsigned int maxSteps) {
+    unsigned int k = 0;
+    while (getTime() >= getStartTime() && getTime() <= getEndTime()) {
+        step();
+        if (maxSteps && ++k >= maxSteps) break;
+    }
+}
+
+FluidModel* Simulation::buildFluidBlock(const std::vector<BlockInfo>& blocks, const std::vector<SphereInfo>& spheres,
+                                        Vector3r smin, Vector3r smax) {
+    std::vector<Vector3r> pos, vel;
+    for (const BlockInfo& b : blocks) {
+        const double mn[3] = {b.min.x, b.min.y, b.min.z}, mx[3] = {b.max.x, b.max.y, b.max.z};
+        const int64_t n = sphg_fluid_block(mn, mx, particleRadius, b.jitter, b.seed, nullptr, 0);
+        if (n < 0) throw SphError("bad fluid block");
+        std::vector<float> p(3 * (size_t)std::max<int64_t>(n, 1));
+        sphg_fluid_block(mn, mx, particleRadius, b.jitter, b.seed, p.data(), n);
+        p.resize(3 * (size_t)n);
+        auto add = fromFloat(p);
+        pos.insert(pos.end(), add.begin(), add.end());
+    }
+    if (!spheres.empty()) {
+        std::vector<double> c, r;
+        for (const SphereInfo& s : spheres) {
+            c.insert(c.end(), {s.center.x, s.center.y, s.center.z});
+            r.push_back(s.radius);
+        }
+        const double mn[3] = {smin.x, smin.y, smin.z}, mx[3] = {smax.x, smax.y, smax.z};
+        const int64_t n = sphg_fluid_spheres(mn, mx, particleRadius, c.data(), r.data(), (int)r.size(), nullptr, 0);
+        if (n < 0) throw SphError("bad fluid spheres");
+        std::vector<float> p(3 * (size_t)std::max<int64_t>(n, 1));
+        sphg_fluid_spheres(mn, mx, particleRadius, c.data(), r.data(), (int)r.size(), p.data(), n);
+        p.resize(3 * (size_t)n);
+        auto add = fromFloat(p);
+        pos.insert(pos.end(), add.begin(), add.end());
+    }
+    vel.assign(pos.size(), Vector3r());
+    return addFluidModel(pos, vel);
+}
+
+std::string Simulation::describe() {
+    static const char* solvers[] = {"Weakly Compressible SPH", "Predictive-Corrective Incompressible SPH",
+                                    "Divergence-Free SPH"};
+    std::ostringstream o;
+    o << "Pressure solver:            " << (current_method >= 0 ? solvers[current_method] : "none") << " (GPU)\n";
+    o << "Fluid models:               " << fluidModels.size() << "\n";
+    for (size_t i = 0; i < fluidModels.size(); ++i)
+        o << "  particles in fm " << i + 1 << ":        " << fluidModels[i]->getNumParticles() << "\n";
+    o << "Boundary models:            " << boundaryModels.size() << "\n";
+    for (size_t i = 0; i < boundaryModels.size(); ++i)
+        o << "  particles in bm " << i + 1 << ":        " << boundaryModels[i]->getNumParticles() << "\n";
+    o << "Particle radius:            " << particleRadius << " m\n";
+    o << "Support radius:             " << supportRadius << " m\n";
+    o << "Gravity:                    (" << gravity.x << "," << gravity.y << "," << gravity.z << ") m/s^2\n";
+    return o.str();
+}

@@ -1,0 +1,225 @@
+"""Pins of the FP64 oracle (no reference run exists: the reference ships no
+tests or fixtures and cannot be compiled here).
+
+Each check is chosen so that a plausible restatement mistake fails it:
+closed forms on exact lattices (tests/golden/closed_forms.json), quadrature
+normalisation, brute-force neighbor search, and invariants (antisymmetric
+pressure pairs conserve momentum, a resting lattice stays at rest).
+"""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+from sph_amd import configs, sphg
+from sph_amd.sphg import SceneParams
+import oracle.pyoracle as po
+
+GOLD = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "closed_forms.json")))
+
+
+def _lattice(n, spacing, origin=0.0):
+    g = np.arange(n) * spacing + origin
+    return np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3)
+
+
+# --------------------------------------------------------------- kernels
+
+@pytest.mark.parametrize("h", [0.1, 0.04, 0.02])
+def test_cubic_W0_and_normalisation(h):
+    assert math.isclose(po.kernel_W(h, 0.0), float(GOLD["cubic_W0_h3"]) / h ** 3, rel_tol=1e-14)
+    # 4 pi int_0^h W(r) r^2 dr = 1 (composite Simpson on 4001 points)
+    r = np.linspace(0.0, h, 4001)
+    w = np.array([po.kernel_W(h, x) for x in r]) * r * r
+    integral = 4 * math.pi * (h / 4000 / 3) * (w[0] + w[-1] + 4 * w[1:-1:2].sum() + 2 * w[2:-1:2].sum())
+    assert abs(integral - 1.0) < 1e-10
+    assert po.kernel_W(h, 1.0000001 * h) == 0.0
+
+
+@pytest.mark.parametrize("q", [0.1, 0.3, 0.5, 0.6, 0.8, 0.99])
+def test_cubic_gradient_is_dW_dr(q):
+    h = 0.05
+    r = q * h
+    eps = 1e-7 * h
+    dwdr = (po.kernel_W(h, r + eps) - po.kernel_W(h, r - eps)) / (2 * eps)
+    g = po.kernel_gradW(h, [r, 0.0, 0.0])
+    assert math.isclose(g[0], dwdr, rel_tol=1e-6)
+    assert g[1] == 0.0 and g[2] == 0.0
+    d = np.array([0.3, -0.5, 0.2])
+    d = d / np.linalg.norm(d) * r
+    g3 = po.kernel_gradW(h, d)
+    assert np.allclose(g3, dwdr * d / r, rtol=1e-6)
+
+
+def test_cubic_gradient_zero_at_origin_and_outside():
+    h = 0.1
+    assert np.all(po.kernel_gradW(h, [0.0, 0.0, 0.0]) == 0)
+    assert np.all(po.kernel_gradW(h, [5e-6, 0.0, 0.0]) == 0)       # |r| <= 1e-5 cut
+    assert np.all(po.kernel_gradW(h, [1.0001 * h, 0.0, 0.0]) == 0)
+
+
+def test_spiky_and_poly6_gradients_closed_form():
+    h = 0.04
+    for r in (0.005, 0.02, 0.035):
+        g = po.kernel_spiky_grad(h, [0.0, r, 0.0])
+        assert math.isclose(g[1], -45.0 / (math.pi * h ** 6) * (h - r) ** 2, rel_tol=1e-13)
+        p = po.kernel_poly6_grad(h, [0.0, 0.0, r])
+        assert math.isclose(p[2], -945.0 / (32 * math.pi * h ** 9) * (h * h - r * r) ** 2 * r, rel_tol=1e-13)
+    assert np.all(po.kernel_spiky_grad(h, [0.0, 0.0, 0.0]) == 0)
+
+
+def test_cohesion_and_adhesion_as_written():
+    h = 0.02
+    c1 = 32.0 / (math.pi * h ** 9)
+    # outer branch (h/2, h]
+    r = 0.75 * h
+    assert math.isclose(po.kernel_cohesion(h, r), c1 * (h - r) ** 3 * r ** 3, rel_tol=1e-13)
+    # inner branch: the h^6/64 offset is NOT scaled by 32/(pi h^9)
+    r = 0.25 * h
+    assert math.isclose(po.kernel_cohesion(h, r), c1 * 2 * (h - r) ** 3 * r ** 3 - h ** 6 / 64, rel_tol=1e-13)
+    # adhesion: (-4r^2/h + 6r - 2h)^(1/4) without its 0.007/h^3.25 factor
+    r = 0.75 * h
+    assert math.isclose(po.kernel_adhesion(h, r), (-4 * r * r / h + 6 * r - 2 * h) ** 0.25, rel_tol=1e-13)
+    assert po.kernel_adhesion(h, 0.4 * h) == 0.0
+
+
+# ------------------------------------------------------- lattice closed forms
+
+def _params(method=sphg.DFSPH, r=0.025, **kw):
+    p = SceneParams(method=method, particle_radius=r, gravity=(0.0, 0.0, 0.0))
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+def test_lattice_density_closed_form():
+    r = 1.0 / 32.0                 # dyadic spacing: the FP32 lattice is exact
+    x = _lattice(9, 2 * r, 1.0)
+    far = np.array([[50.0, 50.0, 50.0]], dtype=np.float32)     # one boundary particle far away
+    o = po.Oracle(_params(r=r), x, far)
+    rho = o.get(po.OF_RHO)
+    centre = np.argmin(np.linalg.norm(x - x.mean(0), axis=1))
+    ratio = float(GOLD["lattice_density_ratio"])
+    assert abs(rho[centre] / 1000.0 - ratio) < 1e-13
+    # counts on the exact lattice
+    cand = o.candidates(int(centre))
+    d = np.linalg.norm(x[cand].astype(np.float64) - x[centre], axis=1) / (4 * r)
+    assert int((d <= 1 + 1e-9).sum()) == GOLD["lattice_counts"]["candidates_q_le_1"]
+
+
+def test_flat_wall_volume_closed_form():
+    h = 0.125                      # dyadic: the sampled wall is an exact h/2 lattice
+    b = sphg.sample_box((0.0, 0.0, 0.0), (1.0, 1.0, 1.0), h)
+    fluid = np.array([[0.5, 0.5, 0.5]], dtype=np.float32)
+    o = po.Oracle(_params(r=h / 4), fluid, b)
+    vb = o.get(po.OF_BVOL)
+    centre = np.argmin(np.linalg.norm(b - np.array([0.5, 0.5, 0.0]), axis=1))
+    assert abs(vb[centre] / h ** 3 - float(GOLD["wall_volume_over_h3"])) < 1e-13
+    # a shell corner sees a quarter disk on each of its 3 faces:
+    # sum W/k = 1 + 3 W(h/2)/k + 6 W(h/sqrt2)/k = 1.75 + 6 (1 - 1/sqrt2)^3
+    corner = np.argmin(np.linalg.norm(b, axis=1))
+    s_corner = 1.75 + 6 * (1 - math.sqrt(0.5)) ** 3
+    assert abs(vb[corner] / h ** 3 - math.pi / (8 * s_corner)) < 1e-13
+
+
+def test_pcisph_delta_closed_form():
+    for r, dt in ((0.005, 2e-4), (0.025, 5e-4)):
+        p = _params(method=sphg.PCISPH, r=r, time_step=dt)
+        o = po.Oracle(p, np.array([[0.5, 0.5, 0.5]], np.float32), np.array([[9.0, 9.0, 9.0]], np.float32))
+        h = 4 * r
+        expect = float(GOLD["pcisph_delta_h2_dt2"]) * h * h / (dt * dt)
+        assert math.isclose(o.scalars()["pcisph_delta"], expect, rel_tol=1e-12)
+
+
+def test_single_particle_and_pair():
+    r = 0.02
+    h = 4 * r
+    m = 1000.0 * (2 * r) ** 3
+    far = np.array([[9.0, 9.0, 9.0]], np.float32)
+    o = po.Oracle(_params(r=r), np.array([[0.5, 0.5, 0.5]], np.float32), far)
+    assert math.isclose(o.get(po.OF_RHO)[0], m * 8 / (math.pi * h ** 3), rel_tol=1e-12)
+    assert o.get(po.OF_ALPHA)[0] == 0.0     # no gradient: denominator 0 -> alpha 0
+    pair = np.array([[0.5, 0.5, 0.5], [0.5 + h / 2, 0.5, 0.5]], np.float32)
+    o2 = po.Oracle(_params(r=r), pair, far)
+    k = 8 / (math.pi * h ** 3)
+    hq = float(np.float32(0.5 + h / 2) - np.float32(0.5)) / h   # q of the FP32 pair
+    w = 6 * hq ** 3 - 6 * hq ** 2 + 1 if hq <= 0.5 else 2 * (1 - hq) ** 3
+    assert np.allclose(o2.get(po.OF_RHO), m * k * (1 + w), rtol=1e-12)
+
+
+def test_tait_eos_closed_form():
+    """WCSPH pressure p = max(0, kappa rho0/gamma ((rho/rho0)^gamma - 1))."""
+    spec = configs.small_scene(sphg.WCSPH, n_side=6, jitter=0.03)
+    o = po.Oracle(spec.params, spec.fluid, spec.boundary)
+    o.step(1)
+    rho = o.get(po.OF_RHO)
+    p = o.get(po.OF_P)
+    kap, gam, r0 = spec.params.stiffness, spec.params.gamma, spec.params.density0
+    assert np.allclose(p, np.maximum(0.0, kap * r0 / gam * ((rho / r0) ** gam - 1)), rtol=1e-12, atol=1e-9)
+
+
+# ----------------------------------------------------------------- grid
+
+def test_grid_keys_buckets_candidates_bruteforce():
+    spec = configs.small_scene(sphg.DFSPH, n_side=9, jitter=0.05)
+    o = po.Oracle(spec.params, spec.fluid, spec.boundary)
+    x = spec.fluid.astype(np.float64)
+    h = spec.params.support_radius
+    keys = np.floor(x / h).astype(np.int64)
+    assert np.array_equal(o.keys(), keys)
+    for i in range(0, spec.n, 7):
+        pts = o.cell_points(i)
+        exp = np.flatnonzero(np.all(keys == keys[i], axis=1))
+        assert np.array_equal(pts, exp)                      # ascending id = insertion order
+        cand = o.candidates(i)
+        near = np.flatnonzero(np.all(np.abs(keys - keys[i]) <= 1, axis=1))
+        assert sorted(cand.tolist()) == near.tolist()         # exactly the 27-cell union
+        within = np.flatnonzero(np.linalg.norm(x - x[i], axis=1) <= h)
+        assert set(within.tolist()) <= set(cand.tolist())    # no neighbor missed
+        # concatenation order: x outermost, z innermost
+        ck = keys[cand] - keys[i]
+        code = (ck[:, 0] + 1) * 9 + (ck[:, 1] + 1) * 3 + (ck[:, 2] + 1)
+        assert np.all(np.diff(code) >= 0)
+
+
+# ----------------------------------------------------------- invariants
+
+def test_pressure_pairs_conserve_momentum():
+    """Fluid-fluid pressure terms are antisymmetric: sum_i m a_i = 0."""
+    spec = configs.small_scene(sphg.WCSPH, n_side=7, jitter=0.05)
+    far = np.array([[50.0, 50.0, 50.0]], np.float32)
+    o = po.Oracle(spec.params, spec.fluid, far)
+    o.step(1)
+    a = o.get(po.OF_ACC_STEP)
+    assert np.abs(a).max() > 1.0
+    assert np.abs(a.sum(0)).max() < 1e-9 * np.abs(a).sum()
+
+
+def test_resting_lattice_stays_at_rest():
+    """No gravity, zero velocity, interior density below rho0: DFSPH must not
+    move anything (rho* clamps to rho0, divergence error is zero)."""
+    r = 0.025
+    x = _lattice(8, 2 * r, 0.5)
+    far = np.array([[50.0, 50.0, 50.0]], np.float32)
+    o = po.Oracle(_params(r=r, time_step=1e-3, min_time_step=1e-6, max_time_step=5e-3, fps=25.0, cfl_factor=0.4,
+                          eta=1e-4, eta_v=1e-3, min_iterations=2, max_iterations=100, min_iterations_v=1,
+                          max_iterations_v=100), x, far)
+    o.step(3)
+    assert np.all(o.get(po.OF_V) == 0.0)
+    assert np.array_equal(o.get(po.OF_X), x.astype(np.float32).astype(np.float64))
+    assert np.all(o.get(po.OF_DIVERR) == 0.0)
+
+
+def test_dfsph_min_iterations_and_dt_rule():
+    spec = configs.small_scene(sphg.DFSPH, n_side=6)
+    o = po.Oracle(spec.params, spec.fluid, spec.boundary)
+    dt0 = spec.params.time_step
+    o.step(1)
+    s = o.scalars()
+    assert s["iterations"] >= spec.params.min_iterations and s["iterations_v"] >= spec.params.min_iterations_v
+    # first step: 0 previous iterations, cfl = 0.4*0.4*2r/max(0.1, sqrt|v|) > dt -> dt * 1.001
+    assert s["cfl_time_step"] > dt0
+    assert math.isclose(s["time_step"], dt0 * 1.001, rel_tol=1e-15)
+    assert math.isclose(s["time"], s["time_step"], rel_tol=1e-15)
