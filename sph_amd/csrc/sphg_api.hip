@@ -67,8 +67,10 @@ struct Profiler {
             pool.pop_back();
             return e;
         }
+        // no system-scope fence on record: a timing event then adds no
+        // cache writeback/invalidate between the kernels it brackets
         hipEvent_t e;
-        (void)hipEventCreate(&e);
+        (void)hipEventCreateWithFlags(&e, hipEventDisableSystemFence);
         return e;
     }
     void collect() {
@@ -119,8 +121,8 @@ struct Scope {
 struct sphg_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
-    int* dActive = nullptr;
-    int* hActive = nullptr;   // pinned
+    int* dActive = nullptr;   // [0] scenes still iterating, [1] largest iteration count reached
+    int* hActive = nullptr;   // pinned copy
     Profiler prof;
     std::vector<sphg_scene*> scenes;
 };
@@ -145,6 +147,7 @@ struct Batch {
     int nP = 0, nC = 0;
     int maxIt = 0;
     int minIt = 0, minItV = 0;   // smallest minimum iteration counts in the batch
+    int lastIt = 0, lastItV = 0; // iterations the loops needed in the previous step
     std::vector<int> parity;   // parity of each scene when the tables were built
     ~Batch() {
         if (dT[0]) (void)hipFree(dT[0]);
@@ -265,6 +268,30 @@ void fill_coef(KCoef& k, double h) {
     k.hhi2 = (float)(h * h * (1.0 + 1e-4));
 }
 
+constexpr int kXcds = 8;
+
+std::vector<Chunk> xcd_order(const std::vector<Chunk>& in) {
+    const size_t n = in.size();
+    std::vector<Chunk> out(n);
+    const size_t per = (n + kXcds - 1) / kXcds;   // chunks per XCD (the last XCD may get fewer)
+    size_t pos = 0;
+    // launch slot L runs on XCD L % 8; fill XCD x's slots with chunks
+    // [x * per, (x + 1) * per) in order
+    std::vector<size_t> next(kXcds);
+    for (int x = 0; x < kXcds; ++x) next[x] = x * per;
+    for (size_t L = 0; L < n; ++L) {
+        int x = (int)(L % kXcds);
+        // an XCD whose range is exhausted hands its slot to the next one with work left
+        int tries = 0;
+        while (next[x] >= std::min(n, (size_t)(x + 1) * per) && tries < kXcds) {
+            x = (x + 1) % kXcds;
+            ++tries;
+        }
+        out[pos++] = in[next[x]++];
+    }
+    return out;
+}
+
 int upload_batch_tables(sphg_ctx* ctx, Batch& B) {
     const int ns = (int)B.scenes.size();
     std::vector<DevScene> t0(ns), t1(ns);
@@ -284,6 +311,12 @@ int upload_batch_tables(sphg_ctx* ctx, Batch& B) {
         B.minIt = std::min(B.minIt, std::min(t0[s].minIt, t0[s].maxIt));
         B.minItV = std::min(B.minItV, std::min(t0[s].minItV, t0[s].maxIt));
     }
+    // Blocks are dispatched round-robin over the XCDs (block b -> XCD b % 8),
+    // each with its own L2: order the table so that XCD k runs one
+    // contiguous, spatially coherent range of chunks and finds its
+    // neighbors' data in its own L2.
+    P = xcd_order(P);
+    Cc = xcd_order(Cc);
     if (!B.dT[0]) {
         HIP_TRY(hipMalloc(&B.dT[0], sizeof(DevScene) * ns));
         HIP_TRY(hipMalloc(&B.dT[1], sizeof(DevScene) * ns));
@@ -314,10 +347,11 @@ void grid_sort(sphg_ctx* ctx, Batch& B, const DevScene* T) {
     { Scope s(P, st, K_REORDER); launch_rank_reorder(st, T, B.dP, B.nP); }
 }
 
-int read_active(sphg_ctx* ctx, int& v, uint32_t& syncs) {
-    HIP_TRY(hipMemcpyAsync(ctx->hActive, ctx->dActive, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+int read_active(sphg_ctx* ctx, int& v, int& maxIt, uint32_t& syncs) {
+    HIP_TRY(hipMemcpyAsync(ctx->hActive, ctx->dActive, 2 * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
-    v = *ctx->hActive;
+    v = ctx->hActive[0];
+    maxIt = ctx->hActive[1];
     ++syncs;
     return SPHG_OK;
 }
@@ -343,17 +377,19 @@ void finalize(sphg_ctx* ctx, Batch& B, const DevScene* T, int kind) {
     launch_finalize(ctx->stream, kind, T, (int)B.scenes.size(), ctx->dActive);
 }
 
-// Whether another iteration of a loop is needed.  Below the smallest
-// minimum iteration count of the batch every scene iterates, so no read
-// is needed; past it, one int (the number of scenes still iterating) is
-// read back.
-int loop_more(sphg_ctx* ctx, int it, int minIt, bool& more, uint32_t& syncs) {
-    if (it < minIt) {
+// Whether iteration `it` of a loop is to be launched.  Below `ahead` --
+// the batch's smallest minimum iteration count, or the iteration count the
+// loop reached in the previous step -- it is launched without asking: a
+// scene that has already converged skips it on the device.  Past that, the
+// number of scenes still iterating (and the largest count reached) is read
+// back.  `seen` returns the largest count reached once the loop is over.
+int loop_more(sphg_ctx* ctx, int it, int ahead, bool& more, int& seen, uint32_t& syncs) {
+    if (it < ahead) {
         more = true;
         return SPHG_OK;
     }
     int act = 0;
-    int rc = read_active(ctx, act, syncs);
+    int rc = read_active(ctx, act, seen, syncs);
     more = act > 0;
     return rc;
 }
@@ -382,34 +418,38 @@ int batch_one_step(sphg_ctx* ctx, Batch& B, uint32_t& syncs) {
         finalize(ctx, B, T, 0);
         { Scope s(P, st, K_DFPREDICT); launch_df_predict(st, T, B.dP, B.nP); }
         { Scope s(P, st, K_DFVEL_WARM); launch_df_vel(st, 0, T, B.dP, B.nP); }
-        launch_fill_int(st, ctx->dActive, 0, 1);
+        launch_fill_int(st, ctx->dActive, 0, 2);
         { Scope s(P, st, K_DFERR_DENS); launch_df_err(st, 0, T, B.dP, B.nP); }
         finalize(ctx, B, T, 1);
+        int seen = 0;
         for (int it = 0; it < B.maxIt; ++it) {
-            if ((rc = loop_more(ctx, it, B.minIt, more, syncs))) return rc;
+            if ((rc = loop_more(ctx, it, std::max(B.minIt, B.lastIt), more, seen, syncs))) return rc;
             if (!more) break;
             { Scope s(P, st, K_DFVEL_DENS); launch_df_vel(st, 1, T, B.dP, B.nP); }
             { Scope s(P, st, K_DFERR_DENS); launch_df_err(st, 1, T, B.dP, B.nP); }
             finalize(ctx, B, T, 2);
         }
+        B.lastIt = seen;
         { Scope s(P, st, K_DFPOS); launch_df_positions(st, T, B.dP, B.nP); }
         grid_sort(ctx, B, T);
         const DevScene* T2 = B.dT[1];
         { Scope s(P, st, K_BUILD); launch_build(st, DFSPH, T2, B.dP, B.nP); }
-        launch_fill_int(st, ctx->dActive, 0, 1);
+        launch_fill_int(st, ctx->dActive, 0, 2);
         { Scope s(P, st, K_DFERR_DIV); launch_df_err(st, 2, T2, B.dP, B.nP); }
         finalize(ctx, B, T2, 3);
+        seen = 0;
         for (int it = 0; it < B.maxIt; ++it) {
-            if ((rc = loop_more(ctx, it, B.minItV, more, syncs))) return rc;
+            if ((rc = loop_more(ctx, it, std::max(B.minItV, B.lastItV), more, seen, syncs))) return rc;
             if (!more) break;
             { Scope s(P, st, K_DFVEL_DIV); launch_df_vel(st, 2, T2, B.dP, B.nP); }
             { Scope s(P, st, K_DFERR_DIV); launch_df_err(st, 3, T2, B.dP, B.nP); }
             finalize(ctx, B, T2, 4);
         }
+        B.lastItV = seen;
         flip(B);
     } else if (method == WCSPH) {
         const DevScene* T = B.dT[0];
-        launch_fill_int(st, ctx->dActive, 0, 1);
+        launch_fill_int(st, ctx->dActive, 0, 2);
         { Scope s(P, st, K_STEPBEGIN); launch_step_begin(st, T, B.dP, B.nP, ctx->dActive); }
         { Scope s(P, st, K_KEYS); launch_keys(st, T, B.dP, B.nP); }
         grid_sort(ctx, B, T);
@@ -421,7 +461,7 @@ int batch_one_step(sphg_ctx* ctx, Batch& B, uint32_t& syncs) {
         flip(B);
     } else {
         const DevScene* T = B.dT[0];
-        launch_fill_int(st, ctx->dActive, 0, 1);
+        launch_fill_int(st, ctx->dActive, 0, 2);
         { Scope s(P, st, K_STEPBEGIN); launch_step_begin(st, T, B.dP, B.nP, ctx->dActive); }
         { Scope s(P, st, K_KEYS); launch_keys(st, T, B.dP, B.nP); }
         grid_sort(ctx, B, T);
@@ -429,13 +469,15 @@ int batch_one_step(sphg_ctx* ctx, Batch& B, uint32_t& syncs) {
         { Scope s(P, st, K_BUILD); launch_build(st, PCISPH, T2, B.dP, B.nP); }
         if (anySt) { Scope s(P, st, K_NORMALS); launch_normals(st, T2, B.dP, B.nP); }
         { Scope s(P, st, K_FORCES); launch_forces(st, PCISPH, T2, B.dP, B.nP); }
+        int seen = 0;
         for (int it = 0; it < B.maxIt; ++it) {
-            if ((rc = loop_more(ctx, it, B.minIt, more, syncs))) return rc;
+            if ((rc = loop_more(ctx, it, std::max(B.minIt, B.lastIt), more, seen, syncs))) return rc;
             if (!more) break;
             { Scope s(P, st, K_PCIDENS); launch_pci_density(st, T2, B.dP, B.nP); }
             finalize(ctx, B, T2, 5);
             { Scope s(P, st, K_PCIPACC); launch_pci_pacc(st, T2, B.dP, B.nP, it); }
         }
+        B.lastIt = seen;
         { Scope s(P, st, K_PCIINT); launch_pci_integrate(st, T2, B.dP, B.nP); }
         flip(B);
     }
@@ -509,8 +551,8 @@ int sphg_ctx_create(int32_t device, sphg_ctx** out) {
     sphg_ctx* c = new sphg_ctx();
     c->device = device;
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
-    if (e == hipSuccess) e = hipMalloc(&c->dActive, sizeof(int));
-    if (e == hipSuccess) e = hipHostMalloc(&c->hActive, sizeof(int), hipHostMallocDefault);
+    if (e == hipSuccess) e = hipMalloc(&c->dActive, 2 * sizeof(int));
+    if (e == hipSuccess) e = hipHostMalloc(&c->hActive, 2 * sizeof(int), hipHostMallocDefault);
     if (e != hipSuccess) {
         delete c;
         return fail(SPHG_EHIP, std::string("context set-up: ") + hipGetErrorString(e));
@@ -632,7 +674,8 @@ int sphg_scene_init(sphg_scene* s) {
     const int nChunks = (int)((n + kBlock - 1) / kBlock);
     const size_t npad = (size_t)nChunks * kBlock;
     const int nCellChunks = (int)((ncells + 1 + kCellChunk - 1) / kCellChunk);
-    const int maxNbr = d.max_neighbors ? (int)d.max_neighbors : kDefaultMaxNbr;
+    // a multiple of 4: the lists are padded to multiples of 4 per group
+    const int maxNbr = ((d.max_neighbors ? (int)d.max_neighbors : kDefaultMaxNbr) + 3) & ~3;
 
     int flags = 0;
     if (d.viscosity_method == SPHG_VISCOSITY_XSPH && d.viscosity != 0.0) flags |= F_XSPH;
@@ -665,6 +708,7 @@ int sphg_scene_init(sphg_scene* s) {
     const size_t oNBR = A.reserve(sizeof(uint32_t) * npad * (size_t)maxNbr);
     const size_t oNNB = A.reserve(sizeof(uint16_t) * npad);
     const size_t oNG = A.reserve(npad);
+    const size_t oGC = A.reserve(sizeof(uint16_t) * (npad / kGroup));
     const size_t oXB = A.reserve(sizeof(float4) * std::max<size_t>(nb, 1));
     const size_t oBCS = A.reserve(sizeof(int) * (size_t)(ncells + 1));
     const size_t oPART = A.reserve(sizeof(double) * (size_t)std::max(nChunks, 1));
@@ -744,6 +788,7 @@ int sphg_scene_init(sphg_scene* s) {
     D.nbr = (uint32_t*)(b + oNBR);
     D.nnb = (uint16_t*)(b + oNNB);
     D.ngrad = (uint8_t*)(b + oNG);
+    D.gcnt = (uint16_t*)(b + oGC);
     D.xb = F4(oXB);
     D.bCellStart = (int*)(b + oBCS);
     D.partial = (double*)(b + oPART);

@@ -20,6 +20,9 @@
 //   nbr           uint32 neighbor list, neighbor-major inside 64-particle groups
 //   nnb, ngrad    uint16/uint8 neighbor count and count of nonzero-gradient
 //                 neighbors (the DFSPH 20-neighbor cutoff)
+//   gcnt          uint16 per 64-particle group: the group's longest list,
+//                 rounded up to 4; shorter lists are padded with the
+//                 particle's own index, whose every term is exactly zero
 // Boundary: xb float4 (x, y, z, V_b) sorted by cell, bCellStart table.
 #pragma once
 
@@ -94,6 +97,7 @@ struct DevScene {
     uint32_t *nbr;
     uint16_t *nnb;
     uint8_t *ngrad;
+    uint16_t *gcnt;             // per 64-particle group: list length padded to a multiple of 4
     float4 *xb;
     int *bCellStart;
     double *partial;            // one per particle chunk

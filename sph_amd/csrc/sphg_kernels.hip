@@ -68,10 +68,30 @@ __device__ inline double scene_partial_max(const double* partial, int n, double*
 // near 1000).  1/|r| is an FP32 rsqrt refined by one FP64 Newton step
 // (relative error ~1e-14), so no FP64 transcendental is issued.
 
-__device__ inline double rsqrt_d(double r2) {
-    double y = (double)rsqrtf((float)r2);
-    return y * (1.5 - 0.5 * r2 * y * y);
+// Global-memory access.  The arrays come out of the scene table, so the
+// compiler cannot prove their address space; these casts make it emit
+// global (not flat) loads and stores.
+typedef float v4f __attribute__((ext_vector_type(4)));
+template <class T> using gptr = __attribute__((address_space(1))) T*;
+__device__ inline float4 ld4(const float4* p, size_t i) {
+    const v4f t = ((gptr<const v4f>)p)[i];
+    return make_float4(t.x, t.y, t.z, t.w);
 }
+__device__ inline void st4(float4* p, size_t i, float4 v) {
+    v4f t;
+    t.x = v.x; t.y = v.y; t.z = v.z; t.w = v.w;
+    ((gptr<v4f>)p)[i] = t;
+}
+template <class T> __device__ inline T ldg(const T* p, size_t i) { return ((gptr<const T>)p)[i]; }
+template <class T> __device__ inline void stg(T* p, size_t i, T v) { ((gptr<T>)p)[i] = v; }
+
+// 1/sqrt(r2), zero for r2 <= 1e-30 (coincident points contribute no gradient)
+__device__ inline double rsqrt_d(double r2) {
+    const double y = (double)rsqrtf((float)r2);
+    const double z = y * (1.5 - 0.5 * r2 * y * y);
+    return (r2 > 1e-30) ? z : 0.0;
+}
+__device__ inline double sel(bool c, double a, double b) { return c ? a : b; }
 
 struct Pair {
     double dx, dy, dz, r2, rl, inv, q;
@@ -83,29 +103,31 @@ __device__ inline Pair pair_of(double xi, double yi, double zi, float4 b, const 
     p.dz = zi - (double)b.z;
     p.r2 = p.dx * p.dx + p.dy * p.dy + p.dz * p.dz;
     p.inv = rsqrt_d(p.r2);
-    p.rl = (p.r2 > 1e-30) ? p.r2 * p.inv : 0.0;
+    p.rl = p.r2 * p.inv;
     p.q = p.rl * k.invh;
     return p;
 }
 // Cubic spline W (src/CubicSpline.h:29-48).
 __device__ inline double W_d(double q, const KCoefD& k) {
     const double om = 1.0 - q;
-    const double w = (q <= 0.5) ? k.ck * ((6.0 * q - 6.0) * q * q + 1.0) : 2.0 * k.ck * om * om * om;
-    return (q <= 1.0) ? w : 0.0;
+    const double w1 = k.ck * ((6.0 * q - 6.0) * q * q + 1.0);
+    const double w2 = 2.0 * k.ck * om * om * om;
+    return sel(q <= 1.0, sel(q <= 0.5, w1, w2), 0.0);
 }
 // Cubic spline gradient factor: gradW(r) = g r, zero for |r| <= 1e-5 or
 // q > 1 (src/CubicSpline.h:55-73).  For q <= 1/2 the reference's
 // l q (3q-2) / (|r| h) equals l (3q-2) / h^2.
 __device__ inline double G_d(const Pair& p, const KCoefD& k) {
     const double om = 1.0 - p.q;
-    const double g = (p.q <= 0.5) ? k.clh2 * (3.0 * p.q - 2.0) : -k.clh * om * om * p.inv;
-    return (p.rl > 1e-5 && p.q <= 1.0) ? g : 0.0;
+    const double g1 = k.clh2 * (3.0 * p.q - 2.0);
+    const double g2 = -k.clh * om * om * p.inv;
+    return sel(p.rl > 1e-5 && p.q <= 1.0, sel(p.q <= 0.5, g1, g2), 0.0);
 }
 // Spiky gradient factor (src/Spiky.h:44-55): -45/(pi h^6) (h-r)^2 / r.
 __device__ inline double Gs_d(const Pair& p, const KCoefD& k) {
     const double d = k.h - p.rl;
     const double g = -k.spg * d * d * p.inv;
-    return (p.r2 > 0.0 && p.rl <= k.h) ? g : 0.0;
+    return sel(p.r2 > 0.0 && p.rl <= k.h, g, 0.0);
 }
 // Cohesion C(r) (src/Cohesion.h:29-40) for 0 < r <= h; `outer` is the
 // branch test 2r > h.  As written there the h^6/64 offset is not scaled by
@@ -153,6 +175,39 @@ __device__ inline bool grad_nonzero64(const Exact& e) {
 __device__ inline const uint32_t* list_of(const DevScene& S, int i) {
     return S.nbr + ((size_t)(i >> 6) * (size_t)S.maxNbr) * kGroup + (i & 63);
 }
+// The group's padded list length, uniform across the wavefront (a
+// wavefront is one 64-particle group: chunks start at multiples of 256).
+__device__ inline int group_count(const DevScene& S, int i) {
+    return __builtin_amdgcn_readfirstlane((int)ldg(S.gcnt, (size_t)(i >> 6)));
+}
+
+// Visit particle i's padded neighbor list four entries at a time: the four
+// indices are loaded, then the four neighbors' data (load(j)), then the
+// four terms are evaluated (body(j, data)), so every lane has several
+// loads in flight.  The trip count is uniform across the wavefront.
+template <class Load, class Body>
+__device__ inline void sweep4(const DevScene& S, int i, Load&& load, Body&& body) {
+    const uint32_t* L = list_of(S, i);
+    const int gc = group_count(S, i);
+    for (int q = 0; q < gc; q += 4) {
+        uint32_t jj[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) jj[u] = ldg(L, (size_t)(q + u) * kGroup);
+        decltype(load(0u)) dd[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) dd[u] = load(jj[u]);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) body(jj[u], dd[u]);
+    }
+}
+
+struct XV {
+    float4 x, v;
+};
+struct NbF {
+    float4 x, v, n;
+    float d;
+};
 __device__ inline uint32_t* list_of_w(const DevScene& S, int i) {
     return S.nbr + ((size_t)(i >> 6) * (size_t)S.maxNbr) * kGroup + (i & 63);
 }
@@ -359,51 +414,61 @@ __global__ __launch_bounds__(kBlock) void k_build(const DevScene* __restrict__ T
     const Chunk c = C[blockIdx.x];
     const DevScene& S = T[c.scene];
     const int i = c.begin + threadIdx.x;
-    if (i >= S.n) return;
+    const bool valid = i < S.n;
     const KCoefD k = S.kd;
-    const float4 xi = S.x[i];
+    const float hh = S.kc.hhi2 * (1.0f + 1e-5f);   // conservative FP32 pre-filter
+    const float4 xi = valid ? ld4(S.x, i) : make_float4(0.f, 0.f, 0.f, 0.f);
     const double xx = xi.x, xy = xi.y, xz = xi.z;
-    const int ci = S.cell[i];
+    const int ci = valid ? ldg(S.cell, i) : S.nx + 1 + S.nx * S.ny;   // any interior cell
     const int nx = S.nx, nxny = S.nx * S.ny;
-    uint32_t* L = list_of_w(S, i);
+    uint32_t* L = list_of_w(S, valid ? i : (i & ~63));
     const int maxN = S.maxNbr;
     int cnt = 0, ngr = 0;
     double ws = 0.0, sgx = 0.0, sgy = 0.0, sgz = 0.0, ssq = 0.0;
-    walk27(S.cellStart, ci, nx, nxny, [&](int j) {
-        if (j == i) return;
-        const float4 xj = S.x[j];
-        {
-            const double ex = xx - (double)xj.x, ey = xy - (double)xj.y, ez = xz - (double)xj.z;
-            if (!(ex * ex + ey * ey + ez * ez <= k.hhi2)) return;
-        }
-        const Pair p = pair_of(xx, xy, xz, xj, k);
-        const double g = G_d(p, k);
-        bool gnz = (g != 0.0);
-        if (p.r2 >= k.hlo2 || fabs(p.q - 0.6666666666666666) < 1e-9 || p.r2 < 1e-9) {
-            const Exact e = exact_q(xi, xj, S.hD);
-            if (!(e.q <= 1.0)) return;
-            gnz = grad_nonzero64(e);
-        }
-        if (cnt < maxN) L[(size_t)cnt * kGroup] = (uint32_t)j;
-        ++cnt;
-        ngr += gnz ? 1 : 0;
-        ws += W_d(p.q, k);
-        sgx += g * p.dx;
-        sgy += g * p.dy;
-        sgz += g * p.dz;
-        ssq += g * g * p.r2;
-    });
+    if (valid) {
+        walk27(S.cellStart, ci, nx, nxny, [&](int j) {
+            const float4 xj = ld4(S.x, j);
+            const float fx = xi.x - xj.x, fy = xi.y - xj.y, fz = xi.z - xj.z;
+            if (!(fx * fx + fy * fy + fz * fz <= hh) || j == i) return;
+            const Pair p = pair_of(xx, xy, xz, xj, k);
+            if (!(p.r2 <= k.hhi2)) return;
+            const double g = G_d(p, k);
+            bool gnz = (g != 0.0);
+            if (p.r2 >= k.hlo2 || fabs(p.q - 0.6666666666666666) < 1e-9 || p.r2 < 1e-9) {
+                const Exact e = exact_q(xi, xj, S.hD);
+                if (!(e.q <= 1.0)) return;
+                gnz = grad_nonzero64(e);
+            }
+            if (cnt < maxN) stg(L, (size_t)cnt * kGroup, (uint32_t)j);
+            ++cnt;
+            ngr += gnz ? 1 : 0;
+            ws += W_d(p.q, k);
+            sgx += g * p.dx;
+            sgy += g * p.dy;
+            sgz += g * p.dz;
+            ssq += g * g * p.r2;
+        });
+    }
+    // pad the list with the particle itself up to the group's longest list
+    // rounded to 4 (a self term is exactly zero in every sweep)
+    const int own = min(cnt, maxN);
+    int pad = min((own + 3) & ~3, maxN);
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) pad = max(pad, __shfl_xor(pad, o));
+    if (valid) {
+        for (int q = own; q < pad; ++q) stg(L, (size_t)q * kGroup, (uint32_t)i);
+        if ((i & 63) == 0) stg(S.gcnt, (size_t)(i >> 6), (uint16_t)pad);
+    }
+    if (!valid) return;
     double bw = 0.0, bgx = 0.0, bgy = 0.0, bgz = 0.0;
     double sx = 0.0, sy = 0.0, sz = 0.0;   // spiky boundary sum (PCISPH)
     double ax = 0.0, ay = 0.0, az = 0.0;   // adhesion sum
     const bool wantAdh = (S.flags & F_ADH) != 0;
     const double rho0 = S.rho0D;
     walk27(S.bCellStart, ci, nx, nxny, [&](int b) {
-        const float4 y = S.xb[b];
-        {
-            const double ex = xx - (double)y.x, ey = xy - (double)y.y, ez = xz - (double)y.z;
-            if (!(ex * ex + ey * ey + ez * ez <= k.hhi2)) return;
-        }
+        const float4 y = ld4(S.xb, b);
+        const float fx = xi.x - y.x, fy = xi.y - y.y, fz = xi.z - y.z;
+        if (!(fx * fx + fy * fy + fz * fz <= hh)) return;
         const Pair p = pair_of(xx, xy, xz, y, k);
         if (!(p.q <= 1.0)) return;
         const double cb = rho0 * (double)y.w;
@@ -426,29 +491,29 @@ __global__ __launch_bounds__(kBlock) void k_build(const DevScene* __restrict__ T
         }
     });
     if (cnt > maxN) atomicMax(&S.ss->overflow, (unsigned)cnt);
-    S.nnb[i] = (uint16_t)min(cnt, maxN);
-    S.ngrad[i] = (uint8_t)min(ngr, 255);
+    stg(S.nnb, (size_t)i, (uint16_t)own);
+    stg(S.ngrad, (size_t)i, (uint8_t)min(ngr, 255));
     const double m = S.mD;
     const double drho = (m * (k.ck + ws) + bw) - rho0;
     const double rho = rho0 + drho;
-    S.drho[i] = (float)drho;
-    S.gb[i] = make_float4((float)bgx, (float)bgy, (float)bgz, (float)bw);
-    if (wantAdh) S.adh[i] = make_float4((float)ax, (float)ay, (float)az, 0.f);
+    stg(S.drho, (size_t)i, (float)drho);
+    st4(S.gb, i, make_float4((float)bgx, (float)bgy, (float)bgz, (float)bw));
+    if (wantAdh) st4(S.adh, i, make_float4((float)ax, (float)ay, (float)az, 0.f));
     if (M == DFSPH) {
         const double tx = m * sgx + bgx, ty = m * sgy + bgy, tz = m * sgz + bgz;
         const double den = tx * tx + ty * ty + tz * tz + m * m * ssq;
-        S.alpha[i] = (fabs(den) > 1e-5) ? (float)(rho / den) : 0.0f;
+        stg(S.alpha, (size_t)i, (fabs(den) > 1e-5) ? (float)(rho / den) : 0.0f);
     } else if (M == WCSPH) {
         // p = max(0, kappa rho0/gamma ((rho/rho0)^gamma - 1)), written
         // (1+x)^g - 1 = expm1(g log1p(x)) with x = (rho - rho0)/rho0
         const double x = drho / rho0;
         const double pr = fmax(0.0, (double)S.stiffness * rho0 / (double)S.gammaEos * expm1((double)S.gammaEos * log1p(x)));
-        S.p[i] = (float)pr;
-        S.xk[i] = make_float4(xi.x, xi.y, xi.z, (float)(pr / (rho * rho)));
-        const float4 vi = S.v[i];
-        S.aux[i] = make_float4(vi.x, vi.y, vi.z, (float)(m / rho));
+        stg(S.p, (size_t)i, (float)pr);
+        st4(S.xk, i, make_float4(xi.x, xi.y, xi.z, (float)(pr / (rho * rho))));
+        const float4 vi = ld4(S.v, i);
+        st4(S.aux, i, make_float4(vi.x, vi.y, vi.z, (float)(m / rho)));
     } else {
-        S.gbs[i] = make_float4((float)sx, (float)sy, (float)sz, 0.f);
+        st4(S.gbs, i, make_float4((float)sx, (float)sy, (float)sz, 0.f));
     }
 }
 
@@ -462,22 +527,20 @@ __global__ __launch_bounds__(kBlock) void k_normals(const DevScene* __restrict__
     const int i = c.begin + threadIdx.x;
     if (i >= S.n) return;
     const KCoefD k = S.kd;
-    const float4 xi = S.x[i];
+    const float4 xi = ld4(S.x, i);
     const double xx = xi.x, xy = xi.y, xz = xi.z;
-    const uint32_t* L = list_of(S, i);
-    const int cnt = S.nnb[i];
     const double m = S.mD, rho0 = S.rho0D;
     double nx = 0.0, ny = 0.0, nz = 0.0;
-    for (int q = 0; q < cnt; ++q) {
-        const int j = (int)L[(size_t)q * kGroup];
-        const Pair p = pair_of(xx, xy, xz, S.x[j], k);
-        const double g = G_d(p, k) * (m / (rho0 + (double)S.drho[j]));
-        nx += g * p.dx;
-        ny += g * p.dy;
-        nz += g * p.dz;
-    }
+    sweep4(S, i, [&](uint32_t j) { XV r; r.x = ld4(S.x, j); r.v.x = ldg(S.drho, j); return r; },
+           [&](uint32_t, const XV& d) {
+               const Pair p = pair_of(xx, xy, xz, d.x, k);
+               const double g = G_d(p, k) * (m / (rho0 + (double)d.v.x));
+               nx += g * p.dx;
+               ny += g * p.dy;
+               nz += g * p.dz;
+           });
     const double h = k.h;
-    S.nrm[i] = make_float4((float)(nx * h), (float)(ny * h), (float)(nz * h), S.drho[i]);
+    st4(S.nrm, i, make_float4((float)(nx * h), (float)(ny * h), (float)(nz * h), ldg(S.drho, i)));
 }
 
 // Non-pressure accelerations (and, for WCSPH, the pressure force and the
@@ -500,58 +563,54 @@ __global__ __launch_bounds__(kBlock) void k_forces(const DevScene* __restrict__ 
     const KCoefD k = S.kd;
     const double dt = S.ss->dt;
     const bool xs = (S.flags & F_XSPH) != 0, st = (S.flags & F_ST) != 0;
-    const float4 xi = (M == WCSPH) ? S.xk[i] : S.x[i];
-    const float4 vi = (M == WCSPH) ? S.aux[i] : S.v[i];
+    const float4 xi = (M == WCSPH) ? ld4(S.xk, i) : ld4(S.x, i);
+    const float4 vi = (M == WCSPH) ? ld4(S.aux, i) : ld4(S.v, i);
     const double xx = xi.x, xy = xi.y, xz = xi.z;
     const double vx0 = vi.x, vy0 = vi.y, vz0 = vi.z;
     const double m = S.mD, rho0 = S.rho0D;
-    const double rhoi = rho0 + (double)S.drho[i];
-    const float4 ni = st ? S.nrm[i] : make_float4(0.f, 0.f, 0.f, 0.f);
-    const uint32_t* L = list_of(S, i);
-    const int cnt = S.nnb[i];
+    const double rhoi = rho0 + (double)ldg(S.drho, i);
+    const float4 ni = st ? ld4(S.nrm, i) : make_float4(0.f, 0.f, 0.f, 0.f);
     const double pri = xi.w;
     double px = 0.0, py = 0.0, pz = 0.0;   // pressure part
     double vx = 0.0, vy = 0.0, vz = 0.0;   // XSPH part (before nu/dt)
     double sx = 0.0, sy = 0.0, sz = 0.0;   // surface tension part (before gamma)
-    for (int q = 0; q < cnt; ++q) {
-        const int j = (int)L[(size_t)q * kGroup];
-        const float4 xj = (M == WCSPH) ? S.xk[j] : S.x[j];
-        const Pair p = pair_of(xx, xy, xz, xj, k);
-        if (M == WCSPH) {
-            const double g = G_d(p, k) * m * (pri + (double)xj.w);
-            px -= g * p.dx;
-            py -= g * p.dy;
-            pz -= g * p.dz;
-        }
-        if (xs) {
-            double wx, wy, wz, mj;
+    sweep4(
+        S, i,
+        [&](uint32_t j) {
+            NbF r;
+            r.x = (M == WCSPH) ? ld4(S.xk, j) : ld4(S.x, j);
+            if (xs) r.v = (M == WCSPH) ? ld4(S.aux, j) : ld4(S.v, j);
+            if (xs && M != WCSPH) r.d = ldg(S.drho, j);
+            if (st) r.n = ld4(S.nrm, j);
+            return r;
+        },
+        [&](uint32_t, const NbF& d) {
+            const Pair p = pair_of(xx, xy, xz, d.x, k);
             if (M == WCSPH) {
-                const float4 vj = S.aux[j];
-                wx = vj.x; wy = vj.y; wz = vj.z;
-                mj = vj.w;
-            } else {
-                const float4 vj = S.v[j];
-                wx = vj.x; wy = vj.y; wz = vj.z;
-                mj = m / (rho0 + (double)S.drho[j]);
+                const double g = G_d(p, k) * m * (pri + (double)d.x.w);
+                px -= g * p.dx;
+                py -= g * p.dy;
+                pz -= g * p.dz;
             }
-            const double w = mj * W_d(p.q, k);
-            vx += w * (wx - vx0);
-            vy += w * (wy - vy0);
-            vz += w * (wz - vz0);
-        }
-        if (st && p.r2 > 0.0) {
-            // list members satisfy |r| <= h exactly (FP64), the reference's
-            // test at src/SurfaceTensionAkinci.h:88
-            const float4 nj = S.nrm[j];
-            const double kij = 2.0 * rho0 / (rhoi + (rho0 + (double)nj.w));
-            bool outer = 2.0 * p.rl > k.h;
-            if (fabs(2.0 * p.rl - k.h) < 1e-9 * k.h) outer = 2.0 * exact_q(xi, xj, S.hD).rl > S.hD;
-            const double cs = -m * C_d(p.rl, outer, k) * p.inv;
-            sx += kij * (cs * p.dx - ((double)ni.x - (double)nj.x));
-            sy += kij * (cs * p.dy - ((double)ni.y - (double)nj.y));
-            sz += kij * (cs * p.dz - ((double)ni.z - (double)nj.z));
-        }
-    }
+            if (xs) {
+                const double mj = (M == WCSPH) ? (double)d.v.w : m / (rho0 + (double)d.d);
+                const double w = mj * W_d(p.q, k);
+                vx += w * ((double)d.v.x - vx0);
+                vy += w * ((double)d.v.y - vy0);
+                vz += w * ((double)d.v.z - vz0);
+            }
+            if (st && p.r2 > 0.0) {
+                // list members satisfy |r| <= h exactly (FP64), the reference's
+                // test at src/SurfaceTensionAkinci.h:88
+                const double kij = 2.0 * rho0 / (rhoi + (rho0 + (double)d.n.w));
+                bool outer = 2.0 * p.rl > k.h;
+                if (fabs(2.0 * p.rl - k.h) < 1e-9 * k.h) outer = 2.0 * exact_q(xi, d.x, S.hD).rl > S.hD;
+                const double cs = -m * C_d(p.rl, outer, k) * p.inv;
+                sx += kij * (cs * p.dx - ((double)ni.x - (double)d.n.x));
+                sy += kij * (cs * p.dy - ((double)ni.y - (double)d.n.y));
+                sz += kij * (cs * p.dz - ((double)ni.z - (double)d.n.z));
+            }
+        });
     double ax = 0.0, ay = 0.0, az = 0.0;
     if (M == WCSPH) {
         const float4 gb = S.gb[i];
@@ -695,25 +754,20 @@ __global__ __launch_bounds__(kBlock) void k_df_vel(const DevScene* __restrict__ 
     const KCoefD k = S.kd;
     const double eps = 1.0e-5;
     const double dt = ss->dt;
-    const float4 xi = S.xk[i];
+    const float4 xi = ld4(S.xk, i);
     const double xx = xi.x, xy = xi.y, xz = xi.z;
     const double si = xi.w;
-    const uint32_t* L = list_of(S, i);
-    const int cnt = S.nnb[i];
     const double m = S.mD;
     double sx = 0.0, sy = 0.0, sz = 0.0;
-    for (int q = 0; q < cnt; ++q) {
-        const int j = (int)L[(size_t)q * kGroup];
-        const float4 xj = S.xk[j];
-        const double ks = si + (double)xj.w;
-        if (fabs(ks) > eps) {
-            const Pair p = pair_of(xx, xy, xz, xj, k);
-            const double g = m * ks * G_d(p, k);
-            sx += g * p.dx;
-            sy += g * p.dy;
-            sz += g * p.dz;
-        }
-    }
+    sweep4(S, i, [&](uint32_t j) { return ld4(S.xk, j); },
+           [&](uint32_t, const float4& xj) {
+               const double ks = si + (double)xj.w;
+               const Pair p = pair_of(xx, xy, xz, xj, k);
+               const double g = sel(fabs(ks) > eps, m * ks * G_d(p, k), 0.0);
+               sx += g * p.dx;
+               sy += g * p.dy;
+               sz += g * p.dz;
+           });
     const double cb = (MODE == 0) ? 2.0 * si : si;
     if (fabs(cb) > eps) {
         const float4 gb = S.gb[i];
@@ -756,21 +810,17 @@ __global__ __launch_bounds__(kBlock) void k_df_err(const DevScene* __restrict__ 
     const double dt = ss->dt;
     double e = 0.0;
     if (i < S.n) {
-        const float4 xi = S.x[i];
-        const float4 vi = S.v[i];
+        const float4 xi = ld4(S.x, i);
+        const float4 vi = ld4(S.v, i);
         const double xx = xi.x, xy = xi.y, xz = xi.z;
         const double ux = vi.x, uy = vi.y, uz = vi.z;
-        const uint32_t* L = list_of(S, i);
-        const int cnt = S.nnb[i];
         double dc = 0.0;
-        for (int q = 0; q < cnt; ++q) {
-            const int j = (int)L[(size_t)q * kGroup];
-            const float4 xj = S.x[j];
-            const float4 vj = S.v[j];
-            const Pair p = pair_of(xx, xy, xz, xj, k);
-            const double g = G_d(p, k);
-            dc += g * ((ux - (double)vj.x) * p.dx + (uy - (double)vj.y) * p.dy + (uz - (double)vj.z) * p.dz);
-        }
+        sweep4(S, i, [&](uint32_t j) { XV r; r.x = ld4(S.x, j); r.v = ld4(S.v, j); return r; },
+               [&](uint32_t, const XV& d) {
+                   const Pair p = pair_of(xx, xy, xz, d.x, k);
+                   const double g = G_d(p, k);
+                   dc += g * ((ux - (double)d.v.x) * p.dx + (uy - (double)d.v.y) * p.dy + (uz - (double)d.v.z) * p.dz);
+               });
         const float4 gb = S.gb[i];
         dc = S.mD * dc + (ux * (double)gb.x + uy * (double)gb.y + uz * (double)gb.z);
         const double drho = (double)S.drho[i];
@@ -885,19 +935,16 @@ __global__ __launch_bounds__(kBlock) void k_pci_pacc(const DevScene* __restrict_
     const float4 xi = S.xk[i];
     const double xx = xi.x, xy = xi.y, xz = xi.z;
     const double pri = xi.w;
-    const uint32_t* L = list_of(S, i);
-    const int cnt = S.nnb[i];
     const double m = S.mD;
     double ax = 0.0, ay = 0.0, az = 0.0;
-    for (int q = 0; q < cnt; ++q) {
-        const int j = (int)L[(size_t)q * kGroup];
-        const float4 xj = S.xk[j];
-        const Pair p = pair_of(xx, xy, xz, xj, k);
-        const double g = m * (pri + (double)xj.w) * Gs_d(p, k);
-        ax -= g * p.dx;
-        ay -= g * p.dy;
-        az -= g * p.dz;
-    }
+    sweep4(S, i, [&](uint32_t j) { return ld4(S.xk, j); },
+           [&](uint32_t, const float4& xj) {
+               const Pair p = pair_of(xx, xy, xz, xj, k);
+               const double g = m * (pri + (double)xj.w) * Gs_d(p, k);
+               ax -= g * p.dx;
+               ay -= g * p.dy;
+               az -= g * p.dz;
+           });
     const float4 gs = S.gbs[i];
     ax -= pri * (double)gs.x;
     ay -= pri * (double)gs.y;
@@ -944,8 +991,8 @@ __global__ __launch_bounds__(kBlock) void k_pci_integrate(const DevScene* __rest
 //             (bounded by maxIterations, :512; time += dt at its start)
 //   FIN_PCI   PCISPH loop test after updatePressure (src/PCISPHSolver.cpp:365)
 // The *0 kinds follow the call before the loop, whose error only feeds the
-// log: the loop's error starts at 0 (:326, :511).  `active` counts the
-// scenes still iterating.
+// log: the loop's error starts at 0 (:326, :511).  active[0] counts the
+// scenes still iterating, active[1] is the largest iteration count reached.
 enum FinKind { FIN_DT = 0, FIN_DENS0, FIN_DENS, FIN_DIV0, FIN_DIV, FIN_PCI };
 
 template <int KIND>
@@ -982,6 +1029,7 @@ __global__ __launch_bounds__(kBlock) void k_finalize(const DevScene* __restrict_
             ss->done = 1;
             ss->sumIt += (unsigned)it;
             atomicSub(active, 1);
+            atomicMax(active + 1, it);
         }
     } else {
         const double avg = (S.n == 0) ? 0.0 : red / (double)S.n;
@@ -1016,6 +1064,7 @@ __global__ __launch_bounds__(kBlock) void k_finalize(const DevScene* __restrict_
         else ss->doneV = more ? 0 : 1;
         if (first && more) atomicAdd(active, 1);
         if (!first && !more) atomicSub(active, 1);
+        if (!more) atomicMax(active + 1, it);
         if (!more) {
             if (dens) ss->sumIt += (unsigned)it;
             else ss->sumItV += (unsigned)it;
