@@ -14,6 +14,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <string>
@@ -247,6 +248,9 @@ void fill_coef_d(KCoefD& k, double h) {
     k.co2 = h6 / 64.0;
     k.hlo2 = h * h * (1.0 - 1e-9);
     k.hhi2 = h * h * (1.0 + 1e-9);
+    k.gA = 3.0 * k.clh2 / h;
+    k.gB = -2.0 * k.clh2;
+    k.hHalf = 0.5 * h;
 }
 
 void fill_coef(KCoef& k, double h) {

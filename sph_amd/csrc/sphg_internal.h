@@ -56,6 +56,7 @@ struct KCoefD {
     double h, invh, h2;
     double ck, clh2, clh, spg, co1, co2;
     double hlo2, hhi2;   // h^2 (1 -/+ 1e-9): band of the operation-exact neighbor test
+    double gA, gB, hHalf;  // cubic gradient for q <= 1/2 as gA r + gB; h/2
 };
 
 // Device-resident control state of one scene (one writer at a time: the

@@ -49,14 +49,30 @@ __device__ inline double block_max(double v, double* sh) {
 }
 
 // Sum (or max) of one scene's chunk partials in chunk order.
+// Each thread combines partials t, t + 256, t + 512, ... in that order; the
+// loads of 8 of them are issued before the first is used, so the loop waits
+// on memory once per 8 partials instead of once per partial.
+constexpr int kPartU = 8;
 __device__ inline double scene_partial_sum(const double* partial, int n, double* sh) {
     double acc = 0.0;
-    for (int t = threadIdx.x; t < n; t += kBlock) acc += partial[t];
+    for (int b = threadIdx.x; b < n; b += kBlock * kPartU) {
+        double v[kPartU];
+#pragma unroll
+        for (int u = 0; u < kPartU; ++u) v[u] = (b + u * kBlock < n) ? partial[b + u * kBlock] : 0.0;
+#pragma unroll
+        for (int u = 0; u < kPartU; ++u) acc += v[u];
+    }
     return block_sum(acc, sh);
 }
 __device__ inline double scene_partial_max(const double* partial, int n, double* sh) {
     double acc = -1e300;
-    for (int t = threadIdx.x; t < n; t += kBlock) acc = fmax(acc, partial[t]);
+    for (int b = threadIdx.x; b < n; b += kBlock * kPartU) {
+        double v[kPartU];
+#pragma unroll
+        for (int u = 0; u < kPartU; ++u) v[u] = (b + u * kBlock < n) ? partial[b + u * kBlock] : -1e300;
+#pragma unroll
+        for (int u = 0; u < kPartU; ++u) acc = fmax(acc, v[u]);
+    }
     return block_max(acc, sh);
 }
 
@@ -83,6 +99,13 @@ __device__ inline void st4(float4* p, size_t i, float4 v) {
     ((gptr<v4f>)p)[i] = t;
 }
 template <class T> __device__ inline T ldg(const T* p, size_t i) { return ((gptr<const T>)p)[i]; }
+// Element j of a float4 array of fewer than 2^28 elements: the 32-bit byte
+// offset lets the load take the uniform-base + 32-bit-offset address form
+// (no 64-bit address arithmetic per element).
+__device__ inline float4 ld4o(const float4* p, uint32_t j) {
+    const v4f t = *(gptr<const v4f>)((gptr<const char>)p + (j << 4));
+    return make_float4(t.x, t.y, t.z, t.w);
+}
 template <class T> __device__ inline void stg(T* p, size_t i, T v) { ((gptr<T>)p)[i] = v; }
 
 // 1/sqrt(r2), zero for r2 <= 1e-30 (coincident points contribute no gradient)
@@ -123,6 +146,33 @@ __device__ inline double G_d(const Pair& p, const KCoefD& k) {
     const double g2 = -k.clh * om * om * p.inv;
     return sel(p.rl > 1e-5 && p.q <= 1.0, sel(p.q <= 0.5, g1, g2), 0.0);
 }
+// Cubic gradient factor of a neighbor-list member for the solver sweeps.
+// List members have q <= 1 by construction and the padding entries are the
+// particle itself (r = 0), so the q <= 1 test of G_d is dropped, and 1/r is
+// the raw FP32 rsqrt refined by one FP64 Newton step with no guard: r = 0
+// gives rl = NaN, which fails rl > 1e-5 and selects 0, as G_d does.
+struct PairG {
+    double dx, dy, dz, g;
+};
+__device__ inline PairG pairg_of(double xi, double yi, double zi, float4 b, const KCoefD& k) {
+    PairG p;
+    p.dx = xi - (double)b.x;
+    p.dy = yi - (double)b.y;
+    p.dz = zi - (double)b.z;
+    const double r2 = p.dx * p.dx + p.dy * p.dy + p.dz * p.dz;
+    const double y = (double)__builtin_amdgcn_rsqf((float)r2);
+    const double e = fma(-r2 * y, y, 1.0);          // 1 - r2 y^2
+    const double inv = fma(0.5 * y, e, y);          // one Newton step
+    const double rl = r2 * inv;
+    // with q = r/h: g = clh2 (3q - 2) = gA r + gB for q <= 1/2, else
+    // -clh (1 - q)^2 / r
+    const double om = fma(-k.invh, rl, 1.0);
+    const double g1 = fma(k.gA, rl, k.gB);
+    const double g2 = (-k.clh * inv) * (om * om);
+    p.g = sel(rl > 1e-5, sel(rl <= k.hHalf, g1, g2), 0.0);
+    return p;
+}
+
 // Spiky gradient factor (src/Spiky.h:44-55): -45/(pi h^6) (h-r)^2 / r.
 __device__ inline double Gs_d(const Pair& p, const KCoefD& k) {
     const double d = k.h - p.rl;
@@ -272,21 +322,6 @@ __global__ __launch_bounds__(kBlock) void k_scan_sum(const DevScene* __restrict_
     if (threadIdx.x == 0) S.cellChunkSum[c.begin / kCellChunk] = (int)tot;
 }
 
-// One block per scene: exclusive scan of the chunk totals, in place.
-__global__ __launch_bounds__(kBlock) void k_scan_offsets(const DevScene* __restrict__ T, int nscenes) {
-    const int s = blockIdx.x;
-    if (s >= nscenes) return;
-    const DevScene& S = T[s];
-    if (threadIdx.x == 0) {
-        int run = 0;
-        for (int t = 0; t < S.nCellChunks; ++t) {
-            const int v = S.cellChunkSum[t];
-            S.cellChunkSum[t] = run;
-            run += v;
-        }
-    }
-}
-
 __device__ inline int wave_incl_scan(int v) {
     const int lane = threadIdx.x & 63;
 #pragma unroll
@@ -295,6 +330,33 @@ __device__ inline int wave_incl_scan(int v) {
         if (lane >= o) v += t;
     }
     return v;
+}
+
+// One block per scene: exclusive scan of the chunk totals, in place, 256
+// totals per pass with the running total carried between passes.
+__global__ __launch_bounds__(kBlock) void k_scan_offsets(const DevScene* __restrict__ T, int nscenes) {
+    const int s = blockIdx.x;
+    if (s >= nscenes) return;
+    const DevScene& S = T[s];
+    __shared__ int wsum[kBlock / 64];
+    __shared__ int carry;
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    const int n = S.nCellChunks;
+    for (int b = 0; b < n; b += kBlock) {
+        const int t = b + threadIdx.x;
+        const int v = (t < n) ? S.cellChunkSum[t] : 0;
+        const int incl = wave_incl_scan(v);
+        const int w = threadIdx.x >> 6;
+        if ((threadIdx.x & 63) == 63) wsum[w] = incl;
+        __syncthreads();
+        int off = carry;
+        for (int q = 0; q < w; ++q) off += wsum[q];
+        if (t < n) S.cellChunkSum[t] = off + incl - v;
+        __syncthreads();
+        if (threadIdx.x == kBlock - 1) carry = off + incl;
+        __syncthreads();
+    }
 }
 
 // Final scan: cellStart[c] = exclusive prefix of counts; counts reset.
@@ -409,6 +471,14 @@ __global__ __launch_bounds__(kBlock) void k_bvolume_pack(const DevScene* __restr
 // plus, per method, the pressure of the Tait EOS (src/WCSPHSolver.cpp:93),
 // the Spiky boundary sum used by PCISPH (src/PCISPHSolver.cpp:342-353) and
 // the adhesion sum (src/AdhesionForce.h:46-59).
+// Fluid neighbors of particle i in two passes.  Pass 1 walks the 27 cells
+// with an FP32 distance test whose bound is 1e-5 above h^2 (FP32 rounding of
+// |x_i - x_j|^2 is ~1e-7 relative) and writes the survivors to the list.
+// Pass 2 runs the FP64 test and sums over those survivors only, compacting
+// the list in place.  Pass 2 thus iterates ~1.1x the neighbor count on
+// every lane of a wavefront, instead of running the FP64 path for every
+// candidate that any lane of the wavefront accepts.  A particle with more
+// survivors than list slots takes the one-pass walk over all candidates.
 template <int M>
 __global__ __launch_bounds__(kBlock) void k_build(const DevScene* __restrict__ T, const Chunk* __restrict__ C) {
     const Chunk c = C[blockIdx.x];
@@ -419,35 +489,54 @@ __global__ __launch_bounds__(kBlock) void k_build(const DevScene* __restrict__ T
     const float hh = S.kc.hhi2 * (1.0f + 1e-5f);   // conservative FP32 pre-filter
     const float4 xi = valid ? ld4(S.x, i) : make_float4(0.f, 0.f, 0.f, 0.f);
     const double xx = xi.x, xy = xi.y, xz = xi.z;
-    const int ci = valid ? ldg(S.cell, i) : S.nx + 1 + S.nx * S.ny;   // any interior cell
+    const int ci = valid ? ldg(S.cell, (size_t)i) : 0;
     const int nx = S.nx, nxny = S.nx * S.ny;
     uint32_t* L = list_of_w(S, valid ? i : (i & ~63));
     const int maxN = S.maxNbr;
     int cnt = 0, ngr = 0;
     double ws = 0.0, sgx = 0.0, sgy = 0.0, sgz = 0.0, ssq = 0.0;
+    // FP64 test and sums of one candidate that passed the FP32 test
+    auto visit = [&](uint32_t j, const float4& xj) {
+        const Pair p = pair_of(xx, xy, xz, xj, k);
+        if (!(p.r2 <= k.hhi2)) return;
+        const double g = G_d(p, k);
+        bool gnz = (g != 0.0);
+        if (p.r2 >= k.hlo2 || fabs(p.q - 0.6666666666666666) < 1e-9 || p.r2 < 1e-9) {
+            const Exact e = exact_q(xi, xj, S.hD);
+            if (!(e.q <= 1.0)) return;
+            gnz = grad_nonzero64(e);
+        }
+        if (cnt < maxN) stg(L, (size_t)cnt * kGroup, j);
+        ++cnt;
+        ngr += gnz ? 1 : 0;
+        ws += W_d(p.q, k);
+        sgx += g * p.dx;
+        sgy += g * p.dy;
+        sgz += g * p.dz;
+        ssq += g * g * p.r2;
+    };
     if (valid) {
+        int nc = 0;
         walk27(S.cellStart, ci, nx, nxny, [&](int j) {
             const float4 xj = ld4(S.x, j);
             const float fx = xi.x - xj.x, fy = xi.y - xj.y, fz = xi.z - xj.z;
             if (!(fx * fx + fy * fy + fz * fz <= hh) || j == i) return;
-            const Pair p = pair_of(xx, xy, xz, xj, k);
-            if (!(p.r2 <= k.hhi2)) return;
-            const double g = G_d(p, k);
-            bool gnz = (g != 0.0);
-            if (p.r2 >= k.hlo2 || fabs(p.q - 0.6666666666666666) < 1e-9 || p.r2 < 1e-9) {
-                const Exact e = exact_q(xi, xj, S.hD);
-                if (!(e.q <= 1.0)) return;
-                gnz = grad_nonzero64(e);
-            }
-            if (cnt < maxN) stg(L, (size_t)cnt * kGroup, (uint32_t)j);
-            ++cnt;
-            ngr += gnz ? 1 : 0;
-            ws += W_d(p.q, k);
-            sgx += g * p.dx;
-            sgy += g * p.dy;
-            sgz += g * p.dz;
-            ssq += g * g * p.r2;
+            if (nc < maxN) stg(L, (size_t)nc * kGroup, (uint32_t)j);
+            ++nc;
         });
+        if (nc <= maxN) {
+            for (int q = 0; q < nc; ++q) {
+                const uint32_t j = ldg((const uint32_t*)L, (size_t)q * kGroup);
+                visit(j, ld4(S.x, j));
+            }
+        } else {
+            walk27(S.cellStart, ci, nx, nxny, [&](int j) {
+                const float4 xj = ld4(S.x, j);
+                const float fx = xi.x - xj.x, fy = xi.y - xj.y, fz = xi.z - xj.z;
+                if (!(fx * fx + fy * fy + fz * fz <= hh) || j == i) return;
+                visit((uint32_t)j, xj);
+            });
+        }
     }
     // pad the list with the particle itself up to the group's longest list
     // rounded to 4 (a self term is exactly zero in every sweep)
@@ -461,8 +550,8 @@ __global__ __launch_bounds__(kBlock) void k_build(const DevScene* __restrict__ T
     }
     if (!valid) return;
     double bw = 0.0, bgx = 0.0, bgy = 0.0, bgz = 0.0;
-    double sx = 0.0, sy = 0.0, sz = 0.0;   // spiky boundary sum (PCISPH)
-    double ax = 0.0, ay = 0.0, az = 0.0;   // adhesion sum
+    double sx2 = 0.0, sy2 = 0.0, sz2 = 0.0;   // spiky boundary sum (PCISPH)
+    double ax = 0.0, ay = 0.0, az = 0.0;      // adhesion sum
     const bool wantAdh = (S.flags & F_ADH) != 0;
     const double rho0 = S.rho0D;
     walk27(S.bCellStart, ci, nx, nxny, [&](int b) {
@@ -479,15 +568,15 @@ __global__ __launch_bounds__(kBlock) void k_build(const DevScene* __restrict__ T
         bgz += g * p.dz;
         if (M == PCISPH) {
             const double gs = cb * Gs_d(p, k);
-            sx += gs * p.dx;
-            sy += gs * p.dy;
-            sz += gs * p.dz;
+            sx2 += gs * p.dx;
+            sy2 += gs * p.dy;
+            sz2 += gs * p.dz;
         }
         if (wantAdh && p.r2 > 0.0) {
-            const double s = cb * A_d(p.rl, k) * p.inv;
-            ax += s * p.dx;
-            ay += s * p.dy;
-            az += s * p.dz;
+            const double sa = cb * A_d(p.rl, k) * p.inv;
+            ax += sa * p.dx;
+            ay += sa * p.dy;
+            az += sa * p.dz;
         }
     });
     if (cnt > maxN) atomicMax(&S.ss->overflow, (unsigned)cnt);
@@ -513,7 +602,7 @@ __global__ __launch_bounds__(kBlock) void k_build(const DevScene* __restrict__ T
         const float4 vi = ld4(S.v, i);
         st4(S.aux, i, make_float4(vi.x, vi.y, vi.z, (float)(m / rho)));
     } else {
-        st4(S.gbs, i, make_float4((float)sx, (float)sy, (float)sz, 0.f));
+        st4(S.gbs, i, make_float4((float)sx2, (float)sy2, (float)sz2, 0.f));
     }
 }
 
@@ -759,11 +848,11 @@ __global__ __launch_bounds__(kBlock) void k_df_vel(const DevScene* __restrict__ 
     const double si = xi.w;
     const double m = S.mD;
     double sx = 0.0, sy = 0.0, sz = 0.0;
-    sweep4(S, i, [&](uint32_t j) { return ld4(S.xk, j); },
+    sweep4(S, i, [&](uint32_t j) { return ld4o(S.xk, j); },
            [&](uint32_t, const float4& xj) {
                const double ks = si + (double)xj.w;
-               const Pair p = pair_of(xx, xy, xz, xj, k);
-               const double g = sel(fabs(ks) > eps, m * ks * G_d(p, k), 0.0);
+               const PairG p = pairg_of(xx, xy, xz, xj, k);
+               const double g = sel(fabs(ks) > eps, m * ks * p.g, 0.0);
                sx += g * p.dx;
                sy += g * p.dy;
                sz += g * p.dz;
@@ -815,11 +904,10 @@ __global__ __launch_bounds__(kBlock) void k_df_err(const DevScene* __restrict__ 
         const double xx = xi.x, xy = xi.y, xz = xi.z;
         const double ux = vi.x, uy = vi.y, uz = vi.z;
         double dc = 0.0;
-        sweep4(S, i, [&](uint32_t j) { XV r; r.x = ld4(S.x, j); r.v = ld4(S.v, j); return r; },
+        sweep4(S, i, [&](uint32_t j) { XV r; r.x = ld4o(S.x, j); r.v = ld4o(S.v, j); return r; },
                [&](uint32_t, const XV& d) {
-                   const Pair p = pair_of(xx, xy, xz, d.x, k);
-                   const double g = G_d(p, k);
-                   dc += g * ((ux - (double)d.v.x) * p.dx + (uy - (double)d.v.y) * p.dy + (uz - (double)d.v.z) * p.dz);
+                   const PairG p = pairg_of(xx, xy, xz, d.x, k);
+                   dc += p.g * ((ux - (double)d.v.x) * p.dx + (uy - (double)d.v.y) * p.dy + (uz - (double)d.v.z) * p.dz);
                });
         const float4 gb = S.gb[i];
         dc = S.mD * dc + (ux * (double)gb.x + uy * (double)gb.y + uz * (double)gb.z);

@@ -105,6 +105,7 @@ def _dfsph_compare(g, o, report):
         "diverr": rel_norm(g.get(sphg.FIELD_DIVERR), o.get(po.OF_DIVERR)),
     }
     report.append(errs)
+    print("errors", {k: f"{v:.2e}" for k, v in errs.items()})
     bad = {k: v for k, v in errs.items() if not v < (TOL_DIVERR if k == "diverr" else TOL)}
     assert not bad, bad
 
@@ -158,6 +159,7 @@ def test_wcsph_steps_identical_state(ctx):
             "v": rel_norm(g.get(sphg.FIELD_V), o.get(po.OF_V)),
             "x": rel_norm(g.get(sphg.FIELD_X), o.get(po.OF_X)),
         }
+        print("errors", {k: f"{v:.2e}" for k, v in errs.items()})
         bad = {k: v for k, v in errs.items() if not v < TOL}
         assert not bad, (step, errs)
     assert abs(g.scalars().time - o.scalars()["time"]) < 1e-12
@@ -188,6 +190,7 @@ def test_pcisph_surface_tension_adhesion(ctx):
             "v": rel_norm(g.get(sphg.FIELD_V), o.get(po.OF_V)),
             "x": rel_norm(g.get(sphg.FIELD_X), o.get(po.OF_X)),
         }
+        print("errors", {k: f"{v:.2e}" for k, v in errs.items()})
         bad = {k: v for k, v in errs.items() if not v < TOL}
         assert not bad, (step, errs)
     assert matched >= 6
