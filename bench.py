@@ -2,19 +2,25 @@
 """Benchmark: DFSPH dam break, 1,000,000 FP32 fluid particles, one MI355X.
 
 A step is one DFSPH solver step (src/DFSPHSolver.cpp:38-112 of the
-reference) of the whole scene, inputs resident in HBM.
+reference) of the whole scene, inputs resident in HBM.  The default run is
+the whole 2000-step dam break after 10 warm-up steps: the density iteration
+count per step changes over the run (about 15 during the first hundred steps
+of the collapse, about 3 afterwards), so a short window is not
+representative.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
 
 Prints one JSON line.  value = fluid particle-steps/s over the K timed steps
 (host wall clock around the step loop, bracketed by device synchronizes).
-The roofline object is measured in a second pass of K steps with a HIP event
-pair around every kernel launch on the library's stream; `achieved` is the
-dominant kernel's algorithmic bytes per launch (per-particle figure x N plus
-boundary and cell terms) divided by its average launch duration.
+During the timed steps a HIP event pair brackets every kernel launch on the
+library's stream; `roofline.achieved` is the dominant kernel's algorithmic
+bytes per launch (per-particle figure x N plus boundary and cell terms)
+divided by its average launch duration.  `roofline.traffic` is the HBM
+traffic per launch of that kernel from the rocprofv3 FETCH_SIZE/WRITE_SIZE
+summary in profiles/ (null when that summary is for another kernel).
 cpu_baseline times the FP64 OpenMP restatement of the reference
-(oracle/liboracle_fast.so) on the host cores, stepping from the GPU's
-warmed-up state.
+(oracle/liboracle_fast.so) on the host cores, stepping from the GPU's state
+at the end of the timed window.
 """
 import argparse
 import json
@@ -26,6 +32,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+PMC_SUMMARY = os.path.join(ROOT, "profiles", "dfsph_1m_pmc.json")
 
 # Algorithmic bytes per fluid particle (FP32) of each kernel, and whether the
 # launch also counts one boundary sweep (16 B per boundary particle) and one
@@ -41,6 +48,9 @@ BALG = {
     "df_positions": (40, 0, 0),
     "rank_reorder": (56, 0, 0),
 }
+KERNELS = ["keys", "scan", "scatter", "rank_reorder", "finalize", "build", "normals", "forces", "dt", "df_predict",
+           "df_vel_warm", "df_vel_density", "df_vel_divergence", "df_err_density", "df_err_divergence",
+           "df_positions"]
 
 
 def balg_step(n, nb, ncells, it_d, it_v):
@@ -52,15 +62,28 @@ def balg_step(n, nb, ncells, it_d, it_v):
     return (264 + 88 * (it_d + it_v)) * n + 16 * nb * sweeps + 4 * ncells * sweeps + 12 * ncells
 
 
+def traffic_for(kernel):
+    """HBM bytes per launch of `kernel` from the committed PMC summary, or None."""
+    try:
+        with open(PMC_SUMMARY) as f:
+            pmc = json.load(f)
+    except (OSError, ValueError):
+        return None
+    k = pmc.get("kernels", {}).get(kernel)
+    if not k:
+        return None
+    return int(k["fetch_bytes"] + k["write_bytes"])
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--n-side", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=2)
-    ap.add_argument("--no-roofline", action="store_true")
+    ap.add_argument("--no-roofline", action="store_true", help="no per-kernel events in the timed steps")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -80,14 +103,25 @@ def main():
     ctx.synchronize()
     print(f"[bench] warmup {args.warmup} done", file=sys.stderr, flush=True)
 
+    n = spec.n
+    if not args.no_roofline:
+        ctx.kernel_timing(True)
     s0 = scene.scalars()
+    syncs = 0
+    done = 0
+    ctx.synchronize()
     t0 = time.perf_counter()
-    st = scene.step(args.steps)
+    while done < args.steps:           # progress lines every 500 steps
+        k = min(500, args.steps - done)
+        st = scene.step(k)
+        syncs += st.host_syncs
+        done += k
+        if done < args.steps:
+            print(f"[bench] {done} steps", file=sys.stderr, flush=True)
     ctx.synchronize()
     t1 = time.perf_counter()
     s1 = scene.scalars()
     wall = t1 - t0
-    n = spec.n
     value = n * args.steps / wall
     it_d = (s1.sum_iterations - s0.sum_iterations) / args.steps
     it_v = (s1.sum_iterations_v - s0.sum_iterations_v) / args.steps
@@ -99,16 +133,8 @@ def main():
     roof = None
     kernels = {}
     if not args.no_roofline:
-        ctx.kernel_timing(True)
-        sa = scene.scalars()
-        scene.step(args.steps)
-        ctx.synchronize()
-        sb = scene.scalars()
         ctx.kernel_timing(False)
-        names = ["keys", "scan", "scatter", "rank_reorder", "finalize", "build", "normals", "forces", "dt",
-                 "df_predict", "df_vel_warm", "df_vel_density", "df_vel_divergence", "df_err_density",
-                 "df_err_divergence", "df_positions"]
-        for nm in names:
+        for nm in KERNELS:
             cnt, ms = ctx.kernel_time(nm)
             if cnt:
                 kernels[nm] = {"launches": cnt, "total_ms": round(ms, 4), "avg_us": round(1e3 * ms / cnt, 3)}
@@ -118,7 +144,7 @@ def main():
         avg_s = kernels[dom]["avg_us"] * 1e-6
         achieved = bytes_launch / avg_s / 1e9
         roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_for(dom), "kernel": dom,
                 "bytes_per_launch": bytes_launch, "avg_launch_us": kernels[dom]["avg_us"]}
         print(f"[bench] kernels: {json.dumps(kernels)}", file=sys.stderr, flush=True)
 
@@ -136,10 +162,12 @@ def main():
         c0 = time.perf_counter()
         o.step(args.cpu_steps)
         c1 = time.perf_counter()
+        oit = o.scalars()
         cpu = {"value": round(n * args.cpu_steps / (c1 - c0), 1), "unit": "particle-steps/s", "cores": threads,
                "kind": "port",
                "sample": f"{args.cpu_steps} DFSPH steps of the same 1M-particle scene from the GPU's state after "
-                         f"{args.warmup + 2 * args.steps} steps, FP64 OpenMP restatement (-O3 -ffast-math -mavx2)"}
+                         f"{args.warmup + args.steps} steps (last step: {int(oit['iterations'])} density iterations), "
+                         "FP64 OpenMP restatement of the reference (-O3 -ffast-math -mavx2)"}
         o.close()
 
     scene.close()
@@ -155,14 +183,16 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32",
+        "dtype": "f64",
         "data": "synthetic",
         "config": {
             "workload": f"DFSPH dam break, {n} fluid + {spec.nb} Akinci boundary particles, spacing 0.02, "
-                        "box 4x2.03x2.03, CFL 0.4, eta 0.01%, eta_V 0.1%",
+                        f"box 4x2.03x2.03, CFL 0.4, eta 0.01%, eta_V 0.1%, steps {args.warmup + 1}-"
+                        f"{args.warmup + args.steps}",
+            "storage": "f32 state, f64 neighbor sums",
             "n_fluid": n, "n_boundary": spec.nb, "cells": info["ncells"],
             "mean_density_iterations": round(it_d, 3), "mean_divergence_iterations": round(it_v, 3),
-            "mean_dt": mean_dt, "host_syncs_per_step": st.host_syncs / max(1, args.steps),
+            "mean_dt": mean_dt, "host_syncs_per_step": round(syncs / max(1, args.steps), 3),
             "balg_bytes_per_step": int(step_bytes),
             "step_roofline_frac": round(step_bytes / (wall / args.steps) / (HBM_PEAK_GBS * 1e9), 4),
         },

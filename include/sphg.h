@@ -41,6 +41,7 @@ enum { SPHG_VISCOSITY_XSPH = 2, SPHG_SURFACE_TENSION_AKINCI = 3, SPHG_ADHESION_A
 
 typedef struct sphg_ctx sphg_ctx;
 typedef struct sphg_scene sphg_scene;
+typedef struct sphg_batch sphg_batch;
 
 /* One scene: the reference's SceneInfo/FluidInfo fields
  * (src/Simulation.h:47-68, read at src/SceneLoader.cpp:38-141) plus the
@@ -161,6 +162,18 @@ int sphg_scene_info(sphg_scene* scene, uint32_t* n, uint32_t* nb, uint32_t* ncel
  * src/Simulation.cpp:84-90). */
 int sphg_batch_step(sphg_ctx* ctx, sphg_scene* const* scenes, uint32_t nscenes, uint32_t nsteps,
                     sphg_step_stats* stats);
+/* The same with the batch kept between calls (its chunk tables and its
+ * iteration-count prediction), for a serving loop that steps a changing set
+ * of resident scenes in quanta: create when the set changes, run per
+ * quantum, destroy before any of its scenes.  No reference counterpart. */
+int sphg_batch_create(sphg_ctx* ctx, sphg_scene* const* scenes, uint32_t nscenes, sphg_batch** out);
+int sphg_batch_run(sphg_batch* batch, uint32_t nsteps, sphg_step_stats* stats);
+int sphg_batch_destroy(sphg_batch* batch);
+/* Device bytes the scene holds once initialised, computed from its fluid and
+ * boundary (after set_fluid / set_boundary) before anything is allocated:
+ * the admission test of a memory-budgeted serving loop.  No reference
+ * counterpart. */
+int sphg_scene_plan_bytes(sphg_scene* scene, uint64_t* bytes);
 
 /* Per-kernel device timing with HIP events on the library's stream.
  * enable != 0 starts recording every launch; read returns, for kernel

@@ -537,9 +537,134 @@ int run_batch(sphg_ctx* ctx, Batch& B, uint32_t nsteps, sphg_step_stats* stats) 
     return SPHG_OK;
 }
 
+// Grid extent and device memory plan of a scene (one allocation).
+struct Plan {
+    int k0[3], dims[3];
+    long ncells = 0;
+    int nChunks = 0, nCellChunks = 0, maxNbr = 0, flags = 0;
+    size_t npad = 0, total = 0;
+    size_t oX[2], oV[2], oK[2] = {0, 0}, oKV[2] = {0, 0}, oID[2];
+    size_t oA, oXK, oGB, oAUX, oXS, oPACC, oGBS, oNRM, oADH, oRHO, oALPHA, oDRHOS, oDIV, oP;
+    size_t oCELLU, oCELL, oPERM, oCC, oCS, oCCS, oNBR, oNNB, oNG, oGC, oXB, oBCS, oPART, oSS, oTMP;
+};
+
+int make_plan(const sphg_scene* s, Plan& P) {
+    const sphg_scene_desc& d = s->desc;
+    const double h = 4.0 * d.particle_radius;
+    const int method = d.method;
+    // grid bounds in key space
+    double lo[3], hi[3];
+    bool given = false;
+    for (int a = 0; a < 3; ++a) given |= (d.domain_min[a] != 0.0 || d.domain_max[a] != 0.0);
+    if (given) {
+        for (int a = 0; a < 3; ++a) { lo[a] = d.domain_min[a]; hi[a] = d.domain_max[a]; }
+    } else {
+        for (int a = 0; a < 3; ++a) { lo[a] = 1e300; hi[a] = -1e300; }
+        auto acc = [&](const std::vector<float>& p) {
+            for (size_t i = 0; i + 2 < p.size(); i += 3)
+                for (int a = 0; a < 3; ++a) {
+                    const double v = p[i + a];
+                    if (std::isfinite(v)) { lo[a] = std::min(lo[a], v); hi[a] = std::max(hi[a], v); }
+                }
+        };
+        acc(s->fxyz);
+        acc(s->bxyz);
+    }
+    int* k0 = P.k0;
+    int* dims = P.dims;
+    long ncells = 1;
+    for (int a = 0; a < 3; ++a) {
+        const long kl = (long)std::floor(lo[a] / h) - 1;
+        const long kh = (long)std::floor(hi[a] / h) + 1;
+        if (kh - kl + 3 > (1L << 20)) return fail(SPHG_EINVAL, "grid too large along one axis");
+        k0[a] = (int)kl;
+        dims[a] = (int)(kh - kl + 1) + 2;
+        ncells *= dims[a];
+    }
+    if (ncells > (1L << 28)) return fail(SPHG_EINVAL, "grid has more than 2^28 cells; set domain_min/max");
+
+    const uint32_t n = s->n, nb = s->nb;
+    const int nChunks = (int)((n + kBlock - 1) / kBlock);
+    const size_t npad = (size_t)nChunks * kBlock;
+    const int nCellChunks = (int)((ncells + 1 + kCellChunk - 1) / kCellChunk);
+    // a multiple of 4: the lists are padded to multiples of 4 per group
+    const int maxNbr = ((d.max_neighbors ? (int)d.max_neighbors : kDefaultMaxNbr) + 3) & ~3;
+
+    int flags = 0;
+    if (d.viscosity_method == SPHG_VISCOSITY_XSPH && d.viscosity != 0.0) flags |= F_XSPH;
+    if (d.surface_tension_method == SPHG_SURFACE_TENSION_AKINCI && d.surface_tension != 0.0) flags |= F_ST;
+    if (d.adhesion_method == SPHG_ADHESION_AKINCI && d.adhesion != 0.0) flags |= F_ADH;
+
+    // memory plan
+    Arena A;
+    const size_t f4 = sizeof(float4) * npad, f1 = sizeof(float) * npad, u1 = sizeof(uint32_t) * npad;
+    size_t* oX = P.oX;
+    size_t* oV = P.oV;
+    size_t* oK = P.oK;
+    size_t* oKV = P.oKV;
+    size_t* oID = P.oID;
+    for (int p = 0; p < 2; ++p) { oX[p] = A.reserve(f4); oV[p] = A.reserve(f4); oID[p] = A.reserve(u1); }
+    if (method == DFSPH)
+        for (int p = 0; p < 2; ++p) { oK[p] = A.reserve(f1); oKV[p] = A.reserve(f1); }
+    P.oA = A.reserve(f4);
+    P.oXK = A.reserve(f4);
+    P.oGB = A.reserve(f4);
+    P.oAUX = (method != DFSPH) ? A.reserve(f4) : 0;
+    P.oXS = (method == PCISPH) ? A.reserve(f4) : 0;
+    P.oPACC = (method == PCISPH) ? A.reserve(f4) : 0;
+    P.oGBS = (method == PCISPH) ? A.reserve(f4) : 0;
+    P.oNRM = (flags & F_ST) ? A.reserve(f4) : 0;
+    P.oADH = (flags & F_ADH) ? A.reserve(f4) : 0;
+    P.oRHO = A.reserve(f1);
+    P.oALPHA = (method == DFSPH) ? A.reserve(f1) : 0;
+    P.oDRHOS = (method == DFSPH) ? A.reserve(f1) : 0;
+    P.oDIV = (method == DFSPH) ? A.reserve(f1) : 0;
+    P.oP = (method != DFSPH) ? A.reserve(f1) : 0;
+    P.oCELLU = A.reserve(u1);
+    P.oCELL = A.reserve(u1);
+    P.oPERM = A.reserve(u1);
+    P.oCC = A.reserve(sizeof(int) * (size_t)ncells);
+    P.oCS = A.reserve(sizeof(int) * (size_t)(ncells + 1));
+    P.oCCS = A.reserve(sizeof(int) * (size_t)nCellChunks);
+    P.oNBR = A.reserve(sizeof(uint32_t) * npad * (size_t)maxNbr);
+    P.oNNB = A.reserve(sizeof(uint16_t) * npad);
+    P.oNG = A.reserve(npad);
+    P.oGC = A.reserve(sizeof(uint16_t) * (npad / kGroup));
+    P.oXB = A.reserve(sizeof(float4) * std::max<size_t>(nb, 1));
+    P.oBCS = A.reserve(sizeof(int) * (size_t)(ncells + 1));
+    P.oPART = A.reserve(sizeof(double) * (size_t)std::max(nChunks, 1));
+    P.oSS = A.reserve(sizeof(Scalars));
+    P.oTMP = A.reserve(std::max(f4, sizeof(float4) * std::max<size_t>(nb, 1)));
+    P.ncells = ncells;
+    P.nChunks = nChunks;
+    P.nCellChunks = nCellChunks;
+    P.maxNbr = maxNbr;
+    P.flags = flags;
+    P.npad = npad;
+    P.total = A.used;
+    return SPHG_OK;
+}
+
+int check_batch(sphg_ctx* ctx, sphg_scene* const* scenes, uint32_t nscenes) {
+    const int method = scenes[0] ? scenes[0]->desc.method : -1;
+    for (uint32_t i = 0; i < nscenes; ++i) {
+        if (!scenes[i] || scenes[i]->ctx != ctx) return fail(SPHG_EINVAL, "scene of another context");
+        if (!scenes[i]->ready) return fail(SPHG_ESTATE, "scene not initialised");
+        if (scenes[i]->desc.method != method) return fail(SPHG_EINVAL, "a batch steps scenes of one method");
+        for (uint32_t j = 0; j < i; ++j)
+            if (scenes[j] == scenes[i]) return fail(SPHG_EINVAL, "scene listed twice");
+    }
+    return SPHG_OK;
+}
+
 }  // namespace
 
 // ================================================================ C API
+
+struct sphg_batch {
+    sphg_ctx* ctx = nullptr;
+    Batch B;
+};
 
 extern "C" {
 
@@ -644,85 +769,21 @@ int sphg_scene_init(sphg_scene* s) {
     const double m = rho0 * std::pow(2.0 * r, 3.0);
     const int method = d.method;
 
-    // grid bounds in key space
-    double lo[3], hi[3];
-    bool given = false;
-    for (int a = 0; a < 3; ++a) given |= (d.domain_min[a] != 0.0 || d.domain_max[a] != 0.0);
-    if (given) {
-        for (int a = 0; a < 3; ++a) { lo[a] = d.domain_min[a]; hi[a] = d.domain_max[a]; }
-    } else {
-        for (int a = 0; a < 3; ++a) { lo[a] = 1e300; hi[a] = -1e300; }
-        auto acc = [&](const std::vector<float>& p) {
-            for (size_t i = 0; i + 2 < p.size(); i += 3)
-                for (int a = 0; a < 3; ++a) {
-                    const double v = p[i + a];
-                    if (std::isfinite(v)) { lo[a] = std::min(lo[a], v); hi[a] = std::max(hi[a], v); }
-                }
-        };
-        acc(s->fxyz);
-        acc(s->bxyz);
+    Plan P;
+    {
+        int prc = make_plan(s, P);
+        if (prc) return prc;
     }
-    int k0[3], dims[3];
-    long ncells = 1;
-    for (int a = 0; a < 3; ++a) {
-        const long kl = (long)std::floor(lo[a] / h) - 1;
-        const long kh = (long)std::floor(hi[a] / h) + 1;
-        if (kh - kl + 3 > (1L << 20)) return fail(SPHG_EINVAL, "grid too large along one axis");
-        k0[a] = (int)kl;
-        dims[a] = (int)(kh - kl + 1) + 2;
-        ncells *= dims[a];
-    }
-    if (ncells > (1L << 28)) return fail(SPHG_EINVAL, "grid has more than 2^28 cells; set domain_min/max");
-
+    const int* k0 = P.k0;
+    const int* dims = P.dims;
+    const long ncells = P.ncells;
     const uint32_t n = s->n, nb = s->nb;
-    const int nChunks = (int)((n + kBlock - 1) / kBlock);
-    const size_t npad = (size_t)nChunks * kBlock;
-    const int nCellChunks = (int)((ncells + 1 + kCellChunk - 1) / kCellChunk);
-    // a multiple of 4: the lists are padded to multiples of 4 per group
-    const int maxNbr = ((d.max_neighbors ? (int)d.max_neighbors : kDefaultMaxNbr) + 3) & ~3;
-
-    int flags = 0;
-    if (d.viscosity_method == SPHG_VISCOSITY_XSPH && d.viscosity != 0.0) flags |= F_XSPH;
-    if (d.surface_tension_method == SPHG_SURFACE_TENSION_AKINCI && d.surface_tension != 0.0) flags |= F_ST;
-    if (d.adhesion_method == SPHG_ADHESION_AKINCI && d.adhesion != 0.0) flags |= F_ADH;
-
-    // memory plan
-    Arena A;
-    const size_t f4 = sizeof(float4) * npad, f1 = sizeof(float) * npad, u1 = sizeof(uint32_t) * npad;
-    size_t oX[2], oV[2], oK[2] = {0, 0}, oKV[2] = {0, 0}, oID[2];
-    for (int p = 0; p < 2; ++p) { oX[p] = A.reserve(f4); oV[p] = A.reserve(f4); oID[p] = A.reserve(u1); }
-    if (method == DFSPH)
-        for (int p = 0; p < 2; ++p) { oK[p] = A.reserve(f1); oKV[p] = A.reserve(f1); }
-    const size_t oA = A.reserve(f4), oXK = A.reserve(f4), oGB = A.reserve(f4);
-    const size_t oAUX = (method != DFSPH) ? A.reserve(f4) : 0;
-    const size_t oXS = (method == PCISPH) ? A.reserve(f4) : 0;
-    const size_t oPACC = (method == PCISPH) ? A.reserve(f4) : 0;
-    const size_t oGBS = (method == PCISPH) ? A.reserve(f4) : 0;
-    const size_t oNRM = (flags & F_ST) ? A.reserve(f4) : 0;
-    const size_t oADH = (flags & F_ADH) ? A.reserve(f4) : 0;
-    const size_t oRHO = A.reserve(f1);
-    const size_t oALPHA = (method == DFSPH) ? A.reserve(f1) : 0;
-    const size_t oDRHOS = (method == DFSPH) ? A.reserve(f1) : 0;
-    const size_t oDIV = (method == DFSPH) ? A.reserve(f1) : 0;
-    const size_t oP = (method != DFSPH) ? A.reserve(f1) : 0;
-    const size_t oCELLU = A.reserve(u1), oCELL = A.reserve(u1), oPERM = A.reserve(u1);
-    const size_t oCC = A.reserve(sizeof(int) * (size_t)ncells);
-    const size_t oCS = A.reserve(sizeof(int) * (size_t)(ncells + 1));
-    const size_t oCCS = A.reserve(sizeof(int) * (size_t)nCellChunks);
-    const size_t oNBR = A.reserve(sizeof(uint32_t) * npad * (size_t)maxNbr);
-    const size_t oNNB = A.reserve(sizeof(uint16_t) * npad);
-    const size_t oNG = A.reserve(npad);
-    const size_t oGC = A.reserve(sizeof(uint16_t) * (npad / kGroup));
-    const size_t oXB = A.reserve(sizeof(float4) * std::max<size_t>(nb, 1));
-    const size_t oBCS = A.reserve(sizeof(int) * (size_t)(ncells + 1));
-    const size_t oPART = A.reserve(sizeof(double) * (size_t)std::max(nChunks, 1));
-    const size_t oSS = A.reserve(sizeof(Scalars));
-    const size_t oTMP = A.reserve(std::max(f4, sizeof(float4) * std::max<size_t>(nb, 1)));
-    hipError_t he = hipMalloc(&s->mem, A.used);
-    if (he != hipSuccess) return fail(SPHG_ENOMEM, "device allocation of " + std::to_string(A.used) + " bytes failed");
-    s->memBytes = A.used;
+    const int nChunks = P.nChunks, nCellChunks = P.nCellChunks, maxNbr = P.maxNbr, flags = P.flags;
+    hipError_t he = hipMalloc(&s->mem, P.total);
+    if (he != hipSuccess) return fail(SPHG_ENOMEM, "device allocation of " + std::to_string(P.total) + " bytes failed");
+    s->memBytes = P.total;
     char* b = s->mem;
-    HIP_TRY(hipMemsetAsync(b, 0, A.used, ctx->stream));
+    HIP_TRY(hipMemsetAsync(b, 0, P.total, ctx->stream));
 
     DevScene D{};
     fill_coef(D.kc, h);
@@ -769,43 +830,43 @@ int sphg_scene_init(sphg_scene* s) {
     D.nCellChunks = nCellChunks;
     auto F4 = [&](size_t o) { return (float4*)(b + o); };
     auto F1 = [&](size_t o) { return (float*)(b + o); };
-    D.a = F4(oA);
-    D.xk = F4(oXK);
-    D.gb = F4(oGB);
-    D.aux = oAUX ? F4(oAUX) : nullptr;
-    D.xs = oXS ? F4(oXS) : nullptr;
-    D.pacc = oPACC ? F4(oPACC) : nullptr;
-    D.gbs = oGBS ? F4(oGBS) : nullptr;
-    D.nrm = oNRM ? F4(oNRM) : nullptr;
-    D.adh = oADH ? F4(oADH) : nullptr;
-    D.drho = F1(oRHO);
-    D.alpha = oALPHA ? F1(oALPHA) : nullptr;
-    D.drhoS = oDRHOS ? F1(oDRHOS) : nullptr;
-    D.divErr = oDIV ? F1(oDIV) : nullptr;
-    D.p = oP ? F1(oP) : nullptr;
-    D.cellU = (int*)(b + oCELLU);
-    D.cell = (int*)(b + oCELL);
-    D.perm = (uint32_t*)(b + oPERM);
-    D.cellCount = (int*)(b + oCC);
-    D.cellStart = (int*)(b + oCS);
-    D.cellChunkSum = (int*)(b + oCCS);
-    D.nbr = (uint32_t*)(b + oNBR);
-    D.nnb = (uint16_t*)(b + oNNB);
-    D.ngrad = (uint8_t*)(b + oNG);
-    D.gcnt = (uint16_t*)(b + oGC);
-    D.xb = F4(oXB);
-    D.bCellStart = (int*)(b + oBCS);
-    D.partial = (double*)(b + oPART);
-    D.ss = (Scalars*)(b + oSS);
+    D.a = F4(P.oA);
+    D.xk = F4(P.oXK);
+    D.gb = F4(P.oGB);
+    D.aux = P.oAUX ? F4(P.oAUX) : nullptr;
+    D.xs = P.oXS ? F4(P.oXS) : nullptr;
+    D.pacc = P.oPACC ? F4(P.oPACC) : nullptr;
+    D.gbs = P.oGBS ? F4(P.oGBS) : nullptr;
+    D.nrm = P.oNRM ? F4(P.oNRM) : nullptr;
+    D.adh = P.oADH ? F4(P.oADH) : nullptr;
+    D.drho = F1(P.oRHO);
+    D.alpha = P.oALPHA ? F1(P.oALPHA) : nullptr;
+    D.drhoS = P.oDRHOS ? F1(P.oDRHOS) : nullptr;
+    D.divErr = P.oDIV ? F1(P.oDIV) : nullptr;
+    D.p = P.oP ? F1(P.oP) : nullptr;
+    D.cellU = (int*)(b + P.oCELLU);
+    D.cell = (int*)(b + P.oCELL);
+    D.perm = (uint32_t*)(b + P.oPERM);
+    D.cellCount = (int*)(b + P.oCC);
+    D.cellStart = (int*)(b + P.oCS);
+    D.cellChunkSum = (int*)(b + P.oCCS);
+    D.nbr = (uint32_t*)(b + P.oNBR);
+    D.nnb = (uint16_t*)(b + P.oNNB);
+    D.ngrad = (uint8_t*)(b + P.oNG);
+    D.gcnt = (uint16_t*)(b + P.oGC);
+    D.xb = F4(P.oXB);
+    D.bCellStart = (int*)(b + P.oBCS);
+    D.partial = (double*)(b + P.oPART);
+    D.ss = (Scalars*)(b + P.oSS);
     s->ss = D.ss;
-    s->tmp = F1(oTMP);
+    s->tmp = F1(P.oTMP);
     for (int p = 0; p < 2; ++p) {
         DevScene v = D;
-        v.x = F4(oX[p]); v.x2 = F4(oX[1 - p]);
-        v.v = F4(oV[p]); v.v2 = F4(oV[1 - p]);
-        v.id = (uint32_t*)(b + oID[p]); v.id2 = (uint32_t*)(b + oID[1 - p]);
-        v.k = oK[p] ? F1(oK[p]) : nullptr; v.k2 = oK[1 - p] ? F1(oK[1 - p]) : nullptr;
-        v.kv = oKV[p] ? F1(oKV[p]) : nullptr; v.kv2 = oKV[1 - p] ? F1(oKV[1 - p]) : nullptr;
+        v.x = F4(P.oX[p]); v.x2 = F4(P.oX[1 - p]);
+        v.v = F4(P.oV[p]); v.v2 = F4(P.oV[1 - p]);
+        v.id = (uint32_t*)(b + P.oID[p]); v.id2 = (uint32_t*)(b + P.oID[1 - p]);
+        v.k = P.oK[p] ? F1(P.oK[p]) : nullptr; v.k2 = P.oK[1 - p] ? F1(P.oK[1 - p]) : nullptr;
+        v.kv = P.oKV[p] ? F1(P.oKV[p]) : nullptr; v.kv2 = P.oKV[1 - p] ? F1(P.oKV[1 - p]) : nullptr;
         s->view[p] = v;
     }
     s->par = 0;
@@ -935,21 +996,73 @@ int sphg_batch_step(sphg_ctx* ctx, sphg_scene* const* scenes, uint32_t nscenes, 
     if (!ctx || (!scenes && nscenes)) return fail(SPHG_EINVAL, "NULL argument");
     if (nscenes == 0) return SPHG_OK;
     HIP_TRY(hipSetDevice(ctx->device));
-    const int method = scenes[0]->desc.method;
-    for (uint32_t i = 0; i < nscenes; ++i) {
-        if (!scenes[i] || scenes[i]->ctx != ctx) return fail(SPHG_EINVAL, "scene of another context");
-        if (!scenes[i]->ready) return fail(SPHG_ESTATE, "scene not initialised");
-        if (scenes[i]->desc.method != method) return fail(SPHG_EINVAL, "a batch steps scenes of one method");
-        for (uint32_t j = 0; j < i; ++j)
-            if (scenes[j] == scenes[i]) return fail(SPHG_EINVAL, "scene listed twice");
-    }
+    int rc = check_batch(ctx, scenes, nscenes);
+    if (rc) return rc;
     Batch B;
     B.scenes.assign(scenes, scenes + nscenes);
-    int rc = upload_batch_tables(ctx, B);
+    rc = upload_batch_tables(ctx, B);
     if (rc) return rc;
     rc = run_batch(ctx, B, nsteps, stats);
     if (rc) return rc;
     HIP_TRY(hipGetLastError());
+    return SPHG_OK;
+}
+
+int sphg_batch_create(sphg_ctx* ctx, sphg_scene* const* scenes, uint32_t nscenes, sphg_batch** out) {
+    if (!ctx || !out || !scenes || nscenes == 0) return fail(SPHG_EINVAL, "NULL argument or empty batch");
+    *out = nullptr;
+    HIP_TRY(hipSetDevice(ctx->device));
+    int rc = check_batch(ctx, scenes, nscenes);
+    if (rc) return rc;
+    sphg_batch* b = new sphg_batch();
+    b->ctx = ctx;
+    b->B.scenes.assign(scenes, scenes + nscenes);
+    rc = upload_batch_tables(ctx, b->B);
+    if (rc) {
+        delete b;
+        return rc;
+    }
+    *out = b;
+    return SPHG_OK;
+}
+
+int sphg_batch_run(sphg_batch* b, uint32_t nsteps, sphg_step_stats* stats) {
+    if (!b) return fail(SPHG_EINVAL, "NULL batch");
+    sphg_ctx* ctx = b->ctx;
+    HIP_TRY(hipSetDevice(ctx->device));
+    Batch& B = b->B;
+    // a scene stepped elsewhere since the tables were built has moved its
+    // double buffers: rebuild the tables from the scenes' current parity
+    bool stale = false;
+    for (size_t i = 0; i < B.scenes.size(); ++i) stale |= (B.parity[i] != B.scenes[i]->par);
+    if (stale) {
+        int rc = upload_batch_tables(ctx, B);
+        if (rc) return rc;
+    }
+    int rc = run_batch(ctx, B, nsteps, stats);
+    if (rc) return rc;
+    HIP_TRY(hipGetLastError());
+    return SPHG_OK;
+}
+
+int sphg_batch_destroy(sphg_batch* b) {
+    if (!b) return SPHG_OK;
+    (void)hipSetDevice(b->ctx->device);
+    delete b;
+    return SPHG_OK;
+}
+
+int sphg_scene_plan_bytes(sphg_scene* s, uint64_t* bytes) {
+    if (!s || !bytes) return fail(SPHG_EINVAL, "NULL argument");
+    if (s->ready) {
+        *bytes = s->memBytes;
+        return SPHG_OK;
+    }
+    if (s->n == 0) return fail(SPHG_ESTATE, "no fluid particles");
+    Plan P;
+    int rc = make_plan(s, P);
+    if (rc) return rc;
+    *bytes = P.total;
     return SPHG_OK;
 }
 

@@ -239,13 +239,27 @@ template <class Load, class Body>
 __device__ inline void sweep4(const DevScene& S, int i, Load&& load, Body&& body) {
     const uint32_t* L = list_of(S, i);
     const int gc = group_count(S, i);
+    if (gc == 0) return;
+    // the indices of the next 4 entries are requested before the neighbor
+    // data of the current 4 is used, so the list stream (HBM) and the
+    // neighbor loads (mostly L2) overlap instead of forming one chain
+    uint32_t jn[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) jn[u] = ldg(L, (size_t)u * kGroup);
     for (int q = 0; q < gc; q += 4) {
         uint32_t jj[4];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) jj[u] = ldg(L, (size_t)(q + u) * kGroup);
+        for (int u = 0; u < 4; ++u) jj[u] = jn[u];
         decltype(load(0u)) dd[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) dd[u] = load(jj[u]);
+        // unconditional (the last pass re-reads its own entries), so the
+        // wait for this pass's data leaves the 4 index loads in flight
+        const int qn = (q + 4 < gc) ? q + 4 : q;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) jn[u] = ldg(L, (size_t)(qn + u) * kGroup);
+        // keeps the compiler from sinking these loads into the next pass
+        asm volatile("" ::: "memory");
 #pragma unroll
         for (int u = 0; u < 4; ++u) body(jj[u], dd[u]);
     }

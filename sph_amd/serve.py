@@ -1,0 +1,376 @@
+"""Serving: simulation requests on one GPU, continuous batching vs plain.
+
+The reference is a one-scene program (its Simulation singleton,
+src/Simulation.cpp:84-90); this module has no counterpart there.  A request
+asks for `steps` solver steps of a seeded scene (method, fluid size,
+geometry); its result is the fluid state after those steps, on the host.
+
+Continuous batching (`serve_batched`):
+  * requests arrive on a wall clock (Poisson process); a waiting request is
+    admitted, first come first served, when the device bytes of its scene
+    (known before allocation, `Scene.plan_bytes`) fit the memory budget
+    beside the scenes already resident;
+  * resident scenes stay on the device; the scenes of one method are stepped
+    together, one launch per solver phase covering all of them
+    (`sphg.Batch`), in quanta of at most `quantum` steps, so that requests
+    that arrive meanwhile join at the next quantum;
+  * a scene that has done its steps is read back, freed and its request
+    completed; the rest continue.
+  The scene geometry of an arriving request is built on a host thread while
+  the GPU steps the resident scenes.
+
+Plain serving (`serve_plain`): the same requests one at a time in arrival
+order, each stepped alone from creation to read-back.
+
+Both report per-request completion latency (arrival to result on the host)
+and the aggregate fluid particle-steps/s over the stream.
+
+  python -m sph_amd.serve [--requests 48] [--rate 2.0] [--nmax 262144] ...
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import threading
+import time
+from collections import deque
+from concurrent.futures import ThreadPoolExecutor
+from dataclasses import dataclass, field
+from typing import Callable, Dict, List, Optional
+
+import numpy as np
+
+from . import configs, sphg
+from .configs import SceneSpec, SplitMix64
+from .sphg import DFSPH, PCISPH, WCSPH, SceneParams
+
+METHODS = (WCSPH, PCISPH, DFSPH)
+METHOD_NAMES = {WCSPH: "WCSPH", PCISPH: "PCISPH", DFSPH: "DFSPH"}
+
+
+@dataclass
+class Request:
+    rid: int
+    arrival: float      # seconds after the stream starts
+    method: int
+    n_target: int
+    steps: int
+    geometry: str       # "block" (dam break) or "drop" (sphere drop)
+    seed: int
+
+
+def request_stream(count: int, rate: float, nmin: int = 2048, nmax: int = 2_000_000, smin: int = 100,
+                   smax: int = 2000, seed: int = 5) -> List[Request]:
+    """Poisson arrivals at `rate` per second (dt = -ln(1 - u) / rate); method
+    uniform over WCSPH, PCISPH, DFSPH; fluid size log-uniform in [nmin, nmax];
+    steps uniform in [smin, smax]; geometry block or sphere drop."""
+    rng = SplitMix64(seed)
+    t = 0.0
+    out = []
+    for i in range(count):
+        t += -math.log(1.0 - rng.uniform()) / rate
+        method = METHODS[min(2, int(rng.uniform() * 3))]
+        n = rng.log_uniform_int(nmin, nmax)
+        steps = smin + min(smax - smin, int(rng.uniform() * (smax - smin + 1)))
+        geometry = "block" if rng.uniform() < 0.5 else "drop"
+        out.append(Request(i, t, method, n, steps, geometry, 10_000 + i))
+    return out
+
+
+def _method_params(method: int, r: float) -> SceneParams:
+    p = SceneParams(method=method, particle_radius=r, gravity=configs.G)
+    if method == DFSPH:
+        return SceneParams(method=DFSPH, particle_radius=r, gravity=configs.G, cfl_factor=0.4, time_step=1e-3,
+                           max_time_step=5e-3, min_time_step=1e-6, eta=1e-4, eta_v=1e-3, min_iterations=2,
+                           max_iterations=100, min_iterations_v=1, max_iterations_v=100)
+    if method == WCSPH:
+        return SceneParams(method=WCSPH, particle_radius=r, gravity=configs.G, stiffness=1000.0, gamma=7.0,
+                           viscosity_method=sphg.VISCOSITY_XSPH, viscosity=0.01, time_step=5e-4,
+                           min_time_step=5e-4, max_time_step=5e-4)
+    if method == PCISPH:
+        return SceneParams(method=PCISPH, particle_radius=r, gravity=configs.G, eta=0.01, min_iterations=3,
+                           max_iterations=100, time_step=5e-4, min_time_step=5e-4, max_time_step=5e-4)
+    return p
+
+
+def build_spec(req: Request, spacing: float = 0.02) -> SceneSpec:
+    """The seeded scene of a request: a jittered lattice block in the corner
+    of a box twice its length ("block"), or a lattice sphere released half a
+    diameter above the floor of a box ("drop"), Akinci walls, spacing 0.02."""
+    r = spacing / 2.0
+    h = 4.0 * r
+    params = _method_params(req.method, r)
+    if req.geometry == "block":
+        ny = max(2, int(round((req.n_target / 2.0) ** (1.0 / 3.0))))
+        nx = max(2, int(round(req.n_target / (ny * ny))))
+        L = (nx, ny, ny)
+        box = (2.0 * nx * spacing + 6 * r, ny * spacing + 6 * r, 1.5 * ny * spacing + 6 * r)
+        spec = configs.dam_break(req.method, L, spacing, box, jitter=0.01, seed=req.seed, gap=2.5)
+        spec.params = params
+    else:
+        # sphere of ~n_target lattice sites: radius from n s^3 = 4/3 pi R^3
+        R = (3.0 * req.n_target / (4.0 * math.pi)) ** (1.0 / 3.0) * spacing
+        side = 2.0 * R + 12 * r
+        c = (side / 2.0, side / 2.0, 2.0 * R + 6 * r)
+        box = (side, side, c[2] + R + 6 * r)
+        fluid = sphg.fluid_spheres((0.0, 0.0, 0.0), box, r, [c], [R])
+        boundary = sphg.sample_box((0.0, 0.0, 0.0), box, h)
+        spec = SceneSpec("drop", params, fluid, boundary)
+    spec.name = f"req{req.rid}_{METHOD_NAMES[req.method]}_{req.geometry}_{spec.n}"
+    return spec
+
+
+# --------------------------------------------------------------- backends
+
+class GpuBackend:
+    """The HIP solver library on one device."""
+
+    def __init__(self, ctx: sphg.Context):
+        self.ctx = ctx
+
+    def create(self, spec: SceneSpec):
+        return sphg.Scene(self.ctx, spec.params, spec.fluid, spec.boundary, init=False)
+
+    def plan_bytes(self, scene) -> int:
+        return scene.plan_bytes()
+
+    def init(self, scene) -> None:
+        scene.init()
+
+    def batch(self, scenes):
+        return sphg.Batch(self.ctx, scenes)
+
+    def step(self, scene, k: int) -> None:
+        scene.step(k)
+
+    def result(self, scene) -> np.ndarray:
+        return scene.get(sphg.FIELD_X)
+
+    def close(self, obj) -> None:
+        obj.close()
+
+
+@dataclass
+class Job:
+    req: Request
+    spec_future: object = None
+    scene: object = None
+    bytes: int = 0
+    n: int = 0
+    remaining: int = 0
+    admitted: float = -1.0
+    done: float = -1.0
+    result: Optional[np.ndarray] = None
+
+
+@dataclass
+class ServeResult:
+    mode: str
+    jobs: List[Job]
+    makespan: float
+    rejected: List[int] = field(default_factory=list)
+
+    def summary(self) -> dict:
+        done = [j for j in self.jobs if j.done >= 0]
+        lat = np.array([j.done - j.req.arrival for j in done]) if done else np.zeros(1)
+        work = float(sum(j.n * j.req.steps for j in done))
+        return {"mode": self.mode, "requests": len(done), "rejected": len(self.rejected),
+                "particle_steps": work, "makespan_s": round(self.makespan, 3),
+                "particle_steps_per_s": round(work / self.makespan, 1) if self.makespan > 0 else None,
+                "latency_p50_s": round(float(np.percentile(lat, 50)), 4),
+                "latency_p99_s": round(float(np.percentile(lat, 99)), 4),
+                "latency_max_s": round(float(lat.max()), 4)}
+
+
+def _spec_pool():
+    return ThreadPoolExecutor(max_workers=2, thread_name_prefix="spec")
+
+
+def serve_batched(backend, requests: List[Request], budget_bytes: int, quantum: int = 25,
+                  clock: Callable[[], float] = time.perf_counter, sleep: Callable[[float], None] = time.sleep,
+                  keep_results: bool = True, log: Callable[[str], None] = None) -> ServeResult:
+    """Continuous batching of `requests` (sorted by arrival) on `backend`."""
+    pending = deque(sorted(requests, key=lambda q: q.arrival))
+    waiting: deque = deque()
+    active: Dict[int, List[Job]] = {m: [] for m in METHODS}
+    batches: Dict[int, object] = {m: None for m in METHODS}
+    jobs: List[Job] = []
+    rejected: List[int] = []
+    used = 0
+    pool = _spec_pool()
+    t0 = clock()
+    try:
+        while pending or waiting or any(active.values()):
+            now = clock() - t0
+            while pending and pending[0].arrival <= now:
+                j = Job(pending.popleft())
+                j.spec_future = pool.submit(build_spec, j.req)
+                waiting.append(j)
+                jobs.append(j)
+            # admission, first come first served, by device bytes
+            while waiting:
+                j = waiting[0]
+                if j.scene is None:
+                    spec = j.spec_future.result()
+                    j.scene = backend.create(spec)
+                    j.n = spec.n
+                    j.bytes = backend.plan_bytes(j.scene)
+                if j.bytes > budget_bytes:
+                    backend.close(j.scene)
+                    j.scene = None
+                    rejected.append(j.req.rid)
+                    waiting.popleft()
+                    continue
+                if used + j.bytes > budget_bytes:
+                    break
+                backend.init(j.scene)
+                used += j.bytes
+                j.remaining = j.req.steps
+                j.admitted = clock() - t0
+                m = j.req.method
+                if batches[m] is not None:
+                    backend.close(batches[m])
+                    batches[m] = None
+                active[m].append(j)
+                waiting.popleft()
+            if not any(active.values()):
+                if pending:
+                    sleep(max(0.0, pending[0].arrival - (clock() - t0)))
+                continue
+            for m in METHODS:
+                js = active[m]
+                if not js:
+                    continue
+                if batches[m] is None:
+                    batches[m] = backend.batch([j.scene for j in js])
+                k = min(quantum, min(j.remaining for j in js))
+                batches[m].run(k)
+                for j in js:
+                    j.remaining -= k
+                fin = [j for j in js if j.remaining == 0]
+                if fin:
+                    backend.close(batches[m])
+                    batches[m] = None
+                    for j in fin:
+                        res = backend.result(j.scene)
+                        j.result = res if keep_results else None
+                        backend.close(j.scene)
+                        j.scene = None
+                        used -= j.bytes
+                        j.done = clock() - t0
+                        if log:
+                            log(f"[serve] done req {j.req.rid} {METHOD_NAMES[m]} n={j.n} steps={j.req.steps} "
+                                f"latency {j.done - j.req.arrival:.3f}s")
+                    active[m] = [j for j in js if j.remaining > 0]
+    finally:
+        for m in METHODS:
+            if batches[m] is not None:
+                backend.close(batches[m])
+        for js in active.values():
+            for j in js:
+                if j.scene is not None:
+                    backend.close(j.scene)
+        for j in waiting:
+            if j.scene is not None:
+                backend.close(j.scene)
+        pool.shutdown(wait=True)
+    done = [j.done for j in jobs if j.done >= 0]
+    first = min(q.arrival for q in requests) if requests else 0.0
+    return ServeResult("batched", jobs, (max(done) - first) if done else 0.0, rejected)
+
+
+def serve_plain(backend, requests: List[Request], budget_bytes: int,
+                clock: Callable[[], float] = time.perf_counter, sleep: Callable[[float], None] = time.sleep,
+                keep_results: bool = True, log: Callable[[str], None] = None) -> ServeResult:
+    """The same requests one at a time, in arrival order, each scene alone."""
+    order = sorted(requests, key=lambda q: q.arrival)
+    jobs: List[Job] = []
+    rejected: List[int] = []
+    pool = _spec_pool()
+    t0 = clock()
+    try:
+        futures = {}
+        for q in order:
+            now = clock() - t0
+            if q.arrival > now:
+                sleep(q.arrival - now)
+            # the geometry of the next arrived requests is built meanwhile
+            for q2 in order:
+                if q2.arrival <= clock() - t0 and q2.rid not in futures:
+                    futures[q2.rid] = pool.submit(build_spec, q2)
+            if q.rid not in futures:
+                futures[q.rid] = pool.submit(build_spec, q)
+            j = Job(q)
+            jobs.append(j)
+            spec = futures.pop(q.rid).result()
+            j.n = spec.n
+            scene = backend.create(spec)
+            j.bytes = backend.plan_bytes(scene)
+            if j.bytes > budget_bytes:
+                backend.close(scene)
+                rejected.append(q.rid)
+                continue
+            backend.init(scene)
+            j.admitted = clock() - t0
+            backend.step(scene, q.steps)
+            res = backend.result(scene)
+            j.result = res if keep_results else None
+            backend.close(scene)
+            j.done = clock() - t0
+            if log:
+                log(f"[plain] done req {q.rid} {METHOD_NAMES[q.method]} n={j.n} steps={q.steps} "
+                    f"latency {j.done - q.arrival:.3f}s")
+    finally:
+        pool.shutdown(wait=True)
+    done = [j.done for j in jobs if j.done >= 0]
+    first = min(q.arrival for q in requests) if requests else 0.0
+    return ServeResult("plain", jobs, (max(done) - first) if done else 0.0, rejected)
+
+
+def main():
+    ap = argparse.ArgumentParser(description="Serving benchmark: continuous batching vs plain, one GPU")
+    ap.add_argument("--requests", type=int, default=48)
+    ap.add_argument("--rate", type=float, default=2.0, help="Poisson arrival rate, requests per second")
+    ap.add_argument("--nmin", type=int, default=2048)
+    ap.add_argument("--nmax", type=int, default=262144)
+    ap.add_argument("--smin", type=int, default=100)
+    ap.add_argument("--smax", type=int, default=500)
+    ap.add_argument("--budget-gb", type=float, default=200.0)
+    ap.add_argument("--quantum", type=int, default=25)
+    ap.add_argument("--seed", type=int, default=5)
+    ap.add_argument("--check", type=int, default=4, help="requests whose batched and plain results are compared")
+    args = ap.parse_args()
+
+    reqs = request_stream(args.requests, args.rate, args.nmin, args.nmax, args.smin, args.smax, args.seed)
+    ctx = sphg.Context(0)
+    be = GpuBackend(ctx)
+    budget = int(args.budget_gb * 2 ** 30)
+    log = lambda s: print(s, flush=True)   # noqa: E731
+    rb = serve_batched(be, reqs, budget, args.quantum, log=log)
+    rp = serve_plain(be, reqs, budget, log=log)
+    sb, sp = rb.summary(), rp.summary()
+    # the first `check` requests: identical results alone and in the mix
+    same = 0
+    checked = 0
+    byid_b = {j.req.rid: j for j in rb.jobs}
+    for j in rp.jobs[: args.check]:
+        jb = byid_b.get(j.req.rid)
+        if jb is not None and jb.result is not None and j.result is not None:
+            checked += 1
+            same += int(np.array_equal(jb.result, j.result))
+    out = {"metric": "served-scene p50/p99 latency and particle-steps/s, continuous batching vs plain GPU",
+           "stream": {"requests": args.requests, "rate_per_s": args.rate, "n": [args.nmin, args.nmax],
+                      "steps": [args.smin, args.smax], "methods": "WCSPH/PCISPH/DFSPH uniform",
+                      "geometry": "dam-break block or sphere drop, spacing 0.02", "seed": args.seed,
+                      "budget_gb": args.budget_gb, "quantum": args.quantum},
+           "batched": sb, "plain": sp,
+           "throughput_ratio": round(sb["particle_steps_per_s"] / sp["particle_steps_per_s"], 3)
+           if sb["particle_steps_per_s"] and sp["particle_steps_per_s"] else None,
+           "identical_results": f"{same}/{checked}"}
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

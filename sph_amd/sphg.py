@@ -178,6 +178,10 @@ def load(path: str = LIB_PATH) -> C.CDLL:
         "sphg_scene_get_neighbors": (C.c_int, [P, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), U32]),
         "sphg_scene_info": (C.c_int, [P, C.POINTER(U32), C.POINTER(U32), C.POINTER(U32), C.POINTER(U64)]),
         "sphg_batch_step": (C.c_int, [P, C.POINTER(P), U32, U32, C.POINTER(StepStats)]),
+        "sphg_batch_create": (C.c_int, [P, C.POINTER(P), U32, C.POINTER(P)]),
+        "sphg_batch_run": (C.c_int, [P, U32, C.POINTER(StepStats)]),
+        "sphg_batch_destroy": (C.c_int, [P]),
+        "sphg_scene_plan_bytes": (C.c_int, [P, C.POINTER(U64)]),
         "sphg_ctx_kernel_timing": (C.c_int, [P, I32]),
         "sphg_ctx_kernel_time": (C.c_int, [P, C.c_char_p, C.POINTER(U64), dp]),
         "sphg_sample_box": (I64, [dp, dp, D, fp, I64]),
@@ -312,6 +316,31 @@ class Context:
         return int(n.value), float(ms.value)
 
 
+class Batch:
+    """Scenes of one method stepped together, kept between calls (sphg_batch).
+
+    Close it before closing any of its scenes."""
+
+    def __init__(self, ctx: "Context", scenes: Sequence["Scene"]):
+        self.ctx = ctx
+        self.lib = ctx.lib
+        self.scenes = list(scenes)
+        arr = (C.c_void_p * len(self.scenes))(*[s.h for s in self.scenes])
+        h = C.c_void_p()
+        _check(self.lib.sphg_batch_create(ctx.h, arr, len(self.scenes), C.byref(h)))
+        self.h = h
+
+    def run(self, nsteps: int = 1) -> StepStats:
+        st = StepStats()
+        _check(self.lib.sphg_batch_run(self.h, int(nsteps), C.byref(st)))
+        return st
+
+    def close(self) -> None:
+        if self.h:
+            self.lib.sphg_batch_destroy(self.h)
+            self.h = None
+
+
 class Scene:
     """One fluid + Akinci boundary scene resident on the device (sphg_scene)."""
 
@@ -333,6 +362,16 @@ class Scene:
         _check(self.lib.sphg_scene_set_boundary(self.h, _fptr(b), self.nb))
         if init:
             _check(self.lib.sphg_scene_init(self.h))
+
+    def init(self) -> None:
+        """Allocate and initialise on the device (when created with init=False)."""
+        _check(self.lib.sphg_scene_init(self.h))
+
+    def plan_bytes(self) -> int:
+        """Device bytes the scene holds once initialised (known before init)."""
+        b = C.c_uint64()
+        _check(self.lib.sphg_scene_plan_bytes(self.h, C.byref(b)))
+        return int(b.value)
 
     def close(self) -> None:
         if self.h:

@@ -1,0 +1,51 @@
+"""Continuous batching on the device: every request's result is the same
+alone (plain serving) and in the mix (batched serving), bit for bit."""
+import numpy as np
+import pytest
+
+from sph_amd import configs, serve, sphg
+
+pytestmark = pytest.mark.gpu
+
+
+def test_persistent_batch_quanta_match_alone(ctx):
+    specs = [configs.small_scene(sphg.DFSPH, n_side=n, seed=s) for n, s in ((6, 11), (9, 12))]
+    alone = [sphg.Scene(ctx, s.params, s.fluid, s.boundary) for s in specs]
+    for a in alone:
+        a.step(9)
+    mixed = [sphg.Scene(ctx, s.params, s.fluid, s.boundary) for s in specs]
+    b = sphg.Batch(ctx, mixed)
+    for k in (2, 3, 4):              # 9 steps in three quanta
+        b.run(k)
+    b.close()
+    for a, m in zip(alone, mixed):
+        assert np.array_equal(a.get(sphg.FIELD_X), m.get(sphg.FIELD_X))
+        assert np.array_equal(a.get(sphg.FIELD_V), m.get(sphg.FIELD_V))
+    for s in alone + mixed:
+        s.close()
+
+
+def test_plan_bytes_matches_allocation(ctx):
+    spec = configs.small_scene(sphg.PCISPH, n_side=8)
+    s = sphg.Scene(ctx, spec.params, spec.fluid, spec.boundary, init=False)
+    planned = s.plan_bytes()
+    s.init()
+    assert s.info()["device_bytes"] == planned > 0
+    s.close()
+
+
+def test_served_results_same_alone_and_in_mix(ctx):
+    reqs = serve.request_stream(6, rate=500.0, nmin=2048, nmax=6000, smin=12, smax=30, seed=21)
+    be = serve.GpuBackend(ctx)
+    rb = serve.serve_batched(be, reqs, 8 << 30, quantum=5)
+    rp = serve.serve_plain(be, reqs, 8 << 30)
+    assert not rb.rejected and not rp.rejected
+    byid = {j.req.rid: j for j in rb.jobs}
+    methods = set()
+    for j in rp.jobs:
+        jb = byid[j.req.rid]
+        assert j.result is not None and jb.result is not None
+        assert np.all(np.isfinite(j.result))
+        assert np.array_equal(j.result, jb.result), j.req
+        methods.add(j.req.method)
+    assert len(methods) >= 2
