@@ -398,6 +398,17 @@ int loop_more(sphg_ctx* ctx, int it, int ahead, bool& more, int& seen, uint32_t&
     return rc;
 }
 
+// Iterations of a density loop launched before the first read-back: the
+// count the loop needed in the previous step plus one.  The count moves by
+// about one between steps; an iteration that a converged scene does not need
+// is three launches that return at once (~12 us), a read-back ~40-50 us of
+// idle device.  The divergence loop (one iteration nearly always) gets no
+// slack.
+constexpr int kDensitySlack = 1;
+int density_ahead(const Batch& B) {
+    return std::max(B.minIt, B.lastIt > 0 ? B.lastIt + kDensitySlack : 0);
+}
+
 // One step of every scene in B (all of one method).
 int batch_one_step(sphg_ctx* ctx, Batch& B, uint32_t& syncs) {
     Profiler* P = &ctx->prof;
@@ -427,7 +438,7 @@ int batch_one_step(sphg_ctx* ctx, Batch& B, uint32_t& syncs) {
         finalize(ctx, B, T, 1);
         int seen = 0;
         for (int it = 0; it < B.maxIt; ++it) {
-            if ((rc = loop_more(ctx, it, std::max(B.minIt, B.lastIt), more, seen, syncs))) return rc;
+            if ((rc = loop_more(ctx, it, density_ahead(B), more, seen, syncs))) return rc;
             if (!more) break;
             { Scope s(P, st, K_DFVEL_DENS); launch_df_vel(st, 1, T, B.dP, B.nP); }
             { Scope s(P, st, K_DFERR_DENS); launch_df_err(st, 1, T, B.dP, B.nP); }
@@ -475,7 +486,7 @@ int batch_one_step(sphg_ctx* ctx, Batch& B, uint32_t& syncs) {
         { Scope s(P, st, K_FORCES); launch_forces(st, PCISPH, T2, B.dP, B.nP); }
         int seen = 0;
         for (int it = 0; it < B.maxIt; ++it) {
-            if ((rc = loop_more(ctx, it, std::max(B.minIt, B.lastIt), more, seen, syncs))) return rc;
+            if ((rc = loop_more(ctx, it, density_ahead(B), more, seen, syncs))) return rc;
             if (!more) break;
             { Scope s(P, st, K_PCIDENS); launch_pci_density(st, T2, B.dP, B.nP); }
             finalize(ctx, B, T2, 5);

@@ -493,18 +493,69 @@ __global__ __launch_bounds__(kBlock) void k_bvolume_pack(const DevScene* __restr
 // every lane of a wavefront, instead of running the FP64 path for every
 // candidate that any lane of the wavefront accepts.  A particle with more
 // survivors than list slots takes the one-pass walk over all candidates.
+//
+// Pass 1 reads its candidates from LDS.  The 64 particles of a wavefront are
+// sorted by cell, so they span one cell range [cF, cL], and each of a lane's
+// 9 candidate rows (cells c-1..c+1 of row (dy, dz)) lies inside that row's
+// range cF-1..cL+1 for the whole wavefront.  The wavefront copies those 9
+// particle ranges (~800 positions in a dense fluid, 12 B each) into LDS
+// once, instead of every lane loading its ~216 candidates from memory.  A
+// wavefront whose ranges exceed the stage reads global memory.
+constexpr int kWaveStage = 816;   // candidates per wavefront: 4 x 9.8 KB per block
+
 template <int M>
 __global__ __launch_bounds__(kBlock) void k_build(const DevScene* __restrict__ T, const Chunk* __restrict__ C) {
     const Chunk c = C[blockIdx.x];
     const DevScene& S = T[c.scene];
+    __shared__ float2 sxy[kBlock / 64][kWaveStage];
+    __shared__ float sz[kBlock / 64][kWaveStage];
     const int i = c.begin + threadIdx.x;
     const bool valid = i < S.n;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const KCoefD k = S.kd;
     const float hh = S.kc.hhi2 * (1.0f + 1e-5f);   // conservative FP32 pre-filter
     const float4 xi = valid ? ld4(S.x, i) : make_float4(0.f, 0.f, 0.f, 0.f);
     const double xx = xi.x, xy = xi.y, xz = xi.z;
     const int ci = valid ? ldg(S.cell, (size_t)i) : 0;
     const int nx = S.nx, nxny = S.nx * S.ny;
+
+    // the wavefront's 9 row ranges: first particle and offset in the stage
+    const int wb = c.begin + (w << 6);
+    const bool live = wb < S.n;
+    int rfirst = 0, rcnt = 0;
+    if (live && lane < 9) {
+        const int cF = ldg(S.cell, (size_t)wb), cL = ldg(S.cell, (size_t)min(wb + 63, S.n - 1));
+        const int off = (lane % 3 - 1) * nx + (lane / 3 - 1) * nxny;
+        rfirst = ldg(S.cellStart, (size_t)(cF - 1 + off));
+        rcnt = ldg(S.cellStart, (size_t)(cL + 2 + off)) - rfirst;
+    }
+    int incl = rcnt;
+#pragma unroll
+    for (int o = 1; o < 16; o <<= 1) {
+        const int t = __shfl_up(incl, o);
+        if (lane >= o) incl += t;
+    }
+    int first[9], base[10];
+    base[0] = 0;
+#pragma unroll
+    for (int r = 0; r < 9; ++r) {
+        first[r] = __builtin_amdgcn_readlane(rfirst, r);
+        base[r + 1] = __builtin_amdgcn_readlane(incl, r);
+    }
+    const bool staged = live && base[9] <= kWaveStage;
+    if (staged) {
+        for (int t = lane; t < base[9]; t += 64) {
+            int f = first[0], b0 = 0;
+#pragma unroll
+            for (int q = 1; q < 9; ++q)
+                if (t >= base[q]) { f = first[q]; b0 = base[q]; }
+            const float4 p = ld4(S.x, (size_t)(f + (t - b0)));
+            sxy[w][t] = make_float2(p.x, p.y);
+            sz[w][t] = p.z;
+        }
+    }
+    __syncthreads();
+
     uint32_t* L = list_of_w(S, valid ? i : (i & ~63));
     const int maxN = S.maxNbr;
     int cnt = 0, ngr = 0;
@@ -529,15 +580,38 @@ __global__ __launch_bounds__(kBlock) void k_build(const DevScene* __restrict__ T
         sgz += g * p.dz;
         ssq += g * g * p.r2;
     };
-    if (valid) {
+    // pass 1 over the 27 cells as 9 rows (walk27 order), candidate positions
+    // from `fetch(j, row)`
+    auto pass1 = [&](auto&& fetch) {
         int nc = 0;
-        walk27(S.cellStart, ci, nx, nxny, [&](int j) {
-            const float4 xj = ld4(S.x, j);
-            const float fx = xi.x - xj.x, fy = xi.y - xj.y, fz = xi.z - xj.z;
-            if (!(fx * fx + fy * fy + fz * fz <= hh) || j == i) return;
-            if (nc < maxN) stg(L, (size_t)nc * kGroup, (uint32_t)j);
-            ++nc;
-        });
+#pragma unroll
+        for (int r = 0; r < 9; ++r) {
+            const int off = (r % 3 - 1) * nx + (r / 3 - 1) * nxny;
+            const int s0 = ldg(S.cellStart, (size_t)(ci - 1 + off)), e0 = ldg(S.cellStart, (size_t)(ci + 2 + off));
+            for (int j = s0; j < e0; ++j) {
+                const float3 q = fetch(j, r);
+                const float fx = xi.x - q.x, fy = xi.y - q.y, fz = xi.z - q.z;
+                if (!(fx * fx + fy * fy + fz * fz <= hh) || j == i) continue;
+                if (nc < maxN) stg(L, (size_t)nc * kGroup, (uint32_t)j);
+                ++nc;
+            }
+        }
+        return nc;
+    };
+    if (valid) {
+        int nc;
+        if (staged) {
+            nc = pass1([&](int j, int r) {
+                const int t = j + (base[r] - first[r]);
+                const float2 a = sxy[w][t];
+                return make_float3(a.x, a.y, sz[w][t]);
+            });
+        } else {
+            nc = pass1([&](int j, int) {
+                const float4 p = ld4(S.x, (size_t)j);
+                return make_float3(p.x, p.y, p.z);
+            });
+        }
         if (nc <= maxN) {
             for (int q = 0; q < nc; ++q) {
                 const uint32_t j = ldg((const uint32_t*)L, (size_t)q * kGroup);

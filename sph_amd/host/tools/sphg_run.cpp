@@ -3,13 +3,16 @@
 // src/InputParser.h) without the GLUT window.
 //
 //   sphg_run -f scene.json [-o true|false] [--steps N] [--info] [--device D]
+//            [--final state.json]
 //
 // --info parses and builds the scene and prints it without touching a GPU.
 // Otherwise the scene is imported (Simulation::importScene), stepped until
 // endTime (or N steps), writing <scene>_NNNN.json frames when -o true, and
 // a final JSON line with steps, simulated time and device milliseconds.
+// --final writes the last state as one frame file (the writeFluid format).
 #include <cstdio>
 #include <cstring>
+#include <fstream>
 #include <string>
 #include <vector>
 
@@ -36,7 +39,8 @@ int main(int argc, char** argv) {
     for (int i = 1; i < argc; ++i) tokens.emplace_back(argv[i]);
     const std::string path = option("-f");
     if (path.empty()) {
-        std::fprintf(stderr, "usage: sphg_run -f scene.json [-o true|false] [--steps N] [--info] [--device D]\n");
+        std::fprintf(stderr, "usage: sphg_run -f scene.json [-o true|false] [--steps N] [--info] [--device D] "
+                             "[--final state.json]\n");
         return 2;
     }
     if (path.find("json") == std::string::npos) {
@@ -70,6 +74,11 @@ int main(int argc, char** argv) {
         std::fprintf(stderr, "%s", sim->describe().c_str());
         const unsigned int nsteps = option("--steps").empty() ? 0u : (unsigned int)std::stoul(option("--steps"));
         sim->run(nsteps);
+        if (!option("--final").empty()) {
+            std::ofstream f(option("--final"));
+            if (!f.is_open()) throw SphError("cannot write " + option("--final"));
+            f << SceneLoader::frameJson();
+        }
         auto* gs = dynamic_cast<GpuSolver*>(sim->getSolver());
         const sphg_scene_scalars& s = gs->scalars();
         std::printf("{\"steps\": %u, \"time\": %.17g, \"time_step\": %.17g, \"frames\": %u, \"device_ms\": %.3f, "

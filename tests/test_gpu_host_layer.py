@@ -1,0 +1,67 @@
+"""Drop-in check of the host layer on the device: a scene JSON run through
+Simulation::importScene -> GpuSolver (the sphg_run front end) ends in the
+same state, bit for bit, as the same scene built through the Python binding
+of the C ABI."""
+import json
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from sph_amd import configs, sphg
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+RUN = os.path.join(ROOT, "sph_amd", "sphg_run")
+SCENES = os.path.join(ROOT, "sph_amd", "scenes")
+STEPS = 5
+
+
+def _cli_state(tmp_path, name):
+    out = tmp_path / "final.json"
+    p = subprocess.run([RUN, "-f", os.path.join(SCENES, name), "--steps", str(STEPS), "--final", str(out)],
+                       capture_output=True, text=True, timeout=110)
+    assert p.returncode == 0, p.stderr
+    summary = json.loads(p.stdout.strip().splitlines()[-1])
+    assert summary["steps"] == STEPS
+    with open(out) as f:
+        st = json.load(f)
+    return np.array(st["positions"], dtype=np.float64), np.array(st["velocities"], dtype=np.float64), summary
+
+
+def _api_state(ctx, spec):
+    s = sphg.Scene(ctx, spec.params, spec.fluid, spec.boundary)
+    s.step(STEPS)
+    x, v = s.get(sphg.FIELD_X), s.get(sphg.FIELD_V)
+    s.close()
+    return x, v
+
+
+def _pcisph_drop():
+    r, h = 0.025, 0.1
+    fluid = sphg.fluid_spheres((0.0, 0.0, 0.0), (1.0, 1.0, 1.0), r, [(0.5, 0.5, 0.6)], [0.25])
+    boundary = np.concatenate([sphg.sample_box((0.0, 0.0, 0.0), (1.0, 1.0, 1.0), h),
+                               sphg.sample_sphere((0.5, 0.5, 0.15), 0.1, r)])
+    p = sphg.SceneParams(method=sphg.PCISPH, particle_radius=r, time_step=5e-4, min_time_step=5e-4,
+                         max_time_step=5e-4, fps=50.0, cfl_factor=1.0, eta=0.01, min_iterations=3,
+                         max_iterations=100, viscosity_method=sphg.VISCOSITY_XSPH, viscosity=0.01,
+                         surface_tension_method=sphg.SURFACE_TENSION_AKINCI, surface_tension=0.05,
+                         adhesion_method=sphg.ADHESION_AKINCI, adhesion=0.1)
+    return configs.SceneSpec("drop", p, fluid, np.ascontiguousarray(boundary))
+
+
+@pytest.mark.parametrize("name", ["dam_break_dfsph_small.json", "sphere_drop_pcisph.json", "dam_break_wcsph_32k.json"])
+def test_scene_json_matches_c_abi(ctx, tmp_path, name):
+    if name.startswith("dam_break_dfsph"):
+        spec = configs.small_scene(sphg.DFSPH, n_side=8, spacing=0.05, jitter=0.01, seed=7)
+    elif name.startswith("sphere_drop"):
+        spec = _pcisph_drop()
+    else:
+        spec = configs.wcsph_dam_32k()
+    xc, vc, _ = _cli_state(tmp_path, name)
+    xa, va = _api_state(ctx, spec)
+    assert xc.shape == xa.shape
+    assert np.array_equal(xc, xa.astype(np.float64))
+    assert np.array_equal(vc, va.astype(np.float64))

@@ -1,0 +1,81 @@
+"""The C++ host layer (reference class surface + scene JSON) on the CPU:
+scene parsing through the sphg_run front end (--info touches no GPU) and its
+error paths.  The GPU run of the same scenes is in test_gpu_host_layer.py."""
+import json
+import os
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+RUN = os.path.join(ROOT, "sph_amd", "sphg_run")
+SCENES = os.path.join(ROOT, "sph_amd", "scenes")
+
+
+def _run(*args):
+    if not os.path.exists(RUN):
+        pytest.fail("sph_amd/sphg_run is not built (make -C sph_amd/host)")
+    return subprocess.run([RUN, *args], capture_output=True, text=True, timeout=60)
+
+
+@pytest.mark.parametrize("name,method,blocks,spheres,visc,st,adh", [
+    ("dam_break_dfsph_small.json", 2, 1, 0, -1, -1, -1),
+    ("dam_break_wcsph_32k.json", 0, 1, 0, 2, -1, -1),
+    ("sphere_drop_pcisph.json", 1, 0, 1, 2, 3, 4),
+])
+def test_scene_info(name, method, blocks, spheres, visc, st, adh):
+    p = _run("-f", os.path.join(SCENES, name), "--info")
+    assert p.returncode == 0, p.stderr
+    info = json.loads(p.stdout)
+    assert info["simulationMethod"] == method and info["boundaryMethod"] == 2
+    assert info["particleRadius"] == 0.025
+    assert (info["fluidBlocks"], info["fluidSpheres"]) == (blocks, spheres)
+    assert (info["viscosityMethod"], info["surfaceTensionMethod"], info["adhesionMethod"]) == (visc, st, adh)
+
+
+def _variant(tmp_path, edit):
+    with open(os.path.join(SCENES, "dam_break_dfsph_small.json")) as f:
+        d = json.load(f)
+    edit(d)
+    p = tmp_path / "scene.json"
+    p.write_text(json.dumps(d))
+    return str(p)
+
+
+def test_missing_required_key_is_an_error(tmp_path):
+    # every Configuration key the method reads is required (src/SceneLoader.cpp:38-80)
+    path = _variant(tmp_path, lambda d: d["Configuration"].pop("etaV"))
+    p = _run("-f", path, "--info")
+    assert p.returncode == 1 and "etaV" in p.stderr
+
+
+def test_wrong_type_is_an_error(tmp_path):
+    path = _variant(tmp_path, lambda d: d["Configuration"].__setitem__("particleRadius", "big"))
+    p = _run("-f", path, "--info")
+    assert p.returncode == 1 and "particleRadius" in p.stderr
+
+
+def test_device_other_than_gpu_is_refused(tmp_path):
+    path = _variant(tmp_path, lambda d: d["Configuration"].__setitem__("device", "cpu"))
+    p = _run("-f", path)
+    assert p.returncode == 1 and "GPU only" in p.stderr
+
+
+def test_unsupported_boundary_method_is_refused(tmp_path):
+    path = _variant(tmp_path, lambda d: d["Configuration"].__setitem__("boundaryMethod", 0))
+    p = _run("-f", path)
+    assert p.returncode == 1 and "boundaryMethod" in p.stderr
+
+
+def test_emitters_are_refused(tmp_path):
+    def edit(d):
+        d["Fluid"]["FluidModel"][0]["emitters"] = [{"position": [0, 0, 0], "rotation": [1, 0, 0, 0], "velocity": 1.0,
+                                                    "numParticles": 10, "startTime": 0.0, "type": 0, "width": 2,
+                                                    "height": 2, "spacing": 0.05}]
+    p = _run("-f", _variant(tmp_path, edit))
+    assert p.returncode == 1 and "emitters" in p.stderr
+
+
+def test_usage_and_extension():
+    assert _run().returncode == 2
+    assert _run("-f", "scene.txt").returncode == 2
