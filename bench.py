@@ -36,14 +36,16 @@ PMC_SUMMARY = os.path.join(ROOT, "profiles", "dfsph_1m_pmc.json")
 
 # Algorithmic bytes per fluid particle (FP32) of each kernel, and whether the
 # launch also counts one boundary sweep (16 B per boundary particle) and one
-# cell-table sweep (4 B per cell).
+# cell-table sweep (4 B per cell).  The DFSPH list build also forms the
+# divergence solve's initial error: 20 B (rho, alpha) + 36 B (divergence
+# error sweep).
 BALG = {
     "df_vel_density": (52, 1, 1),
     "df_vel_divergence": (52, 1, 1),
     "df_err_density": (36, 1, 1),
     "df_err_divergence": (36, 1, 1),
     "df_vel_warm": (44, 1, 1),
-    "build": (20, 1, 1),
+    "build": (56, 1, 1),
     "dt": (12, 0, 0),
     "df_positions": (40, 0, 0),
     "rank_reorder": (56, 0, 0),

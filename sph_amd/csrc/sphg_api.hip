@@ -448,9 +448,10 @@ int batch_one_step(sphg_ctx* ctx, Batch& B, uint32_t& syncs) {
         { Scope s(P, st, K_DFPOS); launch_df_positions(st, T, B.dP, B.nP); }
         grid_sort(ctx, B, T);
         const DevScene* T2 = B.dT[1];
-        { Scope s(P, st, K_BUILD); launch_build(st, DFSPH, T2, B.dP, B.nP); }
         launch_fill_int(st, ctx->dActive, 0, 2);
-        { Scope s(P, st, K_DFERR_DIV); launch_df_err(st, 2, T2, B.dP, B.nP); }
+        // the build also forms the divergence solve's initial error, kappa_v
+        // and the error partials (computeDivergenceError before the loop)
+        { Scope s(P, st, K_BUILD); launch_build(st, DFSPH, T2, B.dP, B.nP); }
         finalize(ctx, B, T2, 3);
         seen = 0;
         for (int it = 0; it < B.maxIt; ++it) {

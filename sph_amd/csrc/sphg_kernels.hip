@@ -501,7 +501,10 @@ __global__ __launch_bounds__(kBlock) void k_bvolume_pack(const DevScene* __restr
 // particle ranges (~800 positions in a dense fluid, 12 B each) into LDS
 // once, instead of every lane loading its ~216 candidates from memory.  A
 // wavefront whose ranges exceed the stage reads global memory.
-constexpr int kWaveStage = 816;   // candidates per wavefront: 4 x 9.8 KB per block
+#ifndef SPHG_BUILD_STAGE
+#define SPHG_BUILD_STAGE 1
+#endif
+constexpr int kWaveStage = SPHG_BUILD_STAGE ? 816 : 1;   // candidates per wavefront: 4 x 9.8 KB per block
 
 template <int M>
 __global__ __launch_bounds__(kBlock) void k_build(const DevScene* __restrict__ T, const Chunk* __restrict__ C) {
@@ -542,7 +545,7 @@ __global__ __launch_bounds__(kBlock) void k_build(const DevScene* __restrict__ T
         first[r] = __builtin_amdgcn_readlane(rfirst, r);
         base[r + 1] = __builtin_amdgcn_readlane(incl, r);
     }
-    const bool staged = live && base[9] <= kWaveStage;
+    const bool staged = SPHG_BUILD_STAGE && live && base[9] <= kWaveStage;
     if (staged) {
         for (int t = lane; t < base[9]; t += 64) {
             int f = first[0], b0 = 0;
@@ -560,8 +563,12 @@ __global__ __launch_bounds__(kBlock) void k_build(const DevScene* __restrict__ T
     const int maxN = S.maxNbr;
     int cnt = 0, ngr = 0;
     double ws = 0.0, sgx = 0.0, sgy = 0.0, sgz = 0.0, ssq = 0.0;
+    // DFSPH: sum_j g_ij (v_i - v_j).(x_i - x_j) of the divergence error
+    double dv = 0.0;
+    const float4 vi = (M == DFSPH && valid) ? ld4(S.v, i) : make_float4(0.f, 0.f, 0.f, 0.f);
+    const double ux = vi.x, uy = vi.y, uz = vi.z;
     // FP64 test and sums of one candidate that passed the FP32 test
-    auto visit = [&](uint32_t j, const float4& xj) {
+    auto visit = [&](uint32_t j, const float4& xj, const float4& vj) {
         const Pair p = pair_of(xx, xy, xz, xj, k);
         if (!(p.r2 <= k.hhi2)) return;
         const double g = G_d(p, k);
@@ -579,21 +586,31 @@ __global__ __launch_bounds__(kBlock) void k_build(const DevScene* __restrict__ T
         sgy += g * p.dy;
         sgz += g * p.dz;
         ssq += g * g * p.r2;
+        if (M == DFSPH) dv += g * ((ux - (double)vj.x) * p.dx + (uy - (double)vj.y) * p.dy + (uz - (double)vj.z) * p.dz);
     };
+    auto vel = [&](uint32_t j) { return (M == DFSPH) ? ld4(S.v, j) : make_float4(0.f, 0.f, 0.f, 0.f); };
     // pass 1 over the 27 cells as 9 rows (walk27 order), candidate positions
     // from `fetch(j, row)`
+    // (4 candidates per pass: their loads are issued before the first test)
     auto pass1 = [&](auto&& fetch) {
         int nc = 0;
 #pragma unroll
         for (int r = 0; r < 9; ++r) {
             const int off = (r % 3 - 1) * nx + (r / 3 - 1) * nxny;
             const int s0 = ldg(S.cellStart, (size_t)(ci - 1 + off)), e0 = ldg(S.cellStart, (size_t)(ci + 2 + off));
-            for (int j = s0; j < e0; ++j) {
-                const float3 q = fetch(j, r);
-                const float fx = xi.x - q.x, fy = xi.y - q.y, fz = xi.z - q.z;
-                if (!(fx * fx + fy * fy + fz * fz <= hh) || j == i) continue;
-                if (nc < maxN) stg(L, (size_t)nc * kGroup, (uint32_t)j);
-                ++nc;
+            for (int j0 = s0; j0 < e0; j0 += 4) {
+                float3 q[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) q[u] = fetch(min(j0 + u, e0 - 1), r);
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int j = j0 + u;
+                    const float fx = xi.x - q[u].x, fy = xi.y - q[u].y, fz = xi.z - q[u].z;
+                    if (j < e0 && j != i && fx * fx + fy * fy + fz * fz <= hh) {
+                        if (nc < maxN) stg(L, (size_t)nc * kGroup, (uint32_t)j);
+                        ++nc;
+                    }
+                }
             }
         }
         return nc;
@@ -613,16 +630,29 @@ __global__ __launch_bounds__(kBlock) void k_build(const DevScene* __restrict__ T
             });
         }
         if (nc <= maxN) {
-            for (int q = 0; q < nc; ++q) {
-                const uint32_t j = ldg((const uint32_t*)L, (size_t)q * kGroup);
-                visit(j, ld4(S.x, j));
+            // 4 survivors per pass, their list entries and positions loaded
+            // first; visit writes slots <= the one it reads, so entries
+            // already loaded are not overwritten before use
+            for (int q0 = 0; q0 < nc; q0 += 4) {
+                uint32_t jj[4];
+                float4 xj[4], vj[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) jj[u] = ldg((const uint32_t*)L, (size_t)min(q0 + u, nc - 1) * kGroup);
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    xj[u] = ld4(S.x, jj[u]);
+                    vj[u] = vel(jj[u]);
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+                    if (q0 + u < nc) visit(jj[u], xj[u], vj[u]);
             }
         } else {
             walk27(S.cellStart, ci, nx, nxny, [&](int j) {
                 const float4 xj = ld4(S.x, j);
                 const float fx = xi.x - xj.x, fy = xi.y - xj.y, fz = xi.z - xj.z;
                 if (!(fx * fx + fy * fy + fz * fz <= hh) || j == i) return;
-                visit((uint32_t)j, xj);
+                visit((uint32_t)j, xj, vel((uint32_t)j));
             });
         }
     }
@@ -636,7 +666,8 @@ __global__ __launch_bounds__(kBlock) void k_build(const DevScene* __restrict__ T
         for (int q = own; q < pad; ++q) stg(L, (size_t)q * kGroup, (uint32_t)i);
         if ((i & 63) == 0) stg(S.gcnt, (size_t)(i >> 6), (uint16_t)pad);
     }
-    if (!valid) return;
+    double e = 0.0;   // DFSPH: this particle's share of the average divergence error
+    if (valid) {
     double bw = 0.0, bgx = 0.0, bgy = 0.0, bgz = 0.0;
     double sx2 = 0.0, sy2 = 0.0, sz2 = 0.0;   // spiky boundary sum (PCISPH)
     double ax = 0.0, ay = 0.0, az = 0.0;      // adhesion sum
@@ -679,7 +710,19 @@ __global__ __launch_bounds__(kBlock) void k_build(const DevScene* __restrict__ T
     if (M == DFSPH) {
         const double tx = m * sgx + bgx, ty = m * sgy + bgy, tz = m * sgz + bgz;
         const double den = tx * tx + ty * ty + tz * tz + m * m * ssq;
-        stg(S.alpha, (size_t)i, (fabs(den) > 1e-5) ? (float)(rho / den) : 0.0f);
+        const float alpha = (fabs(den) > 1e-5) ? (float)(rho / den) : 0.0f;
+        stg(S.alpha, (size_t)i, alpha);
+        // initial divergence error of the step's divergence solve
+        // (computeDivergenceError, src/DFSPHSolver.cpp:408-492, and kappa_v,
+        // :531), from the pairs above; stored values are read back as the
+        // later sweeps read them (FP32 gb, delta rho, alpha)
+        const float4 gbf = make_float4((float)bgx, (float)bgy, (float)bgz, (float)bw);
+        const double dc = m * dv + (ux * (double)gbf.x + uy * (double)gbf.y + uz * (double)gbf.z);
+        const double de = (ngr < 20) ? 0.0 : fmax(dc, 0.0);
+        const double rhof = rho0 + (double)(float)drho;
+        stg(S.divErr, (size_t)i, (float)de);
+        st4(S.xk, i, make_float4(xi.x, xi.y, xi.z, (float)(de * (double)alpha / (S.ss->dt * rhof))));
+        e = de / rho0;
     } else if (M == WCSPH) {
         // p = max(0, kappa rho0/gamma ((rho/rho0)^gamma - 1)), written
         // (1+x)^g - 1 = expm1(g log1p(x)) with x = (rho - rho0)/rho0
@@ -691,6 +734,12 @@ __global__ __launch_bounds__(kBlock) void k_build(const DevScene* __restrict__ T
         st4(S.aux, i, make_float4(vi.x, vi.y, vi.z, (float)(m / rho)));
     } else {
         st4(S.gbs, i, make_float4((float)sx2, (float)sy2, (float)sz2, 0.f));
+    }
+    }
+    if (M == DFSPH) {
+        __shared__ double sh[4];
+        const double bs = block_sum(e, sh);
+        if (threadIdx.x == 0) S.partial[c.begin / kBlock] = bs;
     }
 }
 
