@@ -107,6 +107,9 @@ __device__ inline float4 ld4o(const float4* p, uint32_t j) {
     return make_float4(t.x, t.y, t.z, t.w);
 }
 template <class T> __device__ inline void stg(T* p, size_t i, T v) { ((gptr<T>)p)[i] = v; }
+template <class T> __device__ inline void stg(T* p, uint32_t i, T v) {
+    *(gptr<T>)((gptr<char>)p + (size_t)(i * (uint32_t)sizeof(T))) = v;
+}
 
 // 1/sqrt(r2), zero for r2 <= 1e-30 (coincident points contribute no gradient)
 __device__ inline double rsqrt_d(double r2) {
@@ -561,6 +564,10 @@ __global__ __launch_bounds__(kBlock) void k_build(const DevScene* __restrict__ T
 
     uint32_t* L = list_of_w(S, valid ? i : (i & ~63));
     const int maxN = S.maxNbr;
+    // the same slot as uniform base + 32-bit element offset (lists of a
+    // scene are far below 2^32 elements)
+    uint32_t* const Lg = S.nbr;
+    const uint32_t lbase = (uint32_t)(i >> 6) * (uint32_t)maxN * (uint32_t)kGroup + (uint32_t)(i & 63);
     int cnt = 0, ngr = 0;
     double ws = 0.0, sgx = 0.0, sgy = 0.0, sgz = 0.0, ssq = 0.0;
     // DFSPH: sum_j g_ij (v_i - v_j).(x_i - x_j) of the divergence error
@@ -602,12 +609,20 @@ __global__ __launch_bounds__(kBlock) void k_build(const DevScene* __restrict__ T
                 float3 q[4];
 #pragma unroll
                 for (int u = 0; u < 4; ++u) q[u] = fetch(min(j0 + u, e0 - 1), r);
+                bool ok[4];
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
                     const int j = j0 + u;
                     const float fx = xi.x - q[u].x, fy = xi.y - q[u].y, fz = xi.z - q[u].z;
-                    if (j < e0 && j != i && fx * fx + fy * fy + fz * fz <= hh) {
-                        if (nc < maxN) stg(L, (size_t)nc * kGroup, (uint32_t)j);
+                    // evaluated without short-circuit: one exec-mask section
+                    ok[u] = (j < e0) & (j != i) & (fx * fx + fy * fy + fz * fz <= hh);
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    if (ok[u]) {
+                        // past the last slot the particle takes the one-pass
+                        // walk, which rewrites its list: the slot is clamped
+                        stg(Lg, (uint32_t)min(nc, maxN - 1) * (uint32_t)kGroup + lbase, (uint32_t)(j0 + u));
                         ++nc;
                     }
                 }
