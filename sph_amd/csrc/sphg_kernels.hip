@@ -548,9 +548,17 @@ __global__ __launch_bounds__(kBlock) void k_build(const DevScene* __restrict__ T
         first[r] = __builtin_amdgcn_readlane(rfirst, r);
         base[r + 1] = __builtin_amdgcn_readlane(incl, r);
     }
-    const bool staged = SPHG_BUILD_STAGE && live && base[9] <= kWaveStage;
-    if (staged) {
-        for (int t = lane; t < base[9]; t += 64) {
+    // the leading rows that fit are staged (all 9 in a dense fluid); the
+    // rest of an oversized wavefront's rows are read from global memory
+    int nStaged = 0;
+    if (SPHG_BUILD_STAGE && live) {
+#pragma unroll
+        for (int r = 0; r < 9; ++r)
+            if (base[r + 1] <= kWaveStage) nStaged = r + 1;
+    }
+    {
+        const int tot = base[nStaged];
+        for (int t = lane; t < tot; t += 64) {
             int f = first[0], b0 = 0;
 #pragma unroll
             for (int q = 1; q < 9; ++q)
@@ -599,50 +607,47 @@ __global__ __launch_bounds__(kBlock) void k_build(const DevScene* __restrict__ T
     // pass 1 over the 27 cells as 9 rows (walk27 order), candidate positions
     // from `fetch(j, row)`
     // (4 candidates per pass: their loads are issued before the first test)
-    auto pass1 = [&](auto&& fetch) {
+    auto scanRow = [&](int r, int s0, int e0, int& nc, auto&& fetch) {
+        for (int j0 = s0; j0 < e0; j0 += 4) {
+            float3 q[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) q[u] = fetch(min(j0 + u, e0 - 1), r);
+            bool ok[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int j = j0 + u;
+                const float fx = xi.x - q[u].x, fy = xi.y - q[u].y, fz = xi.z - q[u].z;
+                // evaluated without short-circuit: one exec-mask section
+                ok[u] = (j < e0) & (j != i) & (fx * fx + fy * fy + fz * fz <= hh);
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                if (ok[u]) {
+                    // past the last slot the particle takes the one-pass
+                    // walk, which rewrites its list: the slot is clamped
+                    stg(Lg, (uint32_t)min(nc, maxN - 1) * (uint32_t)kGroup + lbase, (uint32_t)(j0 + u));
+                    ++nc;
+                }
+            }
+        }
+    };
+    auto ldsFetch = [&](int j, int r) {
+        const int t = j + (base[r] - first[r]);
+        const float2 a = sxy[w][t];
+        return make_float3(a.x, a.y, sz[w][t]);
+    };
+    auto gblFetch = [&](int j, int) {
+        const float4 p = ld4(S.x, (size_t)j);
+        return make_float3(p.x, p.y, p.z);
+    };
+    if (valid) {
         int nc = 0;
 #pragma unroll
         for (int r = 0; r < 9; ++r) {
             const int off = (r % 3 - 1) * nx + (r / 3 - 1) * nxny;
             const int s0 = ldg(S.cellStart, (size_t)(ci - 1 + off)), e0 = ldg(S.cellStart, (size_t)(ci + 2 + off));
-            for (int j0 = s0; j0 < e0; j0 += 4) {
-                float3 q[4];
-#pragma unroll
-                for (int u = 0; u < 4; ++u) q[u] = fetch(min(j0 + u, e0 - 1), r);
-                bool ok[4];
-#pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    const int j = j0 + u;
-                    const float fx = xi.x - q[u].x, fy = xi.y - q[u].y, fz = xi.z - q[u].z;
-                    // evaluated without short-circuit: one exec-mask section
-                    ok[u] = (j < e0) & (j != i) & (fx * fx + fy * fy + fz * fz <= hh);
-                }
-#pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    if (ok[u]) {
-                        // past the last slot the particle takes the one-pass
-                        // walk, which rewrites its list: the slot is clamped
-                        stg(Lg, (uint32_t)min(nc, maxN - 1) * (uint32_t)kGroup + lbase, (uint32_t)(j0 + u));
-                        ++nc;
-                    }
-                }
-            }
-        }
-        return nc;
-    };
-    if (valid) {
-        int nc;
-        if (staged) {
-            nc = pass1([&](int j, int r) {
-                const int t = j + (base[r] - first[r]);
-                const float2 a = sxy[w][t];
-                return make_float3(a.x, a.y, sz[w][t]);
-            });
-        } else {
-            nc = pass1([&](int j, int) {
-                const float4 p = ld4(S.x, (size_t)j);
-                return make_float3(p.x, p.y, p.z);
-            });
+            if (r < nStaged) scanRow(r, s0, e0, nc, ldsFetch);
+            else scanRow(r, s0, e0, nc, gblFetch);
         }
         if (nc <= maxN) {
             // 4 survivors per pass, their list entries and positions loaded
@@ -677,10 +682,6 @@ __global__ __launch_bounds__(kBlock) void k_build(const DevScene* __restrict__ T
     int pad = min((own + 3) & ~3, maxN);
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) pad = max(pad, __shfl_xor(pad, o));
-    if (valid) {
-        for (int q = own; q < pad; ++q) stg(L, (size_t)q * kGroup, (uint32_t)i);
-        if ((i & 63) == 0) stg(S.gcnt, (size_t)(i >> 6), (uint16_t)pad);
-    }
     double e = 0.0;   // DFSPH: this particle's share of the average divergence error
     if (valid) {
     double bw = 0.0, bgx = 0.0, bgy = 0.0, bgz = 0.0;
@@ -688,10 +689,9 @@ __global__ __launch_bounds__(kBlock) void k_build(const DevScene* __restrict__ T
     double ax = 0.0, ay = 0.0, az = 0.0;      // adhesion sum
     const bool wantAdh = (S.flags & F_ADH) != 0;
     const double rho0 = S.rho0D;
-    walk27(S.bCellStart, ci, nx, nxny, [&](int b) {
-        const float4 y = ld4(S.xb, b);
-        const float fx = xi.x - y.x, fy = xi.y - y.y, fz = xi.z - y.z;
-        if (!(fx * fx + fy * fy + fz * fz <= hh)) return;
+    // boundary sums, in two passes like the fluid: FP32 survivors go to the
+    // free list slots past the fluid list, then the FP64 pass runs over them
+    auto bvisit = [&](const float4& y) {
         const Pair p = pair_of(xx, xy, xz, y, k);
         if (!(p.q <= 1.0)) return;
         const double cb = rho0 * (double)y.w;
@@ -712,7 +712,38 @@ __global__ __launch_bounds__(kBlock) void k_build(const DevScene* __restrict__ T
             ay += sa * p.dy;
             az += sa * p.dz;
         }
+    };
+    const int room = maxN - own;
+    int nbc = 0;
+    walk27(S.bCellStart, ci, nx, nxny, [&](int b) {
+        const float4 y = ld4(S.xb, b);
+        const float fx = xi.x - y.x, fy = xi.y - y.y, fz = xi.z - y.z;
+        if (fx * fx + fy * fy + fz * fz <= hh) {
+            if (nbc < room) stg(Lg, (uint32_t)(own + nbc) * (uint32_t)kGroup + lbase, (uint32_t)b);
+            ++nbc;
+        }
     });
+    if (nbc <= room) {
+        for (int q0 = 0; q0 < nbc; q0 += 4) {
+            float4 yb[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                yb[u] = ld4(S.xb, ldg((const uint32_t*)L, (size_t)(own + min(q0 + u, nbc - 1)) * kGroup));
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                if (q0 + u < nbc) bvisit(yb[u]);
+        }
+    } else {
+        walk27(S.bCellStart, ci, nx, nxny, [&](int b) {
+            const float4 y = ld4(S.xb, b);
+            const float fx = xi.x - y.x, fy = xi.y - y.y, fz = xi.z - y.z;
+            if (fx * fx + fy * fy + fz * fz <= hh) bvisit(y);
+        });
+    }
+    // pad the list with the particle itself up to the group's longest list
+    // (overwrites the boundary survivors' slots)
+    for (int q = own; q < pad; ++q) stg(L, (size_t)q * kGroup, (uint32_t)i);
+    if ((i & 63) == 0) stg(S.gcnt, (size_t)(i >> 6), (uint16_t)pad);
     if (cnt > maxN) atomicMax(&S.ss->overflow, (unsigned)cnt);
     stg(S.nnb, (size_t)i, (uint16_t)own);
     stg(S.ngrad, (size_t)i, (uint8_t)min(ngr, 255));
