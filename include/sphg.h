@@ -37,7 +37,10 @@ enum {
  * (src/Simulation.cpp:21-33). */
 enum { SPHG_WCSPH = 0, SPHG_PCISPH = 1, SPHG_DFSPH = 2 };
 enum { SPHG_BOUNDARY_AKINCI = 2 };
-enum { SPHG_VISCOSITY_XSPH = 2, SPHG_SURFACE_TENSION_AKINCI = 3, SPHG_ADHESION_AKINCI = 4 };
+/* viscosityMethod (src/Simulation.h enums): standard SPH (src/StandardViscosityForce.h),
+ * artificial (src/ArtificialViscosity.h), XSPH (src/XSPHViscosity.h) */
+enum { SPHG_VISCOSITY_STANDARD = 0, SPHG_VISCOSITY_ARTIFICIAL = 1, SPHG_VISCOSITY_XSPH = 2 };
+enum { SPHG_SURFACE_TENSION_AKINCI = 3, SPHG_ADHESION_AKINCI = 4 };
 
 typedef struct sphg_ctx sphg_ctx;
 typedef struct sphg_scene sphg_scene;

@@ -212,7 +212,7 @@ typedef struct oracle_desc {
     double stiffness, gamma;
     double eta, eta_v;
     unsigned min_iterations, max_iterations, min_iterations_v, max_iterations_v;
-    int viscosity_method;    // -1 none, 2 XSPH
+    int viscosity_method;    // -1 none, 0 standard, 1 artificial, 2 XSPH
     double viscosity, boundary_viscosity;
     int surface_tension_method;  // -1 none, 3 Akinci
     double surface_tension;
@@ -361,6 +361,70 @@ struct Oracle {
             a[i] = ai;
         }
     }
+    // ViscoK::laplW (src/ViscoK.h:28-35): 45/(pi h^6) (h - |r|) for |r| <= h
+    double laplW(const V3& r) const {
+        const double mr = length(r);
+        return (mr <= h) ? (45.0 / (M_PI * h * h * h * h * h * h)) * (h - mr) : 0.0;
+    }
+    // src/StandardViscosityForce.h:30-76
+    void standardViscosity() {
+        const long n = (long)N();
+        const double nu = d.viscosity, nub = d.boundary_viscosity;
+        #pragma omp parallel for
+        for (long i = 0; i < n; ++i) {
+            const V3 ri = x[i], vi = v[i];
+            const Bucket& c = cellOf(ri);
+            V3 f;
+            for (const PointInfo& pj : c.neighbors) {
+                const uint32_t j = pj.id;
+                f += m * (v[j] - vi) / rho[j] * laplW(ri - x[j]);
+            }
+            f *= nu / rho0;
+            V3 ai = a[i] + f;
+            if (nub != 0.0) {
+                V3 fb;
+                for (const PointInfo& pb : c.bneighbors) {
+                    const uint32_t b = pb.id;
+                    fb += rho0 * vb[b] * (-1.0 * vi) / rho[i] * laplW(ri - xb[b]);
+                }
+                fb *= nub / rho0;
+                ai += fb;
+            }
+            a[i] = ai;
+        }
+    }
+    // src/ArtificialViscosity.h:30-84
+    void artificialViscosity() {
+        const long n = (long)N();
+        const double nu = d.viscosity, nub = d.boundary_viscosity;
+        const double h2 = h * h;
+        #pragma omp parallel for
+        for (long i = 0; i < n; ++i) {
+            const V3 ri = x[i], vi = v[i];
+            const double dens_i = rho[i];
+            const Bucket& c = cellOf(ri);
+            V3 ai = a[i];
+            for (const PointInfo& pj : c.neighbors) {
+                const uint32_t j = pj.id;
+                const V3 rirj = ri - x[j];
+                const double dotProd = dot(vi - v[j], rirj);
+                const double dens_j = rho[j];
+                const double sqDist = length(rirj) * length(rirj);
+                const double kij = 2 * rho0 / (dens_i + dens_j);
+                ai += kij * 10.0 * 0.1 * nu * (m / dens_j) * dotProd / (sqDist + 0.01 * h2) * K.gradW(rirj);
+            }
+            if (nub != 0.0) {
+                for (const PointInfo& pb : c.bneighbors) {
+                    const uint32_t b = pb.id;
+                    const V3 rirb = ri - xb[b];
+                    const double dotProd = dot(vi, rirb);
+                    const double sqDist = length(rirb) * length(rirb);
+                    ai += 10.0 * 0.1 * nub * (rho0 * vb[b] / dens_i) * dotProd / (sqDist + 0.01 * h2) * K.gradW(rirb);
+                }
+            }
+            a[i] = ai;
+        }
+    }
     // src/SurfaceTensionAkinci.h:34-98
     void surfaceTension() {
         const long n = (long)N();
@@ -421,6 +485,8 @@ struct Oracle {
     // adhesion (src/Simulation.cpp:414-423), and Simulation::
     // computeNonPressureForces runs them in that order (:313-323).
     void computeNonPressureForces() {
+        if (d.viscosity_method == 0 && d.viscosity != 0.0) standardViscosity();
+        if (d.viscosity_method == 1 && d.viscosity != 0.0) artificialViscosity();
         if (d.viscosity_method == 2 && d.viscosity != 0.0) xsph();
         if (d.surface_tension_method == 3 && d.surface_tension != 0.0) surfaceTension();
         if (d.adhesion_method == 4 && d.adhesion != 0.0) adhesionForce();

@@ -225,8 +225,8 @@ int check_desc(const sphg_scene_desc& d) {
     if (!(d.time_step > 0)) return fail(SPHG_EINVAL, "timeStep must be > 0");
     if (!(d.fps > 0)) return fail(SPHG_EINVAL, "fps must be > 0");
     if (d.method == 0 && !(d.gamma != 0)) return fail(SPHG_EINVAL, "WCSPH gamma must be nonzero");
-    if (d.viscosity_method >= 0 && d.viscosity_method != SPHG_VISCOSITY_XSPH && d.viscosity != 0.0)
-        return fail(SPHG_EINVAL, "only viscosityMethod 2 (XSPH) runs on the GPU solver");
+    if (d.viscosity_method > SPHG_VISCOSITY_XSPH && d.viscosity != 0.0)
+        return fail(SPHG_EINVAL, "viscosityMethod must be -1, 0 (standard), 1 (artificial) or 2 (XSPH)");
     if (d.max_neighbors > 65535) return fail(SPHG_EINVAL, "max_neighbors must be <= 65535");
     return SPHG_OK;
 }
@@ -251,6 +251,7 @@ void fill_coef_d(KCoefD& k, double h) {
     k.gA = 3.0 * k.clh2 / h;
     k.gB = -2.0 * k.clh2;
     k.hHalf = 0.5 * h;
+    k.vl = 45.0 / (M_PI * h * h * h * h * h * h);
 }
 
 void fill_coef(KCoef& k, double h) {
@@ -603,7 +604,13 @@ int make_plan(const sphg_scene* s, Plan& P) {
     const int maxNbr = ((d.max_neighbors ? (int)d.max_neighbors : kDefaultMaxNbr) + 3) & ~3;
 
     int flags = 0;
-    if (d.viscosity_method == SPHG_VISCOSITY_XSPH && d.viscosity != 0.0) flags |= F_XSPH;
+    // a viscosity force exists iff its coefficient is nonzero
+    // (src/Simulation.cpp:423-424)
+    if (d.viscosity != 0.0) {
+        if (d.viscosity_method == SPHG_VISCOSITY_XSPH) flags |= F_XSPH;
+        if (d.viscosity_method == SPHG_VISCOSITY_STANDARD) flags |= F_VSTD;
+        if (d.viscosity_method == SPHG_VISCOSITY_ARTIFICIAL) flags |= F_VART;
+    }
     if (d.surface_tension_method == SPHG_SURFACE_TENSION_AKINCI && d.surface_tension != 0.0) flags |= F_ST;
     if (d.adhesion_method == SPHG_ADHESION_AKINCI && d.adhesion != 0.0) flags |= F_ADH;
 

@@ -37,7 +37,9 @@ constexpr int kCellChunk = 4096;     // cells per scan chunk (256 threads x 16)
 constexpr int kCellPerThread = kCellChunk / kBlock;
 
 enum Method { WCSPH = 0, PCISPH = 1, DFSPH = 2 };
-enum Flags { F_XSPH = 1, F_ST = 2, F_ADH = 4, F_XSPH_B = 8 };
+enum Flags { F_XSPH = 1, F_ST = 2, F_ADH = 4, F_VSTD = 8, F_VART = 16 };
+// any non-pressure force: the step's acceleration a is nonzero
+constexpr int F_FORCES = F_XSPH | F_ST | F_ADH | F_VSTD | F_VART;
 
 // FP32 kernel coefficients (computed in FP64 on the host, then rounded).
 struct KCoef {
@@ -57,6 +59,7 @@ struct KCoefD {
     double ck, clh2, clh, spg, co1, co2;
     double hlo2, hhi2;   // h^2 (1 -/+ 1e-9): band of the operation-exact neighbor test
     double gA, gB, hHalf;  // cubic gradient for q <= 1/2 as gA r + gB; h/2
+    double vl;             // viscosity Laplacian 45/(pi h^6) (src/ViscoK.h:23)
 };
 
 // Device-resident control state of one scene (one writer at a time: the

@@ -827,6 +827,15 @@ __global__ __launch_bounds__(kBlock) void k_normals(const DevScene* __restrict__
 // explicit integration).
 //   pressure  src/WCSPHSolver.cpp:152-186
 //   XSPH      src/XSPHViscosity.h:45-67
+//   standard viscosity   src/StandardViscosityForce.h:37-74:
+//     a_i += nu/rho0 sum_j m (v_j - v_i)/rho_j lapl(r_ij)
+//          + nu_b/rho0 sum_b rho0 V_b (-v_i)/rho_i lapl(r_ib),
+//     lapl(r) = 45/(pi h^6) (h - r) (src/ViscoK.h:28-35)
+//   artificial viscosity src/ArtificialViscosity.h:40-81:
+//     a_i += sum_j k_ij nu (m/rho_j) (v_i - v_j).r_ij/(r^2 + 0.01 h^2) gradW_ij
+//          + sum_b nu_b (rho0 V_b/rho_i) v_i.r_ib/(r^2 + 0.01 h^2) gradW_ib,
+//     k_ij = 2 rho0/(rho_i + rho_j); the reference's factor 10 * 0.1 is 1
+//   (boundary terms only when nu_b != 0, walked here over the boundary cells)
 //   cohesion  src/SurfaceTensionAkinci.h:66-96
 //   adhesion  src/AdhesionForce.h:38-61
 //   integrate src/SPHSolver.cpp:154-167
@@ -843,6 +852,8 @@ __global__ __launch_bounds__(kBlock) void k_forces(const DevScene* __restrict__ 
     const KCoefD k = S.kd;
     const double dt = S.ss->dt;
     const bool xs = (S.flags & F_XSPH) != 0, st = (S.flags & F_ST) != 0;
+    const bool vstd = (S.flags & F_VSTD) != 0, vart = (S.flags & F_VART) != 0;
+    const bool vel = xs || vstd || vart;
     const float4 xi = (M == WCSPH) ? ld4(S.xk, i) : ld4(S.x, i);
     const float4 vi = (M == WCSPH) ? ld4(S.aux, i) : ld4(S.v, i);
     const double xx = xi.x, xy = xi.y, xz = xi.z;
@@ -854,13 +865,15 @@ __global__ __launch_bounds__(kBlock) void k_forces(const DevScene* __restrict__ 
     double px = 0.0, py = 0.0, pz = 0.0;   // pressure part
     double vx = 0.0, vy = 0.0, vz = 0.0;   // XSPH part (before nu/dt)
     double sx = 0.0, sy = 0.0, sz = 0.0;   // surface tension part (before gamma)
+    double wx = 0.0, wy = 0.0, wz = 0.0;   // standard / artificial viscosity part
     sweep4(
         S, i,
         [&](uint32_t j) {
             NbF r;
             r.x = (M == WCSPH) ? ld4(S.xk, j) : ld4(S.x, j);
-            if (xs) r.v = (M == WCSPH) ? ld4(S.aux, j) : ld4(S.v, j);
-            if (xs && M != WCSPH) r.d = ldg(S.drho, j);
+            if (vel) r.v = (M == WCSPH) ? ld4(S.aux, j) : ld4(S.v, j);
+            // rho_j: XSPH under WCSPH takes m/rho_j from aux.w instead
+            if ((vel && M != WCSPH) || vstd || vart) r.d = ldg(S.drho, j);
             if (st) r.n = ld4(S.nrm, j);
             return r;
         },
@@ -878,6 +891,22 @@ __global__ __launch_bounds__(kBlock) void k_forces(const DevScene* __restrict__ 
                 vx += w * ((double)d.v.x - vx0);
                 vy += w * ((double)d.v.y - vy0);
                 vz += w * ((double)d.v.z - vz0);
+            }
+            if (vstd) {
+                // m/rho_j lapl(r) (v_j - v_i); list members have r <= h
+                const double w = (m / (rho0 + (double)d.d)) * (k.vl * (k.h - p.rl));
+                wx += w * ((double)d.v.x - vx0);
+                wy += w * ((double)d.v.y - vy0);
+                wz += w * ((double)d.v.z - vz0);
+            }
+            if (vart) {
+                const double rhoj = rho0 + (double)d.d;
+                const double kij = 2.0 * rho0 / (rhoi + rhoj);
+                const double dot = (vx0 - (double)d.v.x) * p.dx + (vy0 - (double)d.v.y) * p.dy + (vz0 - (double)d.v.z) * p.dz;
+                const double f = kij * (m / rhoj) * dot / (p.rl * p.rl + 0.01 * k.h2) * G_d(p, k);
+                wx += f * p.dx;
+                wy += f * p.dy;
+                wz += f * p.dz;
             }
             if (st && p.r2 > 0.0) {
                 // list members satisfy |r| <= h exactly (FP64), the reference's
@@ -909,6 +938,34 @@ __global__ __launch_bounds__(kBlock) void k_forces(const DevScene* __restrict__ 
             ay -= fb * vy0;
             az -= fb * vz0;
         }
+    }
+    if (vstd || vart) {
+        const double nu = S.nu, nub = S.nub;
+        double bx = 0.0, by = 0.0, bz = 0.0;
+        if (nub != 0.0) {
+            walk27(S.bCellStart, ldg(S.cell, (size_t)i), S.nx, S.nx * S.ny, [&](int b) {
+                const float4 y = ld4(S.xb, b);
+                const Pair p = pair_of(xx, xy, xz, y, k);
+                if (!(p.q <= 1.0)) return;
+                const double cb = rho0 * (double)y.w;
+                if (vstd) {
+                    const double w = cb * (k.vl * (k.h - p.rl)) / rhoi;
+                    bx -= w * vx0;
+                    by -= w * vy0;
+                    bz -= w * vz0;
+                } else {
+                    const double dot = vx0 * p.dx + vy0 * p.dy + vz0 * p.dz;
+                    const double f = (cb / rhoi) * dot / (p.rl * p.rl + 0.01 * k.h2) * G_d(p, k);
+                    bx += f * p.dx;
+                    by += f * p.dy;
+                    bz += f * p.dz;
+                }
+            });
+        }
+        const double fv = vstd ? nu / rho0 : nu, fb = vstd ? nub / rho0 : nub;
+        ax += fv * wx + fb * bx;
+        ay += fv * wy + fb * by;
+        az += fv * wz + fb * bz;
     }
     if (st) {
         ax += (double)S.stCoef * sx;
@@ -975,7 +1032,7 @@ __global__ __launch_bounds__(kBlock) void k_dt(const DevScene* __restrict__ T, c
     if (i < S.n) {
         const float4 v = S.v[i];
         double ax = S.gx, ay = S.gy, az = S.gz;
-        if (S.flags & (F_XSPH | F_ST | F_ADH)) {
+        if (S.flags & F_FORCES) {
             const float4 a = S.a[i];
             ax = (double)a.x + ax;
             ay = (double)a.y + ay;
@@ -999,7 +1056,7 @@ __global__ __launch_bounds__(kBlock) void k_df_predict(const DevScene* __restric
     const double dt = S.ss->dt;
     float4 v = S.v[i];
     double ax = S.gx, ay = S.gy, az = S.gz;
-    if (S.flags & (F_XSPH | F_ST | F_ADH)) {
+    if (S.flags & F_FORCES) {
         const float4 a = S.a[i];
         ax = (double)a.x + ax;
         ay = (double)a.y + ay;

@@ -67,7 +67,9 @@ public:
     virtual void resize(const unsigned int) {}
 };
 
-class XSPHViscosity : public NonPressureForce {   // src/XSPHViscosity.h:13-74
+// The three viscosity forces (src/FluidModel.cpp:73-96) carry their two
+// coefficients; the device kernel applies them.
+class ViscosityForce : public NonPressureForce {
 protected:
     Real viscosity = 0, boundaryViscosity = 0;
 
@@ -76,6 +78,22 @@ public:
     void init(Real v, Real bv) { viscosity = v; boundaryViscosity = bv; }
     Real getViscosity() const { return viscosity; }
     Real getBoundaryViscosity() const { return boundaryViscosity; }
+    virtual int method() const = 0;   // SPHG_VISCOSITY_*
+};
+class StandardViscosityForce : public ViscosityForce {   // src/StandardViscosityForce.h:12-84
+public:
+    using ViscosityForce::ViscosityForce;
+    int method() const override { return SPHG_VISCOSITY_STANDARD; }
+};
+class ArtificialViscosity : public ViscosityForce {   // src/ArtificialViscosity.h:11-90
+public:
+    using ViscosityForce::ViscosityForce;
+    int method() const override { return SPHG_VISCOSITY_ARTIFICIAL; }
+};
+class XSPHViscosity : public ViscosityForce {   // src/XSPHViscosity.h:13-74
+public:
+    using ViscosityForce::ViscosityForce;
+    int method() const override { return SPHG_VISCOSITY_XSPH; }
 };
 
 class SurfaceTensionAkinci : public NonPressureForce {   // src/SurfaceTensionAkinci.h:15-104

@@ -61,12 +61,13 @@ void FluidModel::setMasses(Real m) { std::fill(mass.begin(), mass.end(), m); }
 void FluidModel::setViscosityForce(Real viscosity, Real boundaryViscosity) {
     Simulation* sim = Simulation::getCurrent();
     const int m = sim->getViscosityMethod();
-    if (m == Simulation::XSPH_VISCOSITY_METHOD) {
-        auto* f = new XSPHViscosity(this);
+    ViscosityForce* f = nullptr;
+    if (m == Simulation::STANDARD_VISCOSITY_METHOD) f = new StandardViscosityForce(this);
+    else if (m == Simulation::ARTIFICIAL_VISCOSITY_METHOD) f = new ArtificialViscosity(this);
+    else if (m == Simulation::XSPH_VISCOSITY_METHOD) f = new XSPHViscosity(this);
+    if (f) {
         f->init(viscosity, boundaryViscosity);
         npForces.emplace_back(f);
-    } else if (m == Simulation::STANDARD_VISCOSITY_METHOD || m == Simulation::ARTIFICIAL_VISCOSITY_METHOD) {
-        throw SphError("viscosityMethod " + std::to_string(m) + " is not available on the GPU solver (XSPH = 2 is)");
     }
 }
 
@@ -143,8 +144,8 @@ void GpuSolver::fillDesc(sphg_scene_desc& d) const {
         FluidModel* fm = sim->getFluidModel(0);
         for (unsigned int i = 0; i < fm->numberNonPressureForces(); ++i) {
             NonPressureForce* f = fm->getNonPressureForce(i);
-            if (auto* x = dynamic_cast<XSPHViscosity*>(f)) {
-                d.viscosity_method = SPHG_VISCOSITY_XSPH;
+            if (auto* x = dynamic_cast<ViscosityForce*>(f)) {
+                d.viscosity_method = x->method();
                 d.viscosity = x->getViscosity();
                 d.boundary_viscosity = x->getBoundaryViscosity();
             } else if (auto* s = dynamic_cast<SurfaceTensionAkinci*>(f)) {

@@ -17,7 +17,8 @@ LIB_PATH = os.path.join(_HERE, "libsphg.so")
 
 WCSPH, PCISPH, DFSPH = 0, 1, 2
 BOUNDARY_AKINCI = 2
-VISCOSITY_XSPH, SURFACE_TENSION_AKINCI, ADHESION_AKINCI = 2, 3, 4
+VISCOSITY_STANDARD, VISCOSITY_ARTIFICIAL, VISCOSITY_XSPH = 0, 1, 2
+SURFACE_TENSION_AKINCI, ADHESION_AKINCI = 3, 4
 
 FIELD_X, FIELD_V, FIELD_A, FIELD_RHO, FIELD_P = 0, 1, 2, 3, 4
 FIELD_ALPHA, FIELD_RHOSTAR, FIELD_DIVERR, FIELD_K, FIELD_KV = 5, 6, 7, 8, 9

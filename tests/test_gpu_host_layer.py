@@ -65,3 +65,26 @@ def test_scene_json_matches_c_abi(ctx, tmp_path, name):
     assert xc.shape == xa.shape
     assert np.array_equal(xc, xa.astype(np.float64))
     assert np.array_equal(vc, va.astype(np.float64))
+
+
+def test_viscosity_methods_through_scene_json(ctx, tmp_path):
+    """viscosityMethod 0 and 1 (standard, artificial) selected by the scene
+    file end in the same state as through the C ABI."""
+    with open(os.path.join(SCENES, "dam_break_dfsph_small.json")) as f:
+        base = json.load(f)
+    for method in (sphg.VISCOSITY_STANDARD, sphg.VISCOSITY_ARTIFICIAL):
+        d = json.loads(json.dumps(base))
+        d["Fluid"]["viscosityMethod"] = method
+        d["Fluid"]["FluidModel"][0].update({"viscosity": 0.05, "boundaryViscosity": 0.02})
+        path = tmp_path / f"visc{method}.json"
+        path.write_text(json.dumps(d))
+        out = tmp_path / f"final{method}.json"
+        p = subprocess.run([RUN, "-f", str(path), "--steps", str(STEPS), "--final", str(out)],
+                           capture_output=True, text=True, timeout=110)
+        assert p.returncode == 0, p.stderr
+        with open(out) as f:
+            xc = np.array(json.load(f)["positions"], dtype=np.float64)
+        spec = configs.small_scene(sphg.DFSPH, n_side=8, spacing=0.05, jitter=0.01, seed=7, viscosity_method=method,
+                                   viscosity=0.05, boundary_viscosity=0.02)
+        xa, _ = _api_state(ctx, spec)
+        assert np.array_equal(xc, xa.astype(np.float64)), method

@@ -222,3 +222,41 @@ def test_step_is_deterministic(ctx):
         runs.append((g.get(sphg.FIELD_X), g.get(sphg.FIELD_V)))
         g.close()
     assert np.array_equal(runs[0][0], runs[1][0]) and np.array_equal(runs[0][1], runs[1][1])
+
+
+@pytest.mark.parametrize("method", [sphg.DFSPH, sphg.WCSPH])
+@pytest.mark.parametrize("visc", [sphg.VISCOSITY_STANDARD, sphg.VISCOSITY_ARTIFICIAL])
+def test_standard_and_artificial_viscosity(ctx, method, visc):
+    """Standard (src/StandardViscosityForce.h) and artificial
+    (src/ArtificialViscosity.h) viscosity, fluid and boundary terms: the
+    step's acceleration and the resulting state against the oracle."""
+    spec = configs.small_scene(method, n_side=10, viscosity_method=visc, viscosity=0.05, boundary_viscosity=0.02)
+    g, o = _pair(ctx, spec)
+    a_max = 0.0
+    matched = 0
+    for step in range(6):
+        sc = g.scalars()
+        o.set_state(g.get(sphg.FIELD_X), g.get(sphg.FIELD_V),
+                    g.get(sphg.FIELD_K) if method == sphg.DFSPH else None,
+                    g.get(sphg.FIELD_KV) if method == sphg.DFSPH else None, sc.time_step, sc.iterations)
+        g.step(1)
+        o.step(1)
+        a_g = g.get(sphg.FIELD_A)
+        a_max = max(a_max, float(np.abs(a_g).max()))
+        a_err = rel_norm(a_g, o.get(po.OF_ACC_STEP))
+        assert a_err < TOL, (step, a_err)
+        sg, so = g.scalars(), o.scalars()
+        its = (sg.iterations, int(so["iterations"]), sg.iterations_v, int(so["iterations_v"]))
+        assert abs(its[0] - its[1]) <= 1 and abs(its[2] - its[3]) <= 1, its
+        if its[0] != its[1] or its[2] != its[3]:
+            print("iteration mismatch", its)
+            continue
+        matched += 1
+        v_err = rel_norm(g.get(sphg.FIELD_V), o.get(po.OF_V))
+        print("errors", {"a": f"{a_err:.2e}", "v": f"{v_err:.2e}"})
+        assert v_err < TOL, (step, a_err, v_err)
+    assert matched >= 4
+    if method == sphg.DFSPH:
+        # a is the viscous acceleration alone (the scene starts at rest)
+        assert a_max > 1e-4
+    g.close()

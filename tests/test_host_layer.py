@@ -79,3 +79,13 @@ def test_emitters_are_refused(tmp_path):
 def test_usage_and_extension():
     assert _run().returncode == 2
     assert _run("-f", "scene.txt").returncode == 2
+
+
+@pytest.mark.parametrize("method", [0, 1, 2])
+def test_viscosity_methods_parse(tmp_path, method):
+    def edit(d):
+        d["Fluid"]["viscosityMethod"] = method
+        d["Fluid"]["FluidModel"][0].update({"viscosity": 0.05, "boundaryViscosity": 0.02})
+    p = _run("-f", _variant(tmp_path, edit), "--info")
+    assert p.returncode == 0, p.stderr
+    assert json.loads(p.stdout)["viscosityMethod"] == method

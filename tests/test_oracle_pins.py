@@ -223,3 +223,85 @@ def test_dfsph_min_iterations_and_dt_rule():
     assert s["cfl_time_step"] > dt0
     assert math.isclose(s["time_step"], dt0 * 1.001, rel_tol=1e-15)
     assert math.isclose(s["time"], s["time_step"], rel_tol=1e-15)
+
+
+# ------------------------------------------- standard / artificial viscosity
+# src/StandardViscosityForce.h:37-74, src/ArtificialViscosity.h:40-81, both
+# restated in oracle/sph_oracle.cpp.  The acceleration of the step's
+# non-pressure phase (OF_ACC_STEP of a DFSPH step, which has no other force)
+# is checked against invariants a wrong sign, a missing (v_j - v_i)
+# difference or a wrong vector would break.
+
+def _visc_accel(method, vel, nu=0.05, nub=0.0, seed=3):
+    spec = configs.small_scene(sphg.DFSPH, n_side=7, jitter=0.05, seed=seed, viscosity_method=method, viscosity=nu,
+                               boundary_viscosity=nub)
+    far = np.array([[50.0, 50.0, 50.0]], np.float32)
+    o = po.Oracle(spec.params, spec.fluid, far, velocities=vel(spec.fluid.astype(np.float64)).astype(np.float32))
+    o.step(1)
+    a = o.get(po.OF_ACC_STEP)
+    o.close()
+    return spec, a
+
+
+def _random_vel(x, seed=11):
+    # multiples of 2^-10: exact in FP32, and so are their sums with the
+    # dyadic shift below
+    return np.round(np.random.default_rng(seed).uniform(-1.0, 1.0, size=x.shape) * 1024.0) / 1024.0
+
+
+@pytest.mark.parametrize("method", [sphg.VISCOSITY_STANDARD, sphg.VISCOSITY_ARTIFICIAL])
+def test_viscosity_galilean_invariant_and_linear(method):
+    """The fluid terms depend on velocity differences only (a uniform shift
+    changes nothing) and are linear in nu."""
+    _, a = _visc_accel(method, _random_vel)
+    _, a_shift = _visc_accel(method, lambda x: _random_vel(x) + np.array([0.25, -0.5, 0.125]))
+    _, a2 = _visc_accel(method, _random_vel, nu=0.1)
+    assert np.abs(a).max() > 1e-3
+    assert np.abs(a_shift - a).max() < 1e-12 * np.abs(a).max()
+    assert np.abs(a2 - 2.0 * a).max() < 1e-12 * np.abs(a).max() * 2
+
+
+def test_artificial_viscosity_vanishes_for_rigid_rotation():
+    """(v_i - v_j).r_ij = ((w x r_i) - (w x r_j)).r_ij = (w x r_ij).r_ij = 0;
+    the FP32 rounding of w x r leaves ~1e-8 of the response to a random
+    field (a formula mistake leaves O(1))."""
+    w = np.array([0.3, -1.1, 0.7])
+    _, a_rot = _visc_accel(sphg.VISCOSITY_ARTIFICIAL, lambda x: np.cross(w, x - 0.2))
+    _, a_rnd = _visc_accel(sphg.VISCOSITY_ARTIFICIAL, _random_vel)
+    assert np.abs(a_rot).max() < 1e-6 * np.abs(a_rnd).max()
+
+
+@pytest.mark.parametrize("method", [sphg.VISCOSITY_STANDARD, sphg.VISCOSITY_ARTIFICIAL])
+def test_viscosity_dissipates(method):
+    """Both forms remove kinetic energy from a random velocity field:
+    sum_i m a_i.v_i < 0 (pairwise -|v_i - v_j|^2 L_ij for the standard form,
+    -(v_ij.r_ij)^2 |g_ij| for the artificial one, up to density factors)."""
+    spec, a = _visc_accel(method, _random_vel)
+    v = _random_vel(spec.fluid.astype(np.float64)).astype(np.float32).astype(np.float64)
+    power = float(np.sum(a * v))
+    assert power < 0.0
+    assert abs(power) > 1e-3 * np.abs(a).sum() * np.abs(v).mean()
+
+
+@pytest.mark.parametrize("method", [sphg.VISCOSITY_STANDARD, sphg.VISCOSITY_ARTIFICIAL])
+def test_boundary_viscosity_opposes_motion(method):
+    """With the walls of the box: the boundary term of a fluid moving as a
+    whole opposes its velocity (the fluid terms vanish for uniform v)."""
+    U = np.array([0.4, -0.2, 0.3])
+    spec = configs.small_scene(sphg.DFSPH, n_side=7, jitter=0.0, viscosity_method=method, viscosity=0.05,
+                               boundary_viscosity=0.05)
+    vel = np.tile(U, (spec.n, 1)).astype(np.float32)
+    o = po.Oracle(spec.params, spec.fluid, spec.boundary, velocities=vel)
+    o.step(1)
+    a = o.get(po.OF_ACC_STEP)
+    o.close()
+    near = np.abs(a).sum(1) > 0
+    assert near.any() and not near.all()          # only particles near a wall feel it
+    if method == sphg.VISCOSITY_STANDARD:
+        # standard: -(nu_b/rho0) rho0 V_b L / rho_i v_i, exactly antiparallel to U
+        cos = (a[near] @ U) / (np.linalg.norm(a[near], axis=1) * np.linalg.norm(U))
+        assert np.all(cos < -1 + 1e-9)
+    else:
+        # artificial: sum_b c_b g_b (v.r)/(r^2 + eps) r with g_b < 0: a.v <= 0
+        assert np.all(a[near] @ U <= 1e-12)
+        assert (a[near] @ U).sum() < 0.0
