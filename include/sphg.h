@@ -36,7 +36,10 @@ enum {
 /* Solver and boundary method numbers of the reference scene format
  * (src/Simulation.cpp:21-33). */
 enum { SPHG_WCSPH = 0, SPHG_PCISPH = 1, SPHG_DFSPH = 2 };
-enum { SPHG_BOUNDARY_AKINCI = 2 };
+/* boundaryMethod (src/Simulation.cpp:25-27): cube (a clamp into a box, or
+ * nothing when the scene has no CubeBoundaryModel), Akinci particles.  The
+ * PCISPH boundary method (1) is not available. */
+enum { SPHG_BOUNDARY_CUBE = 0, SPHG_BOUNDARY_AKINCI = 2 };
 /* viscosityMethod (src/Simulation.h enums): standard SPH (src/StandardViscosityForce.h),
  * artificial (src/ArtificialViscosity.h), XSPH (src/XSPHViscosity.h) */
 enum { SPHG_VISCOSITY_STANDARD = 0, SPHG_VISCOSITY_ARTIFICIAL = 1, SPHG_VISCOSITY_XSPH = 2 };
@@ -52,7 +55,7 @@ typedef struct sphg_batch sphg_batch;
  * Simulation::importScene (src/Simulation.cpp:342-424). */
 typedef struct sphg_scene_desc {
     int32_t method;                 /* SPHG_WCSPH / SPHG_PCISPH / SPHG_DFSPH */
-    int32_t boundary_method;        /* must be SPHG_BOUNDARY_AKINCI */
+    int32_t boundary_method;        /* SPHG_BOUNDARY_AKINCI, or SPHG_BOUNDARY_CUBE (no boundary particles) */
     double particle_radius;         /* r; support radius h = 4r (src/Simulation.cpp:197) */
     double density0;                /* rest density rho0 */
     double gravity[3];
@@ -62,7 +65,7 @@ typedef struct sphg_scene_desc {
     double stiffness, gamma;        /* WCSPH Tait EOS */
     double eta, eta_v;              /* PCISPH/DFSPH density, DFSPH divergence tolerance */
     uint32_t min_iterations, max_iterations, min_iterations_v, max_iterations_v;
-    int32_t viscosity_method;       /* -1 or SPHG_VISCOSITY_XSPH */
+    int32_t viscosity_method;       /* -1 or SPHG_VISCOSITY_STANDARD / _ARTIFICIAL / _XSPH */
     double viscosity, boundary_viscosity;
     int32_t surface_tension_method; /* -1 or SPHG_SURFACE_TENSION_AKINCI */
     double surface_tension;
@@ -70,6 +73,14 @@ typedef struct sphg_scene_desc {
     double adhesion;
     uint32_t max_neighbors;         /* neighbor-list capacity; 0 = 96 */
     double domain_min[3], domain_max[3];  /* grid bounds; all zero = fit the particles */
+    /* CubeBoundaryModel (boundary_method SPHG_BOUNDARY_CUBE,
+     * src/CubeBoundaryModel.cpp:5-94): when cube_enabled, every fluid
+     * position is clamped into [cube_min, cube_max] after the step's
+     * integration, the velocity component leaving the box is multiplied by
+     * -cube_normal_fct and the others by cube_tangential_fct (defaults 0, 1). */
+    int32_t cube_enabled;
+    double cube_min[3], cube_max[3];
+    double cube_normal_fct, cube_tangential_fct;
 } sphg_scene_desc;
 
 /* Device-side per-scene state, as the reference's solver members

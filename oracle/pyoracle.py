@@ -44,6 +44,11 @@ class OracleDesc(C.Structure):
         ("surface_tension", C.c_double),
         ("adhesion_method", C.c_int),
         ("adhesion", C.c_double),
+        ("cube_enabled", C.c_int),
+        ("cube_min", C.c_double * 3),
+        ("cube_max", C.c_double * 3),
+        ("cube_normal_fct", C.c_double),
+        ("cube_tangential_fct", C.c_double),
     ]
 
 
@@ -72,6 +77,8 @@ def load(fast: bool = False) -> C.CDLL:
         "oracle_init": (C.c_int, [P]),
         "oracle_set_state": (C.c_int, [P, fp, fp, fp, fp, D, U]),
         "oracle_step": (C.c_int, [P, U]),
+        "oracle_round_state_f32": (C.c_int, [P]),
+        "oracle_set_f32_positions": (C.c_int, [P, C.c_int]),
         "oracle_num_fluid": (U, [P]),
         "oracle_num_boundary": (U, [P]),
         "oracle_get": (C.c_int, [P, C.c_int, dp]),
@@ -110,8 +117,8 @@ def make_desc(params) -> OracleDesc:
     d = OracleDesc()
     for name, _ in OracleDesc._fields_:
         v = getattr(params, name)
-        if name == "gravity":
-            d.gravity[:] = [float(x) for x in v]
+        if name in ("gravity", "cube_min", "cube_max"):
+            getattr(d, name)[:] = [float(x) for x in v]
         else:
             setattr(d, name, v)
     return d
@@ -158,6 +165,17 @@ class Oracle:
         kv = None if kv is None else np.ascontiguousarray(np.asarray(kv, dtype=np.float32))
         dt = self.params.time_step if time_step is None else float(time_step)
         self.lib.oracle_set_state(self.h, _fp(x), _fp(v), _fp(k), _fp(kv), dt, int(last_iterations))
+
+    def round_state_f32(self):
+        """Round x, v, kappa and kappa_v to FP32, keeping the neighbor lists,
+        densities and alpha factors (the state a solver that stores particle
+        fields in FP32 carries into its next step)."""
+        self.lib.oracle_round_state_f32(self.h)
+
+    def set_f32_positions(self, on=True):
+        """Round DFSPH's updated positions to FP32 before the neighbor
+        search, densities and alpha factors of the step."""
+        self.lib.oracle_set_f32_positions(self.h, 1 if on else 0)
 
     def get(self, field: int) -> np.ndarray:
         if field in (OF_BVOL,):

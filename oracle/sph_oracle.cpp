@@ -218,6 +218,9 @@ typedef struct oracle_desc {
     double surface_tension;
     int adhesion_method;     // -1 none, 4 Akinci
     double adhesion;
+    int cube_enabled;        // CubeBoundaryModel (src/CubeBoundaryModel.cpp:5-94)
+    double cube_min[3], cube_max[3];
+    double cube_normal_fct, cube_tangential_fct;
 } oracle_desc;
 
 enum {
@@ -249,6 +252,7 @@ struct Oracle {
     std::vector<double> alpha, divErr, rhoStar, k, kv;
     double avgDensError = 0, avgDivError = 0, avgDensError0 = 0, avgDivError0 = 0;
     unsigned iterations = 0, iterationsV = 0, steps = 0, sumIt = 0, sumItV = 0;
+    bool f32Positions = false;   // round DFSPH's updated positions to FP32 (test emulation of FP32 storage)
     double eps = 1e-5;
     double maxVel = 0, lastCfl = 0;
 
@@ -537,6 +541,53 @@ struct Oracle {
         }
         maxVel = mv;
     }
+    // CubeBoundaryModel::correctPositionAndVelocity (src/CubeBoundaryModel.cpp:16-94)
+    void cubeCorrect() {
+        if (!d.cube_enabled) return;
+        const V3 mn(d.cube_min[0], d.cube_min[1], d.cube_min[2]);
+        const V3 mx(d.cube_max[0], d.cube_max[1], d.cube_max[2]);
+        const double normalFct = d.cube_normal_fct, tangentialFct = d.cube_tangential_fct;
+        const long n = (long)N();
+        #pragma omp parallel for
+        for (long i = 0; i < n; ++i) {
+            V3& pos = x[i];
+            V3& vel = v[i];
+            if (pos.x < mn.x) {
+                pos.x = mn.x;
+                if (vel.x < 0.f) vel.x *= -normalFct;
+                vel.y *= tangentialFct;
+                vel.z *= tangentialFct;
+            } else if (pos.x > mx.x) {
+                pos.x = mx.x;
+                if (vel.x > 0.f) vel.x *= -normalFct;
+                vel.y *= tangentialFct;
+                vel.z *= tangentialFct;
+            }
+            if (pos.y < mn.y) {
+                pos.y = mn.y;
+                if (vel.y < 0.f) vel.y *= -normalFct;
+                vel.x *= tangentialFct;
+                vel.z *= tangentialFct;
+            } else if (pos.y > mx.y) {
+                pos.y = mx.y;
+                if (vel.y > 0.f) vel.y *= -normalFct;
+                vel.x *= tangentialFct;
+                vel.z *= tangentialFct;
+            }
+            if (pos.z < mn.z) {
+                pos.z = mn.z;
+                if (vel.z < 0.f) vel.z *= -normalFct;
+                vel.x *= tangentialFct;
+                vel.y *= tangentialFct;
+            } else if (pos.z > mx.z) {
+                pos.z = mx.z;
+                if (vel.z > 0.f) vel.z *= -normalFct;
+                vel.x *= tangentialFct;
+                vel.y *= tangentialFct;
+            }
+        }
+    }
+
     // src/WCSPHSolver.cpp:14-78
     void wcsphStep() {
         ++steps;
@@ -547,6 +598,7 @@ struct Oracle {
         wcsphPressureForce();
         computeNonPressureForces();
         integrate();
+        cubeCorrect();   // :51-60
         time += ts;
     }
 
@@ -664,6 +716,7 @@ struct Oracle {
         for (size_t i = 0; i < N(); ++i) { p[i] = 0.0; pacc[i] = V3(); }
         pressureSolver();
         pcisphIntegrate();
+        cubeCorrect();   // src/PCISPHSolver.cpp:54-63
         time += ts;
     }
 
@@ -709,7 +762,10 @@ struct Oracle {
         const long n = (long)N();
         const double dt = ts;
         #pragma omp parallel for
-        for (long i = 0; i < n; ++i) x[i] += v[i] * dt;
+        for (long i = 0; i < n; ++i) {
+            x[i] += v[i] * dt;
+            if (f32Positions) x[i] = V3((float)x[i].x, (float)x[i].y, (float)x[i].z);
+        }
     }
     // src/DFSPHSolver.cpp:238-309 (Akinci branch :285-295)
     void predictDensities() {
@@ -885,6 +941,7 @@ struct Oracle {
         computeDensities();
         computeAlpha();
         correctDivergenceError();
+        cubeCorrect();   // src/DFSPHSolver.cpp:99-108
         time += ts;
     }
 
@@ -1007,6 +1064,39 @@ int oracle_set_state(void* h, const float* xyz, const float* vel, const float* k
     o->ts = dt;
     o->iterations = last_iterations;
     o->refresh();
+    return 0;
+}
+
+// Rounds the carried particle state (x, v, kappa, kappa_v) to FP32, as a
+// solver that stores them in FP32 between steps holds them; the neighbor
+// lists, densities and alpha factors are left as they are (unlike
+// oracle_set_state, which recomputes them from the given positions).  With
+// the cube boundary on, a coordinate inside the box whose nearest FP32 value
+// lies outside it takes the nearest FP32 value inside it instead.
+int oracle_round_state_f32(void* h) {
+    Oracle* o = (Oracle*)h;
+    auto r = [](double& a) { a = (double)(float)a; };
+    const bool cube = o->d.cube_enabled != 0;
+    auto rx = [&](double& a, int c) {
+        const double lo = o->d.cube_min[c], hi = o->d.cube_max[c];
+        float f = (float)a;
+        if (cube && a >= lo && (double)f < lo) f = std::nextafter(f, INFINITY);
+        if (cube && a <= hi && (double)f > hi) f = std::nextafter(f, -INFINITY);
+        a = (double)f;
+    };
+    for (size_t i = 0; i < o->N(); ++i) {
+        rx(o->x[i].x, 0); rx(o->x[i].y, 1); rx(o->x[i].z, 2);
+        r(o->v[i].x); r(o->v[i].y); r(o->v[i].z);
+        r(o->k[i]); r(o->kv[i]);
+    }
+    return 0;
+}
+
+// When on, DFSPH's position update (src/DFSPHSolver.cpp:60) rounds the new
+// positions to FP32, so the neighbor search, densities and alpha factors
+// that follow see the positions a solver storing them in FP32 sees.
+int oracle_set_f32_positions(void* h, int on) {
+    ((Oracle*)h)->f32Positions = on != 0;
     return 0;
 }
 

@@ -45,12 +45,12 @@ constexpr int kDefaultMaxNbr = 96;
 enum Kind {
     K_KEYS, K_SCAN, K_SCATTER, K_FINALIZE, K_REORDER, K_BUILD, K_NORMALS, K_FORCES, K_STEPBEGIN, K_DT,
     K_DFPREDICT, K_DFVEL_WARM, K_DFVEL_DENS, K_DFVEL_DIV, K_DFERR_DENS, K_DFERR_DIV, K_DFPOS, K_PCIDENS,
-    K_PCIPACC, K_PCIINT, K_TIME, K_NKINDS
+    K_PCIPACC, K_PCIINT, K_TIME, K_CUBE, K_NKINDS
 };
 const char* kKindNames[K_NKINDS] = {
     "keys", "scan", "scatter", "finalize", "rank_reorder", "build", "normals", "forces", "step_begin", "dt",
     "df_predict", "df_vel_warm", "df_vel_density", "df_vel_divergence", "df_err_density", "df_err_divergence",
-    "df_positions", "pci_density", "pci_pacc", "pci_integrate", "time_advance"};
+    "df_positions", "pci_density", "pci_pacc", "pci_integrate", "time_advance", "cube"};
 
 struct Profiler {
     bool on = false;
@@ -219,7 +219,13 @@ double pcisph_delta(double r, double h, double m, double rho0, double dt) {
 
 int check_desc(const sphg_scene_desc& d) {
     if (d.method < 0 || d.method > 2) return fail(SPHG_EINVAL, "simulationMethod must be 0 (WCSPH), 1 (PCISPH) or 2 (DFSPH)");
-    if (d.boundary_method != SPHG_BOUNDARY_AKINCI) return fail(SPHG_EINVAL, "only boundaryMethod 2 (Akinci) runs on the GPU solver");
+    if (d.boundary_method != SPHG_BOUNDARY_AKINCI && d.boundary_method != SPHG_BOUNDARY_CUBE)
+        return fail(SPHG_EINVAL, "boundaryMethod must be 0 (cube) or 2 (Akinci); the PCISPH boundary method (1) is not available");
+    if (d.cube_enabled && d.boundary_method != SPHG_BOUNDARY_CUBE)
+        return fail(SPHG_EINVAL, "a cube clamp needs boundaryMethod 0 (cube)");
+    if (d.cube_enabled)
+        for (int a = 0; a < 3; ++a)
+            if (!(d.cube_min[a] <= d.cube_max[a])) return fail(SPHG_EINVAL, "cube_min must not exceed cube_max");
     if (!(d.particle_radius > 0)) return fail(SPHG_EINVAL, "particleRadius must be > 0");
     if (!(d.density0 > 0)) return fail(SPHG_EINVAL, "density0 must be > 0");
     if (!(d.time_step > 0)) return fail(SPHG_EINVAL, "timeStep must be > 0");
@@ -416,10 +422,11 @@ int batch_one_step(sphg_ctx* ctx, Batch& B, uint32_t& syncs) {
     hipStream_t st = ctx->stream;
     const int method = B.scenes[0]->desc.method;
     const int ns = (int)B.scenes.size();
-    int anyFlags = 0, anySt = 0;
+    int anyFlags = 0, anySt = 0, anyCube = 0;
     for (auto* s : B.scenes) {
         anyFlags |= s->view[0].flags;
         anySt |= (s->view[0].flags & F_ST);
+        anyCube |= s->view[0].cubeOn;
     }
     int rc;
     bool more = false;
@@ -463,6 +470,8 @@ int batch_one_step(sphg_ctx* ctx, Batch& B, uint32_t& syncs) {
             finalize(ctx, B, T2, 4);
         }
         B.lastItV = seen;
+        // cube boundary after the divergence solve (src/DFSPHSolver.cpp:99-108)
+        if (anyCube) { Scope s(P, st, K_CUBE); launch_cube(st, T2, B.dP, B.nP); }
         flip(B);
     } else if (method == WCSPH) {
         const DevScene* T = B.dT[0];
@@ -474,6 +483,8 @@ int batch_one_step(sphg_ctx* ctx, Batch& B, uint32_t& syncs) {
         { Scope s(P, st, K_BUILD); launch_build(st, WCSPH, T2, B.dP, B.nP); }
         if (anySt) { Scope s(P, st, K_NORMALS); launch_normals(st, T2, B.dP, B.nP); }
         { Scope s(P, st, K_FORCES); launch_forces(st, WCSPH, T2, B.dP, B.nP); }
+        // cube boundary after the integration (src/WCSPHSolver.cpp:51-60)
+        if (anyCube) { Scope s(P, st, K_CUBE); launch_cube(st, T2, B.dP, B.nP); }
         { Scope s(P, st, K_TIME); launch_time_advance(st, T2, ns); }
         flip(B);
     } else {
@@ -496,6 +507,8 @@ int batch_one_step(sphg_ctx* ctx, Batch& B, uint32_t& syncs) {
         }
         B.lastIt = seen;
         { Scope s(P, st, K_PCIINT); launch_pci_integrate(st, T2, B.dP, B.nP); }
+        // cube boundary after the integration (src/PCISPHSolver.cpp:54-63)
+        if (anyCube) { Scope s(P, st, K_CUBE); launch_cube(st, T2, B.dP, B.nP); }
         flip(B);
     }
     return SPHG_OK;
@@ -770,6 +783,8 @@ int sphg_scene_set_fluid(sphg_scene* s, const float* xyz, const float* vel, uint
 int sphg_scene_set_boundary(sphg_scene* s, const float* xyz, uint32_t nb) {
     if (!s || (!xyz && nb)) return fail(SPHG_EINVAL, "NULL argument");
     if (s->ready) return fail(SPHG_ESTATE, "set_boundary after init");
+    if (nb && s->desc.boundary_method != SPHG_BOUNDARY_AKINCI)
+        return fail(SPHG_EINVAL, "boundary particles need boundaryMethod 2 (Akinci)");
     s->bxyz.assign(xyz, xyz + 3 * (size_t)nb);
     s->nb = nb;
     return SPHG_OK;
@@ -835,6 +850,16 @@ int sphg_scene_init(sphg_scene* s) {
     D.gammaEos = (float)d.gamma;
     D.method = method;
     D.flags = flags;
+    D.cubeOn = d.cube_enabled ? 1 : 0;
+    for (int c = 0; c < 3; ++c) {
+        float lo = (float)d.cube_min[c], hi = (float)d.cube_max[c];
+        if ((double)lo < d.cube_min[c]) lo = std::nextafter(lo, INFINITY);
+        if ((double)hi > d.cube_max[c]) hi = std::nextafter(hi, -INFINITY);
+        D.cubeLo[c] = lo;
+        D.cubeHi[c] = hi;
+    }
+    D.cubeNf = d.cube_normal_fct;
+    D.cubeTf = d.cube_tangential_fct;
     D.minIt = (int)d.min_iterations;
     D.maxIt = (int)d.max_iterations;
     D.minItV = (int)d.min_iterations_v;

@@ -82,6 +82,12 @@ struct DevScene {
     float nu, nub, stCoef, adhBeta;
     float stiffness, gammaEos, pad1, pad2;
     int method, flags;
+    int cubeOn;                 // CubeBoundaryModel clamp after the step's integration
+    // box faces as the nearest FP32 values inside the box (min rounded up,
+    // max rounded down): for an FP32 coordinate p, p < cubeLo is the same
+    // test as p < min in FP64, and a clamped particle stays inside
+    float cubeLo[3], cubeHi[3];
+    double cubeNf, cubeTf;
     int minIt, maxIt, minItV, maxItV;
     int kx0, ky0, kz0;          // key of the first unpadded cell
     int nx, ny, nz, ncells;     // padded grid (one empty layer on every side)

@@ -1408,6 +1408,44 @@ __global__ __launch_bounds__(kBlock) void k_finalize(const DevScene* __restrict_
     }
 }
 
+// CubeBoundaryModel::correctPositionAndVelocity (src/CubeBoundaryModel.cpp:
+// 16-94): per axis in x, y, z order, a position below the minimum (above
+// the maximum) is set to it, the axis velocity is multiplied by -normalFct if
+// it points out of the box, and the other two components by tangentialFct.
+// The faces are the nearest FP32 values inside the box, so the set of
+// clamped particles is the FP64 comparison's, and a particle left on a face
+// is not clamped again on later steps (FP64 positions sit exactly on it).
+__global__ __launch_bounds__(kBlock) void k_cube(const DevScene* __restrict__ T, const Chunk* __restrict__ C) {
+    const Chunk c = C[blockIdx.x];
+    const DevScene& S = T[c.scene];
+    const int i = c.begin + threadIdx.x;
+    if (!S.cubeOn || i >= S.n) return;
+    const float4 xf = S.x[i], vf = S.v[i];
+    float p[3] = {xf.x, xf.y, xf.z};
+    double u[3] = {vf.x, vf.y, vf.z};
+    const double nf = S.cubeNf, tf = S.cubeTf;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const int b = (a + 1) % 3, e = (a + 2) % 3;
+        bool hit = false;
+        if (p[a] < S.cubeLo[a]) {
+            p[a] = S.cubeLo[a];
+            if (u[a] < 0.0) u[a] *= -nf;
+            hit = true;
+        } else if (p[a] > S.cubeHi[a]) {
+            p[a] = S.cubeHi[a];
+            if (u[a] > 0.0) u[a] *= -nf;
+            hit = true;
+        }
+        if (hit) {
+            u[b] *= tf;
+            u[e] *= tf;
+        }
+    }
+    S.x[i] = make_float4(p[0], p[1], p[2], xf.w);
+    S.v[i] = make_float4((float)u[0], (float)u[1], (float)u[2], vf.w);
+}
+
 // WCSPH: time += dt after the integration (src/WCSPHSolver.cpp:77).
 __global__ void k_time_advance(const DevScene* __restrict__ T, int nscenes) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1531,6 +1569,7 @@ void launch_df_positions(hipStream_t st, const DevScene* T, const Chunk* C, int 
 void launch_pci_density(hipStream_t st, const DevScene* T, const Chunk* C, int n) { if (n) LAUNCH(k_pci_density, n, st, T, C); }
 void launch_pci_pacc(hipStream_t st, const DevScene* T, const Chunk* C, int n, int iter) { if (n) LAUNCH(k_pci_pacc, n, st, T, C, iter); }
 void launch_pci_integrate(hipStream_t st, const DevScene* T, const Chunk* C, int n) { if (n) LAUNCH(k_pci_integrate, n, st, T, C); }
+void launch_cube(hipStream_t st, const DevScene* T, const Chunk* C, int n) { if (n) LAUNCH(k_cube, n, st, T, C); }
 void launch_time_advance(hipStream_t st, const DevScene* T, int nscenes) {
     hipLaunchKernelGGL(k_time_advance, dim3((nscenes + 255) / 256), dim3(256), 0, st, T, nscenes);
 }

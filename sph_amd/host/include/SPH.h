@@ -178,6 +178,27 @@ public:
     void addSphere(Vector3r origin, Real radius);     // src/AkinciBoundayModel.cpp:60-76
 };
 
+// The box of the cube boundary method: points [min, max] (init), normal and
+// tangential velocity factors 0 and 1 by default.  The clamp runs on the
+// device after each step's integration.
+class CubeBoundaryModel : public BoundaryModel {   // src/CubeBoundaryModel.h:7-32
+protected:
+    Real normalFct = 0, tangentialFct = 1;
+
+public:
+    void init(std::vector<Vector3r>& points) override {   // src/CubeBoundaryModel.cpp:5-14
+        BoundaryModel::init(points);
+        normalFct = 0.0;
+        tangentialFct = 1.0;
+    }
+    Vector3r& getMin() { return r.at(0); }
+    Vector3r& getMax() { return r.at(1); }
+    void setNormalFct(Real f) { normalFct = f; }
+    void setTangentialFct(Real f) { tangentialFct = f; }
+    Real getNormalFct() const { return normalFct; }
+    Real getTangentialFct() const { return tangentialFct; }
+};
+
 // ------------------------------------------------------------ solvers
 
 class SPHSolver {   // src/SPHSolver.h:6-50

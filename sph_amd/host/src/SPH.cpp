@@ -123,6 +123,24 @@ void GpuSolver::fillDesc(sphg_scene_desc& d) const {
     Simulation* sim = Simulation::getCurrent();
     d.method = method;
     d.boundary_method = sim->getBoundaryHandlingMethod();
+    // the cube method's box: one CubeBoundaryModel (the reference's scene
+    // loader creates none, src/Simulation.cpp:435-486, so a scene file with
+    // boundaryMethod 0 has no boundary at all)
+    if (d.boundary_method == Simulation::CUBE_BOUNDARY_METHOD) {
+        d.cube_normal_fct = 0.0;
+        d.cube_tangential_fct = 1.0;
+        unsigned int cubes = 0;
+        for (unsigned int b = 0; b < sim->numberBoundaryModels(); ++b)
+            if (auto* c = dynamic_cast<CubeBoundaryModel*>(sim->getBoundaryModel(b))) {
+                if (++cubes > 1) throw SphError("the GPU solver takes one CubeBoundaryModel");
+                d.cube_enabled = 1;
+                const Vector3r mn = c->getMin(), mx = c->getMax();
+                d.cube_min[0] = mn.x; d.cube_min[1] = mn.y; d.cube_min[2] = mn.z;
+                d.cube_max[0] = mx.x; d.cube_max[1] = mx.y; d.cube_max[2] = mx.z;
+                d.cube_normal_fct = c->getNormalFct();
+                d.cube_tangential_fct = c->getTangentialFct();
+            }
+    }
     d.particle_radius = sim->getParticleRadius();
     d.density0 = sim->numberFluidModels() ? sim->getFluidModel(0)->getRefDensity() : 1000.0;
     const Vector3r g = sim->getGravity();
@@ -178,8 +196,9 @@ void DFSPHSolver::fillDesc(sphg_scene_desc& d) const {
 // handling at src/Simulation.h:24-29 and src/AkinciBoundayModel.cpp:32).
 void GpuSolver::init() {
     Simulation* sim = Simulation::getCurrent();
-    if (sim->getBoundaryHandlingMethod() != Simulation::AKINCI_BOUNDARY_METHOD)
-        throw SphError("the GPU solver needs boundaryMethod 2 (Akinci)");
+    const int bm = sim->getBoundaryHandlingMethod();
+    if (bm != Simulation::AKINCI_BOUNDARY_METHOD && bm != Simulation::CUBE_BOUNDARY_METHOD)
+        throw SphError("the GPU solver needs boundaryMethod 0 (cube) or 2 (Akinci)");
     if (sim->numberFluidModels() != 1) throw SphError("the GPU solver runs one fluid model per scene");
     sphg_ctx*& c = sharedCtx();
     if (!c) check(sphg_ctx_create(sim->getDevice(), &c), "sphg_ctx_create");
@@ -191,10 +210,11 @@ void GpuSolver::init() {
     std::vector<float> x = toFloat(fm->getPositions()), v = toFloat(fm->getVelocitiesVec());
     check(sphg_scene_set_fluid(scene, x.data(), v.data(), fm->getNumParticles()), "set_fluid");
     std::vector<Vector3r> bpts;
-    for (unsigned int b = 0; b < sim->numberBoundaryModels(); ++b) {
-        auto& p = sim->getBoundaryModel(b)->getPosition();
-        bpts.insert(bpts.end(), p.begin(), p.end());
-    }
+    if (bm == Simulation::AKINCI_BOUNDARY_METHOD)
+        for (unsigned int b = 0; b < sim->numberBoundaryModels(); ++b) {
+            auto& p = sim->getBoundaryModel(b)->getPosition();
+            bpts.insert(bpts.end(), p.begin(), p.end());
+        }
     std::vector<float> bx = toFloat(bpts);
     check(sphg_scene_set_boundary(scene, bx.data(), (uint32_t)bpts.size()), "set_boundary");
     check(sphg_scene_init(scene), "sphg_scene_init");
@@ -202,7 +222,7 @@ void GpuSolver::init() {
     std::vector<float> vol(std::max<size_t>(bpts.size(), 1));
     check(sphg_scene_get(scene, SPHG_FIELD_BVOL, vol.data()), "get volumes");
     size_t off = 0;
-    for (unsigned int b = 0; b < sim->numberBoundaryModels(); ++b) {
+    for (unsigned int b = 0; bm == Simulation::AKINCI_BOUNDARY_METHOD && b < sim->numberBoundaryModels(); ++b) {
         auto* abm = dynamic_cast<AkinciBoundaryModel*>(sim->getBoundaryModel(b));
         const unsigned int nb = sim->getBoundaryModel(b)->getNumParticles();
         if (abm)

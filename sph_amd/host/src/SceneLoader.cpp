@@ -275,9 +275,11 @@ bool Simulation::importScene(const std::string& path) {
             for (auto& box : b.box) bm->addCube(box.first.min, box.first.max);
             for (auto& sp : b.sphere) bm->addSphere(sp.first.center, sp.first.radius);
         }
-    } else {
+    } else if (s.boundaryMethod != CUBE_BOUNDARY_METHOD) {
+        // cube (0): the reference's loader creates no boundary model
+        // (src/Simulation.cpp:435-486); the PCISPH method (1) is not built
         throw SphError("boundaryMethod " + std::to_string(s.boundaryMethod) +
-                       " is not supported by the GPU solver (Akinci = 2 is)");
+                       " is not supported by the GPU solver (cube = 0 and Akinci = 2 are)");
     }
     init();
     return true;

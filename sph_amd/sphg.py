@@ -16,7 +16,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libsphg.so")
 
 WCSPH, PCISPH, DFSPH = 0, 1, 2
-BOUNDARY_AKINCI = 2
+BOUNDARY_CUBE, BOUNDARY_AKINCI = 0, 2
 VISCOSITY_STANDARD, VISCOSITY_ARTIFICIAL, VISCOSITY_XSPH = 0, 1, 2
 SURFACE_TENSION_AKINCI, ADHESION_AKINCI = 3, 4
 
@@ -64,6 +64,11 @@ class SceneDesc(C.Structure):
         ("max_neighbors", C.c_uint32),
         ("domain_min", C.c_double * 3),
         ("domain_max", C.c_double * 3),
+        ("cube_enabled", C.c_int32),
+        ("cube_min", C.c_double * 3),
+        ("cube_max", C.c_double * 3),
+        ("cube_normal_fct", C.c_double),
+        ("cube_tangential_fct", C.c_double),
     ]
 
 
@@ -129,6 +134,12 @@ class SceneParams:
     max_neighbors: int = 0
     domain_min: Sequence[float] = (0.0, 0.0, 0.0)
     domain_max: Sequence[float] = (0.0, 0.0, 0.0)
+    boundary_method: int = 2          # BOUNDARY_AKINCI; BOUNDARY_CUBE = no boundary particles
+    cube_enabled: int = 0             # CubeBoundaryModel clamp (boundary_method 0)
+    cube_min: Sequence[float] = (0.0, 0.0, 0.0)
+    cube_max: Sequence[float] = (0.0, 0.0, 0.0)
+    cube_normal_fct: float = 0.0
+    cube_tangential_fct: float = 1.0
 
     @property
     def support_radius(self) -> float:
@@ -138,11 +149,10 @@ class SceneParams:
         d = SceneDesc()
         for f in fields(self):
             v = getattr(self, f.name)
-            if f.name in ("gravity", "domain_min", "domain_max"):
+            if f.name in ("gravity", "domain_min", "domain_max", "cube_min", "cube_max"):
                 getattr(d, f.name)[:] = [float(x) for x in v]
             else:
                 setattr(d, f.name, v)
-        d.boundary_method = BOUNDARY_AKINCI
         return d
 
 

@@ -35,11 +35,27 @@ def test_library_exports_every_declared_symbol():
     assert set(names) <= exported
 
 
-def test_struct_layouts_match_header():
-    # sizes follow the C layout of include/sphg.h on x86-64
-    assert C.sizeof(sphg.SceneDesc) == 248
-    assert C.sizeof(sphg.SceneScalars) == 96
-    assert C.sizeof(sphg.StepStats) == 24
+def test_struct_layouts_match_header(tmp_path):
+    """Every field of the ctypes mirrors sits at the offset the C compiler
+    gives it in include/sphg.h (a probe compiled with gcc)."""
+    structs = {"sphg_scene_desc": sphg.SceneDesc, "sphg_scene_scalars": sphg.SceneScalars,
+               "sphg_step_stats": sphg.StepStats}
+    lines = ['#include "sphg.h"', "#include <stdio.h>", "#include <stddef.h>", "int main(void) {"]
+    for cname, py in structs.items():
+        lines.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')
+        for fname, _ in py._fields_:
+            lines.append(f'printf("{cname} {fname} %zu\\n", offsetof({cname}, {fname}));')
+    lines += ["return 0;", "}"]
+    src = tmp_path / "probe.c"
+    src.write_text("\n".join(lines) + "\n")
+    inc = os.path.join(ROOT, "include")
+    subprocess.run(["gcc", "-I", inc, str(src), "-o", str(tmp_path / "probe")], check=True)
+    out = subprocess.run([str(tmp_path / "probe")], capture_output=True, text=True, check=True).stdout.split("\n")
+    got = {tuple(l.split()[:2]): int(l.split()[2]) for l in out if l.strip()}
+    for cname, py in structs.items():
+        assert got[(cname, "size")] == C.sizeof(py), cname
+        for fname, _ in py._fields_:
+            assert got[(cname, fname)] == getattr(py, fname).offset, (cname, fname)
 
 
 def test_version_and_errors_without_device_calls():

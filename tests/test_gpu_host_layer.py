@@ -88,3 +88,26 @@ def test_viscosity_methods_through_scene_json(ctx, tmp_path):
                                    viscosity=0.05, boundary_viscosity=0.02)
         xa, _ = _api_state(ctx, spec)
         assert np.array_equal(xc, xa.astype(np.float64)), method
+
+
+def test_cube_boundary_method_scene_json(ctx, tmp_path):
+    """boundaryMethod 0 in a scene file: the reference's loader creates no
+    boundary model, so the fluid has no boundary at all; the same through
+    the C ABI (boundary_method cube, no boundary particles, no clamp)."""
+    with open(os.path.join(SCENES, "dam_break_dfsph_small.json")) as f:
+        d = json.load(f)
+    d["Configuration"]["boundaryMethod"] = 0
+    path = tmp_path / "cube.json"
+    path.write_text(json.dumps(d))
+    out = tmp_path / "final_cube.json"
+    p = subprocess.run([RUN, "-f", str(path), "--steps", str(STEPS), "--final", str(out)],
+                       capture_output=True, text=True, timeout=110)
+    assert p.returncode == 0, p.stderr
+    with open(out) as f:
+        xc = np.array(json.load(f)["positions"], dtype=np.float64)
+    spec = configs.small_scene(sphg.DFSPH, n_side=8, spacing=0.05, jitter=0.01, seed=7)
+    from dataclasses import replace
+    spec.params = replace(spec.params, boundary_method=sphg.BOUNDARY_CUBE)
+    spec.boundary = np.zeros((0, 3), np.float32)
+    xa, _ = _api_state(ctx, spec)
+    assert np.array_equal(xc, xa.astype(np.float64))

@@ -260,3 +260,76 @@ def test_standard_and_artificial_viscosity(ctx, method, visc):
         # a is the viscous acceleration alone (the scene starts at rest)
         assert a_max > 1e-4
     g.close()
+
+
+@pytest.mark.parametrize("method", [sphg.DFSPH, sphg.WCSPH, sphg.PCISPH])
+def test_cube_boundary(ctx, method):
+    """Cube boundary method (src/CubeBoundaryModel.cpp:16-94): no boundary
+    particles; the fluid block starts just outside the box's x maximum and z
+    minimum faces (by 0.01 r), so the clamp moves particles every step.
+    Positions are FP32: the clamp stores the nearest FP32 value inside the
+    box, and velocity components normal to a face of particles lying on it
+    are held to 2 ulp(face) / dt (every other component to 1e-5).
+
+    DFSPH clamps after the divergence solve (src/DFSPHSolver.cpp:99-108), so
+    the next step's forces and density solve use the neighbor lists, densities
+    and alpha factors of the positions before the clamp.  A state injected
+    with set_state cannot carry those (the oracle recomputes them from the
+    clamped positions), so for DFSPH the oracle runs on from the common
+    initial state, its updated positions and carried fields rounded to FP32
+    as the GPU stores them."""
+    spec = configs.small_scene(method, n_side=10)
+    f = spec.fluid.astype(np.float64)
+    r = spec.params.particle_radius
+    lo = (f[:, 0].min() - 1.0, f[:, 1].min() - 1.0, f[:, 2].min() + 0.01 * r)
+    hi = (f[:, 0].max() - 0.01 * r, f[:, 1].max() + 1.0, f[:, 2].max() + 1.0)
+    from dataclasses import replace
+    params = replace(spec.params, boundary_method=sphg.BOUNDARY_CUBE, cube_enabled=1, cube_min=lo, cube_max=hi,
+                     cube_normal_fct=0.0, cube_tangential_fct=1.0)
+    empty = np.zeros((0, 3), np.float32)
+    g = sphg.Scene(ctx, params, spec.fluid, empty)
+    o = po.Oracle(params, spec.fluid, empty)
+    matched = 0
+    # the faces the clamp stores: nearest FP32 values inside the box
+    lo32 = np.float32(lo[2]) if np.float64(np.float32(lo[2])) >= lo[2] else np.nextafter(np.float32(lo[2]), np.float32(np.inf))
+    hi32 = np.float32(hi[0]) if np.float64(np.float32(hi[0])) <= hi[0] else np.nextafter(np.float32(hi[0]), np.float32(-np.inf))
+    reseed = method != sphg.DFSPH
+    if not reseed:
+        o.set_f32_positions(True)
+    for step in range(6):
+        if reseed:
+            sc = g.scalars()
+            o.set_state(g.get(sphg.FIELD_X), g.get(sphg.FIELD_V), None, None, sc.time_step, sc.iterations)
+        g.step(1)
+        o.step(1)
+        if not reseed:
+            o.round_state_f32()
+        xg = g.get(sphg.FIELD_X)
+        assert np.all(xg[:, 2] >= lo32) and np.all(xg[:, 0] <= hi32)
+        sg, so = g.scalars(), o.scalars()
+        if sg.iterations != int(so["iterations"]) or (method == sphg.DFSPH and sg.iterations_v != int(so["iterations_v"])):
+            print("iteration mismatch", sg.iterations, so["iterations"])
+            if not reseed:
+                break              # the free-running states part from here
+            continue
+        matched += 1
+        vg, vo = g.get(sphg.FIELD_V).astype(np.float64), o.get(po.OF_V)
+        # A particle resting on a face whose outward velocity moves it less
+        # than one FP32 step is clamped in FP64 and not in FP32: its normal
+        # component may differ by up to 2 ulp(face) / dt.
+        slack = 2.0 * max(np.spacing(hi32), np.spacing(lo32)) / sg.time_step
+        face = {0: xg[:, 0] == hi32, 2: xg[:, 2] == lo32}
+        for a, on in face.items():
+            d = np.abs(vg[on, a] - vo[on, a])
+            assert np.all(d <= slack + TOL * np.abs(vo).max()), (step, a, d.max(), slack)
+            vg[on, a] = vo[on, a]
+        errs = {"x": rel_norm(xg, o.get(po.OF_X)), "v": rel_norm(vg, vo)}
+        print("step", step, "errors", {k: f"{v:.2e}" for k, v in errs.items()}, "face slack", f"{slack:.1e}")
+        assert all(v < TOL for v in errs.values()), (step, errs)
+    assert matched >= 4
+    if not reseed:
+        g.step(20)             # run on past the comparison window
+    # particles sit on the clamped faces
+    xg = g.get(sphg.FIELD_X)
+    assert np.sum(xg[:, 2] == lo32) > 0 and np.sum(xg[:, 0] == hi32) > 0
+    g.close()

@@ -62,9 +62,23 @@ def test_device_other_than_gpu_is_refused(tmp_path):
 
 
 def test_unsupported_boundary_method_is_refused(tmp_path):
-    path = _variant(tmp_path, lambda d: d["Configuration"].__setitem__("boundaryMethod", 0))
+    # the PCISPH boundary method (1) is not built; its scene keys parse
+    def edit(d):
+        d["Configuration"]["boundaryMethod"] = 1
+        for b in d["Boundary"]:
+            b["normalFct"], b["tangFct"] = 0.0, 1.0
+            for box in b["box"]:
+                box["inverted"] = False
+    path = _variant(tmp_path, edit)
     p = _run("-f", path)
     assert p.returncode == 1 and "boundaryMethod" in p.stderr
+
+
+def test_cube_boundary_method_parses(tmp_path):
+    path = _variant(tmp_path, lambda d: d["Configuration"].__setitem__("boundaryMethod", 0))
+    p = _run("-f", path, "--info")
+    assert p.returncode == 0, p.stderr
+    assert json.loads(p.stdout)["boundaryMethod"] == 0
 
 
 def test_emitters_are_refused(tmp_path):

@@ -305,3 +305,73 @@ def test_boundary_viscosity_opposes_motion(method):
         # artificial: sum_b c_b g_b (v.r)/(r^2 + eps) r with g_b < 0: a.v <= 0
         assert np.all(a[near] @ U <= 1e-12)
         assert (a[near] @ U).sum() < 0.0
+
+
+
+# ------------------------------------------------------- cube boundary
+# src/CubeBoundaryModel.cpp:16-94, restated in oracle cubeCorrect().
+
+def test_cube_clamp_single_particle_closed_form():
+    """One particle, no neighbors, no gravity, no pressure (its density m W(0)
+    is rho0/pi < rho0): a WCSPH step moves it by v dt out of the box through
+    the x and y minimum faces.  The clamp sets x, y to the face, and per axis
+    in x, y, z order multiplies the outward component by -normalFct and the
+    other two by tangentialFct, so the y branch rescales the vx the x branch
+    already set: v = (-1, -2, 0.5) -> (0.5, -1.6, 0.4) -> (0.4, 0.8, 0.32)."""
+    dt = 1e-3
+    p = _params(r=0.025, method=sphg.WCSPH, gravity=(0.0, 0.0, 0.0), time_step=dt, min_time_step=dt,
+                max_time_step=dt, boundary_method=sphg.BOUNDARY_CUBE, cube_enabled=1,
+                cube_min=(0.4995, 0.499, 0.0), cube_max=(1.0, 1.0, 1.0), cube_normal_fct=0.5,
+                cube_tangential_fct=0.8)
+    x = np.array([[0.5, 0.5, 0.5]], np.float32)
+    v = np.array([[-1.0, -2.0, 0.5]], np.float32)
+    o = po.Oracle(p, x, np.zeros((0, 3), np.float32), velocities=v)
+    o.step(1)
+    xo, vo = o.get(po.OF_X)[0], o.get(po.OF_V)[0]
+    assert xo[0] == 0.4995 and xo[1] == 0.499 and math.isclose(xo[2], 0.5 + 0.5 * dt, rel_tol=1e-15)
+    assert np.allclose(vo, [0.4, 0.8, 0.32], rtol=1e-15, atol=0)
+    # inside the box nothing changes
+    o2 = po.Oracle(p, np.array([[0.7, 0.7, 0.5]], np.float32), np.zeros((0, 3), np.float32), velocities=v)
+    o2.step(1)
+    assert np.array_equal(o2.get(po.OF_V)[0], v[0].astype(np.float64))
+
+
+def test_fp32_state_rounding_helpers():
+    """The oracle's FP32 storage emulation (used to follow a free-running FP32
+    solver): round_state_f32 leaves every position and velocity an FP32 value,
+    a position on a cube face the nearest FP32 value inside the box, and keeps
+    the densities; set_f32_positions makes DFSPH's updated positions FP32."""
+    spec = configs.small_scene(sphg.DFSPH, n_side=6)
+    f = spec.fluid.astype(np.float64)
+    r = spec.params.particle_radius
+    # faces a quarter ulp inside an FP32 value: the nearest FP32 lies outside
+    zl = np.float32(f[:, 2].min() + 0.01 * r)
+    xh = np.float32(f[:, 0].max() - 0.01 * r)
+    lo = (f[:, 0].min() - 1.0, f[:, 1].min() - 1.0, float(zl) + 0.25 * float(np.spacing(zl)))
+    hi = (float(xh) - 0.25 * float(np.spacing(xh)), f[:, 1].max() + 1.0, f[:, 2].max() + 1.0)
+    assert np.float64(np.float32(hi[0])) > hi[0] and np.float64(np.float32(lo[2])) < lo[2]
+    from dataclasses import replace
+    params = replace(spec.params, boundary_method=sphg.BOUNDARY_CUBE, cube_enabled=1, cube_min=lo, cube_max=hi)
+    empty = np.zeros((0, 3), np.float32)
+    o = po.Oracle(params, spec.fluid, empty)
+    o.step(1)
+    x = o.get(po.OF_X)
+    assert np.any(x[:, 0] == hi[0]) and np.any(x[:, 2] == lo[2])       # clamped onto the faces
+    rho = o.get(po.OF_RHO)
+    o.round_state_f32()
+    x2, v2 = o.get(po.OF_X), o.get(po.OF_V)
+    assert np.array_equal(x2, x2.astype(np.float32).astype(np.float64))
+    assert np.array_equal(v2, v2.astype(np.float32).astype(np.float64))
+    assert np.all(x2[:, 0] <= hi[0]) and np.all(x2[:, 2] >= lo[2])
+    assert np.all(np.abs(x2 - x) <= np.spacing(np.abs(x).astype(np.float32)).astype(np.float64))
+    assert np.array_equal(o.get(po.OF_RHO), rho)
+    # without the cube, DFSPH's updated positions are FP32 values
+    o2 = po.Oracle(spec.params, spec.fluid, spec.boundary)
+    o2.set_f32_positions(True)
+    o2.step(1)
+    x3 = o2.get(po.OF_X)
+    assert np.array_equal(x3, x3.astype(np.float32).astype(np.float64))
+    o3 = po.Oracle(spec.params, spec.fluid, spec.boundary)
+    o3.step(1)
+    assert not np.array_equal(o3.get(po.OF_X), x3)
+    assert np.abs(o3.get(po.OF_X) - x3).max() < 1e-6
