@@ -83,6 +83,27 @@ typedef struct sphg_scene_desc {
     double cube_normal_fct, cube_tangential_fct;
 } sphg_scene_desc;
 
+/* Emitter types (src/Emitter.cpp:7-8). */
+enum { SPHG_EMITTER_SQUARE = 0, SPHG_EMITTER_CIRCLE = 1 };
+
+/* One particle emitter (src/Emitter.h:9-55), with the scene file's emitter
+ * keys (src/SceneLoader.cpp:123-139).  Every time the simulation time
+ * reaches the next emission time and a whole group is left in the budget,
+ * one group (a square lattice of width x height, or a disc of radius width,
+ * at spacing 2r) enters with velocity `velocity` along the emitter's local
+ * +z, rotated by the quaternion and offset to `position`; the next emission
+ * time advances by spacing * 1.105 * 2r / velocity. */
+typedef struct sphg_emitter_desc {
+    uint32_t type;                  /* SPHG_EMITTER_SQUARE or SPHG_EMITTER_CIRCLE */
+    uint32_t num_particles;         /* budget: particles emitted in total */
+    double position[3];
+    double velocity;
+    double rotation[4];             /* quaternion (w, x, y, z), the scene file's order */
+    double start_time;              /* first emission time */
+    double width, height;           /* square sides; a circle's radius is width */
+    double spacing;
+} sphg_emitter_desc;
+
 /* Device-side per-scene state, as the reference's solver members
  * (src/DFSPHSolver.h:26-31, src/PCISPHSolver.h:17, src/TimeManager.h:30-35). */
 typedef struct sphg_scene_scalars {
@@ -142,6 +163,16 @@ int sphg_scene_destroy(sphg_scene* scene);
 int sphg_scene_set_fluid(sphg_scene* scene, const float* xyz, const float* vel, uint32_t n);
 /* BoundaryModel::init / AkinciBoundaryModel::addCube... (src/BoundaryModel.cpp:3-9). */
 int sphg_scene_set_boundary(sphg_scene* scene, const float* xyz, uint32_t nb);
+/* FluidModel::addEmitter (src/FluidModel.cpp:124-143): after set_fluid (or
+ * with no fluid set: an inflow-only scene), before init.  The capacity
+ * grows by the emitter's budget; emitted particles are activated on the
+ * device, in emitter order, with ids continuing from the active count
+ * (Emitter::emitParticles, src/Emitter.cpp:85-162). */
+int sphg_scene_add_emitter(sphg_scene* scene, const sphg_emitter_desc* emitter);
+/* FluidModel::getNumParticles: fluid particles plus every emitter's budget.
+ * sphg_scene_info's n is the active count (getNumActiveParticles), which the
+ * field read-backs cover. */
+int sphg_scene_capacity(sphg_scene* scene, uint32_t* capacity);
 /* Simulation::init -> solver->init(): boundary volumes, and for DFSPH the
  * first grid, densities and alpha (src/Simulation.cpp:102-110,
  * src/SPHSolver.cpp:22-39, src/DFSPHSolver.cpp:24-36). */

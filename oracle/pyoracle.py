@@ -52,6 +52,20 @@ class OracleDesc(C.Structure):
     ]
 
 
+class OracleEmitter(C.Structure):
+    _fields_ = [
+        ("type", C.c_uint),
+        ("num_particles", C.c_uint),
+        ("position", C.c_double * 3),
+        ("velocity", C.c_double),
+        ("rotation", C.c_double * 4),
+        ("start_time", C.c_double),
+        ("width", C.c_double),
+        ("height", C.c_double),
+        ("spacing", C.c_double),
+    ]
+
+
 _libs = {}
 
 
@@ -77,6 +91,7 @@ def load(fast: bool = False) -> C.CDLL:
         "oracle_init": (C.c_int, [P]),
         "oracle_set_state": (C.c_int, [P, fp, fp, fp, fp, D, U]),
         "oracle_step": (C.c_int, [P, U]),
+        "oracle_add_emitter": (C.c_int, [P, C.POINTER(OracleEmitter)]),
         "oracle_round_state_f32": (C.c_int, [P]),
         "oracle_set_f32_positions": (C.c_int, [P, C.c_int]),
         "oracle_num_fluid": (U, [P]),
@@ -157,6 +172,18 @@ class Oracle:
 
     def step(self, n: int = 1):
         self.lib.oracle_step(self.h, int(n))
+        self.n = int(self.lib.oracle_num_fluid(self.h))
+
+    def add_emitter(self, e) -> int:
+        """FluidModel::addEmitter: `e` has the fields of sph_amd.sphg.Emitter.
+        Returns the particles per emitted group."""
+        d = OracleEmitter()
+        d.type, d.num_particles = int(e.type), int(e.num_particles)
+        d.position[:] = [float(x) for x in e.position]
+        d.velocity = float(e.velocity)
+        d.rotation[:] = [float(x) for x in e.rotation]
+        d.start_time, d.width, d.height, d.spacing = float(e.start_time), float(e.width), float(e.height), float(e.spacing)
+        return int(self.lib.oracle_add_emitter(self.h, C.byref(d)))
 
     def set_state(self, x, v, k=None, kv=None, time_step=None, last_iterations=0):
         x = np.ascontiguousarray(np.asarray(x, dtype=np.float32).reshape(-1, 3))

@@ -229,9 +229,81 @@ enum {
     OF_ACC_STEP = 13, OF_BX = 14
 };
 
+// One emitter (src/Emitter.h:9-55, FluidModel::addEmitter
+// src/FluidModel.cpp:124-143).  rotation is the quaternion (w, x, y, z) of
+// the scene file's "rotation" array (src/SceneLoader.cpp:128-131).
+typedef struct oracle_emitter {
+    unsigned type;           // 0 square, 1 circle (src/Emitter.cpp:7-8)
+    unsigned num_particles;
+    double position[3];
+    double velocity;
+    double rotation[4];
+    double start_time;
+    double width, height, spacing;
+} oracle_emitter;
+
 }  // extern "C"
 
 namespace {
+
+inline V3 cross(V3 a, V3 b) { return V3(a.y * b.z - b.y * a.z, a.z * b.x - b.z * a.x, a.x * b.y - b.x * a.y); }
+
+// Quaternion times vector as glm evaluates it: with u = (q.x, q.y, q.z),
+// v + ((u x v) w + u x (u x v)) 2 (a rotation when |q| = 1).
+inline V3 quatRotate(const double q[4], V3 v) {
+    const V3 u(q[1], q[2], q[3]);
+    const V3 uv = cross(u, v);
+    const V3 uuv = cross(u, uv);
+    return v + ((uv * q[0]) + uuv) * 2.0;
+}
+
+struct Emitter {
+    oracle_emitter e;
+    unsigned remaining = 0;      // numParticles: what is still to emit
+    double nextTimeEmit = 0;
+    std::vector<V3> group;
+
+    // src/Emitter.cpp:10-29
+    void setup(const oracle_emitter& ee, double particleRadius) {
+        e = ee;
+        remaining = e.num_particles;
+        nextTimeEmit = e.start_time;
+        if (e.type == 1) buildCircleGroup(particleRadius);
+        else if (e.type == 0) buildSquareGroup(particleRadius);
+    }
+    // src/Emitter.cpp:31-55: the centre, then rings of radius (l+1) dist
+    // holding floor(2 pi radius / dist) particles at angles i 2pi/nParts.
+    void buildCircleGroup(double particleRadius) {
+        const double dist = 2.0 * particleRadius;
+        const double lv = std::floor((e.width - 0.5 * dist) / dist);
+        const unsigned numLevels = lv > 0 ? (unsigned)lv : 0u;
+        group.clear();
+        group.push_back(V3(0.0, 0.0, 0.0));
+        for (unsigned l = 0; l < numLevels; ++l) {
+            const double radius = (l + 1) * dist;
+            const double len = 2.0 * M_PI * radius;
+            const unsigned nParts = (unsigned)std::floor(len / dist);
+            const double inc = 2 * M_PI / nParts;
+            for (unsigned i = 0; i < nParts; ++i) {
+                const double angle = i * inc;
+                group.push_back(V3(radius * std::sin(angle), radius * std::cos(angle), 0.0));
+            }
+        }
+    }
+    // src/Emitter.cpp:57-75: numX x numY lattice at spacing dist, centred,
+    // x index outer, y inner, decreasing from +half size.
+    void buildSquareGroup(double particleRadius) {
+        const double dist = particleRadius * 2.0;
+        const unsigned numX = (unsigned)std::floor(e.width / dist);
+        const unsigned numY = (unsigned)std::floor(e.height / dist);
+        const double realWidth = (double)(numX - 1) * dist;
+        const double realHeight = (double)(numY - 1) * dist;
+        group.clear();
+        for (unsigned i = 0; i < numX; ++i)
+            for (unsigned j = 0; j < numY; ++j)
+                group.push_back(V3(realWidth * 0.5 - i * dist, realHeight * 0.5 - j * dist, 0.0));
+    }
+};
 
 struct Oracle {
     oracle_desc d;
@@ -264,8 +336,47 @@ struct Oracle {
     std::vector<V3> normal;
 
     bool initialised = false;
+    std::vector<Emitter> emitters;
 
     size_t N() const { return x.size(); }
+
+    // Appends one particle to the active set.  The reference's arrays are
+    // sized for every emitter's budget up front and read only below the
+    // active count, so a new particle's fields start at zero (k, kappa_v
+    // included) and its position and velocity are the emitter's.
+    void appendParticle(V3 pos, V3 vel) {
+        x.push_back(pos); v.push_back(vel); a.push_back(V3()); accStep.push_back(V3());
+        rho.push_back(0.0); p.push_back(0.0);
+        alpha.push_back(0.0); divErr.push_back(0.0); rhoStar.push_back(0.0); k.push_back(0.0); kv.push_back(0.0);
+        predR.push_back(V3()); predV.push_back(V3()); pacc.push_back(V3());
+        normal.push_back(V3());
+    }
+    // Simulation::emitParticles (src/Simulation.cpp:325-329) ->
+    // Emitter::emitParticles (src/Emitter.cpp:85-97) and emitSquare /
+    // emitCircle (:99-162), which are the same: when time >= nextTimeEmit
+    // and a whole group is left in the budget, the group is emitted with
+    // velocity (0, 0, v) and position group + t (velocity + g t), t = time -
+    // nextTimeEmit, both rotated by the quaternion, then offset by the
+    // emitter position; nextTimeEmit advances by spacing 1.105 dist / v.
+    void emitParticles() {
+        for (auto& em : emitters) {
+            if (!(time >= em.nextTimeEmit)) continue;
+            const double timeOffset = time - em.nextTimeEmit;
+            const double dist = 2.0 * r;
+            if (em.group.size() <= em.remaining) {
+                for (size_t i = 0; i < em.group.size(); ++i) {
+                    V3 velocity(0, 0, em.e.velocity);
+                    V3 position = em.group[i] + (velocity + g * timeOffset) * timeOffset;
+                    position = quatRotate(em.e.rotation, position);
+                    velocity = quatRotate(em.e.rotation, velocity);
+                    position += V3(em.e.position[0], em.e.position[1], em.e.position[2]);
+                    appendParticle(position, velocity);
+                }
+                em.remaining -= (unsigned)em.group.size();
+                em.nextTimeEmit += em.e.spacing * 1.105 * dist / em.e.velocity;
+            }
+        }
+    }
 
     // ------------------------------------------------------------ grid
     // src/HashTable.h:176-196 clear(): empties per-cell point and neighbor
@@ -599,6 +710,7 @@ struct Oracle {
         computeNonPressureForces();
         integrate();
         cubeCorrect();   // :51-60
+        emitParticles(); // :75
         time += ts;
     }
 
@@ -717,6 +829,7 @@ struct Oracle {
         pressureSolver();
         pcisphIntegrate();
         cubeCorrect();   // src/PCISPHSolver.cpp:54-63
+        emitParticles(); // :78
         time += ts;
     }
 
@@ -936,6 +1049,7 @@ struct Oracle {
         predictVelocities();
         correctDensityError();
         updatePositions();
+        emitParticles();   // src/DFSPHSolver.cpp:77
         insertFluid();
         neighborhoodSearch();
         computeDensities();
@@ -1042,6 +1156,14 @@ int oracle_set_boundary(void* h, const float* xyz, unsigned nb) {
     o->vb.assign(nb, 0.0);
     for (unsigned i = 0; i < nb; ++i) o->xb[i] = V3(xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]);
     return 0;
+}
+// FluidModel::addEmitter (src/FluidModel.cpp:124-143), before init.
+int oracle_add_emitter(void* h, const oracle_emitter* e) {
+    Oracle* o = (Oracle*)h;
+    Emitter em;
+    em.setup(*e, o->r);
+    o->emitters.push_back(em);
+    return (int)em.group.size();
 }
 int oracle_init(void* h) { ((Oracle*)h)->init(); return 0; }
 

@@ -45,12 +45,12 @@ constexpr int kDefaultMaxNbr = 96;
 enum Kind {
     K_KEYS, K_SCAN, K_SCATTER, K_FINALIZE, K_REORDER, K_BUILD, K_NORMALS, K_FORCES, K_STEPBEGIN, K_DT,
     K_DFPREDICT, K_DFVEL_WARM, K_DFVEL_DENS, K_DFVEL_DIV, K_DFERR_DENS, K_DFERR_DIV, K_DFPOS, K_PCIDENS,
-    K_PCIPACC, K_PCIINT, K_TIME, K_CUBE, K_NKINDS
+    K_PCIPACC, K_PCIINT, K_TIME, K_CUBE, K_EMIT, K_NKINDS
 };
 const char* kKindNames[K_NKINDS] = {
     "keys", "scan", "scatter", "finalize", "rank_reorder", "build", "normals", "forces", "step_begin", "dt",
     "df_predict", "df_vel_warm", "df_vel_density", "df_vel_divergence", "df_err_density", "df_err_divergence",
-    "df_positions", "pci_density", "pci_pacc", "pci_integrate", "time_advance", "cube"};
+    "df_positions", "pci_density", "pci_pacc", "pci_integrate", "time_advance", "cube", "emit"};
 
 struct Profiler {
     bool on = false;
@@ -150,6 +150,7 @@ struct Batch {
     int minIt = 0, minItV = 0;   // smallest minimum iteration counts in the batch
     int lastIt = 0, lastItV = 0; // iterations the loops needed in the previous step
     std::vector<int> parity;   // parity of each scene when the tables were built
+    std::vector<uint32_t> nAt; // active count of each scene in the tables
     ~Batch() {
         if (dT[0]) (void)hipFree(dT[0]);
         if (dT[1]) (void)hipFree(dT[1]);
@@ -168,7 +169,9 @@ struct sphg_scene {
     bool ready = false;
     std::vector<float> fxyz, fvel, bxyz;
     std::vector<uint32_t> border;   // sorted boundary slot -> input index
-    uint32_t n = 0, nb = 0;
+    std::vector<sphg_emitter_desc> emitters;
+    uint32_t n = 0, nb = 0;         // n: active particles
+    uint32_t cap = 0;               // particle slots: fluid + every emitter's budget
     double dt0 = 0, dtMin = 0, dtMax = 0;
     char* mem = nullptr;
     size_t memBytes = 0;
@@ -308,6 +311,7 @@ int upload_batch_tables(sphg_ctx* ctx, Batch& B) {
     std::vector<DevScene> t0(ns), t1(ns);
     std::vector<Chunk> P, Cc;
     B.parity.resize(ns);
+    B.nAt.resize(ns);
     B.maxIt = 0;
     B.minIt = 1 << 30;
     B.minItV = 1 << 30;
@@ -316,6 +320,7 @@ int upload_batch_tables(sphg_ctx* ctx, Batch& B) {
         t0[s] = sc->view[sc->par];
         t1[s] = sc->view[1 - sc->par];
         B.parity[s] = sc->par;
+        B.nAt[s] = sc->n;
         for (int c = 0; c < t0[s].nChunks; ++c) P.push_back(Chunk{s, c * kBlock});
         for (int c = 0; c < t0[s].nCellChunks; ++c) Cc.push_back(Chunk{s, c * kCellChunk});
         B.maxIt = std::max(B.maxIt, std::max(t0[s].maxIt, std::max(t0[s].maxItV, 0)));
@@ -378,7 +383,7 @@ void flip(Batch& B) {
 // The one-scene tables of s, refreshed if another batch moved its parity.
 int self_tables(sphg_scene* s) {
     Batch& B = *s->self;
-    if (B.parity[0] != s->par) return upload_batch_tables(s->ctx, B);
+    if (B.parity[0] != s->par || B.nAt[0] != s->n) return upload_batch_tables(s->ctx, B);
     return SPHG_OK;
 }
 
@@ -422,11 +427,12 @@ int batch_one_step(sphg_ctx* ctx, Batch& B, uint32_t& syncs) {
     hipStream_t st = ctx->stream;
     const int method = B.scenes[0]->desc.method;
     const int ns = (int)B.scenes.size();
-    int anyFlags = 0, anySt = 0, anyCube = 0;
+    int anyFlags = 0, anySt = 0, anyCube = 0, anyEmit = 0;
     for (auto* s : B.scenes) {
         anyFlags |= s->view[0].flags;
         anySt |= (s->view[0].flags & F_ST);
         anyCube |= s->view[0].cubeOn;
+        anyEmit |= s->view[0].nEmit;
     }
     int rc;
     bool more = false;
@@ -454,6 +460,9 @@ int batch_one_step(sphg_ctx* ctx, Batch& B, uint32_t& syncs) {
         }
         B.lastIt = seen;
         { Scope s(P, st, K_DFPOS); launch_df_positions(st, T, B.dP, B.nP); }
+        // emitters between the position update and the grid rebuild
+        // (src/DFSPHSolver.cpp:77); the new particles' cells are counted
+        if (anyEmit) { Scope s(P, st, K_EMIT); launch_emit(st, B.dT[0], B.dT[1], ns, 1); }
         grid_sort(ctx, B, T);
         const DevScene* T2 = B.dT[1];
         launch_fill_int(st, ctx->dActive, 0, 2);
@@ -483,8 +492,10 @@ int batch_one_step(sphg_ctx* ctx, Batch& B, uint32_t& syncs) {
         { Scope s(P, st, K_BUILD); launch_build(st, WCSPH, T2, B.dP, B.nP); }
         if (anySt) { Scope s(P, st, K_NORMALS); launch_normals(st, T2, B.dP, B.nP); }
         { Scope s(P, st, K_FORCES); launch_forces(st, WCSPH, T2, B.dP, B.nP); }
-        // cube boundary after the integration (src/WCSPHSolver.cpp:51-60)
+        // cube boundary after the integration (src/WCSPHSolver.cpp:51-60),
+        // then the emitters (:75), then time += dt (:77)
         if (anyCube) { Scope s(P, st, K_CUBE); launch_cube(st, T2, B.dP, B.nP); }
+        if (anyEmit) { Scope s(P, st, K_EMIT); launch_emit(st, B.dT[1], B.dT[0], ns, 0); }
         { Scope s(P, st, K_TIME); launch_time_advance(st, T2, ns); }
         flip(B);
     } else {
@@ -507,8 +518,11 @@ int batch_one_step(sphg_ctx* ctx, Batch& B, uint32_t& syncs) {
         }
         B.lastIt = seen;
         { Scope s(P, st, K_PCIINT); launch_pci_integrate(st, T2, B.dP, B.nP); }
-        // cube boundary after the integration (src/PCISPHSolver.cpp:54-63)
+        // cube boundary after the integration (src/PCISPHSolver.cpp:54-63),
+        // then the emitters (:78), then time += dt (:80)
         if (anyCube) { Scope s(P, st, K_CUBE); launch_cube(st, T2, B.dP, B.nP); }
+        if (anyEmit) { Scope s(P, st, K_EMIT); launch_emit(st, B.dT[1], B.dT[0], ns, 0); }
+        { Scope s(P, st, K_TIME); launch_time_advance(st, T2, ns); }
         flip(B);
     }
     return SPHG_OK;
@@ -556,11 +570,48 @@ int run_batch(sphg_ctx* ctx, Batch& B, uint32_t nsteps, sphg_step_stats* stats) 
             stats->iterations_v += now.sumItV - before[s].sumItV;
         }
         overflow = std::max(overflow, now.overflow);
+        sphg_scene* sc = B.scenes[s];
+        if (!sc->emitters.empty() && now.nActive != sc->n) {
+            sc->n = now.nActive;
+            sc->view[0].n = sc->view[1].n = (int)now.nActive;
+            B.nAt[s] = now.nActive;
+        }
     }
     if (overflow)
         return fail(SPHG_EOVERFLOW, "a particle had " + std::to_string(overflow) +
                                         " neighbors, above max_neighbors; results of that step are invalid");
     return SPHG_OK;
+}
+
+// An emitter's group in local coordinates (src/Emitter.cpp:31-75), at
+// particle spacing dist = 2r.  Circle: the centre, then for ring l = 1 ..
+// floor((width - dist/2) / dist) of radius l dist, floor(2 pi l dist / dist)
+// particles at angles i 2pi/count, placed (radius sin, radius cos, 0).
+// Square: floor(width / dist) x floor(height / dist) particles, x index
+// outer, at (W/2 - i dist, H/2 - j dist, 0) with W, H the lattice extents.
+void emitter_group(const sphg_emitter_desc& e, double particleRadius, std::vector<double>& out) {
+    const double dist = 2.0 * particleRadius;
+    auto put = [&](double x, double y) { out.push_back(x); out.push_back(y); out.push_back(0.0); };
+    if (e.type == SPHG_EMITTER_CIRCLE) {
+        const double lv = std::floor((e.width - 0.5 * dist) / dist);
+        const unsigned levels = lv > 0 ? (unsigned)lv : 0u;
+        put(0.0, 0.0);
+        for (unsigned l = 0; l < levels; ++l) {
+            const double radius = (l + 1) * dist;
+            const unsigned count = (unsigned)std::floor(2.0 * M_PI * radius / dist);
+            const double inc = 2 * M_PI / count;
+            for (unsigned i = 0; i < count; ++i) {
+                const double angle = i * inc;
+                put(radius * std::sin(angle), radius * std::cos(angle));
+            }
+        }
+    } else {
+        const unsigned nx = (unsigned)std::floor(e.width / dist);
+        const unsigned ny = (unsigned)std::floor(e.height / dist);
+        const double w = (double)(nx - 1) * dist, hgt = (double)(ny - 1) * dist;
+        for (unsigned i = 0; i < nx; ++i)
+            for (unsigned j = 0; j < ny; ++j) put(w * 0.5 - i * dist, hgt * 0.5 - j * dist);
+    }
 }
 
 // Grid extent and device memory plan of a scene (one allocation).
@@ -572,6 +623,9 @@ struct Plan {
     size_t oX[2], oV[2], oK[2] = {0, 0}, oKV[2] = {0, 0}, oID[2];
     size_t oA, oXK, oGB, oAUX, oXS, oPACC, oGBS, oNRM, oADH, oRHO, oALPHA, oDRHOS, oDIV, oP;
     size_t oCELLU, oCELL, oPERM, oCC, oCS, oCCS, oNBR, oNNB, oNG, oGC, oXB, oBCS, oPART, oSS, oTMP;
+    size_t oEMIT = 0, oGROUP = 0;
+    std::vector<double> group;          // every emitter's group, 3 doubles per particle
+    std::vector<int> groupOff, groupSize;
 };
 
 int make_plan(const sphg_scene* s, Plan& P) {
@@ -609,7 +663,7 @@ int make_plan(const sphg_scene* s, Plan& P) {
     }
     if (ncells > (1L << 28)) return fail(SPHG_EINVAL, "grid has more than 2^28 cells; set domain_min/max");
 
-    const uint32_t n = s->n, nb = s->nb;
+    const uint32_t n = s->cap, nb = s->nb;
     const int nChunks = (int)((n + kBlock - 1) / kBlock);
     const size_t npad = (size_t)nChunks * kBlock;
     const int nCellChunks = (int)((ncells + 1 + kCellChunk - 1) / kCellChunk);
@@ -667,6 +721,17 @@ int make_plan(const sphg_scene* s, Plan& P) {
     P.oPART = A.reserve(sizeof(double) * (size_t)std::max(nChunks, 1));
     P.oSS = A.reserve(sizeof(Scalars));
     P.oTMP = A.reserve(std::max(f4, sizeof(float4) * std::max<size_t>(nb, 1)));
+    if (!s->emitters.empty()) {
+        P.group.clear();
+        for (const auto& e : s->emitters) {
+            P.groupOff.push_back((int)(P.group.size() / 3));
+            const size_t before = P.group.size();
+            emitter_group(e, d.particle_radius, P.group);
+            P.groupSize.push_back((int)((P.group.size() - before) / 3));
+        }
+        P.oEMIT = A.reserve(sizeof(EmitterDev) * s->emitters.size());
+        P.oGROUP = A.reserve(sizeof(double) * std::max<size_t>(P.group.size(), 1));
+    }
     P.ncells = ncells;
     P.nChunks = nChunks;
     P.nCellChunks = nCellChunks;
@@ -776,7 +841,36 @@ int sphg_scene_set_fluid(sphg_scene* s, const float* xyz, const float* vel, uint
     s->fxyz.assign(xyz, xyz + 3 * (size_t)n);
     if (vel) s->fvel.assign(vel, vel + 3 * (size_t)n);
     else s->fvel.assign(3 * (size_t)n, 0.f);
+    uint64_t cap = n;
+    for (const auto& e : s->emitters) cap += e.num_particles;
+    if (cap > (uint64_t)INT32_MAX) return fail(SPHG_EINVAL, "more than 2^31 - 1 particle slots");
     s->n = n;
+    s->cap = (uint32_t)cap;
+    return SPHG_OK;
+}
+
+int sphg_scene_add_emitter(sphg_scene* s, const sphg_emitter_desc* e) {
+    if (!s || !e) return fail(SPHG_EINVAL, "NULL argument");
+    if (s->ready) return fail(SPHG_ESTATE, "add_emitter after init");
+    if (e->type != SPHG_EMITTER_SQUARE && e->type != SPHG_EMITTER_CIRCLE)
+        return fail(SPHG_EINVAL, "emitter type must be 0 (square) or 1 (circle)");
+    bool finite = std::isfinite(e->velocity) && std::isfinite(e->start_time) && std::isfinite(e->width) &&
+                  std::isfinite(e->height) && std::isfinite(e->spacing);
+    for (int c = 0; c < 3; ++c) finite &= std::isfinite(e->position[c]);
+    for (int c = 0; c < 4; ++c) finite &= std::isfinite(e->rotation[c]);
+    if (!finite) return fail(SPHG_EINVAL, "emitter values must be finite");
+    const double dist = 2.0 * s->desc.particle_radius;
+    if (e->width / dist > 1e6 || e->height / dist > 1e6) return fail(SPHG_EINVAL, "emitter group too large");
+    const uint64_t cap = (uint64_t)s->cap + e->num_particles;
+    if (cap > (uint64_t)INT32_MAX) return fail(SPHG_EINVAL, "more than 2^31 - 1 particle slots");
+    s->emitters.push_back(*e);
+    s->cap = (uint32_t)cap;
+    return SPHG_OK;
+}
+
+int sphg_scene_capacity(sphg_scene* s, uint32_t* capacity) {
+    if (!s || !capacity) return fail(SPHG_EINVAL, "NULL argument");
+    *capacity = s->cap;
     return SPHG_OK;
 }
 
@@ -793,7 +887,7 @@ int sphg_scene_set_boundary(sphg_scene* s, const float* xyz, uint32_t nb) {
 int sphg_scene_init(sphg_scene* s) {
     if (!s || !s->ctx) return fail(SPHG_EINVAL, "NULL scene");
     if (s->ready) return fail(SPHG_ESTATE, "scene already initialised");
-    if (s->n == 0) return fail(SPHG_ESTATE, "no fluid particles");
+    if (s->cap == 0) return fail(SPHG_ESTATE, "no fluid particles and no emitters");
     sphg_ctx* ctx = s->ctx;
     HIP_TRY(hipSetDevice(ctx->device));
     const sphg_scene_desc& d = s->desc;
@@ -868,6 +962,7 @@ int sphg_scene_init(sphg_scene* s) {
     D.nx = dims[0]; D.ny = dims[1]; D.nz = dims[2];
     D.ncells = (int)ncells;
     D.n = (int)n;
+    D.cap = (int)s->cap;
     D.nb = (int)nb;
     D.maxNbr = maxNbr;
     D.nChunks = nChunks;
@@ -903,6 +998,30 @@ int sphg_scene_init(sphg_scene* s) {
     D.partial = (double*)(b + P.oPART);
     D.ss = (Scalars*)(b + P.oSS);
     s->ss = D.ss;
+    if (!s->emitters.empty()) {
+        // emitter constants and state, group table (src/Emitter.cpp:10-29)
+        std::vector<EmitterDev> ed(s->emitters.size());
+        for (size_t e = 0; e < s->emitters.size(); ++e) {
+            const sphg_emitter_desc& E = s->emitters[e];
+            EmitterDev& o = ed[e];
+            for (int c = 0; c < 3; ++c) o.pos[c] = E.position[c];
+            for (int c = 0; c < 4; ++c) o.q[c] = E.rotation[c];
+            o.vel = E.velocity;
+            o.interval = E.spacing * 1.105 * (2.0 * r) / E.velocity;   // src/Emitter.cpp:131,159
+            o.next = E.start_time;
+            o.remaining = E.num_particles;
+            o.group = P.groupOff[e];
+            o.groupSize = P.groupSize[e];
+        }
+        D.nEmit = (int)ed.size();
+        D.emit = (EmitterDev*)(b + P.oEMIT);
+        D.emitGroup = (const double*)(b + P.oGROUP);
+        HIP_TRY(hipMemcpyAsync(D.emit, ed.data(), sizeof(EmitterDev) * ed.size(), hipMemcpyHostToDevice, ctx->stream));
+        if (!P.group.empty())
+            HIP_TRY(hipMemcpyAsync((void*)D.emitGroup, P.group.data(), sizeof(double) * P.group.size(),
+                                   hipMemcpyHostToDevice, ctx->stream));
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+    }
     s->tmp = F1(P.oTMP);
     for (int p = 0; p < 2; ++p) {
         DevScene v = D;
@@ -948,6 +1067,7 @@ int sphg_scene_init(sphg_scene* s) {
     // scalars
     Scalars ss0{};
     ss0.dt = d.time_step;
+    ss0.nActive = n;
     HIP_TRY(hipMemcpyAsync(D.ss, &ss0, sizeof(Scalars), hipMemcpyHostToDevice, ctx->stream));
 
     // tables of a one-scene batch
@@ -960,13 +1080,16 @@ int sphg_scene_init(sphg_scene* s) {
     if (nb) launch_bvolume(ctx->stream, s->self->dT[0], 0, (int)nb, s->tmp);
 
     // fluid upload (staging in tmp is too small for xyz+vel: copy through a scratch allocation)
-    float* dstage = nullptr;
-    HIP_TRY(hipMalloc(&dstage, sizeof(float) * 6 * (size_t)n));
-    HIP_TRY(hipMemcpyAsync(dstage, s->fxyz.data(), sizeof(float) * 3 * n, hipMemcpyHostToDevice, ctx->stream));
-    HIP_TRY(hipMemcpyAsync(dstage + 3 * (size_t)n, s->fvel.data(), sizeof(float) * 3 * n, hipMemcpyHostToDevice, ctx->stream));
-    launch_load_state(ctx->stream, s->self->dT[0], 0, (int)n, dstage, dstage + 3 * (size_t)n, nullptr, nullptr);
-    HIP_TRY(hipStreamSynchronize(ctx->stream));
-    (void)hipFree(dstage);
+    if (n) {
+        float* dstage = nullptr;
+        HIP_TRY(hipMalloc(&dstage, sizeof(float) * 6 * (size_t)n));
+        HIP_TRY(hipMemcpyAsync(dstage, s->fxyz.data(), sizeof(float) * 3 * n, hipMemcpyHostToDevice, ctx->stream));
+        HIP_TRY(hipMemcpyAsync(dstage + 3 * (size_t)n, s->fvel.data(), sizeof(float) * 3 * n, hipMemcpyHostToDevice,
+                               ctx->stream));
+        launch_load_state(ctx->stream, s->self->dT[0], 0, (int)n, dstage, dstage + 3 * (size_t)n, nullptr, nullptr);
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        (void)hipFree(dstage);
+    }
 
     if (method == DFSPH) {
         // DFSPHSolver::init: insert, neighborhoodSearch, densities, alpha
@@ -1078,7 +1201,8 @@ int sphg_batch_run(sphg_batch* b, uint32_t nsteps, sphg_step_stats* stats) {
     // a scene stepped elsewhere since the tables were built has moved its
     // double buffers: rebuild the tables from the scenes' current parity
     bool stale = false;
-    for (size_t i = 0; i < B.scenes.size(); ++i) stale |= (B.parity[i] != B.scenes[i]->par);
+    for (size_t i = 0; i < B.scenes.size(); ++i)
+        stale |= (B.parity[i] != B.scenes[i]->par) || (B.nAt[i] != B.scenes[i]->n);
     if (stale) {
         int rc = upload_batch_tables(ctx, B);
         if (rc) return rc;
@@ -1102,7 +1226,7 @@ int sphg_scene_plan_bytes(sphg_scene* s, uint64_t* bytes) {
         *bytes = s->memBytes;
         return SPHG_OK;
     }
-    if (s->n == 0) return fail(SPHG_ESTATE, "no fluid particles");
+    if (s->cap == 0) return fail(SPHG_ESTATE, "no fluid particles and no emitters");
     Plan P;
     int rc = make_plan(s, P);
     if (rc) return rc;

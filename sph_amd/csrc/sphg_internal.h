@@ -68,7 +68,22 @@ struct Scalars {
     double dt, time, maxVel, cflTs;
     double avgErr, avgErr0, avgErrV, avgErrV0, maxDensErr;
     int it, itV, done, doneV;
-    unsigned steps, sumIt, sumItV, ctr, overflow, pad;
+    unsigned steps, sumIt, sumItV, ctr, overflow;
+    unsigned nActive;           // active particles (the emitters raise it)
+};
+
+// One emitter's constants and state (src/Emitter.h:9-55).  Its group's
+// local positions are rows [group, group + groupSize) of the scene's group
+// table (3 doubles each).
+struct EmitterDev {
+    double pos[3];              // emitter origin
+    double q[4];                // rotation quaternion (w, x, y, z)
+    double vel;                 // speed along local +z
+    double interval;            // spacing 1.105 2r / vel between groups
+    double next;                // next emission time
+    unsigned remaining;         // budget left
+    int group, groupSize;
+    int pad;
 };
 
 struct DevScene {
@@ -92,7 +107,10 @@ struct DevScene {
     int kx0, ky0, kz0;          // key of the first unpadded cell
     int nx, ny, nz, ncells;     // padded grid (one empty layer on every side)
     int n, nb, maxNbr, nChunks;
-    int nCellChunks, pad3;
+    int nCellChunks, cap;       // cap: particle slots (active + every emitter's remaining budget)
+    int nEmit, pad3;
+    EmitterDev *emit;
+    const double *emitGroup;
     // particle arrays; the *2 members are the other half of each double buffer
     float4 *x, *x2, *v, *v2;
     float *k, *k2, *kv, *kv2;

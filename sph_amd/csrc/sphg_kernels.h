@@ -26,6 +26,7 @@ void launch_pci_density(hipStream_t st, const DevScene* T, const Chunk* C, int n
 void launch_pci_pacc(hipStream_t st, const DevScene* T, const Chunk* C, int nchunks, int iter);
 void launch_pci_integrate(hipStream_t st, const DevScene* T, const Chunk* C, int nchunks);
 void launch_cube(hipStream_t st, const DevScene* T, const Chunk* C, int n);
+void launch_emit(hipStream_t st, DevScene* Tc, DevScene* To, int nscenes, int cells);
 void launch_time_advance(hipStream_t st, const DevScene* T, int nscenes);
 void launch_fill_int(hipStream_t st, int* p, int v, long n);
 void launch_unsort_f4(hipStream_t st, const float4* src, const uint32_t* id, float* dst, int n, int comps);

@@ -1310,7 +1310,6 @@ __global__ __launch_bounds__(kBlock) void k_pci_integrate(const DevScene* __rest
     S.v[i] = make_float4((float)vx, (float)vy, (float)vz, 0.f);
     S.x[i] = make_float4((float)((double)x0.x + vx * dt), (float)((double)x0.y + vy * dt),
                          (float)((double)x0.z + vz * dt), 0.f);
-    if (c.begin == 0 && threadIdx.x == 0) S.ss->time += S.ss->dt;
 }
 
 // =========================================================== finalize
@@ -1446,6 +1445,86 @@ __global__ __launch_bounds__(kBlock) void k_cube(const DevScene* __restrict__ T,
     S.v[i] = make_float4((float)u[0], (float)u[1], (float)u[2], vf.w);
 }
 
+// Emitters (Simulation::emitParticles -> Emitter::emitParticles,
+// src/Emitter.cpp:85-162), one block per scene, emitters in order: an
+// emitter whose next emission time has come (time >= next) and whose budget
+// holds a whole group appends the group at slots n, n+1, ... (ids equal to
+// the slots, continuing the active count): velocity (0, 0, vel) and
+// position group + t (velocity + g t), t = time - next, both rotated by the
+// quaternion as glm evaluates it (u = (q.x, q.y, q.z):
+// v + ((u x v) w + u x (u x v)) 2), the position then offset by the
+// emitter origin.  Warm-start values start at zero.  With `cells`, the new
+// particles' cells are counted for the grid sort that follows (DFSPH emits
+// between its position update and the grid rebuild).  Tc is the table whose
+// buffers are current at the call (before or after the step's sort), To
+// the other; the new count goes to both and to the scalars.
+__device__ inline void quat_rotate(const double q[4], double& x, double& y, double& z) {
+    const double ux = q[1], uy = q[2], uz = q[3];
+    const double cx = uy * z - y * uz, cy = uz * x - z * ux, cz = ux * y - x * uy;      // u x v
+    const double dx = uy * cz - cy * uz, dy = uz * cx - cz * ux, dz = ux * cy - cx * uy;  // u x (u x v)
+    x = x + ((cx * q[0]) + dx) * 2.0;
+    y = y + ((cy * q[0]) + dy) * 2.0;
+    z = z + ((cz * q[0]) + dz) * 2.0;
+}
+
+__global__ __launch_bounds__(kBlock) void k_emit(DevScene* __restrict__ Tc, DevScene* __restrict__ To, int nscenes,
+                                                 int cells) {
+    const int sc = blockIdx.x;
+    if (sc >= nscenes) return;
+    const DevScene& S = Tc[sc];
+    if (S.nEmit == 0) return;
+    Scalars* ss = S.ss;
+    const double time = ss->time;
+    const int n0 = S.n;
+    int n = n0;
+    for (int e = 0; e < S.nEmit; ++e) {
+        EmitterDev* E = S.emit + e;
+        const int gs = E->groupSize;
+        const bool fire = (time >= E->next) && (E->remaining >= (unsigned)gs) && (n + gs <= S.cap);
+        if (fire) {
+            const double t = time - E->next;
+            double q[4] = {E->q[0], E->q[1], E->q[2], E->q[3]};
+            for (int g = threadIdx.x; g < gs; g += blockDim.x) {
+                const double* lp = S.emitGroup + 3 * (size_t)(E->group + g);
+                double vx = 0.0, vy = 0.0, vz = E->vel;
+                double px = lp[0] + (vx + (double)S.gx * t) * t;
+                double py = lp[1] + (vy + (double)S.gy * t) * t;
+                double pz = lp[2] + (vz + (double)S.gz * t) * t;
+                quat_rotate(q, px, py, pz);
+                quat_rotate(q, vx, vy, vz);
+                px += E->pos[0];
+                py += E->pos[1];
+                pz += E->pos[2];
+                const int slot = n + g;
+                const float4 xp = make_float4((float)px, (float)py, (float)pz, 0.f);
+                S.x[slot] = xp;
+                S.v[slot] = make_float4((float)vx, (float)vy, (float)vz, 0.f);
+                S.id[slot] = (uint32_t)slot;
+                if (S.k) {
+                    S.k[slot] = 0.f;
+                    S.kv[slot] = 0.f;
+                }
+                if (cells) {
+                    const int cl = cell_of(S, xp);
+                    S.cellU[slot] = cl;
+                    atomicAdd(&S.cellCount[cl], 1);
+                }
+            }
+            n += gs;
+        }
+        __syncthreads();   // every thread has read this emitter's state
+        if (fire && threadIdx.x == 0) {
+            E->remaining -= (unsigned)gs;
+            E->next += E->interval;
+        }
+    }
+    if (threadIdx.x == 0 && n != n0) {
+        Tc[sc].n = n;
+        To[sc].n = n;
+        ss->nActive = (unsigned)n;
+    }
+}
+
 // WCSPH: time += dt after the integration (src/WCSPHSolver.cpp:77).
 __global__ void k_time_advance(const DevScene* __restrict__ T, int nscenes) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1570,6 +1649,9 @@ void launch_pci_density(hipStream_t st, const DevScene* T, const Chunk* C, int n
 void launch_pci_pacc(hipStream_t st, const DevScene* T, const Chunk* C, int n, int iter) { if (n) LAUNCH(k_pci_pacc, n, st, T, C, iter); }
 void launch_pci_integrate(hipStream_t st, const DevScene* T, const Chunk* C, int n) { if (n) LAUNCH(k_pci_integrate, n, st, T, C); }
 void launch_cube(hipStream_t st, const DevScene* T, const Chunk* C, int n) { if (n) LAUNCH(k_cube, n, st, T, C); }
+void launch_emit(hipStream_t st, DevScene* Tc, DevScene* To, int nscenes, int cells) {
+    if (nscenes) hipLaunchKernelGGL(k_emit, dim3(nscenes), dim3(kBlock), 0, st, Tc, To, nscenes, cells);
+}
 void launch_time_advance(hipStream_t st, const DevScene* T, int nscenes) {
     hipLaunchKernelGGL(k_time_advance, dim3((nscenes + 255) / 256), dim3(256), 0, st, T, nscenes);
 }

@@ -45,6 +45,13 @@ struct Vector3r {
     Vector3r operator*(Real s) const { return {x * s, y * s, z * s}; }
 };
 
+// glm::quat's (w, x, y, z) constructor order (src/SceneLoader.cpp:131).
+struct Quat4r {
+    Real w = 1, x = 0, y = 0, z = 0;
+    Quat4r() = default;
+    Quat4r(Real a, Real b, Real c, Real d) : w(a), x(b), y(c), z(d) {}
+};
+
 class SphError : public std::runtime_error {
 public:
     using std::runtime_error::runtime_error;
@@ -118,6 +125,30 @@ public:
 
 // ------------------------------------------------------------ models
 
+// A particle emitter (src/Emitter.h:9-55).  It keeps the reference's
+// parameters; the device activates its groups inside the solver step
+// (sphg_scene_add_emitter), so emitParticles() does nothing on the host.
+class Emitter {
+protected:
+    unsigned int type, numParticles;
+    Vector3r r;
+    Real v;
+    Quat4r rot;
+    Real startTime, width, height, spacing;
+
+public:
+    static unsigned int SQUARE_EMITTER;   // src/Emitter.cpp:7
+    static unsigned int CIRCLE_EMITTER;   // src/Emitter.cpp:8
+    Emitter(FluidModel*, unsigned int type, unsigned int numParticles, Vector3r r, Real v, Quat4r rot, Real startTime,
+            Real w, Real h, Real spacing)
+        : type(type), numParticles(numParticles), r(r), v(v), rot(rot), startTime(startTime), width(w), height(h),
+          spacing(spacing) {}
+    void emitParticles() {}
+    unsigned int getType() const { return type; }
+    unsigned int getNumParticles() const { return numParticles; }
+    sphg_emitter_desc desc() const;
+};
+
 class FluidModel {   // src/FluidModel.h:10-82
 protected:
     Real density_0 = 1000;
@@ -125,6 +156,7 @@ protected:
     std::vector<Vector3r> r, v, a;
     unsigned int numActiveParticles = 0;
     std::vector<std::unique_ptr<NonPressureForce>> npForces;
+    std::vector<Emitter> emitters;
 
 public:
     void init(std::vector<Vector3r>& points, std::vector<Vector3r>& velocities);
@@ -144,6 +176,13 @@ public:
     std::vector<Real>& getPressures() { return pressure; }
     unsigned int getNumParticles() const { return (unsigned int)mass.size(); }
     unsigned int getNumActiveParticles() const { return numActiveParticles; }
+    void setNumActiveParticles(unsigned int n) { if (n <= getNumParticles()) numActiveParticles = n; }   // src/FluidModel.h:63
+
+    // src/FluidModel.cpp:124-143: the capacity grows by the emitter's budget
+    void addEmitter(unsigned int type, unsigned int numParticles, Vector3r r, Real v, Quat4r rot, Real startTime,
+                    Real w, Real h, Real spacing);
+    unsigned int numberEmitters() const { return (unsigned int)emitters.size(); }
+    Emitter& getEmitter(unsigned int i) { return emitters[i]; }
 
     void setViscosityForce(Real viscosity, Real boundaryViscosity);   // src/FluidModel.cpp:73-100
     void setSurfaceTensionForce(Real stCoef);                           // :102-111
@@ -363,12 +402,20 @@ struct SphereInfo {           // scene extension: lattice sites inside a sphere
     Real radius = 0;
 };
 
+struct EmitterInfo {   // src/Simulation.h:76-87
+    unsigned int type = 0, numParticles = 0;
+    Vector3r r;
+    Real v = 0;
+    Quat4r rot;
+    Real startTime = 0, width = 0, height = 0, spacing = 0;
+};
+
 struct Fluid {   // src/Simulation.h:87-98
     std::vector<BlockInfo> fluidBlocks;
     std::vector<SphereInfo> fluidSpheres;
     Vector3r sphereLatticeMin, sphereLatticeMax;
     Real viscosity = 0, boundaryViscosity = 0, surfaceTension = 0, adhesion = 0, density0 = 1000;
-    int numEmitters = 0;
+    std::vector<EmitterInfo> emitters;
 };
 
 struct FluidInfo {

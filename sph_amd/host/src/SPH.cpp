@@ -58,6 +58,29 @@ void FluidModel::init(std::vector<Vector3r>& points, std::vector<Vector3r>& velo
 
 void FluidModel::setMasses(Real m) { std::fill(mass.begin(), mass.end(), m); }
 
+unsigned int Emitter::SQUARE_EMITTER = SPHG_EMITTER_SQUARE;
+unsigned int Emitter::CIRCLE_EMITTER = SPHG_EMITTER_CIRCLE;
+
+sphg_emitter_desc Emitter::desc() const {
+    sphg_emitter_desc d{};
+    d.type = type;
+    d.num_particles = numParticles;
+    d.position[0] = r.x; d.position[1] = r.y; d.position[2] = r.z;
+    d.velocity = v;
+    d.rotation[0] = rot.w; d.rotation[1] = rot.x; d.rotation[2] = rot.y; d.rotation[3] = rot.z;
+    d.start_time = startTime;
+    d.width = width;
+    d.height = height;
+    d.spacing = spacing;
+    return d;
+}
+
+void FluidModel::addEmitter(unsigned int type, unsigned int numParticles, Vector3r pos, Real vel, Quat4r rot,
+                            Real startTime, Real w, Real h, Real spacing) {
+    emitters.emplace_back(this, type, numParticles, pos, vel, rot, startTime, w, h, spacing);
+    resizeFluid(getNumParticles() + numParticles);   // masses: Simulation::init sets every slot
+}
+
 void FluidModel::setViscosityForce(Real viscosity, Real boundaryViscosity) {
     Simulation* sim = Simulation::getCurrent();
     const int m = sim->getViscosityMethod();
@@ -207,8 +230,13 @@ void GpuSolver::init() {
     fillDesc(d);
     check(sphg_scene_create(ctx, &d, &scene), "sphg_scene_create");
     FluidModel* fm = sim->getFluidModel(0);
+    const unsigned int n0 = fm->getNumActiveParticles();
     std::vector<float> x = toFloat(fm->getPositions()), v = toFloat(fm->getVelocitiesVec());
-    check(sphg_scene_set_fluid(scene, x.data(), v.data(), fm->getNumParticles()), "set_fluid");
+    if (n0) check(sphg_scene_set_fluid(scene, x.data(), v.data(), n0), "set_fluid");
+    for (unsigned int e = 0; e < fm->numberEmitters(); ++e) {
+        const sphg_emitter_desc ed = fm->getEmitter(e).desc();
+        check(sphg_scene_add_emitter(scene, &ed), "add_emitter");
+    }
     std::vector<Vector3r> bpts;
     if (bm == Simulation::AKINCI_BOUNDARY_METHOD)
         for (unsigned int b = 0; b < sim->numberBoundaryModels(); ++b) {
@@ -246,6 +274,11 @@ void GpuSolver::step() {
     Simulation* sim = Simulation::getCurrent();
     sim->setTimeStep(last.time_step);
     sim->setTime(last.time);
+    if (sim->getFluidModel(0)->numberEmitters()) {
+        uint32_t n = 0;
+        check(sphg_scene_info(scene, &n, nullptr, nullptr, nullptr), "info");
+        sim->getFluidModel(0)->setNumActiveParticles(n);
+    }
     hostFresh = false;
 }
 
@@ -253,12 +286,18 @@ void GpuSolver::syncToHost() {
     if (!scene || hostFresh) return;
     Simulation* sim = Simulation::getCurrent();
     FluidModel* fm = sim->getFluidModel(0);
-    const unsigned int n = fm->getNumParticles();
+    // the active particles; slots not yet emitted keep their zero state
+    uint32_t n = 0;
+    check(sphg_scene_info(scene, &n, nullptr, nullptr, nullptr), "info");
+    fm->setNumActiveParticles(n);
     std::vector<float> buf(3 * (size_t)n);
+    auto put = [&](std::vector<Vector3r>& dst) {
+        for (uint32_t i = 0; i < n; ++i) dst[i] = Vector3r(buf[3 * i], buf[3 * i + 1], buf[3 * i + 2]);
+    };
     check(sphg_scene_get(scene, SPHG_FIELD_X, buf.data()), "get x");
-    fm->getPositions() = fromFloat(buf);
+    put(fm->getPositions());
     check(sphg_scene_get(scene, SPHG_FIELD_V, buf.data()), "get v");
-    fm->getVelocitiesVec() = fromFloat(buf);
+    put(fm->getVelocitiesVec());
     std::vector<float> s(n);
     check(sphg_scene_get(scene, SPHG_FIELD_RHO, s.data()), "get rho");
     for (unsigned int i = 0; i < n; ++i) fm->getDensity(i) = s[i];

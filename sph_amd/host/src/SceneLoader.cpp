@@ -103,7 +103,25 @@ void readFluidInfo(SimulationInfo& info, const Json& root) {
             fl.sphereLatticeMin = vec3(lat.at("min"), "sphereLattice.min");
             fl.sphereLatticeMax = vec3(lat.at("max"), "sphereLattice.max");
         }
-        fl.numEmitters = (int)m.get("emitters").size();
+        // src/SceneLoader.cpp:107-139
+        const Json& ems = m.get("emitters");
+        for (size_t j = 0; j < ems.size(); ++j) {
+            const Json& e = ems.at(j);
+            EmitterInfo ei;
+            ei.r = vec3(e.at("position"), "emitters.position");
+            const Json& rot = e.at("rotation");
+            if (rot.size() != 4) throw SphError("emitters.rotation: expected 4 numbers (w, x, y, z)");
+            ei.rot = Quat4r(rot.at(0).num("emitters.rotation"), rot.at(1).num("emitters.rotation"),
+                            rot.at(2).num("emitters.rotation"), rot.at(3).num("emitters.rotation"));
+            ei.v = e.at("velocity").num("emitters.velocity");
+            ei.numParticles = (unsigned int)e.at("numParticles").num("emitters.numParticles");
+            ei.startTime = e.at("startTime").num("emitters.startTime");
+            ei.type = (unsigned int)e.at("type").num("emitters.type");
+            ei.width = e.at("width").num("emitters.width");
+            ei.height = e.at("height").num("emitters.height");
+            ei.spacing = e.at("spacing").num("emitters.spacing");
+            fl.emitters.push_back(ei);
+        }
     }
 }
 
@@ -257,11 +275,12 @@ bool Simulation::importScene(const std::string& path) {
     setSurfaceTensionMethod(fi.surfaceTensionMethod);
     setAdhesionMethod(fi.adhesionMethod);
     for (const Fluid& fl : fi.fluids) {
-        if (fl.numEmitters) throw SphError("emitters are not supported by the GPU solver");
         FluidModel* fm = nullptr;
         if (!fl.fluidBlocks.empty() || !fl.fluidSpheres.empty())
             fm = buildFluidBlock(fl.fluidBlocks, fl.fluidSpheres, fl.sphereLatticeMin, fl.sphereLatticeMax);
         if (!fm) fm = addFluidModel();
+        for (const EmitterInfo& e : fl.emitters)   // src/Simulation.cpp:405-417
+            fm->addEmitter(e.type, e.numParticles, e.r, e.v, e.rot, e.startTime, e.width, e.height, e.spacing);
         fm->setRefDensity(fl.density0);
         if (fi.viscosityMethod > -1 && fl.viscosity != 0.0) fm->setViscosityForce(fl.viscosity, fl.boundaryViscosity);
         if (fi.surfaceTensionMethod > -1 && fl.surfaceTension != 0.0) fm->setSurfaceTensionForce(fl.surfaceTension);

@@ -54,16 +54,23 @@ int main(int argc, char** argv) {
                 std::fprintf(stderr, "Cannot read %s\n", path.c_str());
                 return 1;
             }
-            size_t blocks = 0, spheres = 0, boxes = 0;
-            for (auto& f : info.fluidData.fluids) { blocks += f.fluidBlocks.size(); spheres += f.fluidSpheres.size(); }
+            size_t blocks = 0, spheres = 0, boxes = 0, emitters = 0;
+            unsigned long long budget = 0;
+            for (auto& f : info.fluidData.fluids) {
+                blocks += f.fluidBlocks.size();
+                spheres += f.fluidSpheres.size();
+                emitters += f.emitters.size();
+                for (auto& e : f.emitters) budget += e.numParticles;
+            }
             for (auto& b : info.boundaryData.boundaries) boxes += b.box.size();
             std::printf("{\"simulationMethod\": %d, \"boundaryMethod\": %d, \"particleRadius\": %.17g, "
                         "\"fluidModels\": %zu, \"fluidBlocks\": %zu, \"fluidSpheres\": %zu, \"boundaries\": %zu, "
-                        "\"boxes\": %zu, \"viscosityMethod\": %d, \"surfaceTensionMethod\": %d, \"adhesionMethod\": %d}\n",
+                        "\"boxes\": %zu, \"viscosityMethod\": %d, \"surfaceTensionMethod\": %d, \"adhesionMethod\": %d, "
+                        "\"emitters\": %zu, \"emitterParticles\": %llu}\n",
                         info.sceneData.simulationMethod, info.sceneData.boundaryMethod, info.sceneData.particleRadius,
                         info.fluidData.fluids.size(), blocks, spheres, info.boundaryData.boundaries.size(), boxes,
                         info.fluidData.viscosityMethod, info.fluidData.surfaceTensionMethod,
-                        info.fluidData.adhesionMethod);
+                        info.fluidData.adhesionMethod, emitters, budget);
             return 0;
         }
         Simulation* sim = Simulation::getCurrent();

@@ -102,6 +102,51 @@ class StepStats(C.Structure):
     ]
 
 
+class EmitterDesc(C.Structure):
+    _fields_ = [
+        ("type", C.c_uint32),
+        ("num_particles", C.c_uint32),
+        ("position", C.c_double * 3),
+        ("velocity", C.c_double),
+        ("rotation", C.c_double * 4),
+        ("start_time", C.c_double),
+        ("width", C.c_double),
+        ("height", C.c_double),
+        ("spacing", C.c_double),
+    ]
+
+
+EMITTER_SQUARE, EMITTER_CIRCLE = 0, 1
+
+
+@dataclass
+class Emitter:
+    """A particle emitter with the reference scene file's emitter keys
+    (src/SceneLoader.cpp:123-139): a square (width x height) or circle
+    (radius width) group at spacing 2r, emitted along the local +z axis at
+    `velocity`, rotated by the quaternion `rotation` = (w, x, y, z)."""
+
+    type: int = EMITTER_SQUARE
+    num_particles: int = 0
+    position: Sequence[float] = (0.0, 0.0, 0.0)
+    velocity: float = 1.0
+    rotation: Sequence[float] = (1.0, 0.0, 0.0, 0.0)
+    start_time: float = 0.0
+    width: float = 0.1
+    height: float = 0.1
+    spacing: float = 1.0
+
+    def to_desc(self) -> EmitterDesc:
+        d = EmitterDesc()
+        d.type, d.num_particles = int(self.type), int(self.num_particles)
+        d.position[:] = [float(x) for x in self.position]
+        d.velocity = float(self.velocity)
+        d.rotation[:] = [float(x) for x in self.rotation]
+        d.start_time, d.width, d.height, d.spacing = (float(self.start_time), float(self.width), float(self.height),
+                                                      float(self.spacing))
+        return d
+
+
 @dataclass
 class SceneParams:
     """Scene parameters with the reference scene-JSON meaning
@@ -180,6 +225,8 @@ def load(path: str = LIB_PATH) -> C.CDLL:
         "sphg_scene_destroy": (C.c_int, [P]),
         "sphg_scene_set_fluid": (C.c_int, [P, fp, fp, U32]),
         "sphg_scene_set_boundary": (C.c_int, [P, fp, U32]),
+        "sphg_scene_add_emitter": (C.c_int, [P, C.POINTER(EmitterDesc)]),
+        "sphg_scene_capacity": (C.c_int, [P, C.POINTER(U32)]),
         "sphg_scene_init": (C.c_int, [P]),
         "sphg_scene_set_state": (C.c_int, [P, fp, fp, fp, fp, D, U32]),
         "sphg_scene_step": (C.c_int, [P, U32, C.POINTER(StepStats)]),
@@ -355,7 +402,8 @@ class Batch:
 class Scene:
     """One fluid + Akinci boundary scene resident on the device (sphg_scene)."""
 
-    def __init__(self, ctx: Context, params: SceneParams, fluid, boundary, velocities=None, init: bool = True):
+    def __init__(self, ctx: Context, params: SceneParams, fluid, boundary, velocities=None, init: bool = True,
+                 emitters: Sequence[Emitter] = ()):
         self.ctx = ctx
         self.lib = ctx.lib
         self.params = params
@@ -367,12 +415,30 @@ class Scene:
         x = _f32(fluid).reshape(-1, 3)
         v = None if velocities is None else _f32(velocities).reshape(-1, 3)
         b = _f32(boundary).reshape(-1, 3)
-        self.n = x.shape[0]
         self.nb = b.shape[0]
-        _check(self.lib.sphg_scene_set_fluid(self.h, _fptr(x), _fptr(v), self.n))
+        self.emitters = list(emitters)
+        if x.shape[0]:
+            _check(self.lib.sphg_scene_set_fluid(self.h, _fptr(x), _fptr(v), x.shape[0]))
         _check(self.lib.sphg_scene_set_boundary(self.h, _fptr(b), self.nb))
+        for e in self.emitters:
+            d = e.to_desc()
+            _check(self.lib.sphg_scene_add_emitter(self.h, C.byref(d)))
         if init:
             _check(self.lib.sphg_scene_init(self.h))
+
+    @property
+    def n(self) -> int:
+        """Active fluid particles (FluidModel::getNumActiveParticles)."""
+        v = C.c_uint32()
+        _check(self.lib.sphg_scene_info(self.h, C.byref(v), None, None, None))
+        return int(v.value)
+
+    @property
+    def capacity(self) -> int:
+        """Fluid particles plus every emitter's budget (FluidModel::getNumParticles)."""
+        v = C.c_uint32()
+        _check(self.lib.sphg_scene_capacity(self.h, C.byref(v)))
+        return int(v.value)
 
     def init(self) -> None:
         """Allocate and initialise on the device (when created with init=False)."""

@@ -111,3 +111,40 @@ def test_cube_boundary_method_scene_json(ctx, tmp_path):
     spec.boundary = np.zeros((0, 3), np.float32)
     xa, _ = _api_state(ctx, spec)
     assert np.array_equal(xc, xa.astype(np.float64))
+
+
+def test_emitter_scene_json(ctx, tmp_path):
+    """sph_amd/scenes/inflow_dfsph.json: a square emitter shooting along -x
+    and a circle emitter shooting down (quaternion (0, 1, 0, 0): 180 degrees
+    about x).  The scene file through the host layer and the same scene
+    through the C ABI end in bit-identical states; the frame lists every
+    slot (numParticles) with the ones not yet emitted at zero, as the
+    reference's writer does (src/SceneLoader.cpp:224-262)."""
+    steps = 40
+    out = tmp_path / "final_inflow.json"
+    p = subprocess.run([RUN, "-f", os.path.join(SCENES, "inflow_dfsph.json"), "--steps", str(steps), "--final",
+                        str(out)], capture_output=True, text=True, timeout=110)
+    assert p.returncode == 0, p.stderr
+    with open(out) as f:
+        st = json.load(f)
+    spec = configs.small_scene(sphg.DFSPH, n_side=6, spacing=0.05, jitter=0.01, seed=7, box=(1.1, 0.525, 0.7))
+    ems = [sphg.Emitter(type=0, num_particles=400, position=(0.9, 0.26, 0.45), velocity=2.0,
+                        rotation=(0.7071067811865476, 0.0, -0.7071067811865476, 0.0), start_time=0.0, width=0.16,
+                        height=0.16, spacing=1.0),
+           sphg.Emitter(type=1, num_particles=200, position=(0.55, 0.26, 0.6), velocity=1.5,
+                        rotation=(0.0, 1.0, 0.0, 0.0), start_time=0.01, width=0.1, height=0.0, spacing=1.0)]
+    s = sphg.Scene(ctx, spec.params, spec.fluid, spec.boundary, emitters=ems)
+    s.step(steps)
+    n = s.n
+    xa, va = s.get(sphg.FIELD_X), s.get(sphg.FIELD_V)
+    cap = s.capacity
+    s.close()
+    assert st["numParticles"] == cap == spec.n + 600
+    assert st["numActiveParticles"] == n and n > spec.n
+    xc = np.array(st["positions"], dtype=np.float64)
+    vc = np.array(st["velocities"], dtype=np.float64)
+    assert np.array_equal(xc[:n], xa.astype(np.float64)) and np.array_equal(vc[:n], va.astype(np.float64))
+    assert np.all(xc[n:] == 0.0) and np.all(vc[n:] == 0.0)
+    # the circle emitter fired too: its velocity (0, 0, 1.5) turned to -z
+    # (the square's particles fall from rest along z: above -0.6 m/s here)
+    assert np.any(va[spec.n:, 2] < -1.2)

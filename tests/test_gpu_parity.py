@@ -333,3 +333,51 @@ def test_cube_boundary(ctx, method):
     xg = g.get(sphg.FIELD_X)
     assert np.sum(xg[:, 2] == lo32) > 0 and np.sum(xg[:, 0] == hi32) > 0
     g.close()
+
+
+@pytest.mark.parametrize("method", [sphg.DFSPH, sphg.WCSPH, sphg.PCISPH])
+def test_emitter(ctx, method):
+    """Emitters (src/Emitter.cpp:85-162): a 3x3 square group shot along -x
+    (quaternion of -90 degrees about y) into the dam-break box every
+    spacing 1.105 2r / v = 11.05 ms from t = 2.5 dt.  The GPU activates the
+    groups on the device; the active count, the steps that emit and the state
+    of old and new particles are compared with the oracle each step."""
+    spec = configs.small_scene(method, n_side=6)
+    c = np.sqrt(0.5)
+    em = sphg.Emitter(type=sphg.EMITTER_SQUARE, num_particles=9 * 3 + 4, position=(0.9, 0.26, 0.4), velocity=5.0,
+                      rotation=(c, 0.0, -c, 0.0), start_time=2.5 * spec.params.time_step, width=0.16, height=0.16,
+                      spacing=1.0)
+    g = sphg.Scene(ctx, spec.params, spec.fluid, spec.boundary, emitters=[em])
+    o = po.Oracle(spec.params, spec.fluid, spec.boundary)
+    assert o.add_emitter(em) == 9
+    assert g.capacity == spec.n + 31 and g.n == spec.n
+    emitted_at = []
+    matched = 0
+    for step in range(50):
+        assert g.n == o.n
+        sc = g.scalars()
+        dfs = method == sphg.DFSPH
+        o.set_state(g.get(sphg.FIELD_X), g.get(sphg.FIELD_V), g.get(sphg.FIELD_K) if dfs else None,
+                    g.get(sphg.FIELD_KV) if dfs else None, sc.time_step, sc.iterations)
+        n0 = g.n
+        g.step(1)
+        o.step(1)
+        assert g.n == o.n, (step, g.n, o.n)
+        if g.n != n0:
+            emitted_at.append(step)
+        sg, so = g.scalars(), o.scalars()
+        if sg.iterations != int(so["iterations"]) or (dfs and sg.iterations_v != int(so["iterations_v"])):
+            print("iteration mismatch", step)
+            continue
+        matched += 1
+        xg, vg = g.get(sphg.FIELD_X), g.get(sphg.FIELD_V)
+        errs = {"x": rel_norm(xg, o.get(po.OF_X)), "v": rel_norm(vg, o.get(po.OF_V))}
+        if g.n != n0:     # the new group alone
+            errs["x_new"] = rel_norm(xg[n0:], o.get(po.OF_X)[n0:])
+            errs["v_new"] = rel_norm(vg[n0:], o.get(po.OF_V)[n0:])
+        print("step", step, "n", g.n, {k: f"{v:.2e}" for k, v in errs.items()})
+        assert all(v < TOL for v in errs.values()), (step, errs)
+    assert g.n == spec.n + 27                     # three whole groups; 4 particles of budget left
+    assert len(emitted_at) == 3 and emitted_at[0] == 3      # first step whose start time reaches 2.5 dt
+    assert matched >= 40
+    g.close()

@@ -81,13 +81,26 @@ def test_cube_boundary_method_parses(tmp_path):
     assert json.loads(p.stdout)["boundaryMethod"] == 0
 
 
-def test_emitters_are_refused(tmp_path):
+def test_emitters_parse(tmp_path):
+    """The emitter keys of src/SceneLoader.cpp:123-139; every key is required."""
+    em = {"position": [0.5, 0.3, 0.4], "rotation": [1, 0, 0, 0], "velocity": 1.0, "numParticles": 10,
+          "startTime": 0.0, "type": 0, "width": 0.2, "height": 0.2, "spacing": 1.0}
+
     def edit(d):
-        d["Fluid"]["FluidModel"][0]["emitters"] = [{"position": [0, 0, 0], "rotation": [1, 0, 0, 0], "velocity": 1.0,
-                                                    "numParticles": 10, "startTime": 0.0, "type": 0, "width": 2,
-                                                    "height": 2, "spacing": 0.05}]
-    p = _run("-f", _variant(tmp_path, edit))
-    assert p.returncode == 1 and "emitters" in p.stderr
+        d["Fluid"]["FluidModel"][0]["emitters"] = [em, dict(em, type=1, numParticles=7)]
+    p = _run("-f", _variant(tmp_path, edit), "--info")
+    assert p.returncode == 0, p.stderr
+    info = json.loads(p.stdout)
+    assert (info["emitters"], info["emitterParticles"]) == (2, 17)
+    for key in em:
+        def drop(d, key=key):
+            d["Fluid"]["FluidModel"][0]["emitters"] = [{k: v for k, v in em.items() if k != key}]
+        p = _run("-f", _variant(tmp_path, drop), "--info")
+        assert p.returncode == 1 and key in p.stderr, (key, p.stderr)
+    def bad_rot(d):
+        d["Fluid"]["FluidModel"][0]["emitters"] = [dict(em, rotation=[1, 0, 0])]
+    p = _run("-f", _variant(tmp_path, bad_rot), "--info")
+    assert p.returncode == 1 and "rotation" in p.stderr
 
 
 def test_usage_and_extension():

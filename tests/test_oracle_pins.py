@@ -375,3 +375,103 @@ def test_fp32_state_rounding_helpers():
     o3.step(1)
     assert not np.array_equal(o3.get(po.OF_X), x3)
     assert np.abs(o3.get(po.OF_X) - x3).max() < 1e-6
+
+
+# --------------------------------------------------------------- emitters
+
+def _emitter_oracle(emitters, method=sphg.WCSPH, r=2.0 ** -7, dt=2.0 ** -10, gravity=(0.0, 0.0, 0.0)):
+    p = _params(method=method, r=r, gravity=gravity, time_step=dt, min_time_step=dt, max_time_step=dt)
+    empty = np.zeros((0, 3), np.float32)
+    o = po.Oracle(p, empty, empty)
+    sizes = [o.add_emitter(e) for e in emitters]
+    return o, sizes
+
+
+def test_emitter_square_group_exact():
+    """Square group (src/Emitter.cpp:57-75): floor(w/dist) x floor(h/dist)
+    at spacing dist = 2r, centred, x index outer and decreasing from +W/2.
+    Emitted at t = start with zero offset and the identity rotation, the
+    first group sits exactly at origin + group (dyadic values)."""
+    r = 2.0 ** -7
+    dist = 2 * r
+    e = sphg.Emitter(type=sphg.EMITTER_SQUARE, num_particles=15, position=(0.5, 0.25, 0.125), velocity=0.75,
+                     width=5 * dist, height=3 * dist)
+    o, sizes = _emitter_oracle([e], r=r)
+    assert sizes == [15]
+    assert o.n == 0
+    o.step(1)                        # WCSPH emits after the integration: no motion in the emitting step
+    assert o.n == 15
+    x, v = o.get(po.OF_X), o.get(po.OF_V)
+    exp = np.array([[0.5 + 2 * dist - i * dist, 0.25 + dist - j * dist, 0.125] for i in range(5) for j in range(3)])
+    assert np.array_equal(x, exp)
+    assert np.array_equal(v, np.tile([0.0, 0.0, 0.75], (15, 1)))
+
+
+def test_emitter_circle_group_closed_form():
+    """Circle group (src/Emitter.cpp:31-55): the centre plus rings l = 1 ..
+    floor((w - dist/2)/dist) of radius l dist holding floor(2 pi l) particles;
+    the first particle of a ring is at angle 0, i.e. (0, radius)."""
+    r = 2.0 ** -7
+    dist = 2 * r
+    e = sphg.Emitter(type=sphg.EMITTER_CIRCLE, num_particles=1000, width=3.4 * dist, velocity=1.0)
+    o, sizes = _emitter_oracle([e], r=r)
+    assert sizes == [1 + math.floor(2 * math.pi) + math.floor(4 * math.pi)] == [19]
+    o.step(1)
+    x = o.get(po.OF_X)
+    rad = np.hypot(x[:, 0], x[:, 1])
+    assert np.array_equal(x[0], [0.0, 0.0, 0.0])
+    assert np.allclose(rad[1:7], dist, rtol=0, atol=1e-15) and np.allclose(rad[7:], 2 * dist, rtol=0, atol=1e-15)
+    assert np.allclose(x[1], [0.0, dist, 0.0], atol=1e-18) and np.allclose(x[7], [0.0, 2 * dist, 0.0], atol=1e-18)
+    assert np.all(x[:, 2] == 0.0)
+
+
+def test_emitter_schedule_budget_and_time_offset():
+    """Emission k is due at t_k = start + k spacing 1.105 2r / v
+    (src/Emitter.cpp:85-97,131): it happens in the first step whose start
+    time reaches t_k, one group per step, while a whole group is left in the
+    budget.  Its positions carry the offset (v + g t) t, t = time - t_k."""
+    r, dt = 2.0 ** -7, 2.0 ** -10
+    dist = 2 * r
+    interval = 10.37 * dt
+    vel = 1.0 * 1.105 * dist / interval        # spacing 1
+    g = (0.0, 0.0, -9.81)
+    start = 3.3 * dt
+    e = sphg.Emitter(type=sphg.EMITTER_SQUARE, num_particles=int(3.5 * 4), position=(0.0, 0.0, 0.0), velocity=vel,
+                     width=2 * dist, height=2 * dist, start_time=start, spacing=1.0)
+    o, sizes = _emitter_oracle([e], r=r, dt=dt, gravity=g)
+    assert sizes == [4]
+    due = [start + k * interval for k in range(3)]
+    # step s starts at time s dt (fixed dt): emission k at the first s with s dt >= t_k
+    exp_steps = [math.ceil(t / dt - 1e-12) for t in due]
+    counts = []
+    for s in range(60):
+        o.step(1)
+        counts.append(o.n)
+    for k, s in enumerate(exp_steps):
+        assert counts[s] == 4 * (k + 1), (k, s, counts[max(0, s - 1):s + 2])
+        assert s == 0 or counts[s - 1] == 4 * k
+    assert counts[-1] == 12                      # the half group left is never emitted
+    # offset of the first group: emitted at step exp_steps[0], time exp_steps[0] dt
+    o2, _ = _emitter_oracle([e], r=r, dt=dt, gravity=g)
+    o2.step(exp_steps[0] + 1)
+    t = exp_steps[0] * dt - start
+    x = o2.get(po.OF_X)
+    assert np.allclose(x[:, 2], (vel + g[2] * t) * t, rtol=1e-13, atol=0)
+    assert np.allclose(np.sort(x[:, 0]), [-0.5 * dist, -0.5 * dist, 0.5 * dist, 0.5 * dist], rtol=0, atol=1e-15)
+
+
+def test_emitter_rotation_quaternion():
+    """rotation (w, x, y, z) rotates the group and the emission velocity:
+    90 degrees about x maps local (x, y, 0) to (x, 0, y) and +z to -y."""
+    r = 2.0 ** -7
+    dist = 2 * r
+    c = math.cos(math.pi / 4)
+    e = sphg.Emitter(type=sphg.EMITTER_SQUARE, num_particles=6, position=(1.0, 2.0, 3.0), velocity=0.5,
+                     rotation=(c, c, 0.0, 0.0), width=3 * dist, height=2 * dist)
+    o, _ = _emitter_oracle([e], r=r)
+    o.step(1)
+    x, v = o.get(po.OF_X), o.get(po.OF_V)
+    loc = np.array([[dist - i * dist, 0.5 * dist - j * dist] for i in range(3) for j in range(2)])
+    exp = np.stack([1.0 + loc[:, 0], 2.0 + 0.0 * loc[:, 1], 3.0 + loc[:, 1]], -1)
+    assert np.allclose(x, exp, rtol=0, atol=1e-15)
+    assert np.allclose(v, np.tile([0.0, -0.5, 0.0], (6, 1)), rtol=0, atol=1e-15)
