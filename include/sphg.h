@@ -198,6 +198,24 @@ int sphg_scene_get_sorted(sphg_scene* scene, uint32_t* ids, int32_t* cells);
 /* Neighbor lists in id order: counts[n], ids[n*cap] (row i lists the ids of
  * i's neighbors, unordered).  cap >= max_neighbors. */
 int sphg_scene_get_neighbors(sphg_scene* scene, uint32_t* counts, uint32_t* ids, uint32_t cap);
+/* Asynchronous frames (SceneLoader::writeFluid, src/SceneLoader.cpp:204-268:
+ * positions and velocities of the active particles in id order, with the
+ * time and time step).  A snapshot object owns a device staging buffer and
+ * pinned host buffers sized for the scene's capacity.  _begin enqueues the
+ * read-back on the scene's stream and returns at once: the copy to host
+ * memory runs on a second stream while later steps compute.  _wait blocks
+ * until that copy is complete and returns pointers into the snapshot's host
+ * buffers (valid until its next _begin or _destroy); it may be called from
+ * another thread than the one driving the context.  A snapshot's next
+ * _begin waits on the device for its previous copy.  No reference
+ * counterpart for the split: the reference writes frames synchronously. */
+typedef struct sphg_snapshot sphg_snapshot;
+int sphg_snapshot_create(sphg_scene* scene, sphg_snapshot** out);
+int sphg_snapshot_begin(sphg_snapshot* snap);
+int sphg_snapshot_wait(sphg_snapshot* snap, const float** xyz, const float** vel, uint32_t* n, double* time,
+                       double* time_step);
+int sphg_snapshot_destroy(sphg_snapshot* snap);
+
 /* Sizes: fluid, boundary, grid cells, device bytes held. */
 int sphg_scene_info(sphg_scene* scene, uint32_t* n, uint32_t* nb, uint32_t* ncells, uint64_t* device_bytes);
 

@@ -122,6 +122,7 @@ struct Scope {
 struct sphg_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    hipStream_t copyStream = nullptr;   // snapshot read-backs (default priority)
     int* dActive = nullptr;   // [0] scenes still iterating, [1] largest iteration count reached
     int* hActive = nullptr;   // pinned copy
     Profiler prof;
@@ -178,10 +179,34 @@ struct sphg_scene {
     float* tmp = nullptr;      // 4 floats per particle of staging
     Scalars* ss = nullptr;
     Batch* self = nullptr;
+    std::vector<sphg_snapshot*> snaps;
     ~sphg_scene();
 };
 
+struct sphg_snapshot {
+    sphg_scene* scene = nullptr;
+    float* dStage = nullptr;       // x[cap*3], v[cap*3], then a Scalars copy
+    float* hStage = nullptr;       // pinned mirror of dStage
+    size_t capBytes = 0;           // bytes of one of x, v
+    uint32_t n = 0;
+    bool pending = false;
+    hipEvent_t ready = nullptr, done = nullptr;
+    void release() {
+        if (done) (void)hipEventSynchronize(done);
+        if (dStage) (void)hipFree(dStage);
+        if (hStage) (void)hipHostFree(hStage);
+        if (ready) (void)hipEventDestroy(ready);
+        if (done) (void)hipEventDestroy(done);
+        dStage = hStage = nullptr;
+        ready = done = nullptr;
+    }
+};
+
 sphg_scene::~sphg_scene() {
+    for (auto* sn : snaps) {
+        sn->release();
+        sn->scene = nullptr;
+    }
     delete self;
     if (mem) (void)hipFree(mem);
 }
@@ -777,6 +802,7 @@ int sphg_ctx_create(int32_t device, sphg_ctx** out) {
     sphg_ctx* c = new sphg_ctx();
     c->device = device;
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->copyStream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipMalloc(&c->dActive, 2 * sizeof(int));
     if (e == hipSuccess) e = hipHostMalloc(&c->hActive, 2 * sizeof(int), hipHostMallocDefault);
     if (e != hipSuccess) {
@@ -791,6 +817,7 @@ int sphg_ctx_destroy(sphg_ctx* ctx) {
     if (!ctx) return SPHG_OK;
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->copyStream) (void)hipStreamSynchronize(ctx->copyStream);
     for (auto* s : ctx->scenes) {
         s->ctx = nullptr;
         delete s;
@@ -800,6 +827,7 @@ int sphg_ctx_destroy(sphg_ctx* ctx) {
     if (ctx->dActive) (void)hipFree(ctx->dActive);
     if (ctx->hActive) (void)hipHostFree(ctx->hActive);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    if (ctx->copyStream) (void)hipStreamDestroy(ctx->copyStream);
     delete ctx;
     return SPHG_OK;
 }
@@ -1366,6 +1394,88 @@ int sphg_scene_get_neighbors(sphg_scene* s, uint32_t* counts, uint32_t* ids, uin
         const size_t base = (i >> 6) * (size_t)V.maxNbr * kGroup + (i & 63);
         for (uint32_t q = 0; q < c16[i]; ++q) ids[(size_t)me * cap + q] = sid[lst[base + (size_t)q * kGroup]];
     }
+    return SPHG_OK;
+}
+
+int sphg_snapshot_create(sphg_scene* s, sphg_snapshot** out) {
+    if (!s || !s->ctx || !out) return fail(SPHG_EINVAL, "NULL argument");
+    if (!s->ready) return fail(SPHG_ESTATE, "snapshot before init");
+    HIP_TRY(hipSetDevice(s->ctx->device));
+    sphg_snapshot* sn = new sphg_snapshot();
+    sn->scene = s;
+    sn->capBytes = sizeof(float) * 3 * (size_t)std::max<uint32_t>(s->cap, 1);
+    const size_t total = 2 * sn->capBytes + sizeof(Scalars);
+    hipError_t e = hipMalloc(&sn->dStage, total);
+    if (e == hipSuccess) e = hipHostMalloc(&sn->hStage, total, hipHostMallocDefault);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&sn->ready, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&sn->done, hipEventDisableTiming);
+    if (e != hipSuccess) {
+        sn->release();
+        delete sn;
+        return fail(SPHG_ENOMEM, std::string("snapshot buffers: ") + hipGetErrorString(e));
+    }
+    s->snaps.push_back(sn);
+    *out = sn;
+    return SPHG_OK;
+}
+
+int sphg_snapshot_begin(sphg_snapshot* sn) {
+    if (!sn) return fail(SPHG_EINVAL, "NULL snapshot");
+    sphg_scene* s = sn->scene;
+    if (!s || !s->ctx) return fail(SPHG_ESTATE, "the snapshot's scene was destroyed");
+    sphg_ctx* ctx = s->ctx;
+    HIP_TRY(hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    // the staging buffer is free once this snapshot's previous copy is done
+    if (sn->pending) HIP_TRY(hipStreamWaitEvent(st, sn->done, 0));
+    const DevScene& V = s->view[s->par];
+    const int n = (int)s->n;
+    char* d = (char*)sn->dStage;
+    launch_unsort_f4(st, V.x, V.id, (float*)d, n, 3);
+    launch_unsort_f4(st, V.v, V.id, (float*)(d + sn->capBytes), n, 3);
+    HIP_TRY(hipMemcpyAsync(d + 2 * sn->capBytes, s->ss, sizeof(Scalars), hipMemcpyDeviceToDevice, st));
+    HIP_TRY(hipEventRecord(sn->ready, st));
+    hipStream_t cs = ctx->copyStream;
+    HIP_TRY(hipStreamWaitEvent(cs, sn->ready, 0));
+    char* h = (char*)sn->hStage;
+    const size_t b = sizeof(float) * 3 * (size_t)n;
+    if (b) {
+        HIP_TRY(hipMemcpyAsync(h, d, b, hipMemcpyDeviceToHost, cs));
+        HIP_TRY(hipMemcpyAsync(h + sn->capBytes, d + sn->capBytes, b, hipMemcpyDeviceToHost, cs));
+    }
+    HIP_TRY(hipMemcpyAsync(h + 2 * sn->capBytes, d + 2 * sn->capBytes, sizeof(Scalars), hipMemcpyDeviceToHost, cs));
+    HIP_TRY(hipEventRecord(sn->done, cs));
+    HIP_TRY(hipGetLastError());
+    sn->n = s->n;
+    sn->pending = true;
+    return SPHG_OK;
+}
+
+int sphg_snapshot_wait(sphg_snapshot* sn, const float** xyz, const float** vel, uint32_t* n, double* time,
+                       double* time_step) {
+    if (!sn) return fail(SPHG_EINVAL, "NULL snapshot");
+    if (!sn->pending) return fail(SPHG_ESTATE, "snapshot never begun");
+    if (!sn->done) return fail(SPHG_ESTATE, "the snapshot's scene was destroyed");
+    HIP_TRY(hipEventSynchronize(sn->done));
+    const char* h = (const char*)sn->hStage;
+    Scalars sc;
+    std::memcpy(&sc, h + 2 * sn->capBytes, sizeof(Scalars));
+    if (xyz) *xyz = (const float*)h;
+    if (vel) *vel = (const float*)(h + sn->capBytes);
+    if (n) *n = sn->n;
+    if (time) *time = sc.time;
+    if (time_step) *time_step = sc.dt;
+    return SPHG_OK;
+}
+
+int sphg_snapshot_destroy(sphg_snapshot* sn) {
+    if (!sn) return SPHG_OK;
+    if (sn->scene) {
+        auto& v = sn->scene->snaps;
+        v.erase(std::remove(v.begin(), v.end(), sn), v.end());
+    }
+    sn->release();
+    delete sn;
     return SPHG_OK;
 }
 

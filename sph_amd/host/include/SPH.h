@@ -27,7 +27,9 @@
 #include <cmath>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
+#include <thread>
 #include <string>
 #include <vector>
 
@@ -283,6 +285,15 @@ protected:
     bool hostFresh = false;
     sphg_scene_scalars last{};
     double gpuMs = 0;
+    // frames in flight: a device snapshot each, written by its own thread
+    struct FrameSlot {
+        sphg_snapshot* snap = nullptr;
+        std::thread writer;
+    };
+    FrameSlot frames[2];
+    int nextFrame = 0;
+    std::mutex frameMutex;
+    std::string frameError;
 
 public:
     explicit GpuSolver(int method_) : method(method_) {}
@@ -293,6 +304,13 @@ public:
     // Refresh FluidModel's host mirror (positions, velocities, densities,
     // pressures) from the device, in particle-id order.
     void syncToHost();
+    // A frame of the current state (SceneLoader::writeFluid's content) to
+    // `filename` without stopping the host: the read-back runs beside the
+    // next steps and a writer thread formats and writes the file; two
+    // frames are in flight at most.
+    void queueFrame(const std::string& filename);
+    // Waits for every queued frame; a writer's error is rethrown here.
+    void flushFrames();
     const sphg_scene_scalars& scalars() const { return last; }
     double deviceMilliseconds() const { return gpuMs; }
     sphg_scene* handle() { return scene; }
@@ -541,4 +559,8 @@ public:
     // numActiveParticles, time, timeStep, positions, velocities
     static std::string writeFluid();
     static std::string frameJson();
+    // The frame text from the active particles' FP32 positions and
+    // velocities; slots [active, total) are written as zeros.
+    static std::string frameJsonFrom(unsigned int total, unsigned int active, Real time, Real timeStep,
+                                     const float* x, const float* v);
 };

@@ -2,6 +2,7 @@
 #include "SPH.h"
 
 #include <algorithm>
+#include <fstream>
 #include <cstdio>
 #include <sstream>
 
@@ -139,7 +140,51 @@ void AkinciBoundaryModel::addSphere(Vector3r o, Real radius) {
 // ------------------------------------------------------------ solvers
 
 GpuSolver::~GpuSolver() {
+    try {
+        flushFrames();
+    } catch (const std::exception&) {
+    }
+    for (auto& f : frames)
+        if (f.snap) sphg_snapshot_destroy(f.snap);
     if (scene) sphg_scene_destroy(scene);
+}
+
+void GpuSolver::queueFrame(const std::string& filename) {
+    if (!scene) throw SphError("frame before init");
+    FrameSlot& f = frames[nextFrame];
+    nextFrame ^= 1;
+    if (f.writer.joinable()) f.writer.join();
+    if (!f.snap) check(sphg_snapshot_create(scene, &f.snap), "sphg_snapshot_create");
+    check(sphg_snapshot_begin(f.snap), "sphg_snapshot_begin");
+    uint32_t cap = 0;
+    check(sphg_scene_capacity(scene, &cap), "capacity");
+    sphg_snapshot* snap = f.snap;
+    f.writer = std::thread([this, snap, filename, cap]() {
+        try {
+            const float *x = nullptr, *v = nullptr;
+            uint32_t n = 0;
+            double t = 0, dt = 0;
+            if (sphg_snapshot_wait(snap, &x, &v, &n, &t, &dt)) throw SphError(sphg_last_error());
+            std::ofstream out(filename);
+            if (!out.is_open()) throw SphError("cannot write " + filename);
+            out << SceneLoader::frameJsonFrom(cap, n, t, dt, x, v);
+            if (!out) throw SphError("writing " + filename + " failed");
+        } catch (const std::exception& e) {
+            std::lock_guard<std::mutex> g(frameMutex);
+            if (frameError.empty()) frameError = e.what();
+        }
+    });
+}
+
+void GpuSolver::flushFrames() {
+    for (auto& f : frames)
+        if (f.writer.joinable()) f.writer.join();
+    std::lock_guard<std::mutex> g(frameMutex);
+    if (!frameError.empty()) {
+        std::string e;
+        std::swap(e, frameError);
+        throw SphError("frame output: " + e);
+    }
 }
 
 void GpuSolver::fillDesc(sphg_scene_desc& d) const {
@@ -390,6 +435,7 @@ void Simulation::run(unsigned int maxSteps) {
         step();
         if (maxSteps && ++k >= maxSteps) break;
     }
+    if (auto* gs = dynamic_cast<GpuSolver*>(solver.get())) gs->flushFrames();
 }
 
 FluidModel* Simulation::buildFluidBlock(const std::vector<BlockInfo>& blocks, const std::vector<SphereInfo>& spheres,

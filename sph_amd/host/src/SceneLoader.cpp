@@ -216,6 +216,28 @@ std::string SceneLoader::frameJson() {
     return state.dump();
 }
 
+std::string SceneLoader::frameJsonFrom(unsigned int total, unsigned int active, Real time, Real timeStep,
+                                       const float* x, const float* v) {
+    Json state = Json::object();
+    state["numParticles"] = Json::number(total);
+    state["numActiveParticles"] = Json::number(active);
+    state["time"] = Json::number(time);
+    state["timeStep"] = Json::number(timeStep);
+    Json pos = Json::array(), vel = Json::array();
+    for (unsigned int i = 0; i < total; ++i) {
+        Json a = Json::array(), b = Json::array();
+        for (int c = 0; c < 3; ++c) {
+            a.push(Json::number(i < active ? (double)x[3 * i + c] : 0.0));
+            b.push(Json::number(i < active ? (double)v[3 * i + c] : 0.0));
+        }
+        pos.push(a);
+        vel.push(b);
+    }
+    state["positions"] = pos;
+    state["velocities"] = vel;
+    return state.dump();
+}
+
 // <scene>_NNNN.json next to the scene file (src/SceneLoader.cpp:204-268).
 std::string SceneLoader::writeFluid() {
     Simulation* sim = Simulation::getCurrent();
@@ -227,6 +249,10 @@ std::string SceneLoader::writeFluid() {
         filename.insert(filename.size() - ext.size(), "_" + num);
     else
         filename += "_" + num + ext;
+    if (auto* gs = dynamic_cast<GpuSolver*>(sim->getSolver())) {
+        gs->queueFrame(filename);   // written beside the next steps
+        return filename;
+    }
     std::ofstream f(filename);
     if (!f.is_open()) throw SphError("cannot write " + filename);
     f << frameJson();

@@ -227,6 +227,10 @@ def load(path: str = LIB_PATH) -> C.CDLL:
         "sphg_scene_set_boundary": (C.c_int, [P, fp, U32]),
         "sphg_scene_add_emitter": (C.c_int, [P, C.POINTER(EmitterDesc)]),
         "sphg_scene_capacity": (C.c_int, [P, C.POINTER(U32)]),
+        "sphg_snapshot_create": (C.c_int, [P, C.POINTER(P)]),
+        "sphg_snapshot_begin": (C.c_int, [P]),
+        "sphg_snapshot_wait": (C.c_int, [P, C.POINTER(fp), C.POINTER(fp), C.POINTER(U32), dp, dp]),
+        "sphg_snapshot_destroy": (C.c_int, [P]),
         "sphg_scene_init": (C.c_int, [P]),
         "sphg_scene_set_state": (C.c_int, [P, fp, fp, fp, fp, D, U32]),
         "sphg_scene_step": (C.c_int, [P, U32, C.POINTER(StepStats)]),
@@ -399,6 +403,45 @@ class Batch:
             self.h = None
 
 
+class Snapshot:
+    """Asynchronous frame of a scene (sphg_snapshot): begin() enqueues the
+    read-back of the active particles' positions and velocities (id order)
+    and returns at once; wait() returns them with the time and time step of
+    the moment begin() was called."""
+
+    def __init__(self, scene: "Scene"):
+        self.scene = scene
+        self.lib = scene.lib
+        h = C.c_void_p()
+        _check(self.lib.sphg_snapshot_create(scene.h, C.byref(h)))
+        self.h = h
+
+    def begin(self) -> "Snapshot":
+        _check(self.lib.sphg_snapshot_begin(self.h))
+        return self
+
+    def wait(self):
+        x, v = C.POINTER(C.c_float)(), C.POINTER(C.c_float)()
+        n = C.c_uint32()
+        t, dt = C.c_double(), C.c_double()
+        _check(self.lib.sphg_snapshot_wait(self.h, C.byref(x), C.byref(v), C.byref(n), C.byref(t), C.byref(dt)))
+        k = int(n.value)
+        xs = np.ctypeslib.as_array(x, shape=(k, 3)).copy() if k else np.zeros((0, 3), np.float32)
+        vs = np.ctypeslib.as_array(v, shape=(k, 3)).copy() if k else np.zeros((0, 3), np.float32)
+        return xs, vs, float(t.value), float(dt.value)
+
+    def close(self) -> None:
+        if self.h:
+            self.lib.sphg_snapshot_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class Scene:
     """One fluid + Akinci boundary scene resident on the device (sphg_scene)."""
 
@@ -483,6 +526,10 @@ class Scene:
             out = np.zeros(self.n, dtype=np.int32 if fld in _INT else np.float32)
         _check(self.lib.sphg_scene_get(self.h, fld, out.ctypes.data_as(C.c_void_p)))
         return out
+
+    def snapshot(self) -> Snapshot:
+        """A reusable asynchronous frame object for this scene."""
+        return Snapshot(self)
 
     def scalars(self) -> SceneScalars:
         s = SceneScalars()

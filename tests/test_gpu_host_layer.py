@@ -148,3 +148,40 @@ def test_emitter_scene_json(ctx, tmp_path):
     # the circle emitter fired too: its velocity (0, 0, 1.5) turned to -z
     # (the square's particles fall from rest along z: above -0.6 m/s here)
     assert np.any(va[spec.n:, 2] < -1.2)
+
+
+def test_frames_cadence_and_content(ctx, tmp_path):
+    """Frames of `sphg_run -o true` (SceneLoader::writeFluid at
+    TimeManager::hasToSave, src/SceneLoader.cpp:204-268,
+    src/TimeManager.h:101-111), written by the asynchronous path: each
+    frame holds, bit for bit, the C-ABI state at the frame's time, and the
+    frames follow the cadence rule replayed on the C-ABI time sequence."""
+    import shutil
+    scene = tmp_path / "dam.json"
+    shutil.copy(os.path.join(SCENES, "dam_break_dfsph_small.json"), scene)
+    steps = 60
+    p = subprocess.run([RUN, "-f", str(scene), "-o", "true", "--steps", str(steps)], capture_output=True, text=True,
+                       timeout=110)
+    assert p.returncode == 0, p.stderr
+    frames = sorted(tmp_path.glob("dam_*.json"))
+    # replay: time after each step, and hasToSave (lastTimeSave = frame/fps;
+    # the first frame is 1, src/TimeManager.h:31)
+    fps = 50.0
+    spec = configs.small_scene(sphg.DFSPH, n_side=8, spacing=0.05, jitter=0.01, seed=7)
+    s = sphg.Scene(ctx, spec.params, spec.fluid, spec.boundary)
+    last, frame, expect = 0.0, 1, []
+    for k in range(steps):
+        s.step(1)
+        t = s.scalars().time
+        if t - last >= 1.0 / fps:
+            last = frame / fps
+            expect.append((frame, t, s.get(sphg.FIELD_X), s.get(sphg.FIELD_V)))
+            frame += 1
+    s.close()
+    assert len(expect) >= 3 and [f.name for f in frames] == [f"dam_{k:04d}.json" for k, *_ in expect]
+    for f, (k, t, x, v) in zip(frames, expect):
+        with open(f) as fh:
+            st = json.load(fh)
+        assert st["time"] == t and st["numParticles"] == st["numActiveParticles"] == spec.n
+        assert np.array_equal(np.array(st["positions"]), x.astype(np.float64))
+        assert np.array_equal(np.array(st["velocities"]), v.astype(np.float64))

@@ -381,3 +381,40 @@ def test_emitter(ctx, method):
     assert len(emitted_at) == 3 and emitted_at[0] == 3      # first step whose start time reaches 2.5 dt
     assert matched >= 40
     g.close()
+
+
+def test_snapshot_async(ctx):
+    """Asynchronous frames (sphg_snapshot): two snapshots in flight while the
+    scene keeps stepping, one of them waited for from another thread; each
+    holds exactly the state (id order), time and time step of the moment it
+    was begun, and the active count of an emitter scene."""
+    import threading
+    spec = configs.small_scene(sphg.DFSPH, n_side=6)
+    em = sphg.Emitter(type=sphg.EMITTER_SQUARE, num_particles=18, position=(0.9, 0.26, 0.4), velocity=5.0,
+                      rotation=(np.sqrt(0.5), 0.0, -np.sqrt(0.5), 0.0), start_time=0.0, width=0.16, height=0.16)
+    s = sphg.Scene(ctx, spec.params, spec.fluid, spec.boundary, emitters=[em])
+    a, b = s.snapshot(), s.snapshot()
+    s.step(3)
+    a.begin()
+    ref_a = (s.get(sphg.FIELD_X), s.get(sphg.FIELD_V), s.scalars().time, s.scalars().time_step)
+    s.step(4)
+    b.begin()
+    ref_b = (s.get(sphg.FIELD_X), s.get(sphg.FIELD_V), s.scalars().time, s.scalars().time_step)
+    s.step(2)
+    for snap, ref in ((a, ref_a), (b, ref_b)):
+        x, v, t, dt = snap.wait()
+        assert x.shape == ref[0].shape and np.array_equal(x, ref[0]) and np.array_equal(v, ref[1])
+        assert t == ref[2] and dt == ref[3]
+    assert ref_a[0].shape[0] == spec.n + 9        # one group emitted by then
+    # reuse, waited on another thread while this one steps
+    a.begin()
+    ref_c = (s.get(sphg.FIELD_X), s.get(sphg.FIELD_V))
+    got = []
+    th = threading.Thread(target=lambda: got.append(a.wait()))
+    th.start()
+    s.step(3)
+    th.join(timeout=60)
+    assert got and np.array_equal(got[0][0], ref_c[0]) and np.array_equal(got[0][1], ref_c[1])
+    a.close()
+    b.close()
+    s.close()
