@@ -39,7 +39,7 @@ enum { SPHG_WCSPH = 0, SPHG_PCISPH = 1, SPHG_DFSPH = 2 };
 /* boundaryMethod (src/Simulation.cpp:25-27): cube (a clamp into a box, or
  * nothing when the scene has no CubeBoundaryModel), Akinci particles.  The
  * PCISPH boundary method (1) is not available. */
-enum { SPHG_BOUNDARY_CUBE = 0, SPHG_BOUNDARY_AKINCI = 2 };
+enum { SPHG_BOUNDARY_CUBE = 0, SPHG_BOUNDARY_PCISPH = 1, SPHG_BOUNDARY_AKINCI = 2 };
 /* viscosityMethod (src/Simulation.h enums): standard SPH (src/StandardViscosityForce.h),
  * artificial (src/ArtificialViscosity.h), XSPH (src/XSPHViscosity.h) */
 enum { SPHG_VISCOSITY_STANDARD = 0, SPHG_VISCOSITY_ARTIFICIAL = 1, SPHG_VISCOSITY_XSPH = 2 };
@@ -81,6 +81,11 @@ typedef struct sphg_scene_desc {
     int32_t cube_enabled;
     double cube_min[3], cube_max[3];
     double cube_normal_fct, cube_tangential_fct;
+    /* PCISPHBoundaryModel (boundary_method SPHG_BOUNDARY_PCISPH,
+     * src/PCISPHBoundaryModel.h:21-22; the scene file's Boundary[].normalFct
+     * and tangFct): the velocity of a fluid particle projected out of the
+     * boundary becomes tangential_fct * tangential - normal_fct * normal. */
+    double pb_normal_fct, pb_tangential_fct;
 } sphg_scene_desc;
 
 /* Emitter types (src/Emitter.cpp:7-8). */
@@ -143,7 +148,11 @@ enum {
     SPHG_FIELD_PACC = 12,    /* float[n*3] PCISPH pressure acceleration */
     SPHG_FIELD_KEYS = 13,    /* int32[n*3] cell keys floor(x/h) */
     SPHG_FIELD_NNB = 14,     /* int32[n] fluid neighbors with |r| <= h, self excluded */
-    SPHG_FIELD_NGRAD = 15    /* int32[n] of which with nonzero kernel gradient */
+    SPHG_FIELD_NGRAD = 15,   /* int32[n] of which with nonzero kernel gradient */
+    SPHG_FIELD_PBNORMAL = 16,  /* float[n*3] PCISPH boundary method: the fluid normal of the last
+                                  collision projection (FluidModel::n; NaN where nothing collided) */
+    SPHG_FIELD_BDENSITY = 17,  /* float[nb] PCISPH boundary method: boundary densities, input order */
+    SPHG_FIELD_BPRESSURE = 18  /* float[nb] PCISPH boundary method: boundary pressures, input order */
 };
 
 const char* sphg_version(void);
@@ -163,6 +172,14 @@ int sphg_scene_destroy(sphg_scene* scene);
 int sphg_scene_set_fluid(sphg_scene* scene, const float* xyz, const float* vel, uint32_t n);
 /* BoundaryModel::init / AkinciBoundaryModel::addCube... (src/BoundaryModel.cpp:3-9). */
 int sphg_scene_set_boundary(sphg_scene* scene, const float* xyz, uint32_t nb);
+/* PCISPHBoundaryModel normals (src/PCISPHBoundaryModel.cpp:7-52): one per
+ * boundary particle set by set_boundary, required for boundary_method
+ * SPHG_BOUNDARY_PCISPH; see sphg_sample_box_normals / _sphere_normals. */
+int sphg_scene_set_boundary_normals(sphg_scene* scene, const float* nrm, uint32_t nb);
+/* Replace the fluid normal field (id order, n*3) of the PCISPH boundary
+ * method, e.g. with an injected state: DFSPH's velocity correction at the
+ * start of a step reads the normals of the last step's projection. */
+int sphg_scene_set_pb_normals(sphg_scene* scene, const float* nrm);
 /* FluidModel::addEmitter (src/FluidModel.cpp:124-143): after set_fluid (or
  * with no fluid set: an inflow-only scene), before init.  The capacity
  * grows by the emitter's budget; emitted particles are activated on the
@@ -250,6 +267,15 @@ int sphg_ctx_kernel_time(sphg_ctx* ctx, const char* name, uint64_t* launches, do
 int64_t sphg_sample_box(const double* mn, const double* mx, double support_radius, float* out, int64_t cap);
 /* AkinciBoundaryModel::sampleSphere (src/AkinciBoundayModel.cpp:136-164). */
 int64_t sphg_sample_sphere(const double* center, double radius, double particle_radius, float* out, int64_t cap);
+/* PCISPHBoundaryModel::addCube / addSphere (src/PCISPHBoundaryModel.cpp:
+ * 7-52, 60-216): points (box surface at ceil(l / 0.45h) intervals; the
+ * sphere's points as above) and their normals, inward for an inverted box
+ * or sphere; nrm may be NULL.  Returns the point count, or -1 on bad
+ * arguments (a box needs a positive extent on every axis). */
+int64_t sphg_sample_box_normals(const double* min, const double* max, double support_radius, int32_t inverted,
+                                float* xyz, float* nrm, int64_t cap);
+int64_t sphg_sample_sphere_normals(const double* center, double radius, double particle_radius, int32_t inverted,
+                                   float* xyz, float* nrm, int64_t cap);
 /* Simulation::buildFluidBlock (src/Simulation.cpp:512-539) with an optional
  * seeded jitter of +-jitter*2r per coordinate (SplitMix64; 0 = none). */
 int64_t sphg_fluid_block(const double* mn, const double* mx, double particle_radius, double jitter, uint64_t seed,

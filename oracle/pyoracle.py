@@ -15,7 +15,8 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 
 OF_X, OF_V, OF_A, OF_RHO, OF_P, OF_ALPHA, OF_RHOSTAR = 0, 1, 2, 3, 4, 5, 6
 OF_DIVERR, OF_K, OF_KV, OF_BVOL, OF_NORMAL, OF_PACC, OF_ACC_STEP, OF_BX = 7, 8, 9, 10, 11, 12, 13, 14
-_VEC3 = {OF_X, OF_V, OF_A, OF_NORMAL, OF_PACC, OF_ACC_STEP}
+OF_PBN, OF_BDENS, OF_BPRESS, OF_PREDR = 15, 16, 17, 18
+_VEC3 = {OF_X, OF_V, OF_A, OF_NORMAL, OF_PACC, OF_ACC_STEP, OF_PBN, OF_PREDR}
 
 
 class OracleDesc(C.Structure):
@@ -49,6 +50,9 @@ class OracleDesc(C.Structure):
         ("cube_max", C.c_double * 3),
         ("cube_normal_fct", C.c_double),
         ("cube_tangential_fct", C.c_double),
+        ("boundary_method", C.c_int),
+        ("pb_normal_fct", C.c_double),
+        ("pb_tangential_fct", C.c_double),
     ]
 
 
@@ -92,6 +96,9 @@ def load(fast: bool = False) -> C.CDLL:
         "oracle_set_state": (C.c_int, [P, fp, fp, fp, fp, D, U]),
         "oracle_step": (C.c_int, [P, U]),
         "oracle_add_emitter": (C.c_int, [P, C.POINTER(OracleEmitter)]),
+        "oracle_set_boundary_normals": (C.c_int, [P, fp, U]),
+        "oracle_set_pb_normals": (C.c_int, [P, fp]),
+        "oracle_sample_box_normals": (C.c_long, [dp, dp, D, C.c_int, dp, dp, C.c_long]),
         "oracle_round_state_f32": (C.c_int, [P]),
         "oracle_set_f32_positions": (C.c_int, [P, C.c_int]),
         "oracle_num_fluid": (U, [P]),
@@ -143,7 +150,7 @@ class Oracle:
     """FP64 restatement of the reference solver for one scene."""
 
     def __init__(self, params, fluid, boundary, velocities=None, init: bool = True, fast: bool = False,
-                 threads: Optional[int] = None):
+                 threads: Optional[int] = None, boundary_normals=None):
         self.lib = load(fast)
         if threads:
             self.lib.oracle_set_threads(int(threads))
@@ -156,6 +163,10 @@ class Oracle:
         self.n, self.nb = x.shape[0], b.shape[0]
         self.lib.oracle_set_fluid(self.h, _fp(x), _fp(v), self.n)
         self.lib.oracle_set_boundary(self.h, _fp(b), self.nb)
+        if boundary_normals is not None:
+            bn = np.ascontiguousarray(np.asarray(boundary_normals, dtype=np.float32).reshape(-1, 3))
+            if self.lib.oracle_set_boundary_normals(self.h, _fp(bn), bn.shape[0]):
+                raise ValueError("one boundary normal per boundary particle")
         if init:
             self.lib.oracle_init(self.h)
 
@@ -173,6 +184,11 @@ class Oracle:
     def step(self, n: int = 1):
         self.lib.oracle_step(self.h, int(n))
         self.n = int(self.lib.oracle_num_fluid(self.h))
+
+    def set_pb_normals(self, n):
+        """The fluid's normal field of an injected state (PCISPH boundary method)."""
+        a = np.ascontiguousarray(np.asarray(n, dtype=np.float32).reshape(-1, 3))
+        self.lib.oracle_set_pb_normals(self.h, _fp(a))
 
     def add_emitter(self, e) -> int:
         """FluidModel::addEmitter: `e` has the fields of sph_amd.sphg.Emitter.
@@ -205,7 +221,7 @@ class Oracle:
         self.lib.oracle_set_f32_positions(self.h, 1 if on else 0)
 
     def get(self, field: int) -> np.ndarray:
-        if field in (OF_BVOL,):
+        if field in (OF_BVOL, OF_BDENS, OF_BPRESS):
             out = np.zeros(max(self.nb, 1))
             self.lib.oracle_get(self.h, field, _dp(out))
             return out[: self.nb]
@@ -260,6 +276,23 @@ def sample_cube(mn, mx, support_radius):
 def sample_sphere(center, radius, particle_radius):
     lib = load()
     return _sampler(lib.oracle_sample_sphere, _dp(np.asarray(center, float)), float(radius), float(particle_radius))
+
+
+def sample_box_normals(mn, mx, support_radius, inverted):
+    """PCISPHBoundaryModel::sampleCube + addCube: points and their normals."""
+    lib = load()
+    a, b = _dp(np.asarray(mn, float)), _dp(np.asarray(mx, float))
+    n = lib.oracle_sample_box_normals(a, b, float(support_radius), int(bool(inverted)), None, None, 0)
+    pts, nrm = np.zeros((max(n, 1), 3)), np.zeros((max(n, 1), 3))
+    lib.oracle_sample_box_normals(a, b, float(support_radius), int(bool(inverted)), _dp(pts), _dp(nrm), n)
+    return pts[:n], nrm[:n]
+
+
+def sphere_normals(points, center, inverted):
+    """PCISPHBoundaryModel::addSphere (src/PCISPHBoundaryModel.cpp:30-52):
+    n = inv (center - point), inv = 1 if inverted else -1, unnormalised."""
+    inv = 1.0 if inverted else -1.0
+    return inv * (np.asarray(center, float)[None, :] - np.asarray(points, float))
 
 
 def fluid_block(mn, mx, particle_radius):

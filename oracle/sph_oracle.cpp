@@ -154,6 +154,16 @@ struct Kernels {
         }
         return res;
     }
+    // src/Poly6.h:31-42
+    double poly6W(V3 r) const {
+        const double mr = length(r);
+        const double mr2 = mr * mr;
+        if (mr2 <= h2) {
+            const double diff = h2 - mr2;
+            return p6v * diff * diff * diff;
+        }
+        return 0.0;
+    }
     // src/Poly6.h:44-57
     V3 poly6Grad(V3 r) const {
         double mr = length(r);
@@ -221,12 +231,14 @@ typedef struct oracle_desc {
     int cube_enabled;        // CubeBoundaryModel (src/CubeBoundaryModel.cpp:5-94)
     double cube_min[3], cube_max[3];
     double cube_normal_fct, cube_tangential_fct;
+    int boundary_method;     // 0 cube, 1 PCISPH, 2 Akinci (src/Simulation.cpp:25-27)
+    double pb_normal_fct, pb_tangential_fct;   // PCISPHBoundaryModel (src/PCISPHBoundaryModel.h:21-22)
 } oracle_desc;
 
 enum {
     OF_X = 0, OF_V = 1, OF_A = 2, OF_RHO = 3, OF_P = 4, OF_ALPHA = 5, OF_RHOSTAR = 6,
     OF_DIVERR = 7, OF_K = 8, OF_KV = 9, OF_BVOL = 10, OF_NORMAL = 11, OF_PACC = 12,
-    OF_ACC_STEP = 13, OF_BX = 14
+    OF_ACC_STEP = 13, OF_BX = 14, OF_PBN = 15, OF_BDENS = 16, OF_BPRESS = 17, OF_PREDR = 18
 };
 
 // One emitter (src/Emitter.h:9-55, FluidModel::addEmitter
@@ -338,6 +350,14 @@ struct Oracle {
     bool initialised = false;
     std::vector<Emitter> emitters;
 
+    // PCISPH boundary method (src/PCISPHBoundaryModel.h:13-22): boundary
+    // normals, densities and pressures; the fluid's normal field
+    // (FluidModel::n, src/FluidModel.h:22), written by the corrections.
+    bool pbm = false;   // PCISPH boundary method
+    std::vector<V3> nbv;
+    std::vector<double> bdens, bpress;
+    std::vector<V3> pbn;
+
     size_t N() const { return x.size(); }
 
     // Appends one particle to the active set.  The reference's arrays are
@@ -350,6 +370,7 @@ struct Oracle {
         alpha.push_back(0.0); divErr.push_back(0.0); rhoStar.push_back(0.0); k.push_back(0.0); kv.push_back(0.0);
         predR.push_back(V3()); predV.push_back(V3()); pacc.push_back(V3());
         normal.push_back(V3());
+        pbn.push_back(V3());
     }
     // Simulation::emitParticles (src/Simulation.cpp:325-329) ->
     // Emitter::emitParticles (src/Emitter.cpp:85-97) and emitSquare /
@@ -452,6 +473,87 @@ struct Oracle {
         }
     }
 
+    // src/SPHSolver.cpp:93-121: PCISPH boundary particle densities from the
+    // fluid candidates of the boundary particle's cell and the boundary ones
+    // (boundary mass m_b = rho0 (2r)^3, src/Simulation.cpp:465-467).
+    void computeBoundaryDensities() {
+        const long nb = (long)xb.size();
+        const double mb = rho0 * vbPcisph();
+        #pragma omp parallel for
+        for (long b = 0; b < nb; ++b) {
+            const V3 ri = xb[b];
+            const Bucket& c = cellOf(ri);
+            double density = 0;
+            for (const PointInfo& pj : c.neighbors) density += m * K.W(ri - x[pj.id]);
+            for (const PointInfo& pc : c.bneighbors) density += mb * K.W(ri - xb[pc.id]);
+            bdens[b] = density;
+        }
+    }
+    // The PCISPH boundary particles' volume-equivalent: with V_b = (2r)^3
+    // the Akinci-form terms rho0 V_b (...) are the method's m_b (...) terms,
+    // evaluated in the same order (src/DFSPHSolver.cpp:149-160 and the six
+    // other DFSPH sites, src/SPHSolver.cpp:72-80, src/PCISPHSolver.cpp:193-202).
+    double vbPcisph() const { return std::pow(2.0 * r, 3.0); }
+
+    // glm::normalize: v * (1 / sqrt(dot(v, v))) (NaN for v = 0)
+    static V3 normalize(V3 v) { return v * (1.0 / std::sqrt(dot(v, v))); }
+
+    // PCISPHBoundaryModel::correctPositions (src/PCISPHBoundaryModel.cpp:
+    // 219-309) and correctPredPositions (:385-439): for every boundary
+    // candidate of the cell of the current position r_i with |r_i - r_b| <
+    // h/2, w = max(0, (h/2 - |q - r_b|) / (h/2)) with q = r_i (WCSPH, DFSPH)
+    // or the predicted position (PCISPH); n_i = normalize(sum w n_b) is
+    // stored (NaN when the sum is zero) and, when sum w > 0, the target
+    // (r_i, or the predicted position for `pred`) moves by
+    // (1 / sum w) (sum w (h/2 - |q - r_b|)) n_i.
+    void pbCorrect(bool usePred, bool movePred) {
+        const long n = (long)N();
+        const double restRadius = 0.5 * h;
+        #pragma omp parallel for
+        for (long i = 0; i < n; ++i) {
+            const V3 ri = x[i];
+            const Bucket& c = cellOf(ri);
+            V3 avg;
+            double wsum = 0, w2sum = 0;
+            for (const PointInfo& pb_ : c.bneighbors) {
+                const V3 rb = xb[pb_.id];
+                const V3 q = (usePred ? predR[i] : ri) - rb;
+                if (length(ri - rb) < restRadius) {
+                    const double lq = length(q);
+                    const double w = std::max(0.0, (restRadius - lq) / restRadius);
+                    wsum += w;
+                    w2sum += w * (restRadius - lq);
+                    avg += w * nbv[pb_.id];
+                }
+            }
+            const V3 ni = normalize(avg);
+            pbn[i] = ni;
+            if (wsum > 0) {
+                const V3 dlt = ((1.0 / wsum) * w2sum) * ni;
+                if (movePred) predR[i] += dlt;
+                else x[i] += dlt;
+            }
+        }
+    }
+    // PCISPHBoundaryModel::correctVelocities (src/PCISPHBoundaryModel.cpp:
+    // 311-383): where |n_i| > 0, v_i = tangentialFct (u - (u.n) n) -
+    // normalFct (u.n) n with u = v_i (WCSPH, DFSPH) or the predicted
+    // velocity (PCISPH).
+    void pbCorrectVelocities(bool usePred) {
+        const long n = (long)N();
+        const double nf = d.pb_normal_fct, tf = d.pb_tangential_fct;
+        #pragma omp parallel for
+        for (long i = 0; i < n; ++i) {
+            const V3 ni = pbn[i];
+            if (length(ni) > 0) {
+                const V3 u = usePred ? predV[i] : v[i];
+                const V3 nVel = dot(u, ni) * ni;
+                const V3 tVel = u - nVel;
+                v[i] = tf * tVel - nf * nVel;
+            }
+        }
+    }
+
     // ---------------------------------------------------- NP forces
     // src/XSPHViscosity.h:32-69
     void xsph() {
@@ -467,7 +569,7 @@ struct Oracle {
                 const uint32_t j = pj.id;
                 ai += nu * (m / rho[j]) * (v[j] - vi) * K.W(ri - x[j]) / dt;
             }
-            if (nub != 0.0) {
+            if (nub != 0.0 && !pbm) {
                 for (const PointInfo& pb : c.bneighbors) {
                     const uint32_t b = pb.id;
                     ai += nub * (rho0 * vb[b] / rho[i]) * (-vi) * K.W(ri - xb[b]) / dt;
@@ -496,7 +598,7 @@ struct Oracle {
             }
             f *= nu / rho0;
             V3 ai = a[i] + f;
-            if (nub != 0.0) {
+            if (nub != 0.0 && !pbm) {
                 V3 fb;
                 for (const PointInfo& pb : c.bneighbors) {
                     const uint32_t b = pb.id;
@@ -528,7 +630,7 @@ struct Oracle {
                 const double kij = 2 * rho0 / (dens_i + dens_j);
                 ai += kij * 10.0 * 0.1 * nu * (m / dens_j) * dotProd / (sqDist + 0.01 * h2) * K.gradW(rirj);
             }
-            if (nub != 0.0) {
+            if (nub != 0.0 && !pbm) {
                 for (const PointInfo& pb : c.bneighbors) {
                     const uint32_t b = pb.id;
                     const V3 rirb = ri - xb[b];
@@ -615,6 +717,10 @@ struct Oracle {
         #pragma omp parallel for
         for (long i = 0; i < n; ++i)
             p[i] = std::max(0.0, kappa * rho0 / gam * (std::pow(rho[i] / rho0, gam) - 1));
+        // boundary pressures of the PCISPH method (src/WCSPHSolver.cpp:97-114)
+        if (pbm)
+            for (size_t b = 0; b < xb.size(); ++b)
+                bpress[b] = std::max(0.0, kappa * rho0 / gam * (std::pow(bdens[b] / rho0, gam) - 1));
     }
     // src/WCSPHSolver.cpp:132-188 (Akinci branch :174-186)
     void wcsphPressureForce() {
@@ -629,9 +735,17 @@ struct Oracle {
                 const uint32_t j = pj.id;
                 ai -= m * (densPress_i + p[j] / (rho[j] * rho[j])) * K.gradW(ri - x[j]);
             }
-            for (const PointInfo& pb : c.bneighbors) {
-                const uint32_t b = pb.id;
-                ai -= rho0 * vb[b] * (densPress_i * 1.0) * K.gradW(ri - xb[b]);
+            if (pbm) {   // src/WCSPHSolver.cpp:160-172
+                const double mb = rho0 * vbPcisph();
+                for (const PointInfo& pc : c.bneighbors) {
+                    const uint32_t b = pc.id;
+                    ai -= mb * (densPress_i + bpress[b] / (bdens[b] * bdens[b])) * K.gradW(ri - xb[b]);
+                }
+            } else {
+                for (const PointInfo& pc : c.bneighbors) {
+                    const uint32_t b = pc.id;
+                    ai -= rho0 * vb[b] * (densPress_i * 1.0) * K.gradW(ri - xb[b]);
+                }
             }
             a[i] = ai;
         }
@@ -705,11 +819,16 @@ struct Oracle {
         insertFluid();
         neighborhoodSearch();
         computeDensities();
+        if (pbm) computeBoundaryDensities();   // src/SPHSolver.cpp:123-135
         wcsphPressures();
         wcsphPressureForce();
         computeNonPressureForces();
         integrate();
         cubeCorrect();   // :51-60
+        if (pbm) {       // :62-73
+            pbCorrect(false, false);
+            pbCorrectVelocities(false);
+        }
         emitParticles(); // :75
         time += ts;
     }
@@ -758,7 +877,23 @@ struct Oracle {
             rho[i] = density;
         }
     }
-    // src/PCISPHSolver.cpp:249-295, with the max taken as a reduction.
+    // src/PCISPHSolver.cpp:214-247: PCISPH boundary densities from the
+    // predicted fluid positions, with the Poly6 kernel.
+    void predictBoundaryDensities() {
+        const long nb = (long)xb.size();
+        const double mb = rho0 * vbPcisph();
+        #pragma omp parallel for
+        for (long b = 0; b < nb; ++b) {
+            const V3 ri = xb[b];
+            const Bucket& c = cellOf(ri);
+            double density = 0;
+            for (const PointInfo& pj : c.neighbors) density += m * K.poly6W(ri - predR[pj.id]);
+            for (const PointInfo& pc : c.bneighbors) density += mb * K.poly6W(ri - xb[pc.id]);
+            bdens[b] = density;
+        }
+    }
+    // src/PCISPHSolver.cpp:249-295, with the max taken as a reduction;
+    // boundary pressures of the PCISPH method (:276-293, fluid 0's delta).
     void updatePressure() {
         const long n = (long)N();
         double me = 0;
@@ -769,6 +904,8 @@ struct Oracle {
             me = std::max(me, error / rho0);
         }
         maxDensityError = me;
+        if (pbm)
+            for (size_t b = 0; b < xb.size(); ++b) bpress[b] += delta * std::max(0.0, bdens[b] - rho0);
     }
     // src/PCISPHSolver.cpp:297-358 (Akinci branch :342-353)
     void computePressureAcc() {
@@ -783,9 +920,17 @@ struct Oracle {
                 const uint32_t j = pj.id;
                 pa -= m * (densPress_i + p[j] / (rho[j] * rho[j])) * K.spikyGrad(ri - x[j]);
             }
-            for (const PointInfo& pb : c.bneighbors) {
-                const uint32_t b = pb.id;
-                pa -= rho0 * vb[b] * (densPress_i) * K.spikyGrad(ri - xb[b]);
+            if (pbm) {   // src/PCISPHSolver.cpp:330-340
+                const double mb = rho0 * vbPcisph();
+                for (const PointInfo& pc : c.bneighbors) {
+                    const uint32_t b = pc.id;
+                    pa -= mb * (densPress_i + bpress[b] / (bdens[b] * bdens[b])) * K.spikyGrad(ri - xb[b]);
+                }
+            } else {
+                for (const PointInfo& pc : c.bneighbors) {
+                    const uint32_t b = pc.id;
+                    pa -= rho0 * vb[b] * (densPress_i) * K.spikyGrad(ri - xb[b]);
+                }
             }
             pacc[i] = pa;
         }
@@ -795,7 +940,9 @@ struct Oracle {
         iterations = 0;
         while ((maxDensityError > d.eta || iterations < d.min_iterations) && iterations < d.max_iterations) {
             predictVelocityAndPosition();
+            if (pbm) pbCorrect(true, true);        // correctPredPositions (:370-377)
             predictFluidDensities();
+            if (pbm) predictBoundaryDensities();   // predictDensities (:151-162)
             updatePressure();
             computePressureAcc();
             ++iterations;
@@ -824,11 +971,17 @@ struct Oracle {
         insertFluid();
         neighborhoodSearch();
         computeDensities();
+        if (pbm) computeBoundaryDensities();
         computeNonPressureForces();
         for (size_t i = 0; i < N(); ++i) { p[i] = 0.0; pacc[i] = V3(); }
+        if (pbm) std::fill(bpress.begin(), bpress.end(), 0.0);   // initPressure (:105-119)
         pressureSolver();
         pcisphIntegrate();
         cubeCorrect();   // src/PCISPHSolver.cpp:54-63
+        if (pbm) {       // :65-76: with the loop's last predicted position and velocity
+            pbCorrect(true, false);
+            pbCorrectVelocities(true);
+        }
         emitParticles(); // :78
         time += ts;
     }
@@ -1047,8 +1200,10 @@ struct Oracle {
         computeNonPressureForces();
         updateTimeStep();
         predictVelocities();
+        if (pbm) pbCorrectVelocities(false);   // src/DFSPHSolver.cpp:51-58 (normals of the last step)
         correctDensityError();
         updatePositions();
+        if (pbm) pbCorrect(false, false);      // :65-75
         emitParticles();   // src/DFSPHSolver.cpp:77
         insertFluid();
         neighborhoodSearch();
@@ -1062,6 +1217,7 @@ struct Oracle {
     // ------------------------------------------------------------ set-up
     void setDesc(const oracle_desc& dd) {
         d = dd;
+        pbm = d.boundary_method == 1;
         r = d.particle_radius;
         h = 4.0 * r;                      // src/Simulation.cpp:197
         rho0 = d.density0;
@@ -1080,6 +1236,7 @@ struct Oracle {
         alpha.assign(n, 0.0); divErr.assign(n, 0.0); rhoStar.assign(n, 0.0); k.assign(n, 0.0); kv.assign(n, 0.0);
         predR.assign(n, V3()); predV.assign(n, V3()); pacc.assign(n, V3());
         normal.assign(n, V3());
+        pbn.assign(n, V3());
     }
     // Simulation::init (src/Simulation.cpp:102-110) -> solver->init():
     // WCSPH src/WCSPHSolver.cpp:9-12, PCISPH src/PCISPHSolver.cpp:12-19,
@@ -1090,7 +1247,14 @@ struct Oracle {
         if (d.method == 1) computeScalingFactor();
         insertBoundary();
         neighborhoodSearch();
-        computeVolume();
+        if (pbm) {   // no volume pass for the PCISPH method (src/SPHSolver.cpp:22-39)
+            vb.assign(xb.size(), vbPcisph());
+            bdens.assign(xb.size(), 0.0);
+            bpress.assign(xb.size(), 0.0);
+            if (nbv.size() != xb.size()) nbv.assign(xb.size(), V3());
+        } else {
+            computeVolume();
+        }
         if (d.method == 2) {
             insertFluid();
             neighborhoodSearch();
@@ -1157,6 +1321,22 @@ int oracle_set_boundary(void* h, const float* xyz, unsigned nb) {
     for (unsigned i = 0; i < nb; ++i) o->xb[i] = V3(xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]);
     return 0;
 }
+// PCISPH boundary normals, one per boundary particle (before init).
+int oracle_set_boundary_normals(void* h, const float* n, unsigned nb) {
+    Oracle* o = (Oracle*)h;
+    if (nb != o->xb.size()) return -1;
+    o->nbv.resize(nb);
+    for (unsigned i = 0; i < nb; ++i) o->nbv[i] = V3(n[3 * i], n[3 * i + 1], n[3 * i + 2]);
+    return 0;
+}
+// The fluid's normal field (FluidModel::n) of an injected state: DFSPH's
+// velocity correction at the start of a step reads the last step's.
+int oracle_set_pb_normals(void* h, const float* n) {
+    Oracle* o = (Oracle*)h;
+    for (size_t i = 0; i < o->N(); ++i) o->pbn[i] = V3(n[3 * i], n[3 * i + 1], n[3 * i + 2]);
+    return 0;
+}
+
 // FluidModel::addEmitter (src/FluidModel.cpp:124-143), before init.
 int oracle_add_emitter(void* h, const oracle_emitter* e) {
     Oracle* o = (Oracle*)h;
@@ -1249,6 +1429,10 @@ int oracle_get(void* h, int field, double* out) {
         case OF_PACC: put3(o->pacc, out); break;
         case OF_ACC_STEP: put3(o->accStep, out); break;
         case OF_BX: put3(o->xb, out); break;
+        case OF_PBN: put3(o->pbn, out); break;
+        case OF_BDENS: std::copy(o->bdens.begin(), o->bdens.end(), out); break;
+        case OF_BPRESS: std::copy(o->bpress.begin(), o->bpress.end(), out); break;
+        case OF_PREDR: put3(o->predR, out); break;
         default: return -1;
     }
     return 0;
@@ -1318,6 +1502,73 @@ long oracle_sample_cube(const double* mn, const double* mx, double supportRadius
                         out[3 * c] = i * dx + mn[0];
                         out[3 * c + 1] = j * dy + mn[1];
                         out[3 * c + 2] = k * dz + mn[2];
+                    }
+                    ++c;
+                }
+    return c;
+}
+
+// src/PCISPHBoundaryModel.cpp:60-189: the box surface at ceil(l / 0.45h)
+// intervals per axis, each point with the normal of its corner, edge or face
+// (faces: +x at i = 0, -x at i = nx, +y at j = 0, -y at j = ny, +z at k = 0,
+// -z at k = nz; edges and corners glm::normalize of the mean), tested in the
+// reference's order; times -1 unless inverted (:7-27).
+static V3 pcisph_box_normal(unsigned i, unsigned j, unsigned k, unsigned nx, unsigned ny, unsigned nz) {
+    const V3 L(1, 0, 0), R(-1, 0, 0), T(0, -1, 0), Bo(0, 1, 0), F(0, 0, -1), Be(0, 0, 1);
+    auto nrm = [](V3 v) { return v * (1.0 / std::sqrt(dot(v, v))); };
+    auto e2 = [&](V3 a, V3 b) { return nrm((a + b) * 0.5); };
+    auto e3 = [&](V3 a, V3 b, V3 c) { return nrm(((a + b) + c) / 3.0); };
+    const bool i0 = i == 0, iN = i == nx, im = i > 0 && i < nx;
+    const bool j0 = j == 0, jN = j == ny, jm = j > 0 && j < ny;
+    const bool k0 = k == 0, kN = k == nz, km = k > 0 && k < nz;
+    if (i0 && j0 && k0) return e3(L, Be, Bo);
+    if (iN && j0 && k0) return e3(R, Be, Bo);
+    if (i0 && j0 && kN) return e3(L, F, Bo);
+    if (iN && j0 && kN) return e3(R, F, Bo);
+    if (i0 && jN && k0) return e3(L, Be, T);
+    if (iN && jN && k0) return e3(R, Be, T);
+    if (i0 && jN && kN) return e3(L, F, T);
+    if (iN && jN && kN) return e3(R, F, T);
+    if (i0 && j0 && km) return e2(L, Bo);
+    if (im && j0 && k0) return e2(Be, Bo);
+    if (im && j0 && kN) return e2(F, Bo);
+    if (iN && j0 && km) return e2(R, Bo);
+    if (i0 && jN && km) return e2(L, T);
+    if (im && jN && k0) return e2(Be, T);
+    if (im && jN && kN) return e2(F, T);
+    if (iN && jN && km) return e2(R, T);
+    if (i0 && jm && k0) return e2(L, Be);
+    if (iN && jm && k0) return e2(R, Be);
+    if (iN && jm && kN) return e2(R, F);
+    if (i0 && jm && kN) return e2(L, F);
+    if (i0 && jm && km) return L;
+    if (iN && jm && km) return R;
+    if (im && jN && km) return T;
+    if (im && j0 && km) return Bo;
+    if (im && jm && kN) return F;
+    return Be;   // im && jm && k0
+}
+long oracle_sample_box_normals(const double* mn, const double* mx, double supportRadius, int inverted, double* out,
+                               double* nout, long cap) {
+    double lx = std::fabs(mn[0] - mx[0]), ly = std::fabs(mn[1] - mx[1]), lz = std::fabs(mn[2] - mx[2]);
+    unsigned nx = (unsigned)std::ceil(lx / (0.45 * supportRadius));
+    unsigned ny = (unsigned)std::ceil(ly / (0.45 * supportRadius));
+    unsigned nz = (unsigned)std::ceil(lz / (0.45 * supportRadius));
+    double dx = lx / nx, dy = ly / ny, dz = lz / nz;
+    const double inv = inverted ? 1.0 : -1.0;
+    long c = 0;
+    for (unsigned i = 0; i <= nx; i++)
+        for (unsigned j = 0; j <= ny; j++)
+            for (unsigned k = 0; k <= nz; k++)
+                if ((i == 0 || i == nx) || (j == 0 || j == ny) || (k == 0 || k == nz)) {
+                    if (c < cap) {
+                        out[3 * c] = i * dx + mn[0];
+                        out[3 * c + 1] = j * dy + mn[1];
+                        out[3 * c + 2] = k * dz + mn[2];
+                        const V3 nn = inv * pcisph_box_normal(i, j, k, nx, ny, nz);
+                        nout[3 * c] = nn.x;
+                        nout[3 * c + 1] = nn.y;
+                        nout[3 * c + 2] = nn.z;
                     }
                     ++c;
                 }

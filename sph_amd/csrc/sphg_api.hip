@@ -45,12 +45,12 @@ constexpr int kDefaultMaxNbr = 96;
 enum Kind {
     K_KEYS, K_SCAN, K_SCATTER, K_FINALIZE, K_REORDER, K_BUILD, K_NORMALS, K_FORCES, K_STEPBEGIN, K_DT,
     K_DFPREDICT, K_DFVEL_WARM, K_DFVEL_DENS, K_DFVEL_DIV, K_DFERR_DENS, K_DFERR_DIV, K_DFPOS, K_PCIDENS,
-    K_PCIPACC, K_PCIINT, K_TIME, K_CUBE, K_EMIT, K_NKINDS
+    K_PCIPACC, K_PCIINT, K_TIME, K_CUBE, K_EMIT, K_PB, K_NKINDS
 };
 const char* kKindNames[K_NKINDS] = {
     "keys", "scan", "scatter", "finalize", "rank_reorder", "build", "normals", "forces", "step_begin", "dt",
     "df_predict", "df_vel_warm", "df_vel_density", "df_vel_divergence", "df_err_density", "df_err_divergence",
-    "df_positions", "pci_density", "pci_pacc", "pci_integrate", "time_advance", "cube", "emit"};
+    "df_positions", "pci_density", "pci_pacc", "pci_integrate", "time_advance", "cube", "emit", "pcisph_boundary"};
 
 struct Profiler {
     bool on = false;
@@ -171,6 +171,7 @@ struct sphg_scene {
     std::vector<float> fxyz, fvel, bxyz;
     std::vector<uint32_t> border;   // sorted boundary slot -> input index
     std::vector<sphg_emitter_desc> emitters;
+    std::vector<float> bnrm;        // PCISPH boundary method: one normal per boundary particle
     uint32_t n = 0, nb = 0;         // n: active particles
     uint32_t cap = 0;               // particle slots: fluid + every emitter's budget
     double dt0 = 0, dtMin = 0, dtMax = 0;
@@ -247,8 +248,14 @@ double pcisph_delta(double r, double h, double m, double rho0, double dt) {
 
 int check_desc(const sphg_scene_desc& d) {
     if (d.method < 0 || d.method > 2) return fail(SPHG_EINVAL, "simulationMethod must be 0 (WCSPH), 1 (PCISPH) or 2 (DFSPH)");
-    if (d.boundary_method != SPHG_BOUNDARY_AKINCI && d.boundary_method != SPHG_BOUNDARY_CUBE)
-        return fail(SPHG_EINVAL, "boundaryMethod must be 0 (cube) or 2 (Akinci); the PCISPH boundary method (1) is not available");
+    if (d.boundary_method != SPHG_BOUNDARY_AKINCI && d.boundary_method != SPHG_BOUNDARY_CUBE &&
+        d.boundary_method != SPHG_BOUNDARY_PCISPH)
+        return fail(SPHG_EINVAL, "boundaryMethod must be 0 (cube), 1 (PCISPH) or 2 (Akinci)");
+    if (d.boundary_method == SPHG_BOUNDARY_PCISPH && d.adhesion_method == SPHG_ADHESION_AKINCI && d.adhesion != 0.0)
+        return fail(SPHG_EINVAL, "adhesion needs Akinci boundary volumes; with the PCISPH boundary method the reference "
+                                 "reads them from a model that has none (src/AdhesionForce.h:46-50)");
+    if (!std::isfinite(d.pb_normal_fct) || !std::isfinite(d.pb_tangential_fct))
+        return fail(SPHG_EINVAL, "the PCISPH boundary normal and tangential factors must be finite");
     if (d.cube_enabled && d.boundary_method != SPHG_BOUNDARY_CUBE)
         return fail(SPHG_EINVAL, "a cube clamp needs boundaryMethod 0 (cube)");
     if (d.cube_enabled)
@@ -286,6 +293,7 @@ void fill_coef_d(KCoefD& k, double h) {
     k.gB = -2.0 * k.clh2;
     k.hHalf = 0.5 * h;
     k.vl = 45.0 / (M_PI * h * h * h * h * h * h);
+    k.p6v = 315.0 / (64.0 * M_PI * h9);
 }
 
 void fill_coef(KCoef& k, double h) {
@@ -452,13 +460,19 @@ int batch_one_step(sphg_ctx* ctx, Batch& B, uint32_t& syncs) {
     hipStream_t st = ctx->stream;
     const int method = B.scenes[0]->desc.method;
     const int ns = (int)B.scenes.size();
-    int anyFlags = 0, anySt = 0, anyCube = 0, anyEmit = 0;
+    int anyFlags = 0, anySt = 0, anyCube = 0, anyEmit = 0, anyPb = 0;
     for (auto* s : B.scenes) {
         anyFlags |= s->view[0].flags;
         anySt |= (s->view[0].flags & F_ST);
         anyCube |= s->view[0].cubeOn;
         anyEmit |= s->view[0].nEmit;
+        anyPb |= s->view[0].pbOn;
     }
+    // boundary-particle passes of the PCISPH boundary method, one launch per scene
+    auto pbDensity = [&](const DevScene* Tt, int pred, int iter) {
+        for (int k = 0; k < ns; ++k)
+            if (B.scenes[k]->view[0].pbOn) launch_pb_bdensity(st, pred, Tt, k, (int)B.scenes[k]->nb, iter);
+    };
     int rc;
     bool more = false;
     if (method == DFSPH) {
@@ -485,6 +499,9 @@ int batch_one_step(sphg_ctx* ctx, Batch& B, uint32_t& syncs) {
         }
         B.lastIt = seen;
         { Scope s(P, st, K_DFPOS); launch_df_positions(st, T, B.dP, B.nP); }
+        // PCISPH boundary method: projection after the position update
+        // (src/DFSPHSolver.cpp:65-75), which then counts the cells
+        if (anyPb) { Scope s(P, st, K_PB); launch_pb_correct(st, 0, T, B.dP, B.nP); }
         // emitters between the position update and the grid rebuild
         // (src/DFSPHSolver.cpp:77); the new particles' cells are counted
         if (anyEmit) { Scope s(P, st, K_EMIT); launch_emit(st, B.dT[0], B.dT[1], ns, 1); }
@@ -515,11 +532,16 @@ int batch_one_step(sphg_ctx* ctx, Batch& B, uint32_t& syncs) {
         grid_sort(ctx, B, T);
         const DevScene* T2 = B.dT[1];
         { Scope s(P, st, K_BUILD); launch_build(st, WCSPH, T2, B.dP, B.nP); }
+        // PCISPH boundary method: boundary densities and pressures
+        // (src/SPHSolver.cpp:123-135, src/WCSPHSolver.cpp:116-128)
+        if (anyPb) { Scope s(P, st, K_PB); pbDensity(T2, 0, 0); }
         if (anySt) { Scope s(P, st, K_NORMALS); launch_normals(st, T2, B.dP, B.nP); }
         { Scope s(P, st, K_FORCES); launch_forces(st, WCSPH, T2, B.dP, B.nP); }
         // cube boundary after the integration (src/WCSPHSolver.cpp:51-60),
-        // then the emitters (:75), then time += dt (:77)
+        // or the PCISPH boundary projection (:62-73), then the emitters
+        // (:75), then time += dt (:77)
         if (anyCube) { Scope s(P, st, K_CUBE); launch_cube(st, T2, B.dP, B.nP); }
+        if (anyPb) { Scope s(P, st, K_PB); launch_pb_correct(st, 1, T2, B.dP, B.nP); }
         if (anyEmit) { Scope s(P, st, K_EMIT); launch_emit(st, B.dT[1], B.dT[0], ns, 0); }
         { Scope s(P, st, K_TIME); launch_time_advance(st, T2, ns); }
         flip(B);
@@ -537,15 +559,22 @@ int batch_one_step(sphg_ctx* ctx, Batch& B, uint32_t& syncs) {
         for (int it = 0; it < B.maxIt; ++it) {
             if ((rc = loop_more(ctx, it, density_ahead(B), more, seen, syncs))) return rc;
             if (!more) break;
+            // PCISPH boundary method: correctPredPositions after the
+            // prediction (src/PCISPHSolver.cpp:370-377)
+            if (anyPb) { Scope s(P, st, K_PB); launch_pb_predcorrect(st, T2, B.dP, B.nP); }
             { Scope s(P, st, K_PCIDENS); launch_pci_density(st, T2, B.dP, B.nP); }
             finalize(ctx, B, T2, 5);
+            // boundary densities and pressures of the iteration (:151-162, :276-293)
+            if (anyPb) { Scope s(P, st, K_PB); pbDensity(T2, 1, it); }
             { Scope s(P, st, K_PCIPACC); launch_pci_pacc(st, T2, B.dP, B.nP, it); }
         }
         B.lastIt = seen;
         { Scope s(P, st, K_PCIINT); launch_pci_integrate(st, T2, B.dP, B.nP); }
         // cube boundary after the integration (src/PCISPHSolver.cpp:54-63),
-        // then the emitters (:78), then time += dt (:80)
+        // or the PCISPH boundary projection (:65-76), then the emitters
+        // (:78), then time += dt (:80)
         if (anyCube) { Scope s(P, st, K_CUBE); launch_cube(st, T2, B.dP, B.nP); }
+        if (anyPb) { Scope s(P, st, K_PB); launch_pb_correct(st, 2, T2, B.dP, B.nP); }
         if (anyEmit) { Scope s(P, st, K_EMIT); launch_emit(st, B.dT[1], B.dT[0], ns, 0); }
         { Scope s(P, st, K_TIME); launch_time_advance(st, T2, ns); }
         flip(B);
@@ -649,6 +678,8 @@ struct Plan {
     size_t oA, oXK, oGB, oAUX, oXS, oPACC, oGBS, oNRM, oADH, oRHO, oALPHA, oDRHOS, oDIV, oP;
     size_t oCELLU, oCELL, oPERM, oCC, oCS, oCCS, oNBR, oNNB, oNG, oGC, oXB, oBCS, oPART, oSS, oTMP;
     size_t oEMIT = 0, oGROUP = 0;
+    size_t oBN = 0, oBSELF = 0, oBDENS = 0, oBPRES = 0, oBPR = 0, oPBN[2] = {0, 0}, oPV = 0, oPRS = 0, oPVS = 0,
+           oX0S = 0, oOCC = 0;
     std::vector<double> group;          // every emitter's group, 3 doubles per particle
     std::vector<int> groupOff, groupSize;
 };
@@ -746,6 +777,23 @@ int make_plan(const sphg_scene* s, Plan& P) {
     P.oPART = A.reserve(sizeof(double) * (size_t)std::max(nChunks, 1));
     P.oSS = A.reserve(sizeof(Scalars));
     P.oTMP = A.reserve(std::max(f4, sizeof(float4) * std::max<size_t>(nb, 1)));
+    if (d.boundary_method == SPHG_BOUNDARY_PCISPH) {
+        const size_t nb1 = std::max<size_t>(nb, 1);
+        P.oBN = A.reserve(sizeof(float4) * nb1);
+        P.oBSELF = A.reserve(sizeof(double) * 2 * nb1);
+        P.oBDENS = A.reserve(sizeof(float) * nb1);
+        P.oBPRES = A.reserve(sizeof(float) * nb1);
+        P.oBPR = A.reserve(sizeof(float) * nb1);
+        P.oPBN[0] = A.reserve(f4);
+        P.oPBN[1] = (method == DFSPH) ? A.reserve(f4) : 0;
+        if (method == PCISPH) {
+            P.oPV = A.reserve(f4);
+            P.oPRS = A.reserve(f4);
+            P.oPVS = A.reserve(f4);
+            P.oX0S = A.reserve(f4);
+        }
+        P.oOCC = A.reserve((size_t)ncells);
+    }
     if (!s->emitters.empty()) {
         P.group.clear();
         for (const auto& e : s->emitters) {
@@ -905,10 +953,42 @@ int sphg_scene_capacity(sphg_scene* s, uint32_t* capacity) {
 int sphg_scene_set_boundary(sphg_scene* s, const float* xyz, uint32_t nb) {
     if (!s || (!xyz && nb)) return fail(SPHG_EINVAL, "NULL argument");
     if (s->ready) return fail(SPHG_ESTATE, "set_boundary after init");
-    if (nb && s->desc.boundary_method != SPHG_BOUNDARY_AKINCI)
-        return fail(SPHG_EINVAL, "boundary particles need boundaryMethod 2 (Akinci)");
+    if (nb && s->desc.boundary_method != SPHG_BOUNDARY_AKINCI && s->desc.boundary_method != SPHG_BOUNDARY_PCISPH)
+        return fail(SPHG_EINVAL, "boundary particles need boundaryMethod 1 (PCISPH) or 2 (Akinci)");
     s->bxyz.assign(xyz, xyz + 3 * (size_t)nb);
     s->nb = nb;
+    s->bnrm.clear();
+    return SPHG_OK;
+}
+
+int sphg_scene_set_boundary_normals(sphg_scene* s, const float* nrm, uint32_t nb) {
+    if (!s || (!nrm && nb)) return fail(SPHG_EINVAL, "NULL argument");
+    if (s->ready) return fail(SPHG_ESTATE, "set_boundary_normals after init");
+    if (s->desc.boundary_method != SPHG_BOUNDARY_PCISPH)
+        return fail(SPHG_EINVAL, "boundary normals belong to boundaryMethod 1 (PCISPH)");
+    if (nb != s->nb) return fail(SPHG_EINVAL, "one normal per boundary particle");
+    for (size_t i = 0; i < 3 * (size_t)nb; ++i)
+        if (!std::isfinite(nrm[i])) return fail(SPHG_EINVAL, "boundary normals must be finite");
+    s->bnrm.assign(nrm, nrm + 3 * (size_t)nb);
+    return SPHG_OK;
+}
+
+int sphg_scene_set_pb_normals(sphg_scene* s, const float* nrm) {
+    if (!s || !nrm) return fail(SPHG_EINVAL, "NULL argument");
+    if (!s->ready) return fail(SPHG_ESTATE, "set_pb_normals before init");
+    if (!s->view[0].pbOn) return fail(SPHG_EINVAL, "fluid normals belong to boundaryMethod 1 (PCISPH)");
+    sphg_ctx* ctx = s->ctx;
+    HIP_TRY(hipSetDevice(ctx->device));
+    const size_t n = s->n;
+    if (!n) return SPHG_OK;
+    float* d = nullptr;
+    HIP_TRY(hipMalloc(&d, sizeof(float) * 3 * n));
+    HIP_TRY(hipMemcpyAsync(d, nrm, sizeof(float) * 3 * n, hipMemcpyHostToDevice, ctx->stream));
+    const DevScene& V = s->view[s->par];
+    launch_gather_f3(ctx->stream, V.pbn, d, V.id, (int)n);
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    (void)hipFree(d);
+    HIP_TRY(hipGetLastError());
     return SPHG_OK;
 }
 
@@ -916,6 +996,8 @@ int sphg_scene_init(sphg_scene* s) {
     if (!s || !s->ctx) return fail(SPHG_EINVAL, "NULL scene");
     if (s->ready) return fail(SPHG_ESTATE, "scene already initialised");
     if (s->cap == 0) return fail(SPHG_ESTATE, "no fluid particles and no emitters");
+    if (s->desc.boundary_method == SPHG_BOUNDARY_PCISPH && s->bnrm.size() != 3 * (size_t)s->nb)
+        return fail(SPHG_ESTATE, "the PCISPH boundary method needs sphg_scene_set_boundary_normals");
     sphg_ctx* ctx = s->ctx;
     HIP_TRY(hipSetDevice(ctx->device));
     const sphg_scene_desc& d = s->desc;
@@ -982,6 +1064,12 @@ int sphg_scene_init(sphg_scene* s) {
     }
     D.cubeNf = d.cube_normal_fct;
     D.cubeTf = d.cube_tangential_fct;
+    D.pbOn = d.boundary_method == SPHG_BOUNDARY_PCISPH ? 1 : 0;
+    D.pbNf = d.pb_normal_fct;
+    D.pbTf = d.pb_tangential_fct;
+    // the viscosity forces' boundary terms exist for Akinci boundaries only
+    // (src/XSPHViscosity.h:58, src/StandardViscosityForce.h:62, src/ArtificialViscosity.h:66)
+    if (D.pbOn) D.nub = 0.f;
     D.minIt = (int)d.min_iterations;
     D.maxIt = (int)d.max_iterations;
     D.minItV = (int)d.min_iterations_v;
@@ -1026,6 +1114,18 @@ int sphg_scene_init(sphg_scene* s) {
     D.partial = (double*)(b + P.oPART);
     D.ss = (Scalars*)(b + P.oSS);
     s->ss = D.ss;
+    if (D.pbOn) {
+        D.bn = F4(P.oBN);
+        D.bself = (double*)(b + P.oBSELF);
+        D.bdens = F1(P.oBDENS);
+        D.bpres = F1(P.oBPRES);
+        D.bpr = F1(P.oBPR);
+        D.pv = P.oPV ? F4(P.oPV) : nullptr;
+        D.prS = P.oPRS ? F4(P.oPRS) : nullptr;
+        D.pvS = P.oPVS ? F4(P.oPVS) : nullptr;
+        D.x0s = P.oX0S ? F4(P.oX0S) : nullptr;
+        D.everOcc = (uint8_t*)(b + P.oOCC);
+    }
     if (!s->emitters.empty()) {
         // emitter constants and state, group table (src/Emitter.cpp:10-29)
         std::vector<EmitterDev> ed(s->emitters.size());
@@ -1058,6 +1158,11 @@ int sphg_scene_init(sphg_scene* s) {
         v.id = (uint32_t*)(b + P.oID[p]); v.id2 = (uint32_t*)(b + P.oID[1 - p]);
         v.k = P.oK[p] ? F1(P.oK[p]) : nullptr; v.k2 = P.oK[1 - p] ? F1(P.oK[1 - p]) : nullptr;
         v.kv = P.oKV[p] ? F1(P.oKV[p]) : nullptr; v.kv2 = P.oKV[1 - p] ? F1(P.oKV[1 - p]) : nullptr;
+        if (D.pbOn) {
+            const size_t a0 = P.oPBN[1] ? P.oPBN[p] : P.oPBN[0], a1 = P.oPBN[1] ? P.oPBN[1 - p] : P.oPBN[0];
+            v.pbn = F4(a0);
+            v.pbn2 = F4(a1);
+        }
         s->view[p] = v;
     }
     s->par = 0;
@@ -1077,18 +1182,35 @@ int sphg_scene_init(sphg_scene* s) {
             keyed[i] = {lin, i};
         }
         std::stable_sort(keyed.begin(), keyed.end(), [](auto& x, auto& y) { return x.first < y.first; });
-        std::vector<float4> xb(std::max<size_t>(nb, 1));
+        std::vector<float4> xb(std::max<size_t>(nb, 1)), bn;
         std::vector<int> bcs((size_t)ncells + 1, 0);
+        std::vector<uint8_t> occ;
+        // PCISPH boundary method: V_b = (2r)^3, so rho0 V_b is the boundary
+        // mass rho0 (2r)^3 (src/Simulation.cpp:465-467); its particles' cells
+        // exist in the hash table from the start (src/SPHSolver.cpp:26)
+        const float vPb = (float)std::pow(2.0 * r, 3.0);
+        if (D.pbOn) {
+            bn.resize(std::max<size_t>(nb, 1));
+            occ.assign((size_t)ncells, 0);
+        }
         s->border.resize(nb);
         for (uint32_t i = 0; i < nb; ++i) {
             const uint32_t o = keyed[i].second;
             s->border[i] = o;
-            xb[i] = make_float4(s->bxyz[3 * o], s->bxyz[3 * o + 1], s->bxyz[3 * o + 2], 0.f);
+            xb[i] = make_float4(s->bxyz[3 * o], s->bxyz[3 * o + 1], s->bxyz[3 * o + 2], D.pbOn ? vPb : 0.f);
             bcs[keyed[i].first + 1] += 1;
+            if (D.pbOn) {
+                bn[i] = make_float4(s->bnrm[3 * o], s->bnrm[3 * o + 1], s->bnrm[3 * o + 2], 0.f);
+                occ[(size_t)keyed[i].first] = 1;
+            }
         }
         for (long c = 0; c < ncells; ++c) bcs[c + 1] += bcs[c];
         HIP_TRY(hipMemcpyAsync(D.xb, xb.data(), sizeof(float4) * nb, hipMemcpyHostToDevice, ctx->stream));
         HIP_TRY(hipMemcpyAsync(D.bCellStart, bcs.data(), sizeof(int) * bcs.size(), hipMemcpyHostToDevice, ctx->stream));
+        if (D.pbOn) {
+            HIP_TRY(hipMemcpyAsync(D.bn, bn.data(), sizeof(float4) * nb, hipMemcpyHostToDevice, ctx->stream));
+            HIP_TRY(hipMemcpyAsync(D.everOcc, occ.data(), occ.size(), hipMemcpyHostToDevice, ctx->stream));
+        }
         HIP_TRY(hipStreamSynchronize(ctx->stream));
     }
 
@@ -1104,8 +1226,10 @@ int sphg_scene_init(sphg_scene* s) {
     int rc = upload_batch_tables(ctx, *s->self);
     if (rc) return rc;
 
-    // boundary volumes (src/SPHSolver.cpp:22-39)
-    if (nb) launch_bvolume(ctx->stream, s->self->dT[0], 0, (int)nb, s->tmp);
+    // boundary volumes (src/SPHSolver.cpp:22-39); the PCISPH method's
+    // boundary-to-boundary density sums instead
+    if (nb && !D.pbOn) launch_bvolume(ctx->stream, s->self->dT[0], 0, (int)nb, s->tmp);
+    if (nb && D.pbOn) launch_pb_bself(ctx->stream, s->self->dT[0], 0, (int)nb);
 
     // fluid upload (staging in tmp is too small for xyz+vel: copy through a scratch allocation)
     if (n) {
@@ -1320,6 +1444,20 @@ int sphg_scene_get(sphg_scene* s, int32_t field, void* out) {
             HIP_TRY(hipStreamSynchronize(st));
             int32_t* o = (int32_t*)out;
             for (int i = 0; i < n; ++i) o[ids[i]] = field == SPHG_FIELD_NNB ? (int32_t)c16[i] : (int32_t)c8[i];
+            return SPHG_OK;
+        }
+        case SPHG_FIELD_PBNORMAL:
+            if (!V.pbOn) return fail(SPHG_EINVAL, "fluid normals belong to boundaryMethod 1 (PCISPH)");
+            launch_unsort_f4(st, V.pbn, V.id, s->tmp, n, 3); bytes = 12ull * n; break;
+        case SPHG_FIELD_BDENSITY:
+        case SPHG_FIELD_BPRESSURE: {
+            if (!V.pbOn) return fail(SPHG_EINVAL, "boundary densities belong to boundaryMethod 1 (PCISPH)");
+            std::vector<float> tmp(std::max<uint32_t>(s->nb, 1));
+            HIP_TRY(hipMemcpyAsync(tmp.data(), field == SPHG_FIELD_BDENSITY ? V.bdens : V.bpres,
+                                   sizeof(float) * s->nb, hipMemcpyDeviceToHost, st));
+            HIP_TRY(hipStreamSynchronize(st));
+            float* o = (float*)out;
+            for (uint32_t i = 0; i < s->nb; ++i) o[s->border[i]] = tmp[i];
             return SPHG_OK;
         }
         case SPHG_FIELD_BVOL: {
@@ -1557,6 +1695,83 @@ int64_t sphg_sample_sphere(const double* o, double radius, double pr, float* out
                 out[3 * c] = (float)(radius * (std::sin(th) * std::cos(ph)) + o[0]);
                 out[3 * c + 1] = (float)(radius * (std::sin(th) * std::sin(ph)) + o[1]);
                 out[3 * c + 2] = (float)(radius * std::cos(th) + o[2]);
+            }
+            ++c;
+        }
+    }
+    return c;
+}
+
+// PCISPHBoundaryModel::sampleCube + addCube (src/PCISPHBoundaryModel.cpp:
+// 7-27, 60-189): the box surface at ceil(l / 0.45h) intervals per axis.
+// A point's normal is the mean of the inward normals of the faces it lies on
+// (+x at the minimum x face, -x at the maximum, and so on), glm::normalize'd;
+// the face normals are axis unit vectors, so their sums are exact in any
+// order.  Inverted keeps these normals, otherwise they are negated.
+int64_t sphg_sample_box_normals(const double* mn, const double* mx, double supportRadius, int32_t inverted, float* out,
+                                float* nrm, int64_t cap) {
+    if (!mn || !mx || !(supportRadius > 0)) return -1;
+    const double l[3] = {std::fabs(mn[0] - mx[0]), std::fabs(mn[1] - mx[1]), std::fabs(mn[2] - mx[2])};
+    if (!(l[0] > 0 && l[1] > 0 && l[2] > 0)) return -1;
+    unsigned nn[3];
+    double dd[3];
+    for (int a = 0; a < 3; ++a) {
+        nn[a] = (unsigned)std::ceil(l[a] / (0.45 * supportRadius));
+        dd[a] = l[a] / nn[a];
+    }
+    const double inv = inverted ? 1.0 : -1.0;
+    int64_t c = 0;
+    for (unsigned i = 0; i <= nn[0]; i++)
+        for (unsigned j = 0; j <= nn[1]; j++)
+            for (unsigned k = 0; k <= nn[2]; k++) {
+                const unsigned ix[3] = {i, j, k};
+                double v[3] = {0, 0, 0};
+                int faces = 0;
+                for (int a = 0; a < 3; ++a) {
+                    if (ix[a] == 0) { v[a] = 1.0; ++faces; }
+                    else if (ix[a] == nn[a]) { v[a] = -1.0; ++faces; }
+                }
+                if (!faces) continue;
+                if (out && c < cap) {
+                    for (int a = 0; a < 3; ++a) out[3 * c + a] = (float)(ix[a] * dd[a] + mn[a]);
+                    if (faces == 2) for (double& e : v) e *= 0.5;
+                    if (faces == 3) for (double& e : v) e /= 3.0;
+                    const double f = 1.0 / std::sqrt((v[0] * v[0] + v[1] * v[1]) + v[2] * v[2]);
+                    if (nrm)
+                        for (int a = 0; a < 3; ++a) nrm[3 * c + a] = (float)(inv * (v[a] * f));
+                }
+                ++c;
+            }
+    return c;
+}
+
+// PCISPHBoundaryModel::addSphere (src/PCISPHBoundaryModel.cpp:29-52): the
+// sphere points of sphg_sample_sphere with normals inv (center - point)
+// (unnormalised), inv = 1 if inverted else -1.
+int64_t sphg_sample_sphere_normals(const double* o, double radius, double pr, int32_t inverted, float* out,
+                                   float* nrm, int64_t cap) {
+    if (!o || !(radius > 0) || !(pr > 0)) return -1;
+    const double sep = 1.5;
+    const unsigned num = (unsigned)std::floor(4.0 * M_PI * radius * radius / (M_PI * pr * pr) * sep);
+    const double a = 4.0 * M_PI / num;
+    const double dd = std::sqrt(a);
+    const int mt = int(std::round(M_PI / dd));
+    const double dth = M_PI / mt;
+    const double dph = a / dth;
+    const double inv = inverted ? 1.0 : -1.0;
+    int64_t c = 0;
+    for (int m = 0; m < mt; ++m) {
+        const double th = M_PI * (m + 0.5) / mt;
+        const int mp = int(std::round(2.0 * M_PI * std::sin(th) / dph));
+        for (int q = 0; q < mp; ++q) {
+            const double ph = 2.0 * M_PI * q / mp;
+            if (out && c < cap) {
+                const double p[3] = {radius * (std::sin(th) * std::cos(ph)) + o[0],
+                                     radius * (std::sin(th) * std::sin(ph)) + o[1], radius * std::cos(th) + o[2]};
+                for (int e = 0; e < 3; ++e) {
+                    out[3 * c + e] = (float)p[e];
+                    if (nrm) nrm[3 * c + e] = (float)(inv * (o[e] - p[e]));
+                }
             }
             ++c;
         }

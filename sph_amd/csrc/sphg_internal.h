@@ -60,6 +60,7 @@ struct KCoefD {
     double hlo2, hhi2;   // h^2 (1 -/+ 1e-9): band of the operation-exact neighbor test
     double gA, gB, hHalf;  // cubic gradient for q <= 1/2 as gA r + gB; h/2
     double vl;             // viscosity Laplacian 45/(pi h^6) (src/ViscoK.h:23)
+    double p6v;            // Poly6 value coefficient 315/(64 pi h^9) (src/Poly6.h:27)
 };
 
 // Device-resident control state of one scene (one writer at a time: the
@@ -111,6 +112,20 @@ struct DevScene {
     int nEmit, pad3;
     EmitterDev *emit;
     const double *emitGroup;
+    // PCISPH boundary method (src/PCISPHBoundaryModel.h:10-52): boundary
+    // particles carry normals, densities and pressures (xb.w = (2r)^3, so
+    // rho0 xb.w is the boundary mass); the fluid carries the normal the
+    // collision projection last wrote (NaN where nothing collided)
+    int pbOn, pad4;
+    double pbNf, pbTf;
+    float4 *bn;                 // boundary normals (sorted like xb)
+    double *bself;              // per boundary particle: boundary sums of m_b W (cubic) and m_b Poly6
+    float *bdens, *bpres, *bpr; // boundary density, pressure, pressure / density^2
+    float4 *pbn, *pbn2;         // fluid normals; DFSPH carries them through the sort
+    float4 *pv;                 // PCISPH predicted velocity of the next density sweep
+    float4 *prS, *pvS, *x0s;    // PCISPH: last iteration's corrected displacement and predicted
+                                // velocity; positions before the integration
+    uint8_t *everOcc;           // per cell: a particle was ever inserted there
     // particle arrays; the *2 members are the other half of each double buffer
     float4 *x, *x2, *v, *v2;
     float *k, *k2, *kv, *kv2;

@@ -27,6 +27,12 @@ void launch_pci_pacc(hipStream_t st, const DevScene* T, const Chunk* C, int nchu
 void launch_pci_integrate(hipStream_t st, const DevScene* T, const Chunk* C, int nchunks);
 void launch_cube(hipStream_t st, const DevScene* T, const Chunk* C, int n);
 void launch_emit(hipStream_t st, DevScene* Tc, DevScene* To, int nscenes, int cells);
+// PCISPH boundary method
+void launch_pb_bself(hipStream_t st, const DevScene* T, int scene, int nb);
+void launch_pb_bdensity(hipStream_t st, int pred, const DevScene* T, int scene, int nb, int iter);
+void launch_gather_f3(hipStream_t st, float4* dst, const float* src, const uint32_t* id, int n);
+void launch_pb_correct(hipStream_t st, int mode, const DevScene* T, const Chunk* C, int n);
+void launch_pb_predcorrect(hipStream_t st, const DevScene* T, const Chunk* C, int n);
 void launch_time_advance(hipStream_t st, const DevScene* T, int nscenes);
 void launch_fill_int(hipStream_t st, int* p, int v, long n);
 void launch_unsort_f4(hipStream_t st, const float4* src, const uint32_t* id, float* dst, int n, int comps);

@@ -318,6 +318,69 @@ __device__ inline void walk27(const int* __restrict__ start, int c, int nx, int 
     }
 }
 
+// Poly6 W (src/Poly6.h:31-42) of a squared distance.
+__device__ inline double P6_d(double r2, const KCoefD& k) {
+    const double d = k.h2 - r2;
+    return sel(r2 <= k.h2, k.p6v * d * d * d, 0.0);
+}
+
+// The collision projection of PCISPHBoundaryModel (src/PCISPHBoundaryModel.
+// cpp:219-309, 385-439) for one fluid particle at position (px, py, pz):
+// boundary candidates of the position's cell (none when no particle was
+// ever inserted in that cell: the reference's hash table has no entry for
+// it) with |x - r_b| < h/2 weigh w = max(0, (h/2 - |q_b|) / (h/2)), q_b =
+// (qx, qy, qz) - r_b; returns n = glm::normalize(sum w n_b) (NaN for a zero
+// sum) and the shift (1 / sum w)(sum w (h/2 - |q_b|)), applied when sum w > 0.
+struct PbHit {
+    double nx, ny, nz, s;
+    bool hit;
+};
+__device__ inline PbHit pb_project(const DevScene& S, double px, double py, double pz, float4 pf, double qx, double qy,
+                                   double qz) {
+    const KCoefD k = S.kd;
+    const double R = 0.5 * k.h;
+    double ax = 0.0, ay = 0.0, az = 0.0, ws = 0.0, w2 = 0.0;
+    const int cl = cell_of(S, pf);
+    if (S.everOcc[cl]) {
+        walk27(S.bCellStart, cl, S.nx, S.nx * S.ny, [&](int b) {
+            const float4 y = S.xb[b];
+            const double dx = px - (double)y.x, dy = py - (double)y.y, dz = pz - (double)y.z;
+            if (sqrt(dx * dx + dy * dy + dz * dz) < R) {
+                const double ex = qx - (double)y.x, ey = qy - (double)y.y, ez = qz - (double)y.z;
+                const double lq = sqrt(ex * ex + ey * ey + ez * ez);
+                const double w = fmax(0.0, (R - lq) / R);
+                ws += w;
+                w2 += w * (R - lq);
+                const float4 nb = S.bn[b];
+                ax += w * (double)nb.x;
+                ay += w * (double)nb.y;
+                az += w * (double)nb.z;
+            }
+        });
+    }
+    const double inv = 1.0 / sqrt(ax * ax + ay * ay + az * az);
+    PbHit h;
+    h.nx = ax * inv;
+    h.ny = ay * inv;
+    h.nz = az * inv;
+    h.hit = ws > 0.0;
+    h.s = h.hit ? (1.0 / ws) * w2 : 0.0;
+    return h;
+}
+// correctVelocities (src/PCISPHBoundaryModel.cpp:311-383): where |n| > 0
+// (false for NaN), v = tangentialFct (u - (u.n) n) - normalFct (u.n) n.
+__device__ inline bool pb_velocity(const DevScene& S, double nx, double ny, double nz, double& ux, double& uy,
+                                   double& uz) {
+    if (!(sqrt(nx * nx + ny * ny + nz * nz) > 0.0)) return false;
+    const double d = ux * nx + uy * ny + uz * nz;
+    const double nvx = d * nx, nvy = d * ny, nvz = d * nz;
+    ux = S.pbTf * (ux - nvx) - S.pbNf * nvx;
+    uy = S.pbTf * (uy - nvy) - S.pbNf * nvy;
+    uz = S.pbTf * (uz - nvz) - S.pbNf * nvz;
+    return true;
+}
+
+
 // ================================================================ grid
 
 // Cell key + histogram (src/HashTable.h:79-86 insertion, as a count).
@@ -453,7 +516,9 @@ __global__ __launch_bounds__(kBlock) void k_rank_reorder(const DevScene* __restr
     if (S.method == DFSPH) {
         S.k2[d] = S.k[i];
         S.kv2[d] = S.kv[i];
+        if (S.pbOn) S.pbn2[d] = S.pbn[i];
     }
+    if (S.pbOn) S.everOcc[cl] = 1;
     S.cellCount[cl] = 0;
 }
 
@@ -926,6 +991,18 @@ __global__ __launch_bounds__(kBlock) void k_forces(const DevScene* __restrict__ 
         ax = px - pri * (double)gb.x;
         ay = py - pri * (double)gb.y;
         az = pz - pri * (double)gb.z;
+        if (S.pbOn) {
+            // the boundary particles' own pressure term m_b p_b / rho_b^2
+            // gradW of the PCISPH method (src/WCSPHSolver.cpp:160-172)
+            walk27(S.bCellStart, ldg(S.cell, (size_t)i), S.nx, S.nx * S.ny, [&](int b) {
+                const float4 y = ld4(S.xb, b);
+                const Pair p = pair_of(xx, xy, xz, y, k);
+                const double g = S.rho0D * (double)y.w * (double)S.bpr[b] * G_d(p, k);
+                ax -= g * p.dx;
+                ay -= g * p.dy;
+                az -= g * p.dz;
+            });
+        }
     }
     if (xs) {
         const double f = (double)S.nu / dt;
@@ -990,6 +1067,7 @@ __global__ __launch_bounds__(kBlock) void k_forces(const DevScene* __restrict__ 
         const double wy = vy0 + (ay + (double)S.gy) * dt;
         const double wz = vz0 + (az + (double)S.gz) * dt;
         S.xs[i] = make_float4((float)(wx * dt), (float)(wy * dt), (float)(wz * dt), 0.f);
+        if (S.pbOn) S.pv[i] = make_float4((float)wx, (float)wy, (float)wz, 0.f);
     }
 }
 
@@ -1065,6 +1143,13 @@ __global__ __launch_bounds__(kBlock) void k_df_predict(const DevScene* __restric
     v.x = (float)((double)v.x + ax * dt);
     v.y = (float)((double)v.y + ay * dt);
     v.z = (float)((double)v.z + az * dt);
+    if (S.pbOn) {
+        // correctVelocities with the normals of the last step's projection
+        // (src/DFSPHSolver.cpp:51-58)
+        const float4 n = S.pbn[i];
+        double ux = v.x, uy = v.y, uz = v.z;
+        if (pb_velocity(S, n.x, n.y, n.z, ux, uy, uz)) v = make_float4((float)ux, (float)uy, (float)uz, v.w);
+    }
     S.v[i] = v;
     const double rho = S.rho0D + (double)S.drho[i];
     const double kw = -fmax(-(double)S.k[i] / rho, -1.0e-9) / (dt * dt);
@@ -1194,6 +1279,7 @@ __global__ __launch_bounds__(kBlock) void k_df_positions(const DevScene* __restr
     x.y = (float)((double)x.y + (double)v.y * dt);
     x.z = (float)((double)x.z + (double)v.z * dt);
     S.x[i] = x;
+    if (S.pbOn) return;   // the boundary projection moves it, then counts its cell
     const int cl = cell_of(S, x);
     S.cellU[i] = cl;
     atomicAdd(&S.cellCount[cl], 1);
@@ -1285,6 +1371,18 @@ __global__ __launch_bounds__(kBlock) void k_pci_pacc(const DevScene* __restrict_
     ax -= pri * (double)gs.x;
     ay -= pri * (double)gs.y;
     az -= pri * (double)gs.z;
+    if (S.pbOn) {
+        // m_b p_b / rho_b^2 Spiky gradW of the PCISPH method
+        // (src/PCISPHSolver.cpp:330-340)
+        walk27(S.bCellStart, S.cell[i], S.nx, S.nx * S.ny, [&](int b) {
+            const float4 y = S.xb[b];
+            const Pair p = pair_of(xx, xy, xz, y, k);
+            const double g = S.rho0D * (double)y.w * (double)S.bpr[b] * Gs_d(p, k);
+            ax -= g * p.dx;
+            ay -= g * p.dy;
+            az -= g * p.dz;
+        });
+    }
     S.pacc[i] = make_float4((float)ax, (float)ay, (float)az, 0.f);
     const float4 a = S.a[i];
     const float4 v = S.v[i];
@@ -1292,6 +1390,7 @@ __global__ __launch_bounds__(kBlock) void k_pci_pacc(const DevScene* __restrict_
     const double wy = (double)v.y + (((double)a.y + ay) + (double)S.gy) * dt;
     const double wz = (double)v.z + (((double)a.z + az) + (double)S.gz) * dt;
     S.xs[i] = make_float4((float)(wx * dt), (float)(wy * dt), (float)(wz * dt), 0.f);
+    if (S.pbOn) S.pv[i] = make_float4((float)wx, (float)wy, (float)wz, 0.f);
 }
 
 // PCISPH integrate (src/PCISPHSolver.cpp:444-474): a += g + pacc.
@@ -1304,6 +1403,7 @@ __global__ __launch_bounds__(kBlock) void k_pci_integrate(const DevScene* __rest
     const float4 a = S.a[i], pa = S.pacc[i];
     const float4 v0 = S.v[i];
     const float4 x0 = S.x[i];
+    if (S.pbOn) S.x0s[i] = x0;
     const double vx = (double)v0.x + ((double)a.x + ((double)S.gx + (double)pa.x)) * dt;
     const double vy = (double)v0.y + ((double)a.y + ((double)S.gy + (double)pa.y)) * dt;
     const double vz = (double)v0.z + ((double)a.z + ((double)S.gz + (double)pa.z)) * dt;
@@ -1525,6 +1625,150 @@ __global__ __launch_bounds__(kBlock) void k_emit(DevScene* __restrict__ Tc, DevS
     }
 }
 
+// ===================================================== PCISPH boundary method
+
+// The boundary particles' sums over the (static) boundary, m_b W and m_b
+// Poly6 with the boundary candidates of the particle's cell
+// (src/SPHSolver.cpp:115-120, src/PCISPHSolver.cpp:239-244).  One scene.
+__global__ __launch_bounds__(kBlock) void k_pb_bself(const DevScene* __restrict__ T, int scene) {
+    const DevScene& S = T[scene];
+    const int b = blockIdx.x * kBlock + threadIdx.x;
+    if (b >= S.nb) return;
+    const KCoefD k = S.kd;
+    const float4 yb = S.xb[b];
+    const double px = yb.x, py = yb.y, pz = yb.z;
+    double wc = 0.0, wp = 0.0;
+    walk27(S.bCellStart, cell_of(S, yb), S.nx, S.nx * S.ny, [&](int c) {
+        const float4 y = S.xb[c];
+        const Pair p = pair_of(px, py, pz, y, k);
+        const double mb = S.rho0D * (double)y.w;
+        wc += mb * W_d(p.q, k);
+        wp += mb * P6_d(p.r2, k);
+    });
+    S.bself[2 * b] = wc;
+    S.bself[2 * b + 1] = wp;
+}
+
+// Boundary densities and pressures (one scene).  PRED = 0, WCSPH
+// (src/SPHSolver.cpp:93-121, src/WCSPHSolver.cpp:97-114): rho_b = sum over
+// the fluid candidates of the particle's cell of m W(r_b - x_j) plus the
+// boundary sum, p_b = max(0, kappa rho0/gamma ((rho_b/rho0)^gamma - 1)).
+// PRED = 1, PCISPH loop (src/PCISPHSolver.cpp:214-247, :276-293): Poly6
+// with the predicted fluid positions, p_b += delta max(0, rho_b - rho0),
+// from zero at the first iteration.  Both store p_b / rho_b^2 for the
+// fluid's pressure sums.  In the loop it runs after the finalize of
+// iteration `iter`, for scenes that ran that iteration.
+template <int PRED>
+__global__ __launch_bounds__(kBlock) void k_pb_bdensity(const DevScene* __restrict__ T, int scene, int iter) {
+    const DevScene& S = T[scene];
+    if (PRED && S.ss->it <= iter) return;
+    const int first = iter == 0;
+    const int b = blockIdx.x * kBlock + threadIdx.x;
+    if (b >= S.nb) return;
+    const KCoefD k = S.kd;
+    const float4 yb = S.xb[b];
+    const double px = yb.x, py = yb.y, pz = yb.z;
+    const double m = S.mD, rho0 = S.rho0D;
+    double s = 0.0;
+    walk27(S.cellStart, cell_of(S, yb), S.nx, S.nx * S.ny, [&](int j) {
+        const float4 xj = S.x[j];
+        if (PRED) {
+            const float4 dj = S.xs[j];
+            const double dx = (px - (double)xj.x) - (double)dj.x;
+            const double dy = (py - (double)xj.y) - (double)dj.y;
+            const double dz = (pz - (double)xj.z) - (double)dj.z;
+            s += m * P6_d(dx * dx + dy * dy + dz * dz, k);
+        } else {
+            const Pair p = pair_of(px, py, pz, xj, k);
+            s += m * W_d(p.q, k);
+        }
+    });
+    const double rho = s + S.bself[2 * b + PRED];
+    double pr;
+    if (PRED) {
+        pr = (first ? 0.0 : (double)S.bpres[b]) + S.pciDeltaD * fmax(0.0, rho - rho0);
+    } else {
+        const double x = (rho - rho0) / rho0;
+        pr = fmax(0.0, (double)S.stiffness * rho0 / (double)S.gammaEos * expm1((double)S.gammaEos * log1p(x)));
+    }
+    S.bdens[b] = (float)rho;
+    S.bpres[b] = (float)pr;
+    S.bpr[b] = (float)(pr / (rho * rho));
+}
+
+// Position (and velocity) projection after the step's position update.
+// MODE 0, DFSPH (src/DFSPHSolver.cpp:65-75): q = x; the particle's cell is
+// then counted for the grid rebuild.  MODE 1, WCSPH (src/WCSPHSolver.cpp:
+// 62-73): q = x, then the velocity.  MODE 2, PCISPH (src/PCISPHSolver.cpp:
+// 65-76): q = the loop's last corrected predicted position, and the velocity
+// correction starts from its predicted velocity.
+template <int MODE>
+__global__ __launch_bounds__(kBlock) void k_pb_correct(const DevScene* __restrict__ T, const Chunk* __restrict__ C) {
+    const Chunk c = C[blockIdx.x];
+    const DevScene& S = T[c.scene];
+    if (!S.pbOn) return;
+    const int i = c.begin + threadIdx.x;
+    if (i >= S.n) return;
+    float4 xf = S.x[i];
+    const double px = xf.x, py = xf.y, pz = xf.z;
+    double qx = px, qy = py, qz = pz;
+    if (MODE == 2) {
+        const float4 x0 = S.x0s[i], d = S.prS[i];
+        qx = (double)x0.x + (double)d.x;
+        qy = (double)x0.y + (double)d.y;
+        qz = (double)x0.z + (double)d.z;
+    }
+    const PbHit h = pb_project(S, px, py, pz, xf, qx, qy, qz);
+    S.pbn[i] = make_float4((float)h.nx, (float)h.ny, (float)h.nz, 0.f);
+    if (h.hit) {
+        xf.x = (float)(px + h.s * h.nx);
+        xf.y = (float)(py + h.s * h.ny);
+        xf.z = (float)(pz + h.s * h.nz);
+        S.x[i] = xf;
+    }
+    if (MODE == 0) {
+        const int cl = cell_of(S, xf);
+        S.cellU[i] = cl;
+        atomicAdd(&S.cellCount[cl], 1);
+    } else {
+        double ux, uy, uz;
+        if (MODE == 1) {
+            const float4 v = S.v[i];
+            ux = v.x; uy = v.y; uz = v.z;
+        } else {
+            const float4 v = S.pvS[i];
+            ux = v.x; uy = v.y; uz = v.z;
+        }
+        if (pb_velocity(S, h.nx, h.ny, h.nz, ux, uy, uz)) S.v[i] = make_float4((float)ux, (float)uy, (float)uz, 0.f);
+    }
+}
+
+// correctPredPositions (src/PCISPHBoundaryModel.cpp:385-439), each PCISPH
+// loop iteration after the prediction: collision test with the current
+// position, weights and shift on the predicted one (x + d), d corrected.
+// Keeps the corrected d and the predicted velocity for the projection after
+// the integration.
+__global__ __launch_bounds__(kBlock) void k_pb_predcorrect(const DevScene* __restrict__ T, const Chunk* __restrict__ C) {
+    const Chunk c = C[blockIdx.x];
+    const DevScene& S = T[c.scene];
+    if (!S.pbOn || S.ss->done) return;
+    const int i = c.begin + threadIdx.x;
+    if (i >= S.n) return;
+    const float4 xf = S.x[i];
+    float4 d = S.xs[i];
+    const double px = xf.x, py = xf.y, pz = xf.z;
+    const PbHit h = pb_project(S, px, py, pz, xf, px + (double)d.x, py + (double)d.y, pz + (double)d.z);
+    S.pbn[i] = make_float4((float)h.nx, (float)h.ny, (float)h.nz, 0.f);
+    if (h.hit) {
+        d.x = (float)((double)d.x + h.s * h.nx);
+        d.y = (float)((double)d.y + h.s * h.ny);
+        d.z = (float)((double)d.z + h.s * h.nz);
+        S.xs[i] = d;
+    }
+    S.prS[i] = d;
+    S.pvS[i] = S.pv[i];
+}
+
 // WCSPH: time += dt after the integration (src/WCSPHSolver.cpp:77).
 __global__ void k_time_advance(const DevScene* __restrict__ T, int nscenes) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1649,6 +1893,35 @@ void launch_pci_density(hipStream_t st, const DevScene* T, const Chunk* C, int n
 void launch_pci_pacc(hipStream_t st, const DevScene* T, const Chunk* C, int n, int iter) { if (n) LAUNCH(k_pci_pacc, n, st, T, C, iter); }
 void launch_pci_integrate(hipStream_t st, const DevScene* T, const Chunk* C, int n) { if (n) LAUNCH(k_pci_integrate, n, st, T, C); }
 void launch_cube(hipStream_t st, const DevScene* T, const Chunk* C, int n) { if (n) LAUNCH(k_cube, n, st, T, C); }
+void launch_pb_bself(hipStream_t st, const DevScene* T, int scene, int nb) {
+    if (nb) hipLaunchKernelGGL(k_pb_bself, dim3((nb + kBlock - 1) / kBlock), dim3(kBlock), 0, st, T, scene);
+}
+void launch_pb_bdensity(hipStream_t st, int pred, const DevScene* T, int scene, int nb, int iter) {
+    if (!nb) return;
+    const dim3 g((nb + kBlock - 1) / kBlock), b(kBlock);
+    if (pred) hipLaunchKernelGGL(k_pb_bdensity<1>, g, b, 0, st, T, scene, iter);
+    else hipLaunchKernelGGL(k_pb_bdensity<0>, g, b, 0, st, T, scene, iter);
+}
+// dst[s] = src[id[s]] (an id-ordered float3 array into the working order).
+__global__ void k_gather_f3(float4* __restrict__ dst, const float* __restrict__ src, const uint32_t* __restrict__ id,
+                            int n) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const float* q = src + 3 * (size_t)id[s];
+    dst[s] = make_float4(q[0], q[1], q[2], 0.f);
+}
+void launch_gather_f3(hipStream_t st, float4* dst, const float* src, const uint32_t* id, int n) {
+    if (n) hipLaunchKernelGGL(k_gather_f3, dim3((n + 255) / 256), dim3(256), 0, st, dst, src, id, n);
+}
+void launch_pb_correct(hipStream_t st, int mode, const DevScene* T, const Chunk* C, int n) {
+    if (!n) return;
+    if (mode == 0) LAUNCH(k_pb_correct<0>, n, st, T, C);
+    else if (mode == 1) LAUNCH(k_pb_correct<1>, n, st, T, C);
+    else LAUNCH(k_pb_correct<2>, n, st, T, C);
+}
+void launch_pb_predcorrect(hipStream_t st, const DevScene* T, const Chunk* C, int n) {
+    if (n) LAUNCH(k_pb_predcorrect, n, st, T, C);
+}
 void launch_emit(hipStream_t st, DevScene* Tc, DevScene* To, int nscenes, int cells) {
     if (nscenes) hipLaunchKernelGGL(k_emit, dim3(nscenes), dim3(kBlock), 0, st, Tc, To, nscenes, cells);
 }

@@ -219,6 +219,41 @@ public:
     void addSphere(Vector3r origin, Real radius);     // src/AkinciBoundayModel.cpp:60-76
 };
 
+// The PCISPH boundary method's model (src/PCISPHBoundaryModel.h:10-52):
+// points sampled at 0.45h on boxes and spheres with their normals (inverted
+// keeps the sampler's inward normals), boundary mass and rest density, the
+// normal and tangential velocity factors of the collision projection, and
+// a host mirror of the boundary densities and pressures.  The device
+// computes those and runs the projection inside the solver step.
+class PCISPHBoundaryModel : public BoundaryModel {
+protected:
+    std::vector<Vector3r> n;
+    std::vector<Real> density, pressure;
+    Real density0 = 1000, mass = 0, normalFct = 0, tangentialFct = 1;
+
+public:
+    void setNormalFct(Real f) { normalFct = f; }
+    void setTangentialFct(Real f) { tangentialFct = f; }
+    void setRefDensity(Real d) { density0 = d; }
+    void setMass(Real m) { mass = m; }
+    Real getNormalFct() const { return normalFct; }
+    Real getTangentialFct() const { return tangentialFct; }
+    Real getRefDensity() const { return density0; }
+    Real getMass() const { return mass; }
+    Real& getDensity(unsigned int i) { return density[i]; }
+    Real& getPressure(unsigned int i) { return pressure[i]; }
+    Vector3r& getNormal(unsigned int i) { return n[i]; }
+    std::vector<Vector3r>& getNormals() { return n; }
+    void resizeBoundary(unsigned int k) override {
+        BoundaryModel::resizeBoundary(k);
+        n.resize(k);
+        density.resize(k);
+        pressure.resize(k);
+    }
+    void addCube(Vector3r min, Vector3r max, bool inverted);         // src/PCISPHBoundaryModel.cpp:7-27
+    void addSphere(Vector3r origin, Real radius, bool inverted);     // src/PCISPHBoundaryModel.cpp:29-52
+};
+
 // The box of the cube boundary method: points [min, max] (init), normal and
 // tangential velocity factors 0 and 1 by default.  The clamp runs on the
 // device after each step's integration.
@@ -445,6 +480,7 @@ struct Boundary {   // src/Simulation.h:119-127
     std::vector<std::pair<BlockInfo, bool>> box;
     std::vector<std::pair<SphereInfo, bool>> sphere;
     int numGeometry = 0;
+    Real normalFct = 0, tangFct = 1;   // PCISPH boundary method
 };
 
 struct BoundaryInfo {

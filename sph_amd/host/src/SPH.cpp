@@ -137,6 +137,34 @@ void AkinciBoundaryModel::addSphere(Vector3r o, Real radius) {
     std::copy(add.begin(), add.end(), r.begin() + base);
 }
 
+void PCISPHBoundaryModel::addCube(Vector3r mn, Vector3r mx, bool inverted) {
+    const double a[3] = {mn.x, mn.y, mn.z}, b[3] = {mx.x, mx.y, mx.z};
+    const double h = Simulation::getCurrent()->getSupportRadius();
+    const int64_t k = sphg_sample_box_normals(a, b, h, inverted ? 1 : 0, nullptr, nullptr, 0);
+    if (k < 0) throw SphError("bad box (the PCISPH boundary sampler needs a positive extent on every axis)");
+    std::vector<float> pts(3 * (size_t)k), nrm(3 * (size_t)k);
+    sphg_sample_box_normals(a, b, h, inverted ? 1 : 0, pts.data(), nrm.data(), k);
+    auto p = fromFloat(pts), q = fromFloat(nrm);
+    const size_t base = r.size();
+    resizeBoundary((unsigned int)(base + p.size()));
+    std::copy(p.begin(), p.end(), r.begin() + base);
+    std::copy(q.begin(), q.end(), n.begin() + base);
+}
+
+void PCISPHBoundaryModel::addSphere(Vector3r o, Real radius, bool inverted) {
+    const double c[3] = {o.x, o.y, o.z};
+    const double pr = Simulation::getCurrent()->getParticleRadius();
+    const int64_t k = sphg_sample_sphere_normals(c, radius, pr, inverted ? 1 : 0, nullptr, nullptr, 0);
+    if (k < 0) throw SphError("bad sphere");
+    std::vector<float> pts(3 * (size_t)k), nrm(3 * (size_t)k);
+    sphg_sample_sphere_normals(c, radius, pr, inverted ? 1 : 0, pts.data(), nrm.data(), k);
+    auto p = fromFloat(pts), q = fromFloat(nrm);
+    const size_t base = r.size();
+    resizeBoundary((unsigned int)(base + p.size()));
+    std::copy(p.begin(), p.end(), r.begin() + base);
+    std::copy(q.begin(), q.end(), n.begin() + base);
+}
+
 // ------------------------------------------------------------ solvers
 
 GpuSolver::~GpuSolver() {
@@ -209,6 +237,20 @@ void GpuSolver::fillDesc(sphg_scene_desc& d) const {
                 d.cube_tangential_fct = c->getTangentialFct();
             }
     }
+    // the PCISPH method's velocity factors: one PCISPHBoundaryModel (its
+    // corrections run model after model in the reference,
+    // src/PCISPHSolver.cpp:67-74, which one model on the device reproduces)
+    d.pb_normal_fct = 0.0;
+    d.pb_tangential_fct = 1.0;
+    if (d.boundary_method == Simulation::PCISPH_BOUNDARY_METHOD) {
+        unsigned int models = 0;
+        for (unsigned int b = 0; b < sim->numberBoundaryModels(); ++b)
+            if (auto* m = dynamic_cast<PCISPHBoundaryModel*>(sim->getBoundaryModel(b))) {
+                if (++models > 1) throw SphError("the GPU solver takes one PCISPHBoundaryModel (one Boundary entry)");
+                d.pb_normal_fct = m->getNormalFct();
+                d.pb_tangential_fct = m->getTangentialFct();
+            }
+    }
     d.particle_radius = sim->getParticleRadius();
     d.density0 = sim->numberFluidModels() ? sim->getFluidModel(0)->getRefDensity() : 1000.0;
     const Vector3r g = sim->getGravity();
@@ -265,8 +307,9 @@ void DFSPHSolver::fillDesc(sphg_scene_desc& d) const {
 void GpuSolver::init() {
     Simulation* sim = Simulation::getCurrent();
     const int bm = sim->getBoundaryHandlingMethod();
-    if (bm != Simulation::AKINCI_BOUNDARY_METHOD && bm != Simulation::CUBE_BOUNDARY_METHOD)
-        throw SphError("the GPU solver needs boundaryMethod 0 (cube) or 2 (Akinci)");
+    if (bm != Simulation::AKINCI_BOUNDARY_METHOD && bm != Simulation::CUBE_BOUNDARY_METHOD &&
+        bm != Simulation::PCISPH_BOUNDARY_METHOD)
+        throw SphError("the GPU solver needs boundaryMethod 0 (cube), 1 (PCISPH) or 2 (Akinci)");
     if (sim->numberFluidModels() != 1) throw SphError("the GPU solver runs one fluid model per scene");
     sphg_ctx*& c = sharedCtx();
     if (!c) check(sphg_ctx_create(sim->getDevice(), &c), "sphg_ctx_create");
@@ -282,18 +325,24 @@ void GpuSolver::init() {
         const sphg_emitter_desc ed = fm->getEmitter(e).desc();
         check(sphg_scene_add_emitter(scene, &ed), "add_emitter");
     }
-    std::vector<Vector3r> bpts;
-    if (bm == Simulation::AKINCI_BOUNDARY_METHOD)
+    std::vector<Vector3r> bpts, bnrm;
+    if (bm == Simulation::AKINCI_BOUNDARY_METHOD || bm == Simulation::PCISPH_BOUNDARY_METHOD)
         for (unsigned int b = 0; b < sim->numberBoundaryModels(); ++b) {
             auto& p = sim->getBoundaryModel(b)->getPosition();
             bpts.insert(bpts.end(), p.begin(), p.end());
+            if (auto* m = dynamic_cast<PCISPHBoundaryModel*>(sim->getBoundaryModel(b)))
+                bnrm.insert(bnrm.end(), m->getNormals().begin(), m->getNormals().end());
         }
     std::vector<float> bx = toFloat(bpts);
     check(sphg_scene_set_boundary(scene, bx.data(), (uint32_t)bpts.size()), "set_boundary");
+    if (bm == Simulation::PCISPH_BOUNDARY_METHOD) {
+        std::vector<float> bn = toFloat(bnrm);
+        check(sphg_scene_set_boundary_normals(scene, bn.data(), (uint32_t)bnrm.size()), "set_boundary_normals");
+    }
     check(sphg_scene_init(scene), "sphg_scene_init");
     // boundary volumes back into the Akinci models
     std::vector<float> vol(std::max<size_t>(bpts.size(), 1));
-    check(sphg_scene_get(scene, SPHG_FIELD_BVOL, vol.data()), "get volumes");
+    if (bm != Simulation::PCISPH_BOUNDARY_METHOD) check(sphg_scene_get(scene, SPHG_FIELD_BVOL, vol.data()), "get volumes");
     size_t off = 0;
     for (unsigned int b = 0; bm == Simulation::AKINCI_BOUNDARY_METHOD && b < sim->numberBoundaryModels(); ++b) {
         auto* abm = dynamic_cast<AkinciBoundaryModel*>(sim->getBoundaryModel(b));
@@ -350,6 +399,20 @@ void GpuSolver::syncToHost() {
         check(sphg_scene_get(scene, SPHG_FIELD_P, s.data()), "get p");
         for (unsigned int i = 0; i < n; ++i) fm->getPressure(i) = s[i];
     }
+    // PCISPH boundary method: the boundary densities and pressures the
+    // WCSPH and PCISPH steps form (DFSPH does not use them)
+    if (sim->getBoundaryHandlingMethod() == Simulation::PCISPH_BOUNDARY_METHOD && method != SPHG_DFSPH)
+        for (unsigned int b = 0; b < sim->numberBoundaryModels(); ++b)
+            if (auto* m = dynamic_cast<PCISPHBoundaryModel*>(sim->getBoundaryModel(b))) {
+                const unsigned int nb = m->getNumParticles();
+                std::vector<float> bd(std::max(nb, 1u)), bp(std::max(nb, 1u));
+                check(sphg_scene_get(scene, SPHG_FIELD_BDENSITY, bd.data()), "get boundary densities");
+                check(sphg_scene_get(scene, SPHG_FIELD_BPRESSURE, bp.data()), "get boundary pressures");
+                for (unsigned int i = 0; i < nb; ++i) {
+                    m->getDensity(i) = bd[i];
+                    m->getPressure(i) = bp[i];
+                }
+            }
     hostFresh = true;
 }
 

@@ -150,6 +150,10 @@ void readBoundaryInfo(SimulationInfo& info, const Json& root) {
             bool inverted = pcisphBoundary ? spheres.at(j).at("inverted").flag("inverted") : false;
             bd.sphere.emplace_back(s, inverted);
         }
+        if (pcisphBoundary) {   // src/SceneLoader.cpp:161-165
+            bd.normalFct = b.at("normalFct").num("Boundary.normalFct");
+            bd.tangFct = b.at("tangFct").num("Boundary.tangFct");
+        }
         bd.numGeometry = (int)b.get("geometry").size();
     }
 }
@@ -320,11 +324,27 @@ bool Simulation::importScene(const std::string& path) {
             for (auto& box : b.box) bm->addCube(box.first.min, box.first.max);
             for (auto& sp : b.sphere) bm->addSphere(sp.first.center, sp.first.radius);
         }
+    } else if (s.boundaryMethod == PCISPH_BOUNDARY_METHOD) {
+        // src/Simulation.cpp:459-486: one model per Boundary entry, boundary
+        // mass rho0 (2r)^3 and rest density of the (single) fluid
+        if (numberFluidModels() != 1) throw SphError("the PCISPH boundary method takes one fluid model");
+        const Real rho0 = getFluidModel(0)->getRefDensity();
+        for (const Boundary& b : info.boundaryData.boundaries) {
+            if (b.numGeometry) throw SphError("OBJ geometry boundaries are not supported by the GPU solver");
+            auto* bm = new PCISPHBoundaryModel();
+            addBoundaryModel(bm);
+            bm->setMass(rho0 * std::pow(2.0 * getParticleRadius(), 3.0));
+            bm->setRefDensity(rho0);
+            bm->setNormalFct(b.normalFct);
+            bm->setTangentialFct(b.tangFct);
+            for (auto& box : b.box) bm->addCube(box.first.min, box.first.max, box.second);
+            for (auto& sp : b.sphere) bm->addSphere(sp.first.center, sp.first.radius, sp.second);
+        }
     } else if (s.boundaryMethod != CUBE_BOUNDARY_METHOD) {
         // cube (0): the reference's loader creates no boundary model
-        // (src/Simulation.cpp:435-486); the PCISPH method (1) is not built
+        // (src/Simulation.cpp:435-486)
         throw SphError("boundaryMethod " + std::to_string(s.boundaryMethod) +
-                       " is not supported by the GPU solver (cube = 0 and Akinci = 2 are)");
+                       " is not one of 0 (cube), 1 (PCISPH), 2 (Akinci)");
     }
     init();
     return true;

@@ -16,15 +16,16 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libsphg.so")
 
 WCSPH, PCISPH, DFSPH = 0, 1, 2
-BOUNDARY_CUBE, BOUNDARY_AKINCI = 0, 2
+BOUNDARY_CUBE, BOUNDARY_PCISPH, BOUNDARY_AKINCI = 0, 1, 2
 VISCOSITY_STANDARD, VISCOSITY_ARTIFICIAL, VISCOSITY_XSPH = 0, 1, 2
 SURFACE_TENSION_AKINCI, ADHESION_AKINCI = 3, 4
 
 FIELD_X, FIELD_V, FIELD_A, FIELD_RHO, FIELD_P = 0, 1, 2, 3, 4
 FIELD_ALPHA, FIELD_RHOSTAR, FIELD_DIVERR, FIELD_K, FIELD_KV = 5, 6, 7, 8, 9
 FIELD_BVOL, FIELD_NORMAL, FIELD_PACC, FIELD_KEYS, FIELD_NNB, FIELD_NGRAD = 10, 11, 12, 13, 14, 15
+FIELD_PBNORMAL, FIELD_BDENSITY, FIELD_BPRESSURE = 16, 17, 18
 
-_VEC3 = {FIELD_X, FIELD_V, FIELD_A, FIELD_NORMAL, FIELD_PACC}
+_VEC3 = {FIELD_X, FIELD_V, FIELD_A, FIELD_NORMAL, FIELD_PACC, FIELD_PBNORMAL}
 _INT = {FIELD_KEYS, FIELD_NNB, FIELD_NGRAD}
 
 
@@ -69,6 +70,8 @@ class SceneDesc(C.Structure):
         ("cube_max", C.c_double * 3),
         ("cube_normal_fct", C.c_double),
         ("cube_tangential_fct", C.c_double),
+        ("pb_normal_fct", C.c_double),
+        ("pb_tangential_fct", C.c_double),
     ]
 
 
@@ -185,6 +188,8 @@ class SceneParams:
     cube_max: Sequence[float] = (0.0, 0.0, 0.0)
     cube_normal_fct: float = 0.0
     cube_tangential_fct: float = 1.0
+    pb_normal_fct: float = 0.0        # PCISPHBoundaryModel (boundary_method 1)
+    pb_tangential_fct: float = 1.0
 
     @property
     def support_radius(self) -> float:
@@ -226,6 +231,10 @@ def load(path: str = LIB_PATH) -> C.CDLL:
         "sphg_scene_set_fluid": (C.c_int, [P, fp, fp, U32]),
         "sphg_scene_set_boundary": (C.c_int, [P, fp, U32]),
         "sphg_scene_add_emitter": (C.c_int, [P, C.POINTER(EmitterDesc)]),
+        "sphg_scene_set_boundary_normals": (C.c_int, [P, fp, U32]),
+        "sphg_scene_set_pb_normals": (C.c_int, [P, fp]),
+        "sphg_sample_box_normals": (I64, [dp, dp, D, I32, fp, fp, I64]),
+        "sphg_sample_sphere_normals": (I64, [dp, D, D, I32, fp, fp, I64]),
         "sphg_scene_capacity": (C.c_int, [P, C.POINTER(U32)]),
         "sphg_snapshot_create": (C.c_int, [P, C.POINTER(P)]),
         "sphg_snapshot_begin": (C.c_int, [P]),
@@ -304,6 +313,33 @@ def sample_sphere(center, radius: float, particle_radius: float) -> np.ndarray:
     lib = load()
     c = np.asarray(center, dtype=np.float64)
     return _sample(lib.sphg_sample_sphere, _dptr(c), float(radius), float(particle_radius))
+
+
+def _sample_normals(fn, *args):
+    n = fn(*args, None, None, 0)
+    if n < 0:
+        raise SphgError("invalid sampler arguments")
+    pts = np.zeros((max(n, 1), 3), dtype=np.float32)
+    nrm = np.zeros((max(n, 1), 3), dtype=np.float32)
+    fn(*args, _fptr(pts), _fptr(nrm), n)
+    return pts[:n], nrm[:n]
+
+
+def sample_box_normals(mn, mx, support_radius: float, inverted: bool):
+    """PCISPHBoundaryModel::addCube (src/PCISPHBoundaryModel.cpp:7-27,60-189):
+    points and normals (inward when inverted)."""
+    lib = load()
+    a = np.asarray(mn, dtype=np.float64)
+    b = np.asarray(mx, dtype=np.float64)
+    return _sample_normals(lib.sphg_sample_box_normals, _dptr(a), _dptr(b), float(support_radius), int(bool(inverted)))
+
+
+def sample_sphere_normals(center, radius: float, particle_radius: float, inverted: bool):
+    """PCISPHBoundaryModel::addSphere (src/PCISPHBoundaryModel.cpp:29-52)."""
+    lib = load()
+    c = np.asarray(center, dtype=np.float64)
+    return _sample_normals(lib.sphg_sample_sphere_normals, _dptr(c), float(radius), float(particle_radius),
+                           int(bool(inverted)))
 
 
 def fluid_block(mn, mx, particle_radius: float, jitter: float = 0.0, seed: int = 0) -> np.ndarray:
@@ -446,7 +482,7 @@ class Scene:
     """One fluid + Akinci boundary scene resident on the device (sphg_scene)."""
 
     def __init__(self, ctx: Context, params: SceneParams, fluid, boundary, velocities=None, init: bool = True,
-                 emitters: Sequence[Emitter] = ()):
+                 emitters: Sequence[Emitter] = (), boundary_normals=None):
         self.ctx = ctx
         self.lib = ctx.lib
         self.params = params
@@ -463,6 +499,9 @@ class Scene:
         if x.shape[0]:
             _check(self.lib.sphg_scene_set_fluid(self.h, _fptr(x), _fptr(v), x.shape[0]))
         _check(self.lib.sphg_scene_set_boundary(self.h, _fptr(b), self.nb))
+        if boundary_normals is not None:
+            bn = _f32(boundary_normals).reshape(-1, 3)
+            _check(self.lib.sphg_scene_set_boundary_normals(self.h, _fptr(bn), bn.shape[0]))
         for e in self.emitters:
             d = e.to_desc()
             _check(self.lib.sphg_scene_add_emitter(self.h, C.byref(d)))
@@ -514,8 +553,13 @@ class Scene:
         _check(self.lib.sphg_scene_set_state(self.h, _fptr(x), _fptr(v), _fptr(k), _fptr(kv), dt,
                                              int(last_iterations)))
 
+    def set_pb_normals(self, n) -> None:
+        """The fluid normal field of the PCISPH boundary method (id order)."""
+        a = _f32(n).reshape(-1, 3)
+        _check(self.lib.sphg_scene_set_pb_normals(self.h, _fptr(a)))
+
     def get(self, fld: int) -> np.ndarray:
-        if fld == FIELD_BVOL:
+        if fld in (FIELD_BVOL, FIELD_BDENSITY, FIELD_BPRESSURE):
             out = np.zeros(max(self.nb, 1), dtype=np.float32)
             _check(self.lib.sphg_scene_get(self.h, fld, out.ctypes.data_as(C.c_void_p)))
             return out[: self.nb]

@@ -185,3 +185,33 @@ def test_frames_cadence_and_content(ctx, tmp_path):
         assert st["time"] == t and st["numParticles"] == st["numActiveParticles"] == spec.n
         assert np.array_equal(np.array(st["positions"]), x.astype(np.float64))
         assert np.array_equal(np.array(st["velocities"]), v.astype(np.float64))
+
+
+def test_pcisph_boundary_method_scene_json(ctx, tmp_path):
+    """boundaryMethod 1 in a scene file (one Boundary entry, an inverted box,
+    normalFct 0.5, tangFct 0.9) through the host layer ends in the same
+    state, bit for bit, as the same scene through the C ABI."""
+    with open(os.path.join(SCENES, "dam_break_dfsph_small.json")) as f:
+        d = json.load(f)
+    d["Configuration"]["boundaryMethod"] = 1
+    b = d["Boundary"][0]
+    b["normalFct"], b["tangFct"] = 0.5, 0.9
+    b["box"][0]["inverted"] = True
+    path = tmp_path / "pb.json"
+    path.write_text(json.dumps(d))
+    out = tmp_path / "final_pb.json"
+    p = subprocess.run([RUN, "-f", str(path), "--steps", str(STEPS), "--final", str(out)],
+                       capture_output=True, text=True, timeout=110)
+    assert p.returncode == 0, p.stderr
+    with open(out) as f:
+        st = json.load(f)
+    spec = configs.small_scene(sphg.DFSPH, n_side=8, spacing=0.05, jitter=0.01, seed=7)
+    from dataclasses import replace
+    params = replace(spec.params, boundary_method=sphg.BOUNDARY_PCISPH, pb_normal_fct=0.5, pb_tangential_fct=0.9)
+    pts, nrm = sphg.sample_box_normals(b["box"][0]["min"], b["box"][0]["max"], spec.params.support_radius, True)
+    s = sphg.Scene(ctx, params, spec.fluid, pts, boundary_normals=nrm)
+    s.step(STEPS)
+    xa, va = s.get(sphg.FIELD_X), s.get(sphg.FIELD_V)
+    s.close()
+    assert np.array_equal(np.array(st["positions"]), xa.astype(np.float64))
+    assert np.array_equal(np.array(st["velocities"]), va.astype(np.float64))

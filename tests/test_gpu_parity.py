@@ -418,3 +418,66 @@ def test_snapshot_async(ctx):
     a.close()
     b.close()
     s.close()
+
+
+@pytest.mark.parametrize("method", [sphg.DFSPH, sphg.WCSPH, sphg.PCISPH])
+def test_pcisph_boundary_method(ctx, method):
+    """PCISPH boundary method (src/PCISPHBoundaryModel.cpp, and its terms in
+    src/SPHSolver.cpp, src/WCSPHSolver.cpp, src/PCISPHSolver.cpp,
+    src/DFSPHSolver.cpp): the dam-break box sampled at 0.45h with inward
+    normals and the block's first layer inside h/2 = 2r of the walls, so the
+    collision projection acts from the first step.  Each step starts from
+    the GPU's state (DFSPH also from its fluid normals); positions,
+    velocities and densities (1e-5), boundary densities and pressures
+    (1e-5) and the fluid normals are compared.  The normals get 1e-4: near
+    the cut-off a weight w ~ (h/2 - |q|) is close to zero and the normal's
+    direction follows FP32 rounding there, while the shift it drives
+    vanishes with w.
+
+    With the PCISPH solver the method's boundary pressures grow by
+    delta max(0, rho_b - rho0) every loop iteration (a boundary particle's
+    Poly6 sum over its own 0.45h lattice exceeds rho0 and it cannot move),
+    so with the density tolerance loop the step is chaotic (iteration counts
+    up to the maximum, speeds of thousands of m/s, in the oracle as well);
+    that case runs a fixed two iterations (eta = 1) at dt = 2.5e-4."""
+    from dataclasses import replace
+    pci = method == sphg.PCISPH
+    spec = configs.small_scene(method, n_side=8, gap=1.8 if pci else 1.5)
+    box = tuple(float(x) for x in spec.boundary.max(0))
+    h = spec.params.support_radius
+    pts, nrm = sphg.sample_box_normals((0.0, 0.0, 0.0), box, h, True)
+    kw = dict(eta=1.0, min_iterations=2, time_step=2.5e-4, min_time_step=2.5e-4, max_time_step=2.5e-4) if pci else {}
+    params = replace(spec.params, boundary_method=sphg.BOUNDARY_PCISPH, pb_normal_fct=0.5, pb_tangential_fct=0.9, **kw)
+    g = sphg.Scene(ctx, params, spec.fluid, pts, boundary_normals=nrm)
+    o = po.Oracle(params, spec.fluid, pts, boundary_normals=nrm)
+    dfs = method == sphg.DFSPH
+    matched = 0
+    hits = 0
+    for step in range(10):
+        sc = g.scalars()
+        o.set_state(g.get(sphg.FIELD_X), g.get(sphg.FIELD_V), g.get(sphg.FIELD_K) if dfs else None,
+                    g.get(sphg.FIELD_KV) if dfs else None, sc.time_step, sc.iterations)
+        if dfs:
+            o.set_pb_normals(g.get(sphg.FIELD_PBNORMAL))
+        g.step(1)
+        o.step(1)
+        sg, so = g.scalars(), o.scalars()
+        if sg.iterations != int(so["iterations"]) or (dfs and sg.iterations_v != int(so["iterations_v"])):
+            print("iteration mismatch", step)
+            continue
+        matched += 1
+        ng, no = g.get(sphg.FIELD_PBNORMAL), o.get(po.OF_PBN)
+        assert np.array_equal(np.isnan(ng), np.isnan(no)), step
+        ok = ~np.isnan(no[:, 0])
+        hits += int(ok.sum())
+        errs = {"x": rel_norm(g.get(sphg.FIELD_X), o.get(po.OF_X)), "v": rel_norm(g.get(sphg.FIELD_V), o.get(po.OF_V)),
+                "rho": rel_elem(g.get(sphg.FIELD_RHO), o.get(po.OF_RHO))}
+        if not dfs:
+            errs["bdens"] = rel_elem(g.get(sphg.FIELD_BDENSITY), o.get(po.OF_BDENS))
+            errs["bpres"] = rel_norm(g.get(sphg.FIELD_BPRESSURE), o.get(po.OF_BPRESS))
+        dn = float(np.abs(ng[ok] - no[ok]).max()) if ok.any() else 0.0
+        print("step", step, "hits", int(ok.sum()), {k: f"{v:.2e}" for k, v in errs.items()}, f"normals {dn:.1e}")
+        assert all(v < TOL for v in errs.values()), (step, errs)
+        assert dn < 1e-4, (step, dn)
+    assert matched >= 7 and hits > 0
+    g.close()

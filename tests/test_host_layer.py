@@ -62,16 +62,28 @@ def test_device_other_than_gpu_is_refused(tmp_path):
 
 
 def test_unsupported_boundary_method_is_refused(tmp_path):
-    # the PCISPH boundary method (1) is not built; its scene keys parse
-    def edit(d):
-        d["Configuration"]["boundaryMethod"] = 1
-        for b in d["Boundary"]:
-            b["normalFct"], b["tangFct"] = 0.0, 1.0
-            for box in b["box"]:
-                box["inverted"] = False
-    path = _variant(tmp_path, edit)
+    path = _variant(tmp_path, lambda d: d["Configuration"].__setitem__("boundaryMethod", 3))
     p = _run("-f", path)
     assert p.returncode == 1 and "boundaryMethod" in p.stderr
+
+
+def test_pcisph_boundary_method_parses(tmp_path):
+    """boundaryMethod 1: each Boundary entry needs normalFct and tangFct, and
+    each box and sphere its inverted flag (src/SceneLoader.cpp:161-190)."""
+    def edit(d, drop=None):
+        d["Configuration"]["boundaryMethod"] = 1
+        for b in d["Boundary"]:
+            b["normalFct"], b["tangFct"] = 0.5, 0.9
+            for box in b["box"]:
+                box["inverted"] = True
+            if drop:
+                del b[drop]
+    p = _run("-f", _variant(tmp_path, edit), "--info")
+    assert p.returncode == 0, p.stderr
+    assert json.loads(p.stdout)["boundaryMethod"] == 1
+    for key in ("normalFct", "tangFct"):
+        p = _run("-f", _variant(tmp_path, lambda d, k=key: edit(d, k)), "--info")
+        assert p.returncode == 1 and key in p.stderr, (key, p.stderr)
 
 
 def test_cube_boundary_method_parses(tmp_path):

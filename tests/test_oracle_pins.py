@@ -475,3 +475,123 @@ def test_emitter_rotation_quaternion():
     exp = np.stack([1.0 + loc[:, 0], 2.0 + 0.0 * loc[:, 1], 3.0 + loc[:, 1]], -1)
     assert np.allclose(x, exp, rtol=0, atol=1e-15)
     assert np.allclose(v, np.tile([0.0, -0.5, 0.0], (6, 1)), rtol=0, atol=1e-15)
+
+
+# ------------------------------------------------- PCISPH boundary method
+
+def _cubic_np(r, h):
+    q = r / h
+    k = 8.0 / (math.pi * h ** 3)
+    return np.where(q <= 0.5, k * (6 * q ** 3 - 6 * q ** 2 + 1), np.where(q <= 1.0, k * 2 * (1 - q) ** 3, 0.0))
+
+
+def _pb_scene(method, fluid, vel, dt=2.0 ** -6, **kw):
+    r = 1.0 / 32.0
+    h = 4 * r
+    pts, nrm = sphg.sample_box_normals((0.0, 0.0, 0.0), (1.0, 1.0, 1.0), h, True)
+    p = _params(method=method, r=r, time_step=dt, min_time_step=dt, max_time_step=dt, boundary_method=1, **kw)
+    o = po.Oracle(p, np.asarray(fluid, np.float32), pts, velocities=np.asarray(vel, np.float32),
+                  boundary_normals=nrm)
+    return o, pts.astype(np.float64), nrm.astype(np.float64), h
+
+
+def test_pcisph_box_sampler_normals():
+    """PCISPHBoundaryModel::sampleCube/addCube (src/PCISPHBoundaryModel.cpp:
+    7-27, 60-189): the product sampler equals the oracle's restatement bit
+    for bit (as FP32); spacing ceil(l / 0.45h); unit normals pointing into
+    an inverted box (faces, edges (1,1,0)/sqrt2, corners (1,1,1)/sqrt3)."""
+    for mn, mx, h, inv in (((0, 0, 0), (1, 1, 1), 0.125, True), ((0.1, -0.2, 0.3), (0.9, 0.45, 1.7), 0.1, False)):
+        p, n = sphg.sample_box_normals(mn, mx, h, inv)
+        po_p, po_n = po.sample_box_normals(mn, mx, h, inv)
+        assert np.array_equal(p, po_p.astype(np.float32)) and np.array_equal(n, po_n.astype(np.float32))
+        cnt = [math.ceil(abs(b - a) / (0.45 * h)) for a, b in zip(mn, mx)]
+        tot = (cnt[0] + 1) * (cnt[1] + 1) * (cnt[2] + 1) - (cnt[0] - 1) * (cnt[1] - 1) * (cnt[2] - 1)
+        assert len(p) == tot
+        assert np.allclose(np.linalg.norm(n, axis=1), 1.0, atol=1e-7)
+        c = 0.5 * (np.asarray(mn) + np.asarray(mx))
+        inward = np.einsum("ij,ij->i", n.astype(np.float64), c - p) > 0
+        assert np.all(inward == inv)
+        faces = np.sum(np.isclose(np.abs(n), 1.0, atol=1e-7), axis=1) == 1
+        corners = np.all(np.isclose(np.abs(n), 1 / math.sqrt(3), atol=1e-7), axis=1)
+        assert faces.sum() > 0 and corners.sum() == 8
+    # sphere: the Akinci sampler's points, normals inv (center - point)
+    ps, ns = sphg.sample_sphere_normals((0.3, 0.4, 0.5), 0.2, 0.02, False)
+    assert np.array_equal(ps, sphg.sample_sphere((0.3, 0.4, 0.5), 0.2, 0.02))
+    assert np.allclose(ns, po.sphere_normals(ps.astype(np.float64), (0.3, 0.4, 0.5), False), atol=1e-7)
+
+
+def test_pcisph_boundary_projection_flat_wall():
+    """A particle moving into the x = 1 face (inverted box, no forces): after
+    the integration it lies within h/2 of the face particles and is moved
+    along the face normal (-1, 0, 0) by sum w (h/2 - |q|) / sum w, w =
+    (h/2 - |q|) / (h/2) (src/PCISPHBoundaryModel.cpp:282-307); the velocity
+    becomes tangentialFct v_t - normalFct v_n (:339-355)."""
+    x0 = np.array([[0.965, 0.5, 0.5]])
+    v0 = np.array([[1.0, 0.3, 0.0]])
+    o, pts, nrm, h = _pb_scene(sphg.WCSPH, x0, v0, stiffness=0.0, pb_normal_fct=0.5, pb_tangential_fct=0.8)
+    dt = 2.0 ** -6
+    o.step(1)
+    v32 = v0.astype(np.float32).astype(np.float64)[0]
+    x1 = x0.astype(np.float32).astype(np.float64)[0] + v32 * dt
+    R = 0.5 * h
+    q = np.linalg.norm(x1 - pts, axis=1)
+    hit = q < R
+    assert hit.sum() >= 4 and np.allclose(nrm[hit], [-1.0, 0.0, 0.0])
+    w = (R - q[hit]) / R
+    s = (w * (R - q[hit])).sum() / w.sum()
+    assert np.allclose(o.get(po.OF_X)[0], x1 + s * np.array([-1.0, 0.0, 0.0]), rtol=0, atol=1e-14)
+    assert np.allclose(o.get(po.OF_V)[0], [-0.5 * v32[0], 0.8 * v32[1], 0.0], rtol=0, atol=1e-15)
+    assert np.allclose(o.get(po.OF_PBN)[0], [-1.0, 0.0, 0.0], atol=1e-15)
+
+
+def test_pcisph_boundary_needs_an_existing_cell():
+    """The projection looks the position's cell up in the hash table: a cell
+    no particle was ever inserted in has no boundary candidates, so a
+    particle that enters one within h/2 of the wall is not projected in that
+    step (and its normal is NaN); once inserted there it is.  The same
+    motion starting inside that cell is projected at once."""
+    dt = 2.0 ** -6
+    # from cell 6 (x < 0.875) into cell 7 (0.875 <= x < 1), 0.056 from the wall
+    o, pts, _, h = _pb_scene(sphg.WCSPH, [[0.874, 0.5, 0.5]], [[4.5, 0.0, 0.0]], stiffness=0.0)
+    o.step(1)
+    x1 = np.float64(np.float32(0.874)) + 4.5 * dt
+    assert x1 >= 0.875 and 1.0 - x1 < 0.5 * h
+    assert o.get(po.OF_X)[0, 0] == x1 and np.all(np.isnan(o.get(po.OF_PBN)[0]))
+    o.step(1)                     # inserted in cell 7 last step: now projected
+    assert o.get(po.OF_X)[0, 0] < x1 + 4.5 * dt and np.allclose(o.get(po.OF_PBN)[0], [-1, 0, 0])
+    # control: starts in cell 7 (visited at the first insert)
+    o2, _, _, _ = _pb_scene(sphg.WCSPH, [[0.876, 0.5, 0.5]], [[4.5, 0.0, 0.0]], stiffness=0.0)
+    o2.step(1)
+    assert o2.get(po.OF_X)[0, 0] < np.float64(np.float32(0.876)) + 4.5 * dt
+    assert np.allclose(o2.get(po.OF_PBN)[0], [-1, 0, 0])
+
+
+@pytest.mark.parametrize("method", [sphg.WCSPH, sphg.PCISPH])
+def test_pcisph_boundary_densities_brute_force(method):
+    """Boundary densities of the PCISPH method with the fluid far from the
+    wall: WCSPH sums m_b W (cubic) over the boundary (src/SPHSolver.cpp:
+    93-121) and its pressure is the Tait EOS; the PCISPH loop sums m_b Poly6
+    (src/PCISPHSolver.cpp:214-247).  Brute force over all pairs."""
+    o, pts, _, h = _pb_scene(method, [[0.5, 0.5, 0.5]], [[0.0, 0.0, 0.0]], stiffness=1000.0, gamma=7.0,
+                             eta=0.01, min_iterations=2)
+    o.step(1)
+    bd = o.get(po.OF_BDENS)
+    far = np.linalg.norm(pts - 0.5, axis=1) > 1.5 * h
+    mb = 1000.0 * (2 * (1 / 32)) ** 3
+    sel = np.flatnonzero(far)[::37]
+    d = np.linalg.norm(pts[sel, None, :] - pts[None, :, :], axis=2)
+    if method == sphg.WCSPH:
+        exp = (mb * _cubic_np(d, h)).sum(1)
+        assert np.allclose(bd[sel], exp, rtol=1e-12)
+        bp = o.get(po.OF_BPRESS)
+        assert np.allclose(bp[sel], np.maximum(0.0, 1000.0 * 1000.0 / 7.0 * ((exp / 1000.0) ** 7 - 1)), rtol=1e-10)
+    else:
+        c6 = 315.0 / (64 * math.pi * h ** 9)
+        exp = (mb * np.where(d <= h, c6 * (h * h - d * d) ** 3, 0.0)).sum(1)
+        assert np.allclose(bd[sel], exp, rtol=1e-12)
+
+
+def test_pcisph_boundary_normals_nan_without_collision():
+    o, _, _, _ = _pb_scene(sphg.DFSPH, [[0.5, 0.5, 0.5], [0.52, 0.5, 0.5]], [[0, 0, 0], [0, 0, 0]])
+    o.step(1)
+    assert np.all(np.isnan(o.get(po.OF_PBN)))
