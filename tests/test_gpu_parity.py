@@ -481,3 +481,51 @@ def test_pcisph_boundary_method(ctx, method):
         assert dn < 1e-4, (step, dn)
     assert matched >= 7 and hits > 0
     g.close()
+
+
+@pytest.mark.parametrize("method", [sphg.DFSPH, sphg.WCSPH, sphg.PCISPH])
+def test_batch_of_mixed_features_alone_vs_mix(ctx, method):
+    """One batch holding a plain Akinci scene, a scene with an emitter, a
+    PCISPH-boundary scene and a cube-clamp scene: each ends, bit for bit, where
+    it ends stepped alone (persistent batch, run in quanta)."""
+    from dataclasses import replace
+    c = np.sqrt(0.5)
+    em = sphg.Emitter(type=sphg.EMITTER_SQUARE, num_particles=27, position=(0.9, 0.26, 0.4), velocity=5.0,
+                      rotation=(c, 0.0, -c, 0.0), start_time=0.0, width=0.16, height=0.16)
+
+    def build():
+        out = []
+        a = configs.small_scene(method, n_side=6, seed=3)
+        out.append(sphg.Scene(ctx, a.params, a.fluid, a.boundary))
+        b = configs.small_scene(method, n_side=5, seed=4)
+        out.append(sphg.Scene(ctx, b.params, b.fluid, b.boundary, emitters=[em]))
+        p = configs.small_scene(method, n_side=6, gap=1.8, seed=5)
+        kw = dict(eta=1.0, min_iterations=2) if method == sphg.PCISPH else {}
+        pp = replace(p.params, boundary_method=sphg.BOUNDARY_PCISPH, pb_normal_fct=0.5, pb_tangential_fct=0.9, **kw)
+        pts, nrm = sphg.sample_box_normals((0.0, 0.0, 0.0), tuple(float(x) for x in p.boundary.max(0)),
+                                           p.params.support_radius, True)
+        out.append(sphg.Scene(ctx, pp, p.fluid, pts, boundary_normals=nrm))
+        q = configs.small_scene(method, n_side=5, seed=6)
+        f = q.fluid.astype(np.float64)
+        qp = replace(q.params, boundary_method=sphg.BOUNDARY_CUBE, cube_enabled=1,
+                     cube_min=tuple(f.min(0) - (0.0, 0.0, -0.01 * q.params.particle_radius)),
+                     cube_max=tuple(f.max(0) + 0.5))
+        out.append(sphg.Scene(ctx, qp, q.fluid, np.zeros((0, 3), np.float32)))
+        return out
+
+    alone = build()
+    for s in alone:
+        s.step(30)
+    mixed = build()
+    b = sphg.Batch(ctx, mixed)
+    for _ in range(3):
+        b.run(10)
+    b.close()
+    for k, (a, m) in enumerate(zip(alone, mixed)):
+        assert a.n == m.n, k
+        assert np.array_equal(a.get(sphg.FIELD_X), m.get(sphg.FIELD_X)), k
+        assert np.array_equal(a.get(sphg.FIELD_V), m.get(sphg.FIELD_V)), k
+        assert a.scalars().sum_iterations == m.scalars().sum_iterations, k
+    assert mixed[1].n > configs.small_scene(method, n_side=5).n     # the emitter fired
+    for s in alone + mixed:
+        s.close()
