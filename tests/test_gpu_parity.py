@@ -529,3 +529,39 @@ def test_batch_of_mixed_features_alone_vs_mix(ctx, method):
     assert mixed[1].n > configs.small_scene(method, n_side=5).n     # the emitter fired
     for s in alone + mixed:
         s.close()
+
+
+@pytest.mark.parametrize("method", [sphg.DFSPH, sphg.WCSPH, sphg.PCISPH])
+def test_inflow_only_scene(ctx, method):
+    """A scene with no fluid at start, only an emitter (a FluidModel with no
+    fluid blocks, src/Simulation.cpp:395-405): the solver steps an empty
+    active set until the first group enters, then the state follows the
+    oracle."""
+    spec = configs.small_scene(method, n_side=4)
+    em = sphg.Emitter(type=sphg.EMITTER_CIRCLE, num_particles=70, position=(0.5, 0.26, 0.5), velocity=5.0,
+                      rotation=(0.0, 1.0, 0.0, 0.0), start_time=3.5 * spec.params.time_step, width=0.1, spacing=1.0)
+    empty = np.zeros((0, 3), np.float32)
+    g = sphg.Scene(ctx, spec.params, empty, spec.boundary, emitters=[em])
+    o = po.Oracle(spec.params, empty, spec.boundary)
+    assert o.add_emitter(em) == 7
+    assert g.n == 0 and g.capacity == 70
+    dfs = method == sphg.DFSPH
+    compared = 0
+    for step in range(30):
+        if g.n:
+            sc = g.scalars()
+            o.set_state(g.get(sphg.FIELD_X), g.get(sphg.FIELD_V), g.get(sphg.FIELD_K) if dfs else None,
+                        g.get(sphg.FIELD_KV) if dfs else None, sc.time_step, sc.iterations)
+        g.step(1)
+        o.step(1)
+        assert g.n == o.n, (step, g.n, o.n)
+        if g.n == 0:
+            continue
+        sg, so = g.scalars(), o.scalars()
+        if sg.iterations != int(so["iterations"]) or (dfs and sg.iterations_v != int(so["iterations_v"])):
+            continue
+        compared += 1
+        errs = {"x": rel_norm(g.get(sphg.FIELD_X), o.get(po.OF_X)), "v": rel_norm(g.get(sphg.FIELD_V), o.get(po.OF_V))}
+        assert all(v < TOL for v in errs.values()), (step, errs)
+    assert g.n > 7 and compared >= 10
+    g.close()
