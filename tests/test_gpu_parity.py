@@ -1,6 +1,6 @@
 """HIP solver vs the FP64 oracle on identical states.
 
-Bars (the north-star contract):
+Bars:
   * cell keys, sort order, bucket membership and neighbor sets: bit-exact;
   * per-step density, pressure / kappa, acceleration and velocity: within
     1e-5 relative in FP32 (density and boundary volume elementwise; signed
