@@ -12,8 +12,11 @@ representative.
 
 Prints one JSON line.  value = fluid particle-steps/s over the K timed steps
 (host wall clock around the step loop, bracketed by device synchronizes).
-During the timed steps a HIP event pair brackets every kernel launch on the
-library's stream; `roofline.achieved` is the dominant kernel's algorithmic
+The warm-up steps time every kernel with HIP event pairs on the library's
+stream (`kernels_warmup`); the dominant one is then the only kernel an event
+pair brackets during the timed steps (an event pair costs the stream a few
+microseconds: timing every launch would slow the step ~11%).
+`roofline.achieved` is the dominant kernel's algorithmic
 bytes per launch (per-particle figure x N plus boundary and cell terms)
 divided by its average duration per launch that does work (solver loop
 sweeps are also launched ahead for converged scenes, which return at once:
@@ -103,14 +106,28 @@ def main():
     ctx = sphg.Context(0)
     scene = sphg.Scene(ctx, spec.params, spec.fluid, spec.boundary)
     info = scene.info()
+    # the warm-up steps time every kernel; the dominant one (largest total
+    # time among the kernels with a byte model) is the one timed in the
+    # measured steps
+    warm = {}
+    if not args.no_roofline:
+        ctx.kernel_timing(True)
     if args.warmup:
         scene.step(args.warmup)
     ctx.synchronize()
+    if not args.no_roofline:
+        ctx.kernel_timing(False)
+        for nm in KERNELS:
+            cnt, ms = ctx.kernel_time(nm)
+            if cnt:
+                warm[nm] = {"launches": cnt, "total_ms": round(ms, 4), "avg_us": round(1e3 * ms / cnt, 3)}
     print(f"[bench] warmup {args.warmup} done", file=sys.stderr, flush=True)
 
     n = spec.n
+    dom = None
     if not args.no_roofline:
-        ctx.kernel_timing(True)
+        dom = max((k for k in warm if k in BALG), key=lambda k: warm[k]["total_ms"], default="df_err_density")
+        ctx.kernel_timing(True, only=[dom])
     s0 = scene.scalars()
     syncs = 0
     done = 0
@@ -139,10 +156,8 @@ def main():
     kernels = {}
     if not args.no_roofline:
         ctx.kernel_timing(False)
-        for nm in KERNELS:
-            cnt, ms = ctx.kernel_time(nm)
-            if cnt:
-                kernels[nm] = {"launches": cnt, "total_ms": round(ms, 4), "avg_us": round(1e3 * ms / cnt, 3)}
+        cnt, ms = ctx.kernel_time(dom)
+        kernels[dom] = {"launches": cnt, "total_ms": round(ms, 4), "avg_us": round(1e3 * ms / cnt, 3) if cnt else 0.0}
         # Loop sweeps are launched ahead of the convergence read-back; a
         # launch for a converged scene returns at once.  The launches that do
         # the work are (1 + it_d) density error sweeps, it_d density velocity
@@ -150,7 +165,6 @@ def main():
         # launches' time is charged to them.
         useful = {"df_err_density": args.steps * (1.0 + it_d), "df_vel_density": args.steps * it_d,
                   "df_err_divergence": args.steps * it_v, "df_vel_divergence": args.steps * it_v}
-        dom = max((k for k in kernels if k in BALG), key=lambda k: kernels[k]["total_ms"])
         per, bsw, csw = BALG[dom]
         bytes_launch = per * n + 16 * spec.nb * bsw + 4 * info["ncells"] * csw
         n_useful = useful.get(dom, kernels[dom]["launches"])
@@ -215,6 +229,8 @@ def main():
     }
     if kernels:
         out["kernels"] = kernels
+    if warm:
+        out["kernels_warmup"] = warm
     print(json.dumps(out), flush=True)
 
 

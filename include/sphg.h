@@ -162,6 +162,14 @@ const char* sphg_last_error(void);
 int sphg_ctx_create(int32_t device, sphg_ctx** out);
 int sphg_ctx_destroy(sphg_ctx* ctx);
 int sphg_ctx_synchronize(sphg_ctx* ctx);
+/* Replay of captured step segments (default off).  On, the fixed launch
+ * sequences of a step (everything but the host-decided iterations of the
+ * pressure loops) are captured once into HIP graphs per scene / per
+ * persistent batch and replayed; off, every kernel is launched one by one.
+ * Results are identical either way; kernel timing launches one by one.  No
+ * counterpart in the reference (its substep is a sequence of OpenMP loops,
+ * src/DFSPHSolver.cpp:33-87). */
+int sphg_ctx_graphs(sphg_ctx* ctx, int32_t enable);
 
 /* Scenes.  create -> set_boundary / set_fluid -> init -> step ...
  * sphg_scene_create replaces Simulation::setSimulationMethod and the solver
@@ -260,6 +268,10 @@ int sphg_scene_plan_bytes(sphg_scene* scene, uint64_t* bytes);
  * `name`, the launch count and the summed milliseconds since enabling. */
 int sphg_ctx_kernel_timing(sphg_ctx* ctx, int32_t enable);
 int sphg_ctx_kernel_time(sphg_ctx* ctx, const char* name, uint64_t* launches, double* total_ms);
+/* As sphg_ctx_kernel_timing(ctx, 1), recording only the `count` kernels
+ * named: each event pair costs the stream a few microseconds, so a
+ * throughput measurement times only the kernel it needs. */
+int sphg_ctx_kernel_timing_only(sphg_ctx* ctx, const char* const* names, uint32_t count);
 
 /* Scene geometry (set-up path, FP64 then rounded to float):
  * AkinciBoundaryModel::sampleCube (src/AkinciBoundayModel.cpp:100-134);

@@ -54,6 +54,7 @@ const char* kKindNames[K_NKINDS] = {
 
 struct Profiler {
     bool on = false;
+    uint64_t mask = ~0ull;      // kinds recorded while on
     struct Rec {
         int kind;
         hipEvent_t a, b;
@@ -101,14 +102,15 @@ struct Scope {
     hipStream_t st;
     int kind;
     hipEvent_t a{}, b{};
-    Scope(Profiler* p_, hipStream_t s, int k) : p(p_), st(s), kind(k) {
-        if (p->on) {
+    bool rec;
+    Scope(Profiler* p_, hipStream_t s, int k) : p(p_), st(s), kind(k), rec(p_->on && ((p_->mask >> k) & 1)) {
+        if (rec) {
             a = p->get();
             (void)hipEventRecord(a, st);
         }
     }
     ~Scope() {
-        if (p->on) {
+        if (rec) {
             b = p->get();
             (void)hipEventRecord(b, st);
             p->pending.push_back({kind, a, b});
@@ -123,6 +125,7 @@ struct sphg_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     hipStream_t copyStream = nullptr;   // snapshot read-backs (default priority)
+    bool graphs = false;                // replay captured step segments (persistent batches)
     int* dActive = nullptr;   // [0] scenes still iterating, [1] largest iteration count reached
     int* hActive = nullptr;   // pinned copy
     Profiler prof;
@@ -152,7 +155,16 @@ struct Batch {
     int lastIt = 0, lastItV = 0; // iterations the loops needed in the previous step
     std::vector<int> parity;   // parity of each scene when the tables were built
     std::vector<uint32_t> nAt; // active count of each scene in the tables
+    // Fixed launch sequences of the step, captured once per table parity
+    // (which of the two tables is dT[0]) and replayed; only for batches that
+    // live across calls
+    bool graphs = false;
+    DevScene* tab0 = nullptr;
+    hipGraphExec_t seg[3][2] = {};
     ~Batch() {
+        for (auto& a : seg)
+            for (auto& e : a)
+                if (e) (void)hipGraphExecDestroy(e);
         if (dT[0]) (void)hipFree(dT[0]);
         if (dT[1]) (void)hipFree(dT[1]);
         if (dP) (void)hipFree(dP);
@@ -369,6 +381,7 @@ int upload_batch_tables(sphg_ctx* ctx, Batch& B) {
     if (!B.dT[0]) {
         HIP_TRY(hipMalloc(&B.dT[0], sizeof(DevScene) * ns));
         HIP_TRY(hipMalloc(&B.dT[1], sizeof(DevScene) * ns));
+        B.tab0 = B.dT[0];
     }
     if (!B.dP) {
         HIP_TRY(hipMalloc(&B.dP, sizeof(Chunk) * std::max<size_t>(P.size(), 1)));
@@ -378,6 +391,12 @@ int upload_batch_tables(sphg_ctx* ctx, Batch& B) {
     HIP_TRY(hipMemcpyAsync(B.dT[1], t1.data(), sizeof(DevScene) * ns, hipMemcpyHostToDevice, ctx->stream));
     HIP_TRY(hipMemcpyAsync(B.dP, P.data(), sizeof(Chunk) * P.size(), hipMemcpyHostToDevice, ctx->stream));
     HIP_TRY(hipMemcpyAsync(B.dC, Cc.data(), sizeof(Chunk) * Cc.size(), hipMemcpyHostToDevice, ctx->stream));
+    if (B.nP != (int)P.size() || B.nC != (int)Cc.size()) {
+        // the captured segments hold the launch sizes
+        for (auto& a : B.seg)
+            for (auto& e : a)
+                if (e) { (void)hipGraphExecDestroy(e); e = nullptr; }
+    }
     B.nP = (int)P.size();
     B.nC = (int)Cc.size();
     HIP_TRY(hipStreamSynchronize(ctx->stream));
@@ -454,6 +473,34 @@ int density_ahead(const Batch& B) {
     return std::max(B.minIt, B.lastIt > 0 ? B.lastIt + kDensitySlack : 0);
 }
 
+// A fixed launch sequence of the step (kernels only, no copies or
+// memsets): launched as issued while kernel timing is on (its events need
+// the single launches) or for a batch that lives for one call; otherwise
+// captured once per table parity into a graph and replayed.
+template <class F>
+int run_segment(sphg_ctx* ctx, Batch& B, int which, F&& issue) {
+    if (!B.graphs || !ctx->graphs || ctx->prof.on) {
+        issue();
+        return SPHG_OK;
+    }
+    const int k = B.dT[0] == B.tab0 ? 0 : 1;
+    hipGraphExec_t& e = B.seg[which][k];
+    if (!e) {
+        hipGraph_t g = nullptr;
+        HIP_TRY(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+        issue();
+        HIP_TRY(hipStreamEndCapture(ctx->stream, &g));
+        const hipError_t ie = hipGraphInstantiate(&e, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (ie != hipSuccess) {
+            e = nullptr;
+            return fail(SPHG_EHIP, std::string("graph instantiation: ") + hipGetErrorString(ie));
+        }
+    }
+    HIP_TRY(hipGraphLaunch(e, ctx->stream));
+    return SPHG_OK;
+}
+
 // One step of every scene in B (all of one method).
 int batch_one_step(sphg_ctx* ctx, Batch& B, uint32_t& syncs) {
     Profiler* P = &ctx->prof;
@@ -477,18 +524,21 @@ int batch_one_step(sphg_ctx* ctx, Batch& B, uint32_t& syncs) {
     bool more = false;
     if (method == DFSPH) {
         const DevScene* T = B.dT[0];
-        if (anyFlags) {
-            if (anySt) { Scope s(P, st, K_NORMALS); launch_normals(st, T, B.dP, B.nP); }
-            Scope s(P, st, K_FORCES);
-            launch_forces(st, DFSPH, T, B.dP, B.nP);
-        }
-        { Scope s(P, st, K_DT); launch_dt(st, T, B.dP, B.nP); }
-        finalize(ctx, B, T, 0);
-        { Scope s(P, st, K_DFPREDICT); launch_df_predict(st, T, B.dP, B.nP); }
-        { Scope s(P, st, K_DFVEL_WARM); launch_df_vel(st, 0, T, B.dP, B.nP); }
-        launch_fill_int(st, ctx->dActive, 0, 2);
-        { Scope s(P, st, K_DFERR_DENS); launch_df_err(st, 0, T, B.dP, B.nP); }
-        finalize(ctx, B, T, 1);
+        rc = run_segment(ctx, B, 0, [&] {
+            if (anyFlags) {
+                if (anySt) { Scope s(P, st, K_NORMALS); launch_normals(st, T, B.dP, B.nP); }
+                Scope s(P, st, K_FORCES);
+                launch_forces(st, DFSPH, T, B.dP, B.nP);
+            }
+            { Scope s(P, st, K_DT); launch_dt(st, T, B.dP, B.nP); }
+            finalize(ctx, B, T, 0);
+            { Scope s(P, st, K_DFPREDICT); launch_df_predict(st, T, B.dP, B.nP); }
+            { Scope s(P, st, K_DFVEL_WARM); launch_df_vel(st, 0, T, B.dP, B.nP); }
+            launch_fill_int(st, ctx->dActive, 0, 2);
+            { Scope s(P, st, K_DFERR_DENS); launch_df_err(st, 0, T, B.dP, B.nP); }
+            finalize(ctx, B, T, 1);
+        });
+        if (rc) return rc;
         int seen = 0;
         for (int it = 0; it < B.maxIt; ++it) {
             if ((rc = loop_more(ctx, it, density_ahead(B), more, seen, syncs))) return rc;
@@ -498,20 +548,24 @@ int batch_one_step(sphg_ctx* ctx, Batch& B, uint32_t& syncs) {
             finalize(ctx, B, T, 2);
         }
         B.lastIt = seen;
-        { Scope s(P, st, K_DFPOS); launch_df_positions(st, T, B.dP, B.nP); }
-        // PCISPH boundary method: projection after the position update
-        // (src/DFSPHSolver.cpp:65-75), which then counts the cells
-        if (anyPb) { Scope s(P, st, K_PB); launch_pb_correct(st, 0, T, B.dP, B.nP); }
-        // emitters between the position update and the grid rebuild
-        // (src/DFSPHSolver.cpp:77); the new particles' cells are counted
-        if (anyEmit) { Scope s(P, st, K_EMIT); launch_emit(st, B.dT[0], B.dT[1], ns, 1); }
-        grid_sort(ctx, B, T);
         const DevScene* T2 = B.dT[1];
-        launch_fill_int(st, ctx->dActive, 0, 2);
-        // the build also forms the divergence solve's initial error, kappa_v
-        // and the error partials (computeDivergenceError before the loop)
-        { Scope s(P, st, K_BUILD); launch_build(st, DFSPH, T2, B.dP, B.nP); }
-        finalize(ctx, B, T2, 3);
+        rc = run_segment(ctx, B, 1, [&] {
+            { Scope s(P, st, K_DFPOS); launch_df_positions(st, T, B.dP, B.nP); }
+            // PCISPH boundary method: projection after the position update
+            // (src/DFSPHSolver.cpp:65-75), which then counts the cells
+            if (anyPb) { Scope s(P, st, K_PB); launch_pb_correct(st, 0, T, B.dP, B.nP); }
+            // emitters between the position update and the grid rebuild
+            // (src/DFSPHSolver.cpp:77); the new particles' cells are counted
+            if (anyEmit) { Scope s(P, st, K_EMIT); launch_emit(st, B.dT[0], B.dT[1], ns, 1); }
+            grid_sort(ctx, B, T);
+            launch_fill_int(st, ctx->dActive, 0, 2);
+            // the build also forms the divergence solve's initial error,
+            // kappa_v and the error partials (computeDivergenceError before
+            // the loop)
+            { Scope s(P, st, K_BUILD); launch_build(st, DFSPH, T2, B.dP, B.nP); }
+            finalize(ctx, B, T2, 3);
+        });
+        if (rc) return rc;
         seen = 0;
         for (int it = 0; it < B.maxIt; ++it) {
             if ((rc = loop_more(ctx, it, std::max(B.minItV, B.lastItV), more, seen, syncs))) return rc;
@@ -526,35 +580,41 @@ int batch_one_step(sphg_ctx* ctx, Batch& B, uint32_t& syncs) {
         flip(B);
     } else if (method == WCSPH) {
         const DevScene* T = B.dT[0];
-        launch_fill_int(st, ctx->dActive, 0, 2);
-        { Scope s(P, st, K_STEPBEGIN); launch_step_begin(st, T, B.dP, B.nP, ctx->dActive); }
-        { Scope s(P, st, K_KEYS); launch_keys(st, T, B.dP, B.nP); }
-        grid_sort(ctx, B, T);
         const DevScene* T2 = B.dT[1];
-        { Scope s(P, st, K_BUILD); launch_build(st, WCSPH, T2, B.dP, B.nP); }
-        // PCISPH boundary method: boundary densities and pressures
-        // (src/SPHSolver.cpp:123-135, src/WCSPHSolver.cpp:116-128)
-        if (anyPb) { Scope s(P, st, K_PB); pbDensity(T2, 0, 0); }
-        if (anySt) { Scope s(P, st, K_NORMALS); launch_normals(st, T2, B.dP, B.nP); }
-        { Scope s(P, st, K_FORCES); launch_forces(st, WCSPH, T2, B.dP, B.nP); }
-        // cube boundary after the integration (src/WCSPHSolver.cpp:51-60),
-        // or the PCISPH boundary projection (:62-73), then the emitters
-        // (:75), then time += dt (:77)
-        if (anyCube) { Scope s(P, st, K_CUBE); launch_cube(st, T2, B.dP, B.nP); }
-        if (anyPb) { Scope s(P, st, K_PB); launch_pb_correct(st, 1, T2, B.dP, B.nP); }
-        if (anyEmit) { Scope s(P, st, K_EMIT); launch_emit(st, B.dT[1], B.dT[0], ns, 0); }
-        { Scope s(P, st, K_TIME); launch_time_advance(st, T2, ns); }
+        rc = run_segment(ctx, B, 0, [&] {
+            launch_fill_int(st, ctx->dActive, 0, 2);
+            { Scope s(P, st, K_STEPBEGIN); launch_step_begin(st, T, B.dP, B.nP, ctx->dActive); }
+            { Scope s(P, st, K_KEYS); launch_keys(st, T, B.dP, B.nP); }
+            grid_sort(ctx, B, T);
+            { Scope s(P, st, K_BUILD); launch_build(st, WCSPH, T2, B.dP, B.nP); }
+            // PCISPH boundary method: boundary densities and pressures
+            // (src/SPHSolver.cpp:123-135, src/WCSPHSolver.cpp:116-128)
+            if (anyPb) { Scope s(P, st, K_PB); pbDensity(T2, 0, 0); }
+            if (anySt) { Scope s(P, st, K_NORMALS); launch_normals(st, T2, B.dP, B.nP); }
+            { Scope s(P, st, K_FORCES); launch_forces(st, WCSPH, T2, B.dP, B.nP); }
+            // cube boundary after the integration (src/WCSPHSolver.cpp:51-60),
+            // or the PCISPH boundary projection (:62-73), then the emitters
+            // (:75), then time += dt (:77)
+            if (anyCube) { Scope s(P, st, K_CUBE); launch_cube(st, T2, B.dP, B.nP); }
+            if (anyPb) { Scope s(P, st, K_PB); launch_pb_correct(st, 1, T2, B.dP, B.nP); }
+            if (anyEmit) { Scope s(P, st, K_EMIT); launch_emit(st, B.dT[1], B.dT[0], ns, 0); }
+            { Scope s(P, st, K_TIME); launch_time_advance(st, T2, ns); }
+        });
+        if (rc) return rc;
         flip(B);
     } else {
         const DevScene* T = B.dT[0];
-        launch_fill_int(st, ctx->dActive, 0, 2);
-        { Scope s(P, st, K_STEPBEGIN); launch_step_begin(st, T, B.dP, B.nP, ctx->dActive); }
-        { Scope s(P, st, K_KEYS); launch_keys(st, T, B.dP, B.nP); }
-        grid_sort(ctx, B, T);
         const DevScene* T2 = B.dT[1];
-        { Scope s(P, st, K_BUILD); launch_build(st, PCISPH, T2, B.dP, B.nP); }
-        if (anySt) { Scope s(P, st, K_NORMALS); launch_normals(st, T2, B.dP, B.nP); }
-        { Scope s(P, st, K_FORCES); launch_forces(st, PCISPH, T2, B.dP, B.nP); }
+        rc = run_segment(ctx, B, 0, [&] {
+            launch_fill_int(st, ctx->dActive, 0, 2);
+            { Scope s(P, st, K_STEPBEGIN); launch_step_begin(st, T, B.dP, B.nP, ctx->dActive); }
+            { Scope s(P, st, K_KEYS); launch_keys(st, T, B.dP, B.nP); }
+            grid_sort(ctx, B, T);
+            { Scope s(P, st, K_BUILD); launch_build(st, PCISPH, T2, B.dP, B.nP); }
+            if (anySt) { Scope s(P, st, K_NORMALS); launch_normals(st, T2, B.dP, B.nP); }
+            { Scope s(P, st, K_FORCES); launch_forces(st, PCISPH, T2, B.dP, B.nP); }
+        });
+        if (rc) return rc;
         int seen = 0;
         for (int it = 0; it < B.maxIt; ++it) {
             if ((rc = loop_more(ctx, it, density_ahead(B), more, seen, syncs))) return rc;
@@ -569,14 +629,17 @@ int batch_one_step(sphg_ctx* ctx, Batch& B, uint32_t& syncs) {
             { Scope s(P, st, K_PCIPACC); launch_pci_pacc(st, T2, B.dP, B.nP, it); }
         }
         B.lastIt = seen;
-        { Scope s(P, st, K_PCIINT); launch_pci_integrate(st, T2, B.dP, B.nP); }
-        // cube boundary after the integration (src/PCISPHSolver.cpp:54-63),
-        // or the PCISPH boundary projection (:65-76), then the emitters
-        // (:78), then time += dt (:80)
-        if (anyCube) { Scope s(P, st, K_CUBE); launch_cube(st, T2, B.dP, B.nP); }
-        if (anyPb) { Scope s(P, st, K_PB); launch_pb_correct(st, 2, T2, B.dP, B.nP); }
-        if (anyEmit) { Scope s(P, st, K_EMIT); launch_emit(st, B.dT[1], B.dT[0], ns, 0); }
-        { Scope s(P, st, K_TIME); launch_time_advance(st, T2, ns); }
+        rc = run_segment(ctx, B, 1, [&] {
+            { Scope s(P, st, K_PCIINT); launch_pci_integrate(st, T2, B.dP, B.nP); }
+            // cube boundary after the integration (src/PCISPHSolver.cpp:54-63),
+            // or the PCISPH boundary projection (:65-76), then the emitters
+            // (:78), then time += dt (:80)
+            if (anyCube) { Scope s(P, st, K_CUBE); launch_cube(st, T2, B.dP, B.nP); }
+            if (anyPb) { Scope s(P, st, K_PB); launch_pb_correct(st, 2, T2, B.dP, B.nP); }
+            if (anyEmit) { Scope s(P, st, K_EMIT); launch_emit(st, B.dT[1], B.dT[0], ns, 0); }
+            { Scope s(P, st, K_TIME); launch_time_advance(st, T2, ns); }
+        });
+        if (rc) return rc;
         flip(B);
     }
     return SPHG_OK;
@@ -877,6 +940,12 @@ int sphg_ctx_destroy(sphg_ctx* ctx) {
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     if (ctx->copyStream) (void)hipStreamDestroy(ctx->copyStream);
     delete ctx;
+    return SPHG_OK;
+}
+
+int sphg_ctx_graphs(sphg_ctx* ctx, int32_t enable) {
+    if (!ctx) return fail(SPHG_EINVAL, "ctx is NULL");
+    ctx->graphs = enable != 0;
     return SPHG_OK;
 }
 
@@ -1222,6 +1291,7 @@ int sphg_scene_init(sphg_scene* s) {
 
     // tables of a one-scene batch
     s->self = new Batch();
+    s->self->graphs = true;
     s->self->scenes = {s};
     int rc = upload_batch_tables(ctx, *s->self);
     if (rc) return rc;
@@ -1336,6 +1406,7 @@ int sphg_batch_create(sphg_ctx* ctx, sphg_scene* const* scenes, uint32_t nscenes
     sphg_batch* b = new sphg_batch();
     b->ctx = ctx;
     b->B.scenes.assign(scenes, scenes + nscenes);
+    b->B.graphs = true;
     rc = upload_batch_tables(ctx, b->B);
     if (rc) {
         delete b;
@@ -1631,12 +1702,29 @@ int sphg_ctx_kernel_timing(sphg_ctx* ctx, int32_t enable) {
     HIP_TRY(hipSetDevice(ctx->device));
     ctx->prof.collect();
     ctx->prof.on = enable != 0;
+    ctx->prof.mask = ~0ull;
     if (enable) {
         for (int k = 0; k < K_NKINDS; ++k) {
             ctx->prof.count[k] = 0;
             ctx->prof.ms[k] = 0.0;
         }
     }
+    return SPHG_OK;
+}
+
+int sphg_ctx_kernel_timing_only(sphg_ctx* ctx, const char* const* names, uint32_t count) {
+    if (!ctx || (count && !names)) return fail(SPHG_EINVAL, "NULL argument");
+    uint64_t mask = 0;
+    for (uint32_t i = 0; i < count; ++i) {
+        int k = 0;
+        while (k < K_NKINDS && (!names[i] || std::strcmp(names[i], kKindNames[k]) != 0)) ++k;
+        if (k == K_NKINDS)
+            return fail(SPHG_EINVAL, std::string("unknown kernel name ") + (names[i] ? names[i] : "(null)"));
+        mask |= 1ull << k;
+    }
+    const int rc = sphg_ctx_kernel_timing(ctx, 1);
+    if (rc) return rc;
+    ctx->prof.mask = mask;
     return SPHG_OK;
 }
 

@@ -226,6 +226,7 @@ def load(path: str = LIB_PATH) -> C.CDLL:
         "sphg_ctx_create": (C.c_int, [I32, C.POINTER(P)]),
         "sphg_ctx_destroy": (C.c_int, [P]),
         "sphg_ctx_synchronize": (C.c_int, [P]),
+        "sphg_ctx_graphs": (C.c_int, [P, C.c_int32]),
         "sphg_scene_create": (C.c_int, [P, C.POINTER(SceneDesc), C.POINTER(P)]),
         "sphg_scene_destroy": (C.c_int, [P]),
         "sphg_scene_set_fluid": (C.c_int, [P, fp, fp, U32]),
@@ -255,6 +256,7 @@ def load(path: str = LIB_PATH) -> C.CDLL:
         "sphg_scene_plan_bytes": (C.c_int, [P, C.POINTER(U64)]),
         "sphg_ctx_kernel_timing": (C.c_int, [P, I32]),
         "sphg_ctx_kernel_time": (C.c_int, [P, C.c_char_p, C.POINTER(U64), dp]),
+        "sphg_ctx_kernel_timing_only": (C.c_int, [P, C.POINTER(C.c_char_p), U32]),
         "sphg_sample_box": (I64, [dp, dp, D, fp, I64]),
         "sphg_sample_sphere": (I64, [dp, D, D, fp, I64]),
         "sphg_fluid_block": (I64, [dp, dp, D, D, U64, fp, I64]),
@@ -395,6 +397,11 @@ class Context:
     def synchronize(self) -> None:
         _check(self.lib.sphg_ctx_synchronize(self.h))
 
+    def set_graphs(self, enable: bool) -> None:
+        """Replay captured step segments, or launch every kernel one by one
+        (default); results are identical either way."""
+        _check(self.lib.sphg_ctx_graphs(self.h, 1 if enable else 0))
+
     def scene(self, params: SceneParams, fluid, boundary, velocities=None) -> "Scene":
         return Scene(self, params, fluid, boundary, velocities)
 
@@ -404,8 +411,14 @@ class Context:
         _check(self.lib.sphg_batch_step(self.h, arr, len(scenes), int(nsteps), C.byref(st)))
         return st
 
-    def kernel_timing(self, enable: bool) -> None:
-        _check(self.lib.sphg_ctx_kernel_timing(self.h, 1 if enable else 0))
+    def kernel_timing(self, enable: bool, only: Optional[Sequence[str]] = None) -> None:
+        """Start (clearing the totals) or stop per-kernel event timing; with
+        `only`, record just the kernels named."""
+        if enable and only is not None:
+            names = (C.c_char_p * max(len(only), 1))(*[n.encode() for n in only])
+            _check(self.lib.sphg_ctx_kernel_timing_only(self.h, names, len(only)))
+        else:
+            _check(self.lib.sphg_ctx_kernel_timing(self.h, 1 if enable else 0))
 
     def kernel_time(self, name: str):
         n = C.c_uint64()

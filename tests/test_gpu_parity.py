@@ -483,35 +483,40 @@ def test_pcisph_boundary_method(ctx, method):
     g.close()
 
 
+def _feature_scenes(ctx, method):
+    """A plain Akinci scene, a scene with an emitter, a PCISPH-boundary scene
+    and a cube-clamp scene."""
+    from dataclasses import replace
+    c = np.sqrt(0.5)
+    em = sphg.Emitter(type=sphg.EMITTER_SQUARE, num_particles=27, position=(0.9, 0.26, 0.4), velocity=5.0,
+                      rotation=(c, 0.0, -c, 0.0), start_time=0.0, width=0.16, height=0.16)
+    out = []
+    a = configs.small_scene(method, n_side=6, seed=3)
+    out.append(sphg.Scene(ctx, a.params, a.fluid, a.boundary))
+    b = configs.small_scene(method, n_side=5, seed=4)
+    out.append(sphg.Scene(ctx, b.params, b.fluid, b.boundary, emitters=[em]))
+    p = configs.small_scene(method, n_side=6, gap=1.8, seed=5)
+    kw = dict(eta=1.0, min_iterations=2) if method == sphg.PCISPH else {}
+    pp = replace(p.params, boundary_method=sphg.BOUNDARY_PCISPH, pb_normal_fct=0.5, pb_tangential_fct=0.9, **kw)
+    pts, nrm = sphg.sample_box_normals((0.0, 0.0, 0.0), tuple(float(x) for x in p.boundary.max(0)),
+                                       p.params.support_radius, True)
+    out.append(sphg.Scene(ctx, pp, p.fluid, pts, boundary_normals=nrm))
+    q = configs.small_scene(method, n_side=5, seed=6)
+    f = q.fluid.astype(np.float64)
+    qp = replace(q.params, boundary_method=sphg.BOUNDARY_CUBE, cube_enabled=1,
+                 cube_min=tuple(f.min(0) - (0.0, 0.0, -0.01 * q.params.particle_radius)),
+                 cube_max=tuple(f.max(0) + 0.5))
+    out.append(sphg.Scene(ctx, qp, q.fluid, np.zeros((0, 3), np.float32)))
+    return out
+
+
 @pytest.mark.parametrize("method", [sphg.DFSPH, sphg.WCSPH, sphg.PCISPH])
 def test_batch_of_mixed_features_alone_vs_mix(ctx, method):
     """One batch holding a plain Akinci scene, a scene with an emitter, a
     PCISPH-boundary scene and a cube-clamp scene: each ends, bit for bit, where
     it ends stepped alone (persistent batch, run in quanta)."""
-    from dataclasses import replace
-    c = np.sqrt(0.5)
-    em = sphg.Emitter(type=sphg.EMITTER_SQUARE, num_particles=27, position=(0.9, 0.26, 0.4), velocity=5.0,
-                      rotation=(c, 0.0, -c, 0.0), start_time=0.0, width=0.16, height=0.16)
-
     def build():
-        out = []
-        a = configs.small_scene(method, n_side=6, seed=3)
-        out.append(sphg.Scene(ctx, a.params, a.fluid, a.boundary))
-        b = configs.small_scene(method, n_side=5, seed=4)
-        out.append(sphg.Scene(ctx, b.params, b.fluid, b.boundary, emitters=[em]))
-        p = configs.small_scene(method, n_side=6, gap=1.8, seed=5)
-        kw = dict(eta=1.0, min_iterations=2) if method == sphg.PCISPH else {}
-        pp = replace(p.params, boundary_method=sphg.BOUNDARY_PCISPH, pb_normal_fct=0.5, pb_tangential_fct=0.9, **kw)
-        pts, nrm = sphg.sample_box_normals((0.0, 0.0, 0.0), tuple(float(x) for x in p.boundary.max(0)),
-                                           p.params.support_radius, True)
-        out.append(sphg.Scene(ctx, pp, p.fluid, pts, boundary_normals=nrm))
-        q = configs.small_scene(method, n_side=5, seed=6)
-        f = q.fluid.astype(np.float64)
-        qp = replace(q.params, boundary_method=sphg.BOUNDARY_CUBE, cube_enabled=1,
-                     cube_min=tuple(f.min(0) - (0.0, 0.0, -0.01 * q.params.particle_radius)),
-                     cube_max=tuple(f.max(0) + 0.5))
-        out.append(sphg.Scene(ctx, qp, q.fluid, np.zeros((0, 3), np.float32)))
-        return out
+        return _feature_scenes(ctx, method)
 
     alone = build()
     for s in alone:
@@ -529,6 +534,73 @@ def test_batch_of_mixed_features_alone_vs_mix(ctx, method):
     assert mixed[1].n > configs.small_scene(method, n_side=5).n     # the emitter fired
     for s in alone + mixed:
         s.close()
+
+
+@pytest.mark.parametrize("method", [sphg.DFSPH, sphg.WCSPH, sphg.PCISPH])
+def test_graph_replay_matches_eager(ctx, method):
+    """The step's captured launch segments, replayed, give what the same
+    kernels launched one by one give, bit for bit: replays of both table
+    parities, uneven quanta with host read-backs between them, a scene moved
+    between its own tables and a persistent batch and back, emitters raising
+    the particle count between replays."""
+    quanta = (1, 3, 7, 2, 10, 1, 6)
+    try:
+        ctx.set_graphs(False)
+        eager = _feature_scenes(ctx, method)
+        for q in quanta:
+            for s in eager:
+                s.step(q)
+        ref = [(s.n, s.get(sphg.FIELD_X), s.get(sphg.FIELD_V), s.scalars().sum_iterations) for s in eager]
+        ctx.set_graphs(True)
+        replay = _feature_scenes(ctx, method)
+        b = sphg.Batch(ctx, replay)
+        for k, q in enumerate(quanta):
+            if k % 3 == 1:
+                b.run(q)
+            else:
+                for s in replay:
+                    s.step(q)
+            for s in replay:
+                s.get(sphg.FIELD_X)
+        b.close()
+        for k, (r, s) in enumerate(zip(ref, replay)):
+            assert s.n == r[0], k
+            assert np.array_equal(s.get(sphg.FIELD_X), r[1]), k
+            assert np.array_equal(s.get(sphg.FIELD_V), r[2]), k
+            assert s.scalars().sum_iterations == r[3], k
+        assert replay[1].n > configs.small_scene(method, n_side=5).n     # the emitter fired
+    finally:
+        ctx.set_graphs(False)
+    for s in eager + replay:
+        s.close()
+
+
+def test_kernel_timing_counts_and_selection(ctx):
+    """Per-kernel event timing counts every launch of each kind (one build,
+    one dt per DFSPH step; at least one density error sweep more than
+    velocity sweeps), records only the kinds named when asked to, and
+    changes no result."""
+    spec = configs.small_scene(sphg.DFSPH, n_side=10, seed=7)
+    a = sphg.Scene(ctx, spec.params, spec.fluid, spec.boundary)
+    b = sphg.Scene(ctx, spec.params, spec.fluid, spec.boundary)
+    try:
+        ctx.kernel_timing(True)
+        a.step(6)
+        ctx.kernel_timing(False)
+        assert ctx.kernel_time("build")[0] == 6 and ctx.kernel_time("dt")[0] == 6
+        ne, nv = ctx.kernel_time("df_err_density")[0], ctx.kernel_time("df_vel_density")[0]
+        assert ne >= nv + 6 and ctx.kernel_time("build")[1] > 0.0
+        ctx.kernel_timing(True, only=["df_err_density"])
+        b.step(6)
+        ctx.kernel_timing(False)
+        assert ctx.kernel_time("build")[0] == 0 and ctx.kernel_time("df_err_density")[0] >= 6
+        with pytest.raises(sphg.SphgError):
+            ctx.kernel_timing(True, only=["no_such_kernel"])
+        assert np.array_equal(a.get(sphg.FIELD_X), b.get(sphg.FIELD_X))
+    finally:
+        ctx.kernel_timing(False)
+        a.close()
+        b.close()
 
 
 @pytest.mark.parametrize("method", [sphg.DFSPH, sphg.WCSPH, sphg.PCISPH])
