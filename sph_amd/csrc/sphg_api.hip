@@ -1600,8 +1600,10 @@ int sphg_scene_get_neighbors(sphg_scene* s, uint32_t* counts, uint32_t* ids, uin
     for (size_t i = 0; i < n; ++i) {
         const uint32_t me = sid[i];
         counts[me] = c16[i];
-        const size_t base = (i >> 6) * (size_t)V.maxNbr * kGroup + (i & 63);
-        for (uint32_t q = 0; q < c16[i]; ++q) ids[(size_t)me * cap + q] = sid[lst[base + (size_t)q * kGroup]];
+        // blocks of 4 slots, lane-major within a block (see slot_of)
+        const size_t base = (i >> 6) * (size_t)V.maxNbr * kGroup + (i & 63) * 4;
+        for (uint32_t q = 0; q < c16[i]; ++q)
+            ids[(size_t)me * cap + q] = sid[lst[base + (size_t)(q >> 2) * kGroup * 4 + (q & 3)]];
     }
     return SPHG_OK;
 }

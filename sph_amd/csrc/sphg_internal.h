@@ -17,7 +17,9 @@
 //   k, kv         float (DFSPH warm-start accumulators, double-buffered)
 //   id            uint32 original particle id (double-buffered)
 //   cell          int    linear cell of each sorted particle
-//   nbr           uint32 neighbor list, neighbor-major inside 64-particle groups
+//   nbr           uint32 neighbor list inside 64-particle groups, in blocks of
+//                 4 slots, lane-major within a block (a lane's 4 next slots
+//                 are one 16-byte load)
 //   nnb, ngrad    uint16/uint8 neighbor count and count of nonzero-gradient
 //                 neighbors (the DFSPH 20-neighbor cutoff)
 //   gcnt          uint16 per 64-particle group: the group's longest list,

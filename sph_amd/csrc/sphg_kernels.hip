@@ -233,9 +233,19 @@ __device__ inline bool grad_nonzero64(const Exact& e) {
     return __dmul_rn(f, f) != 0.0;
 }
 
-__device__ inline const uint32_t* list_of(const DevScene& S, int i) {
-    return S.nbr + ((size_t)(i >> 6) * (size_t)S.maxNbr) * kGroup + (i & 63);
+// List layout inside a 64-particle group: blocks of 4 slots, lane-major
+// within a block, so slot s of lane l is element (s/4 * 64 + l) * 4 + s%4
+// of the group's maxNbr * 64 entries, and a lane reads its next 4 slots
+// with one 16-byte load (a wavefront reads 1 KB contiguous).
+__device__ inline uint32_t slot_of(int s) { return (uint32_t)(((s >> 2) * kGroup) << 2) + (uint32_t)(s & 3); }
+// Element offset of lane (i & 63)'s first slot (maxNbr is a multiple of 4).
+__device__ inline uint32_t lane_base(const DevScene& S, int i) {
+    return (uint32_t)(i >> 6) * (uint32_t)S.maxNbr * (uint32_t)kGroup + ((uint32_t)(i & 63) << 2);
 }
+__device__ inline const uint32_t* list_of(const DevScene& S, int i) { return S.nbr + lane_base(S, i); }
+typedef uint32_t v4u __attribute__((ext_vector_type(4)));
+// slots 4b .. 4b+3 of a lane
+__device__ inline v4u ld_block(const uint32_t* L, int b) { return ((gptr<const v4u>)L)[(size_t)b * kGroup]; }
 // The group's padded list length, uniform across the wavefront (a
 // wavefront is one 64-particle group: chunks start at multiples of 256).
 __device__ inline int group_count(const DevScene& S, int i) {
@@ -254,21 +264,16 @@ __device__ inline void sweep4(const DevScene& S, int i, Load&& load, Body&& body
     // the indices of the next 4 entries are requested before the neighbor
     // data of the current 4 is used, so the list stream (HBM) and the
     // neighbor loads (mostly L2) overlap instead of forming one chain
-    uint32_t jn[4];
-#pragma unroll
-    for (int u = 0; u < 4; ++u) jn[u] = ldg(L, (size_t)u * kGroup);
+    v4u jn = ld_block(L, 0);
     for (int q = 0; q < gc; q += 4) {
-        uint32_t jj[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) jj[u] = jn[u];
+        const uint32_t jj[4] = {jn.x, jn.y, jn.z, jn.w};
         decltype(load(0u)) dd[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) dd[u] = load(jj[u]);
         // unconditional (the last pass re-reads its own entries), so the
         // wait for this pass's data leaves the 4 index loads in flight
         const int qn = (q + 4 < gc) ? q + 4 : q;
-#pragma unroll
-        for (int u = 0; u < 4; ++u) jn[u] = ldg(L, (size_t)(qn + u) * kGroup);
+        jn = ld_block(L, qn >> 2);
         // keeps the compiler from sinking these loads into the next pass
         asm volatile("" ::: "memory");
 #pragma unroll
@@ -283,9 +288,7 @@ struct NbF {
     float4 x, v, n;
     float d;
 };
-__device__ inline uint32_t* list_of_w(const DevScene& S, int i) {
-    return S.nbr + ((size_t)(i >> 6) * (size_t)S.maxNbr) * kGroup + (i & 63);
-}
+__device__ inline uint32_t* list_of_w(const DevScene& S, int i) { return S.nbr + lane_base(S, i); }
 
 // Linear cell of a position: key floor(x/h) in FP64 (src/HashTable.h:81),
 // clamped into the padded grid.  Clamping keeps any two particles closer
@@ -648,7 +651,7 @@ __global__ __launch_bounds__(kBlock) void k_build(const DevScene* __restrict__ T
     // the same slot as uniform base + 32-bit element offset (lists of a
     // scene are far below 2^32 elements)
     uint32_t* const Lg = S.nbr;
-    const uint32_t lbase = (uint32_t)(i >> 6) * (uint32_t)maxN * (uint32_t)kGroup + (uint32_t)(i & 63);
+    const uint32_t lbase = lane_base(S, i);
     int cnt = 0, ngr = 0;
     double ws = 0.0, sgx = 0.0, sgy = 0.0, sgz = 0.0, ssq = 0.0;
     // DFSPH: sum_j g_ij (v_i - v_j).(x_i - x_j) of the divergence error
@@ -666,7 +669,7 @@ __global__ __launch_bounds__(kBlock) void k_build(const DevScene* __restrict__ T
             if (!(e.q <= 1.0)) return;
             gnz = grad_nonzero64(e);
         }
-        if (cnt < maxN) stg(L, (size_t)cnt * kGroup, j);
+        if (cnt < maxN) stg(L, (size_t)slot_of(cnt), j);
         ++cnt;
         ngr += gnz ? 1 : 0;
         ws += W_d(p.q, k);
@@ -698,7 +701,7 @@ __global__ __launch_bounds__(kBlock) void k_build(const DevScene* __restrict__ T
                 if (ok[u]) {
                     // past the last slot the particle takes the one-pass
                     // walk, which rewrites its list: the slot is clamped
-                    stg(Lg, (uint32_t)min(nc, maxN - 1) * (uint32_t)kGroup + lbase, (uint32_t)(j0 + u));
+                    stg(Lg, slot_of(min(nc, maxN - 1)) + lbase, (uint32_t)(j0 + u));
                     ++nc;
                 }
             }
@@ -727,10 +730,12 @@ __global__ __launch_bounds__(kBlock) void k_build(const DevScene* __restrict__ T
             // first; visit writes slots <= the one it reads, so entries
             // already loaded are not overwritten before use
             for (int q0 = 0; q0 < nc; q0 += 4) {
-                uint32_t jj[4];
+                // one block of 4 slots; those past the last survivor hold
+                // stale values and are replaced by the first (a valid index)
+                const v4u blk = ld_block(L, q0 >> 2);
+                const uint32_t jj[4] = {blk.x, q0 + 1 < nc ? blk.y : blk.x, q0 + 2 < nc ? blk.z : blk.x,
+                                        q0 + 3 < nc ? blk.w : blk.x};
                 float4 xj[4], vj[4];
-#pragma unroll
-                for (int u = 0; u < 4; ++u) jj[u] = ldg((const uint32_t*)L, (size_t)min(q0 + u, nc - 1) * kGroup);
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
                     xj[u] = ld3o(S.x, jj[u]);
@@ -792,7 +797,7 @@ __global__ __launch_bounds__(kBlock) void k_build(const DevScene* __restrict__ T
         const float4 y = ld4(S.xb, b);
         const float fx = xi.x - y.x, fy = xi.y - y.y, fz = xi.z - y.z;
         if (fx * fx + fy * fy + fz * fz <= hh) {
-            if (nbc < room) stg(Lg, (uint32_t)(own + nbc) * (uint32_t)kGroup + lbase, (uint32_t)b);
+            if (nbc < room) stg(Lg, slot_of(own + nbc) + lbase, (uint32_t)b);
             ++nbc;
         }
     });
@@ -801,7 +806,7 @@ __global__ __launch_bounds__(kBlock) void k_build(const DevScene* __restrict__ T
             float4 yb[4];
 #pragma unroll
             for (int u = 0; u < 4; ++u)
-                yb[u] = ld4(S.xb, ldg((const uint32_t*)L, (size_t)(own + min(q0 + u, nbc - 1)) * kGroup));
+                yb[u] = ld4(S.xb, ldg((const uint32_t*)L, (size_t)slot_of(own + min(q0 + u, nbc - 1))));
 #pragma unroll
             for (int u = 0; u < 4; ++u)
                 if (q0 + u < nbc) bvisit(yb[u]);
@@ -815,7 +820,7 @@ __global__ __launch_bounds__(kBlock) void k_build(const DevScene* __restrict__ T
     }
     // pad the list with the particle itself up to the group's longest list
     // (overwrites the boundary survivors' slots)
-    for (int q = own; q < pad; ++q) stg(L, (size_t)q * kGroup, (uint32_t)i);
+    for (int q = own; q < pad; ++q) stg(L, (size_t)slot_of(q), (uint32_t)i);
     if ((i & 63) == 0) stg(S.gcnt, (size_t)(i >> 6), (uint16_t)pad);
     if (cnt > maxN) atomicMax(&S.ss->overflow, (unsigned)cnt);
     stg(S.nnb, (size_t)i, (uint16_t)own);
