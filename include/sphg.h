@@ -257,6 +257,31 @@ int sphg_batch_step(sphg_ctx* ctx, sphg_scene* const* scenes, uint32_t nscenes, 
 int sphg_batch_create(sphg_ctx* ctx, sphg_scene* const* scenes, uint32_t nscenes, sphg_batch** out);
 int sphg_batch_run(sphg_batch* batch, uint32_t nsteps, sphg_step_stats* stats);
 int sphg_batch_destroy(sphg_batch* batch);
+/* Launch-schedule autotuner of a scene's own stepping or of a persistent
+ * batch.  The schedule is how many pressure-loop iterations are launched
+ * past the previous step's count before the first convergence read-back
+ * (density loop 0-3, DFSPH divergence loop 0-1) and whether the step's
+ * fixed launch segments are replayed as HIP graphs.  Every schedule gives
+ * the same results, so the trials are ordinary steps: `rounds` rounds each
+ * run every candidate for `steps_per_trial` steps (the scenes advance by
+ * all of them); with two or more rounds the first is not counted.  The
+ * fastest schedule by wall time is kept for later steps.  Block size
+ * (256 particles = 4 wavefronts, one neighbor-list group per wavefront),
+ * cells per scan block (4096) and the reorder every step (the reference's
+ * bucket order needs it) are fixed by the data layout and not tuned.  No
+ * reference counterpart. */
+typedef struct sphg_tune_result {
+    int32_t density_slack;
+    int32_t divergence_slack;
+    int32_t replay;                 /* 1: captured segments replayed */
+    uint32_t candidates;            /* schedules tried */
+    uint32_t steps;                 /* steps run while tuning */
+    double ms_per_step;             /* the chosen schedule's mean wall time per step */
+    double default_ms_per_step;     /* the default schedule's (1, 0, no replay); -1 if not tried */
+} sphg_tune_result;
+int sphg_scene_autotune(sphg_scene* scene, uint32_t steps_per_trial, uint32_t rounds, sphg_tune_result* out);
+int sphg_batch_autotune(sphg_batch* batch, uint32_t steps_per_trial, uint32_t rounds, sphg_tune_result* out);
+
 /* Device bytes the scene holds once initialised, computed from its fluid and
  * boundary (after set_fluid / set_boundary) before anything is allocated:
  * the admission test of a memory-budgeted serving loop.  No reference

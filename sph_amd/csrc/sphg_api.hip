@@ -12,6 +12,7 @@
 #include "sphg_kernels.h"
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -160,6 +161,10 @@ struct Batch {
     // live across calls
     bool graphs = false;
     DevScene* tab0 = nullptr;
+    // launch schedule (any choice gives the same results): loop iterations
+    // launched past the previous step's count before the first read-back,
+    // and segment replay (-1: the context's setting)
+    int slackD = 1, slackV = 0, replay = -1;
     hipGraphExec_t seg[3][2] = {};
     ~Batch() {
         for (auto& a : seg)
@@ -468,9 +473,11 @@ int loop_more(sphg_ctx* ctx, int it, int ahead, bool& more, int& seen, uint32_t&
 // is three launches that return at once (~12 us), a read-back ~40-50 us of
 // idle device.  The divergence loop (one iteration nearly always) gets no
 // slack.
-constexpr int kDensitySlack = 1;
 int density_ahead(const Batch& B) {
-    return std::max(B.minIt, B.lastIt > 0 ? B.lastIt + kDensitySlack : 0);
+    return std::max(B.minIt, B.lastIt > 0 ? B.lastIt + B.slackD : 0);
+}
+int divergence_ahead(const Batch& B) {
+    return std::max(B.minItV, B.lastItV > 0 ? B.lastItV + B.slackV : 0);
 }
 
 // A fixed launch sequence of the step (kernels only, no copies or
@@ -479,7 +486,8 @@ int density_ahead(const Batch& B) {
 // captured once per table parity into a graph and replayed.
 template <class F>
 int run_segment(sphg_ctx* ctx, Batch& B, int which, F&& issue) {
-    if (!B.graphs || !ctx->graphs || ctx->prof.on) {
+    const bool replay = B.replay >= 0 ? B.replay != 0 : ctx->graphs;
+    if (!B.graphs || !replay || ctx->prof.on) {
         issue();
         return SPHG_OK;
     }
@@ -568,7 +576,7 @@ int batch_one_step(sphg_ctx* ctx, Batch& B, uint32_t& syncs) {
         if (rc) return rc;
         seen = 0;
         for (int it = 0; it < B.maxIt; ++it) {
-            if ((rc = loop_more(ctx, it, std::max(B.minItV, B.lastItV), more, seen, syncs))) return rc;
+            if ((rc = loop_more(ctx, it, divergence_ahead(B), more, seen, syncs))) return rc;
             if (!more) break;
             { Scope s(P, st, K_DFVEL_DIV); launch_df_vel(st, 2, T2, B.dP, B.nP); }
             { Scope s(P, st, K_DFERR_DIV); launch_df_err(st, 3, T2, B.dP, B.nP); }
@@ -697,6 +705,62 @@ int run_batch(sphg_ctx* ctx, Batch& B, uint32_t nsteps, sphg_step_stats* stats) 
     if (overflow)
         return fail(SPHG_EOVERFLOW, "a particle had " + std::to_string(overflow) +
                                         " neighbors, above max_neighbors; results of that step are invalid");
+    return SPHG_OK;
+}
+
+// Launch-schedule autotuner.  Every candidate schedule gives the same
+// results -- the slack only moves read-backs and launches that return at
+// once for converged scenes, and replay launches the same kernels -- so the
+// trials are ordinary steps of the simulation.  Each round runs every
+// candidate for `per` steps in turn (so all of them see about the same
+// physics), and the candidate with the least wall time summed over the
+// rounds is kept; with two or more rounds the first is a warm-up (graph
+// capture, caches) and is not counted.
+int tune_batch(sphg_ctx* ctx, Batch& B, uint32_t per, uint32_t rounds, sphg_tune_result* out) {
+    if (per == 0 || rounds == 0) return fail(SPHG_EINVAL, "steps_per_trial and rounds must be positive");
+    const int method = B.scenes[0]->desc.method;
+    struct Cand {
+        int d, v, r;
+        double ms;
+    };
+    const std::vector<int> ds = method == WCSPH ? std::vector<int>{B.slackD} : std::vector<int>{0, 1, 2, 3};
+    const std::vector<int> vs = method == DFSPH ? std::vector<int>{0, 1} : std::vector<int>{B.slackV};
+    const std::vector<int> rs = B.graphs ? std::vector<int>{0, 1} : std::vector<int>{0};
+    std::vector<Cand> cs;
+    for (int d : ds)
+        for (int v : vs)
+            for (int r : rs) cs.push_back(Cand{d, v, r, 0.0});
+    for (uint32_t round = 0; round < rounds; ++round) {
+        for (auto& c : cs) {
+            B.slackD = c.d;
+            B.slackV = c.v;
+            B.replay = c.r;
+            const auto t0 = std::chrono::steady_clock::now();
+            const int rc = run_batch(ctx, B, per, nullptr);
+            if (rc) return rc;
+            const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+            if (rounds < 2 || round > 0) c.ms += ms;
+        }
+    }
+    const double counted = (double)per * (rounds < 2 ? rounds : rounds - 1);
+    size_t best = 0;
+    double def = -1.0;
+    for (size_t k = 0; k < cs.size(); ++k) {
+        if (cs[k].ms < cs[best].ms) best = k;
+        if (cs[k].d == 1 && cs[k].v == 0 && cs[k].r == 0) def = cs[k].ms / counted;
+    }
+    B.slackD = cs[best].d;
+    B.slackV = cs[best].v;
+    B.replay = cs[best].r;
+    if (out) {
+        out->density_slack = B.slackD;
+        out->divergence_slack = B.slackV;
+        out->replay = B.replay;
+        out->candidates = (uint32_t)cs.size();
+        out->steps = per * rounds * (uint32_t)cs.size();
+        out->ms_per_step = cs[best].ms / counted;
+        out->default_ms_per_step = def;
+    }
     return SPHG_OK;
 }
 
@@ -1380,6 +1444,18 @@ int sphg_scene_step(sphg_scene* s, uint32_t nsteps, sphg_step_stats* stats) {
     return SPHG_OK;
 }
 
+int sphg_scene_autotune(sphg_scene* s, uint32_t steps_per_trial, uint32_t rounds, sphg_tune_result* out) {
+    if (!s || !s->ctx) return fail(SPHG_EINVAL, "NULL scene");
+    if (!s->ready) return fail(SPHG_ESTATE, "autotune before init");
+    HIP_TRY(hipSetDevice(s->ctx->device));
+    int rc = self_tables(s);
+    if (rc) return rc;
+    rc = tune_batch(s->ctx, *s->self, steps_per_trial, rounds, out);
+    if (rc) return rc;
+    HIP_TRY(hipGetLastError());
+    return SPHG_OK;
+}
+
 int sphg_batch_step(sphg_ctx* ctx, sphg_scene* const* scenes, uint32_t nscenes, uint32_t nsteps,
                     sphg_step_stats* stats) {
     if (!ctx || (!scenes && nscenes)) return fail(SPHG_EINVAL, "NULL argument");
@@ -1416,21 +1492,34 @@ int sphg_batch_create(sphg_ctx* ctx, sphg_scene* const* scenes, uint32_t nscenes
     return SPHG_OK;
 }
 
+// A scene stepped elsewhere since the batch's tables were built has moved
+// its double buffers: rebuild the tables from the scenes' current parity.
+static int batch_tables_current(sphg_batch* b) {
+    Batch& B = b->B;
+    bool stale = false;
+    for (size_t i = 0; i < B.scenes.size(); ++i)
+        stale |= (B.parity[i] != B.scenes[i]->par) || (B.nAt[i] != B.scenes[i]->n);
+    return stale ? upload_batch_tables(b->ctx, B) : SPHG_OK;
+}
+
 int sphg_batch_run(sphg_batch* b, uint32_t nsteps, sphg_step_stats* stats) {
     if (!b) return fail(SPHG_EINVAL, "NULL batch");
     sphg_ctx* ctx = b->ctx;
     HIP_TRY(hipSetDevice(ctx->device));
-    Batch& B = b->B;
-    // a scene stepped elsewhere since the tables were built has moved its
-    // double buffers: rebuild the tables from the scenes' current parity
-    bool stale = false;
-    for (size_t i = 0; i < B.scenes.size(); ++i)
-        stale |= (B.parity[i] != B.scenes[i]->par) || (B.nAt[i] != B.scenes[i]->n);
-    if (stale) {
-        int rc = upload_batch_tables(ctx, B);
-        if (rc) return rc;
-    }
-    int rc = run_batch(ctx, B, nsteps, stats);
+    int rc = batch_tables_current(b);
+    if (rc) return rc;
+    rc = run_batch(ctx, b->B, nsteps, stats);
+    if (rc) return rc;
+    HIP_TRY(hipGetLastError());
+    return SPHG_OK;
+}
+
+int sphg_batch_autotune(sphg_batch* b, uint32_t steps_per_trial, uint32_t rounds, sphg_tune_result* out) {
+    if (!b) return fail(SPHG_EINVAL, "NULL batch");
+    HIP_TRY(hipSetDevice(b->ctx->device));
+    int rc = batch_tables_current(b);
+    if (rc) return rc;
+    rc = tune_batch(b->ctx, b->B, steps_per_trial, rounds, out);
     if (rc) return rc;
     HIP_TRY(hipGetLastError());
     return SPHG_OK;

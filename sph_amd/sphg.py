@@ -105,6 +105,19 @@ class StepStats(C.Structure):
     ]
 
 
+class TuneResult(C.Structure):
+    """The launch schedule an autotune kept (sphg_tune_result)."""
+    _fields_ = [
+        ("density_slack", C.c_int32),
+        ("divergence_slack", C.c_int32),
+        ("replay", C.c_int32),
+        ("candidates", C.c_uint32),
+        ("steps", C.c_uint32),
+        ("ms_per_step", C.c_double),
+        ("default_ms_per_step", C.c_double),
+    ]
+
+
 class EmitterDesc(C.Structure):
     _fields_ = [
         ("type", C.c_uint32),
@@ -253,6 +266,8 @@ def load(path: str = LIB_PATH) -> C.CDLL:
         "sphg_batch_create": (C.c_int, [P, C.POINTER(P), U32, C.POINTER(P)]),
         "sphg_batch_run": (C.c_int, [P, U32, C.POINTER(StepStats)]),
         "sphg_batch_destroy": (C.c_int, [P]),
+        "sphg_scene_autotune": (C.c_int, [P, U32, U32, C.POINTER(TuneResult)]),
+        "sphg_batch_autotune": (C.c_int, [P, U32, U32, C.POINTER(TuneResult)]),
         "sphg_scene_plan_bytes": (C.c_int, [P, C.POINTER(U64)]),
         "sphg_ctx_kernel_timing": (C.c_int, [P, I32]),
         "sphg_ctx_kernel_time": (C.c_int, [P, C.c_char_p, C.POINTER(U64), dp]),
@@ -446,6 +461,13 @@ class Batch:
         _check(self.lib.sphg_batch_run(self.h, int(nsteps), C.byref(st)))
         return st
 
+    def autotune(self, steps_per_trial: int = 5, rounds: int = 3) -> TuneResult:
+        """Pick the fastest launch schedule for this batch; the trials are
+        real steps (every scene advances by TuneResult.steps)."""
+        r = TuneResult()
+        _check(self.lib.sphg_batch_autotune(self.h, int(steps_per_trial), int(rounds), C.byref(r)))
+        return r
+
     def close(self) -> None:
         if self.h:
             self.lib.sphg_batch_destroy(self.h)
@@ -556,6 +578,13 @@ class Scene:
         st = StepStats()
         _check(self.lib.sphg_scene_step(self.h, int(nsteps), C.byref(st)))
         return st
+
+    def autotune(self, steps_per_trial: int = 5, rounds: int = 3) -> TuneResult:
+        """Pick the fastest launch schedule for this scene's stepping; the
+        trials are real steps (the scene advances by TuneResult.steps)."""
+        r = TuneResult()
+        _check(self.lib.sphg_scene_autotune(self.h, int(steps_per_trial), int(rounds), C.byref(r)))
+        return r
 
     def set_state(self, x, v, k=None, kv=None, time_step: float = None, last_iterations: int = 0) -> None:
         x = _f32(x).reshape(-1, 3)

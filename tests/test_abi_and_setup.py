@@ -39,7 +39,8 @@ def test_struct_layouts_match_header(tmp_path):
     """Every field of the ctypes mirrors sits at the offset the C compiler
     gives it in include/sphg.h (a probe compiled with gcc)."""
     structs = {"sphg_scene_desc": sphg.SceneDesc, "sphg_scene_scalars": sphg.SceneScalars,
-               "sphg_step_stats": sphg.StepStats, "sphg_emitter_desc": sphg.EmitterDesc}
+               "sphg_step_stats": sphg.StepStats, "sphg_emitter_desc": sphg.EmitterDesc,
+               "sphg_tune_result": sphg.TuneResult}
     lines = ['#include "sphg.h"', "#include <stdio.h>", "#include <stddef.h>", "int main(void) {"]
     for cname, py in structs.items():
         lines.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')

@@ -575,6 +575,42 @@ def test_graph_replay_matches_eager(ctx, method):
         s.close()
 
 
+@pytest.mark.parametrize("method", [sphg.DFSPH, sphg.WCSPH, sphg.PCISPH])
+def test_autotune_keeps_results(ctx, method):
+    """Autotuning the launch schedule (loop slack, segment replay) runs real
+    steps under every candidate and keeps one; a scene and a persistent
+    batch tuned that way end, bit for bit, where untuned stepping of the
+    same number of steps ends."""
+    spec = configs.small_scene(method, n_side=9, seed=11)
+    tuned = sphg.Scene(ctx, spec.params, spec.fluid, spec.boundary)
+    r = tuned.autotune(steps_per_trial=2, rounds=2)
+    expect = {sphg.DFSPH: 16, sphg.WCSPH: 2, sphg.PCISPH: 8}[method]
+    assert r.candidates == expect and r.steps == 4 * expect
+    assert r.ms_per_step > 0 and r.default_ms_per_step > 0
+    assert 0 <= r.density_slack <= 3 and 0 <= r.divergence_slack <= 1 and r.replay in (0, 1)
+    tuned.step(7)
+    plain = sphg.Scene(ctx, spec.params, spec.fluid, spec.boundary)
+    plain.step(r.steps + 7)
+    assert np.array_equal(tuned.get(sphg.FIELD_X), plain.get(sphg.FIELD_X))
+    assert np.array_equal(tuned.get(sphg.FIELD_V), plain.get(sphg.FIELD_V))
+    assert tuned.scalars().sum_iterations == plain.scalars().sum_iterations
+    # a persistent batch of the feature scenes (emitter, PCISPH boundary, cube)
+    alone = _feature_scenes(ctx, method)
+    mixed = _feature_scenes(ctx, method)
+    b = sphg.Batch(ctx, mixed)
+    rb = b.autotune(steps_per_trial=1, rounds=2)
+    b.run(5)
+    b.close()
+    for s in alone:
+        s.step(rb.steps + 5)
+    for k, (a, m) in enumerate(zip(alone, mixed)):
+        assert a.n == m.n, k
+        assert np.array_equal(a.get(sphg.FIELD_X), m.get(sphg.FIELD_X)), k
+        assert np.array_equal(a.get(sphg.FIELD_V), m.get(sphg.FIELD_V)), k
+    for s in [tuned, plain] + alone + mixed:
+        s.close()
+
+
 def test_kernel_timing_counts_and_selection(ctx):
     """Per-kernel event timing counts every launch of each kind (one build,
     one dt per DFSPH step; at least one density error sweep more than
