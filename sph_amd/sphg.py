@@ -16,6 +16,8 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libsphg.so")
 
 WCSPH, PCISPH, DFSPH = 0, 1, 2
+# status codes (SPHG_OK, SPHG_E*)
+OK, EINVAL, EHIP, ENOMEM, ESTATE, EOVERFLOW = 0, -1, -2, -3, -4, -5
 BOUNDARY_CUBE, BOUNDARY_PCISPH, BOUNDARY_AKINCI = 0, 1, 2
 VISCOSITY_STANDARD, VISCOSITY_ARTIFICIAL, VISCOSITY_XSPH = 0, 1, 2
 SURFACE_TENSION_AKINCI, ADHESION_AKINCI = 3, 4

@@ -611,6 +611,24 @@ def test_autotune_keeps_results(ctx, method):
         s.close()
 
 
+@pytest.mark.parametrize("method", [sphg.DFSPH, sphg.WCSPH, sphg.PCISPH])
+def test_neighbor_overflow_is_reported(ctx, method):
+    """More neighbors than max_neighbors: the list build keeps its writes
+    inside the lists (the excess is dropped), and the step reports
+    SPHG_EOVERFLOW with the count it saw instead of returning quietly."""
+    from dataclasses import replace
+    spec = configs.small_scene(method, n_side=8, seed=5)
+    p = replace(spec.params, max_neighbors=8)
+    g = sphg.Scene(ctx, p, spec.fluid, spec.boundary)
+    with pytest.raises(sphg.SphgError) as e:
+        g.step(1)
+    assert e.value.code == sphg.EOVERFLOW and "max_neighbors" in str(e.value)
+    g.close()
+    ok = sphg.Scene(ctx, spec.params, spec.fluid, spec.boundary)   # the default limit holds
+    ok.step(2)
+    ok.close()
+
+
 def test_kernel_timing_counts_and_selection(ctx):
     """Per-kernel event timing counts every launch of each kind (one build,
     one dt per DFSPH step; at least one density error sweep more than
