@@ -98,6 +98,18 @@ struct Profiler {
     }
 };
 
+// A temporary device allocation freed on every return path.
+struct DevTemp {
+    void* p = nullptr;
+    DevTemp() = default;
+    DevTemp(const DevTemp&) = delete;
+    DevTemp& operator=(const DevTemp&) = delete;
+    ~DevTemp() {
+        if (p) (void)hipFree(p);
+    }
+    float* f() const { return (float*)p; }
+};
+
 struct Scope {
     Profiler* p;
     hipStream_t st;
@@ -497,12 +509,17 @@ int run_segment(sphg_ctx* ctx, Batch& B, int which, F&& issue) {
         hipGraph_t g = nullptr;
         HIP_TRY(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
         issue();
-        HIP_TRY(hipStreamEndCapture(ctx->stream, &g));
-        const hipError_t ie = hipGraphInstantiate(&e, g, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(g);
-        if (ie != hipSuccess) {
+        hipError_t ce = hipStreamEndCapture(ctx->stream, &g);
+        if (ce == hipSuccess) ce = hipGraphInstantiate(&e, g, nullptr, nullptr, 0);
+        if (g) (void)hipGraphDestroy(g);
+        if (ce != hipSuccess) {
+            // nothing ran: launch the segment one kernel at a time, and do
+            // not try to capture for this batch again
             e = nullptr;
-            return fail(SPHG_EHIP, std::string("graph instantiation: ") + hipGetErrorString(ie));
+            (void)hipGetLastError();
+            B.graphs = false;
+            issue();
+            return SPHG_OK;
         }
     }
     HIP_TRY(hipGraphLaunch(e, ctx->stream));
@@ -1114,13 +1131,12 @@ int sphg_scene_set_pb_normals(sphg_scene* s, const float* nrm) {
     HIP_TRY(hipSetDevice(ctx->device));
     const size_t n = s->n;
     if (!n) return SPHG_OK;
-    float* d = nullptr;
-    HIP_TRY(hipMalloc(&d, sizeof(float) * 3 * n));
-    HIP_TRY(hipMemcpyAsync(d, nrm, sizeof(float) * 3 * n, hipMemcpyHostToDevice, ctx->stream));
+    DevTemp d;
+    HIP_TRY(hipMalloc(&d.p, sizeof(float) * 3 * n));
+    HIP_TRY(hipMemcpyAsync(d.p, nrm, sizeof(float) * 3 * n, hipMemcpyHostToDevice, ctx->stream));
     const DevScene& V = s->view[s->par];
-    launch_gather_f3(ctx->stream, V.pbn, d, V.id, (int)n);
+    launch_gather_f3(ctx->stream, V.pbn, d.f(), V.id, (int)n);
     HIP_TRY(hipStreamSynchronize(ctx->stream));
-    (void)hipFree(d);
     HIP_TRY(hipGetLastError());
     return SPHG_OK;
 }
@@ -1367,14 +1383,14 @@ int sphg_scene_init(sphg_scene* s) {
 
     // fluid upload (staging in tmp is too small for xyz+vel: copy through a scratch allocation)
     if (n) {
-        float* dstage = nullptr;
-        HIP_TRY(hipMalloc(&dstage, sizeof(float) * 6 * (size_t)n));
+        DevTemp stage;
+        HIP_TRY(hipMalloc(&stage.p, sizeof(float) * 6 * (size_t)n));
+        float* dstage = stage.f();
         HIP_TRY(hipMemcpyAsync(dstage, s->fxyz.data(), sizeof(float) * 3 * n, hipMemcpyHostToDevice, ctx->stream));
         HIP_TRY(hipMemcpyAsync(dstage + 3 * (size_t)n, s->fvel.data(), sizeof(float) * 3 * n, hipMemcpyHostToDevice,
                                ctx->stream));
         launch_load_state(ctx->stream, s->self->dT[0], 0, (int)n, dstage, dstage + 3 * (size_t)n, nullptr, nullptr);
         HIP_TRY(hipStreamSynchronize(ctx->stream));
-        (void)hipFree(dstage);
     }
 
     if (method == DFSPH) {
@@ -1402,8 +1418,9 @@ int sphg_scene_set_state(sphg_scene* s, const float* xyz, const float* vel, cons
     sphg_ctx* ctx = s->ctx;
     HIP_TRY(hipSetDevice(ctx->device));
     const size_t n = s->n;
-    float* dstage = nullptr;
-    HIP_TRY(hipMalloc(&dstage, sizeof(float) * 8 * n));
+    DevTemp stage;
+    HIP_TRY(hipMalloc(&stage.p, sizeof(float) * 8 * n));
+    float* dstage = stage.f();
     HIP_TRY(hipMemcpyAsync(dstage, xyz, sizeof(float) * 3 * n, hipMemcpyHostToDevice, ctx->stream));
     HIP_TRY(hipMemcpyAsync(dstage + 3 * n, vel, sizeof(float) * 3 * n, hipMemcpyHostToDevice, ctx->stream));
     if (k) HIP_TRY(hipMemcpyAsync(dstage + 6 * n, k, sizeof(float) * n, hipMemcpyHostToDevice, ctx->stream));
@@ -1426,7 +1443,6 @@ int sphg_scene_set_state(sphg_scene* s, const float* xyz, const float* vel, cons
         flip(B);
     }
     HIP_TRY(hipStreamSynchronize(ctx->stream));
-    (void)hipFree(dstage);
     HIP_TRY(hipGetLastError());
     return SPHG_OK;
 }

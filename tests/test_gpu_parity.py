@@ -629,6 +629,61 @@ def test_neighbor_overflow_is_reported(ctx, method):
     ok.close()
 
 
+@pytest.mark.parametrize("method", [sphg.DFSPH, sphg.WCSPH, sphg.PCISPH])
+def test_whole_step_invariants_on_device(ctx, method):
+    """The HIP path keeps the whole-step invariants that pin the oracle, to
+    FP32 storage: a mirror-symmetric free fluid block stays symmetric, free
+    fluid without gravity conserves momentum, and (DFSPH, WCSPH, whose
+    neighbor sets do not depend on where the grid lies) a uniform velocity
+    added to every particle only translates the result."""
+    from dataclasses import replace
+    from test_oracle_pins import _sym_block, _solver_params
+    none = np.zeros((0, 3), np.float32)
+
+    def scene(p, x, v):
+        p = replace(p, boundary_method=sphg.BOUNDARY_CUBE)      # no walls, no clamp
+        return sphg.Scene(ctx, p, x.astype(np.float32), none, v.astype(np.float32))
+
+    x, v, mirror, c = _sym_block()
+    g = scene(_solver_params(method, 1.0 / 64.0, gravity=(0.0, 0.0, -9.81)), x, v)
+    g.step(3)
+    xa, va = g.get(sphg.FIELD_X).astype(np.float64), g.get(sphg.FIELD_V).astype(np.float64)
+    assert np.abs(va - v).max() > 1e-2
+    img_x = np.column_stack([2.0 * c - xa[:, 0], xa[:, 1], xa[:, 2]])
+    img_v = np.column_stack([-va[:, 0], va[:, 1], va[:, 2]])
+    assert np.abs(xa[mirror] - img_x).max() < 1e-6
+    assert np.abs(va[mirror] - img_v).max() < 1e-5 * np.abs(va).max()
+    g.close()
+
+    x, v, _, _ = _sym_block(seed=22)
+    v = v * np.array([1.0, 0.5, 0.25])
+    g = scene(_solver_params(method, 1.0 / 64.0, xsph=False), x, v)
+    g.step(3)
+    va = g.get(sphg.FIELD_V).astype(np.float64)
+    assert np.abs(va - v).max() > 1e-2
+    assert np.abs(va.sum(0) - v.sum(0)).max() < 1e-5 * np.abs(va).sum()
+    g.close()
+
+    if method != sphg.PCISPH:
+        x, v, _, _ = _sym_block(seed=23)
+        u0 = np.array([0.25, -0.125, 0.5])
+        p = _solver_params(method, 1.0 / 64.0, gravity=(0.0, 0.0, -9.81), min_time_step=5e-4, max_time_step=5e-4)
+        a, b = scene(p, x, v), scene(p, x, v + u0)
+        a.step(3)
+        b.step(3)
+        t = a.scalars().time
+        va, vb = a.get(sphg.FIELD_V).astype(np.float64), b.get(sphg.FIELD_V).astype(np.float64)
+        assert np.abs(va - v).max() > 1e-2
+        # WCSPH's stiff equation of state (kappa 1000, gamma 7) amplifies
+        # the FP32 rounding of v + u0 most
+        tol = 1e-4 if method == sphg.WCSPH else 1e-5
+        assert np.abs(vb - u0 - va).max() < tol * np.abs(va).max()
+        xa, xb = a.get(sphg.FIELD_X).astype(np.float64), b.get(sphg.FIELD_X).astype(np.float64)
+        assert np.abs(xb - u0 * t - xa).max() < 1e-6
+        a.close()
+        b.close()
+
+
 def test_kernel_timing_counts_and_selection(ctx):
     """Per-kernel event timing counts every launch of each kind (one build,
     one dt per DFSPH step; at least one density error sweep more than

@@ -595,3 +595,116 @@ def test_pcisph_boundary_normals_nan_without_collision():
     o, _, _, _ = _pb_scene(sphg.DFSPH, [[0.5, 0.5, 0.5], [0.52, 0.5, 0.5]], [[0, 0, 0], [0, 0, 0]])
     o.step(1)
     assert np.all(np.isnan(o.get(po.OF_PBN)))
+
+
+# ------------------------------------------------ whole-step invariants
+# Full solver steps -- every phase, both DFSPH loops, the PCISPH loop, the
+# Akinci surface tension -- against what the physics fixes for any correct
+# implementation: a mirror-symmetric state stays mirror-symmetric (a swapped
+# or transposed vector component breaks it), free fluid conserves momentum
+# (a pressure or cohesion term that is not antisymmetric in i, j breaks it),
+# and a uniform velocity added to every particle only translates the result
+# (an absolute velocity where a relative one belongs breaks it).  The states
+# are dyadic (exact in FP32), and the one boundary particle is far away.
+# PCISPH's predicted densities take their candidates from the cells of the
+# current positions (src/PCISPHSolver.cpp:181), so they depend on where the
+# particles sit in the fixed grid: the mirror plane lies on a cell boundary
+# and the velocity shift moves every particle by exactly one cell per step.
+
+_FAR = np.array([[50.0, 50.0, 50.0]], np.float32)
+_Q = 2.0 ** -20
+
+
+def _sym_block(n=8, r=1.0 / 64.0, seed=21):
+    """A jittered lattice block mirror-symmetric about the plane x = c = 10 h
+    (a cell boundary, h = 4r), with mirrored velocities (vx -> -vx):
+    particle k and mirror[k] are images.  No particle is within 2r/2 of a
+    cell boundary."""
+    s = 2.0 * r
+    rng = np.random.default_rng(seed)
+    c = 10.0 * 4.0 * r
+    g = c - (n - 1) * s / 2.0 + np.arange(n) * s
+    xs, vs, idx = [], [], {}
+    for ix in range(n // 2):
+        for iy in range(n):
+            for iz in range(n):
+                j = np.round(rng.uniform(-0.05, 0.05, 3) * s / _Q) * _Q
+                v = np.round(rng.uniform(-1.0, 1.0, 3) * 1024.0) / 1024.0
+                p = np.array([g[ix], g[iy], g[iz]]) + j
+                idx[(ix, iy, iz)] = len(xs)
+                xs.append(p)
+                vs.append(v)
+                idx[(n - 1 - ix, iy, iz)] = len(xs)
+                xs.append(np.array([2.0 * c - p[0], p[1], p[2]]))
+                vs.append(np.array([-v[0], v[1], v[2]]))
+    x, v = np.array(xs), np.array(vs)
+    assert np.array_equal(x.astype(np.float32).astype(np.float64), x)
+    mirror = np.arange(len(x)) ^ 1
+    return x, v, mirror, c
+
+
+def _solver_params(method, r, xsph=True, **kw):
+    """DFSPH; WCSPH with XSPH viscosity (unless xsph is False: its m/rho_j
+    weights are not antisymmetric, so it does not conserve momentum); PCISPH
+    with Akinci surface tension."""
+    base = dict(time_step=5e-4, min_time_step=1e-6, max_time_step=5e-3, cfl_factor=0.4, eta=1e-4, eta_v=1e-3,
+                min_iterations=2, max_iterations=100, min_iterations_v=1, max_iterations_v=100)
+    if method == sphg.WCSPH and xsph:
+        base.update(viscosity_method=sphg.VISCOSITY_XSPH, viscosity=0.01)
+    if method == sphg.PCISPH:
+        base.update(eta=0.01, min_iterations=3, surface_tension_method=sphg.SURFACE_TENSION_AKINCI,
+                    surface_tension=0.05)
+    base.update(kw)
+    return _params(method=method, r=r, **base)
+
+
+_METHODS = [sphg.DFSPH, sphg.WCSPH, sphg.PCISPH]
+
+
+@pytest.mark.parametrize("method", _METHODS)
+def test_whole_steps_keep_mirror_symmetry(method):
+    x, v, mirror, c = _sym_block()
+    o = po.Oracle(_solver_params(method, 1.0 / 64.0, gravity=(0.0, 0.0, -9.81)), x, _FAR, velocities=v)
+    o.step(3)
+    xa, va = o.get(po.OF_X), o.get(po.OF_V)
+    assert np.abs(va - v).max() > 1e-2          # the solver moved things
+    img_x = np.column_stack([2.0 * c - xa[:, 0], xa[:, 1], xa[:, 2]])
+    img_v = np.column_stack([-va[:, 0], va[:, 1], va[:, 2]])
+    assert np.abs(xa[mirror] - img_x).max() < 1e-12
+    assert np.abs(va[mirror] - img_v).max() < 1e-9 * np.abs(va).max()
+    o.close()
+
+
+@pytest.mark.parametrize("method", _METHODS)
+def test_whole_steps_conserve_momentum(method):
+    x, v, _, _ = _sym_block(seed=22)
+    v = v * np.array([1.0, 0.5, 0.25])           # break the mirror: net momentum is nonzero
+    o = po.Oracle(_solver_params(method, 1.0 / 64.0, xsph=False), x, _FAR, velocities=v)
+    p0 = v.sum(0)
+    o.step(3)
+    va = o.get(po.OF_V)
+    assert np.abs(va - v).max() > 1e-2
+    assert np.abs(va.sum(0) - p0).max() < 1e-11 * np.abs(va).sum()
+    o.close()
+
+
+@pytest.mark.parametrize("method", _METHODS)
+def test_whole_steps_galilean_invariant(method):
+    x, v, _, _ = _sym_block(seed=23)
+    dt = 2.0 ** -11
+    u0 = np.array([(1.0 / 16.0) / dt, 0.0, 0.0])     # one cell (h = 4r = 1/16) per step
+    p = _solver_params(method, 1.0 / 64.0, gravity=(0.0, 0.0, -9.81), time_step=dt, min_time_step=dt,
+                       max_time_step=dt)
+    a = po.Oracle(p, x, _FAR, velocities=v)
+    b = po.Oracle(p, x, _FAR, velocities=v + u0)
+    a.step(3)
+    b.step(3)
+    t = a.scalars()["time"]
+    assert t == 3 * dt and t == b.scalars()["time"]
+    assert a.scalars()["iterations"] == b.scalars()["iterations"]
+    va, vb = a.get(po.OF_V), b.get(po.OF_V)
+    assert np.abs(va - v).max() > 1e-2
+    assert np.abs(vb - u0 - va).max() < 1e-9 * np.abs(va).max()
+    assert np.abs(b.get(po.OF_X) - u0 * t - a.get(po.OF_X)).max() < 1e-12
+    a.close()
+    b.close()
