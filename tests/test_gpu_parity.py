@@ -684,6 +684,35 @@ def test_whole_step_invariants_on_device(ctx, method):
         b.close()
 
 
+def test_divergence_error_20_neighbor_cutoff_on_device(ctx):
+    """The HIP path applies the DFSPH 20-neighbor cutoff of the divergence
+    error (src/DFSPHSolver.cpp:480-483) at the same counts as a brute-force
+    FP64 test: both loops skipped, a converging velocity field, counts on
+    both sides of 20 (the configuration of the oracle pin)."""
+    from dataclasses import replace
+    from test_oracle_pins import _lattice, _params, _Q
+    r = 1.0 / 64.0
+    s, h = 2.0 * r, 4.0 * r
+    rng = np.random.default_rng(31)
+    x = _lattice(6, s, 0.5) + np.round(rng.uniform(-0.15, 0.15, (216, 3)) * s / _Q) * _Q
+    v = np.round(-(x - x.mean(0)) * 0.5 / _Q) * _Q
+    p = _params(r=r, time_step=5e-4, min_time_step=5e-4, max_time_step=5e-4, cfl_factor=0.4, eta=1e9, eta_v=1e9,
+                min_iterations=0, max_iterations=100, min_iterations_v=0, max_iterations_v=100)
+    p = replace(p, boundary_method=sphg.BOUNDARY_CUBE)
+    g = sphg.Scene(ctx, p, x.astype(np.float32), np.zeros((0, 3), np.float32), v.astype(np.float32))
+    st = g.step(1)
+    assert st.iterations == 0 and st.iterations_v == 0
+    xa = g.get(sphg.FIELD_X).astype(np.float64)
+    de = g.get(sphg.FIELD_DIVERR)
+    d = np.linalg.norm(xa[:, None, :] - xa[None, :, :], axis=2)
+    cnt = ((d > 1e-5) & (d / h < 1.0)).sum(1)
+    assert set(range(17, 24)) <= set(cnt.tolist())
+    assert np.array_equal(g.get(sphg.FIELD_NGRAD), cnt)
+    assert np.all(de[cnt < 20] == 0.0)
+    assert np.all(de[cnt >= 20] > 0.0)
+    g.close()
+
+
 def test_kernel_timing_counts_and_selection(ctx):
     """Per-kernel event timing counts every launch of each kind (one build,
     one dt per DFSPH step; at least one density error sweep more than

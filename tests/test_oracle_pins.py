@@ -708,3 +708,34 @@ def test_whole_steps_galilean_invariant(method):
     assert np.abs(b.get(po.OF_X) - u0 * t - a.get(po.OF_X)).max() < 1e-12
     a.close()
     b.close()
+
+
+def test_divergence_error_20_neighbor_cutoff():
+    """computeDivergenceError zeroes the error of a particle with fewer than
+    20 fluid neighbors of nonzero kernel gradient (src/DFSPHSolver.cpp:446-447,
+    480-483).  Both loops are skipped (min 0 iterations, huge tolerances)
+    and k starts at 0, so the step only moves x by v dt and the error is
+    formed once from the new positions.  A converging velocity field makes
+    every error positive before the cutoff; the neighbor counts come from a
+    brute-force FP64 distance test on the returned positions."""
+    r = 1.0 / 64.0
+    s = 2.0 * r
+    h = 4.0 * r
+    rng = np.random.default_rng(31)
+    x = _lattice(6, s, 0.5) + np.round(rng.uniform(-0.15, 0.15, (216, 3)) * s / _Q) * _Q
+    centre = x.mean(0)
+    v = np.round(-(x - centre) * 0.5 / _Q) * _Q
+    p = _params(r=r, time_step=5e-4, min_time_step=5e-4, max_time_step=5e-4, cfl_factor=0.4, eta=1e9, eta_v=1e9,
+                min_iterations=0, max_iterations=100, min_iterations_v=0, max_iterations_v=100)
+    o = po.Oracle(p, x, _FAR, velocities=v)
+    o.step(1)
+    assert o.scalars()["iterations"] == 0 and o.scalars()["iterations_v"] == 0
+    xa = o.get(po.OF_X)
+    de = o.get(po.OF_DIVERR)
+    d = np.linalg.norm(xa[:, None, :] - xa[None, :, :], axis=2)
+    q = d / h
+    cnt = ((d > 1e-5) & (q < 1.0)).sum(1)
+    assert set(range(17, 24)) <= set(cnt.tolist())        # counts on both sides of the cutoff
+    assert np.all(de[cnt < 20] == 0.0)
+    assert np.all(de[cnt >= 20] > 0.0)
+    o.close()
