@@ -713,6 +713,42 @@ def test_divergence_error_20_neighbor_cutoff_on_device(ctx):
     g.close()
 
 
+def test_warm_start_eps_branch_linearity_and_clamp_on_device(ctx):
+    """The HIP warm start (k_df_predict + k_df_vel<0>) against the pin the
+    oracle carries: nothing moves below the 1e-5 eps branch, doubling k
+    doubles the change bit for bit, every k above the 1e-9 clamp gives the
+    same change, and that change is within ~1% of k_i = 0.99e-9 rho_i's."""
+    from dataclasses import replace
+    from test_oracle_pins import _lattice, _params, _Q
+    r = 1.0 / 64.0
+    s = 2.0 * r
+    rng = np.random.default_rng(41)
+    x = (_lattice(6, s, 0.5) + np.round(rng.uniform(-0.15, 0.15, (216, 3)) * s / _Q) * _Q).astype(np.float32)
+    v0 = np.zeros_like(x)
+    dt = 5e-4
+    p = _params(r=r, time_step=dt, min_time_step=dt, max_time_step=dt, cfl_factor=0.4, eta=1e9, eta_v=1e9,
+                min_iterations=0, max_iterations=100, min_iterations_v=0, max_iterations_v=100)
+    g = sphg.Scene(ctx, replace(p, boundary_method=sphg.BOUNDARY_CUBE), x, np.zeros((0, 3), np.float32))
+    rho = g.get(sphg.FIELD_RHO).astype(np.float64)
+
+    def dv(k):
+        g.set_state(x, v0, k=np.broadcast_to(np.asarray(k, np.float32), (len(x),)).copy(), time_step=dt)
+        st = g.step(1)
+        assert st.iterations == 0 and st.iterations_v == 0
+        return g.get(sphg.FIELD_V).astype(np.float64)
+
+    assert np.all(dv(1e-10) == 0.0)
+    a1, a2 = dv(1e-8), dv(2e-8)
+    assert np.abs(a1).max() > 1e-9
+    assert np.array_equal(a2, 2.0 * a1)
+    c1, c2 = dv(1e-3), dv(1.0)
+    assert np.array_equal(c1, c2) and np.abs(c1).max() > 30.0 * np.abs(a1).max()
+    below = dv((0.99e-9 * rho).astype(np.float32))
+    rel = np.abs(c1 - below).max() / np.abs(c1).max()
+    assert 0.005 < rel < 0.02
+    g.close()
+
+
 def test_kernel_timing_counts_and_selection(ctx):
     """Per-kernel event timing counts every launch of each kind (one build,
     one dt per DFSPH step; at least one density error sweep more than

@@ -739,3 +739,41 @@ def test_divergence_error_20_neighbor_cutoff():
     assert np.all(de[cnt < 20] == 0.0)
     assert np.all(de[cnt >= 20] > 0.0)
     o.close()
+
+
+def test_warm_start_eps_branch_linearity_and_clamp():
+    """densityWarmStart (src/DFSPHSolver.cpp:602-689): k_i <- min(k_i/rho_i,
+    1e-9)/dt^2, then v_i -= dt sum_j m (k_i + k_j) gradW_ij, a term kept only
+    where |k_i + k_j| > 1e-5.  With both loops skipped and v = 0 it is the
+    step's only velocity change, so: below the eps branch nothing moves;
+    between eps and the clamp the change is linear in k (doubling k doubles
+    it bit for bit); above the clamp every k gives the same change, which is
+    the change of k_i just below 1e-9 rho_i to within its 1% offset."""
+    r = 1.0 / 64.0
+    s = 2.0 * r
+    rng = np.random.default_rng(41)
+    x = _lattice(6, s, 0.5) + np.round(rng.uniform(-0.15, 0.15, (216, 3)) * s / _Q) * _Q
+    v0 = np.zeros_like(x)
+    dt = 5e-4
+    p = _params(r=r, time_step=dt, min_time_step=dt, max_time_step=dt, cfl_factor=0.4, eta=1e9, eta_v=1e9,
+                min_iterations=0, max_iterations=100, min_iterations_v=0, max_iterations_v=100)
+    o = po.Oracle(p, x, _FAR)
+    rho = o.get(po.OF_RHO)
+    assert rho.min() > 400.0 and rho.max() < 1200.0    # keeps k = 1e-10 below and 1e-8 above the eps branch
+
+    def dv(k):
+        o.set_state(x, v0, k=np.broadcast_to(np.asarray(k, np.float32), (len(x),)).copy(), time_step=dt)
+        o.step(1)
+        assert o.scalars()["iterations"] == 0 and o.scalars()["iterations_v"] == 0
+        return o.get(po.OF_V)
+
+    assert np.all(dv(1e-10) == 0.0)                       # 2 k/rho/dt^2 < 1e-5: every term dropped
+    a1, a2 = dv(1e-8), dv(2e-8)
+    assert np.abs(a1).max() > 1e-9
+    assert np.array_equal(a2, 2.0 * a1)
+    c1, c2 = dv(1e-3), dv(1.0)
+    assert np.array_equal(c1, c2) and np.abs(c1).max() > 30.0 * np.abs(a1).max()   # 1e-9 against ~1e-11
+    below = dv((0.99e-9 * rho).astype(np.float32))
+    rel = np.abs(c1 - below).max() / np.abs(c1).max()
+    assert 0.005 < rel < 0.02
+    o.close()
