@@ -387,14 +387,38 @@ __device__ inline bool pb_velocity(const DevScene& S, double nx, double ny, doub
 // ================================================================ grid
 
 // Cell key + histogram (src/HashTable.h:79-86 insertion, as a count).
+// Adds one to counter[key] for every lane with key >= 0 and returns the
+// counter's value before this lane's unit, a slot unique to the lane.  The
+// particles of a wavefront are in the previous step's cell order and most
+// keep their cell, so runs of consecutive lanes share a key: each run adds
+// its length with one atomic (a dense fluid has ~8 lanes per run), and its
+// lanes take consecutive slots.  Every lane of the wavefront calls it; lanes
+// with nothing to count pass -1 and must all come after the counting ones.
+__device__ inline int wave_count_add(int* counter, int key) {
+    const int lane = threadIdx.x & 63;
+    const int prev = __shfl_up(key, 1);
+    const bool head = (lane == 0) || (key != prev);
+    const uint64_t heads = __ballot(head);
+    const uint64_t upto = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);   // lanes 0 .. lane
+    const int first = 63 - __clzll((long long)(heads & upto));             // this lane's run head
+    const uint64_t later = heads & ~upto;
+    const int end = later ? __ffsll((long long)later) - 1 : 64;             // one past the run
+    int base = 0;
+    if (head && key >= 0) base = atomicAdd(&counter[key], end - lane);
+    base = __shfl(base, first);
+    return base + (lane - first);
+}
+
 __global__ __launch_bounds__(kBlock) void k_keys(const DevScene* __restrict__ T, const Chunk* __restrict__ C) {
     const Chunk c = C[blockIdx.x];
     const DevScene& S = T[c.scene];
     const int i = c.begin + threadIdx.x;
-    if (i >= S.n) return;
-    const int cl = cell_of(S, S.x[i]);
-    S.cellU[i] = cl;
-    atomicAdd(&S.cellCount[cl], 1);
+    int cl = -1;
+    if (i < S.n) {
+        cl = cell_of(S, S.x[i]);
+        S.cellU[i] = cl;
+    }
+    (void)wave_count_add(S.cellCount, cl);
 }
 
 // Per 4096-cell chunk: total count.
@@ -489,10 +513,9 @@ __global__ __launch_bounds__(kBlock) void k_scatter(const DevScene* __restrict__
     const Chunk c = C[blockIdx.x];
     const DevScene& S = T[c.scene];
     const int i = c.begin + threadIdx.x;
-    if (i >= S.n) return;
-    const int cl = S.cellU[i];
-    const int slot = S.cellStart[cl] + atomicAdd(&S.cellCount[cl], 1);
-    S.perm[slot] = S.id[i];
+    const int cl = (i < S.n) ? S.cellU[i] : -1;
+    const int r = wave_count_add(S.cellCount, cl);
+    if (i < S.n) S.perm[S.cellStart[cl] + r] = S.id[i];
 }
 
 // Final slot of each particle = cell start + number of particles of its
@@ -1276,18 +1299,23 @@ __global__ __launch_bounds__(kBlock) void k_df_positions(const DevScene* __restr
     const Chunk c = C[blockIdx.x];
     const DevScene& S = T[c.scene];
     const int i = c.begin + threadIdx.x;
-    if (i >= S.n) return;
-    const double dt = S.ss->dt;
-    float4 x = S.x[i];
-    const float4 v = S.v[i];
-    x.x = (float)((double)x.x + (double)v.x * dt);
-    x.y = (float)((double)x.y + (double)v.y * dt);
-    x.z = (float)((double)x.z + (double)v.z * dt);
-    S.x[i] = x;
-    if (S.pbOn) return;   // the boundary projection moves it, then counts its cell
-    const int cl = cell_of(S, x);
-    S.cellU[i] = cl;
-    atomicAdd(&S.cellCount[cl], 1);
+    int cl = -1;
+    if (i < S.n) {
+        const double dt = S.ss->dt;
+        float4 x = S.x[i];
+        const float4 v = S.v[i];
+        x.x = (float)((double)x.x + (double)v.x * dt);
+        x.y = (float)((double)x.y + (double)v.y * dt);
+        x.z = (float)((double)x.z + (double)v.z * dt);
+        S.x[i] = x;
+        if (!S.pbOn) {
+            cl = cell_of(S, x);
+            S.cellU[i] = cl;
+        }
+    }
+    // with the PCISPH boundary method the projection moves the particle
+    // first, then counts its cell (a scene-uniform branch)
+    if (!S.pbOn) (void)wave_count_add(S.cellCount, cl);
 }
 
 // =============================================================== PCISPH
