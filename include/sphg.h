@@ -267,7 +267,7 @@ int sphg_batch_destroy(sphg_batch* batch);
  * all of them); with two or more rounds the first is not counted.  The
  * fastest schedule by wall time is kept for later steps.  Block size
  * (256 particles = 4 wavefronts, one neighbor-list group per wavefront),
- * cells per scan block (4096) and the reorder every step (the reference's
+ * cells per scan block (1024) and the reorder every step (the reference's
  * bucket order needs it) are fixed by the data layout and not tuned.  No
  * reference counterpart. */
 typedef struct sphg_tune_result {

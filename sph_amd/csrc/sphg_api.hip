@@ -559,7 +559,7 @@ int batch_one_step(sphg_ctx* ctx, Batch& B, uint32_t& syncs) {
             finalize(ctx, B, T, 0);
             { Scope s(P, st, K_DFPREDICT); launch_df_predict(st, T, B.dP, B.nP); }
             { Scope s(P, st, K_DFVEL_WARM); launch_df_vel(st, 0, T, B.dP, B.nP); }
-            launch_fill_int(st, ctx->dActive, 0, 2);
+            // the loop counters were cleared by the dt finalize
             { Scope s(P, st, K_DFERR_DENS); launch_df_err(st, 0, T, B.dP, B.nP); }
             finalize(ctx, B, T, 1);
         });
@@ -575,7 +575,7 @@ int batch_one_step(sphg_ctx* ctx, Batch& B, uint32_t& syncs) {
         B.lastIt = seen;
         const DevScene* T2 = B.dT[1];
         rc = run_segment(ctx, B, 1, [&] {
-            { Scope s(P, st, K_DFPOS); launch_df_positions(st, T, B.dP, B.nP); }
+            { Scope s(P, st, K_DFPOS); launch_df_positions(st, T, B.dP, B.nP, ctx->dActive); }
             // PCISPH boundary method: projection after the position update
             // (src/DFSPHSolver.cpp:65-75), which then counts the cells
             if (anyPb) { Scope s(P, st, K_PB); launch_pb_correct(st, 0, T, B.dP, B.nP); }
@@ -583,7 +583,7 @@ int batch_one_step(sphg_ctx* ctx, Batch& B, uint32_t& syncs) {
             // (src/DFSPHSolver.cpp:77); the new particles' cells are counted
             if (anyEmit) { Scope s(P, st, K_EMIT); launch_emit(st, B.dT[0], B.dT[1], ns, 1); }
             grid_sort(ctx, B, T);
-            launch_fill_int(st, ctx->dActive, 0, 2);
+            // (the loop counters were cleared by the position update)
             // the build also forms the divergence solve's initial error,
             // kappa_v and the error partials (computeDivergenceError before
             // the loop)

@@ -35,7 +35,7 @@ namespace sphg {
 
 constexpr int kBlock = 256;          // threads per block = particles per chunk
 constexpr int kGroup = 64;           // particles per neighbor-list group (one wavefront)
-constexpr int kCellChunk = 4096;     // cells per scan chunk (256 threads x 16)
+constexpr int kCellChunk = 1024;     // cells per scan chunk (256 threads x 4: ~300 blocks for a 1M-particle grid)
 constexpr int kCellPerThread = kCellChunk / kBlock;
 
 enum Method { WCSPH = 0, PCISPH = 1, DFSPH = 2 };

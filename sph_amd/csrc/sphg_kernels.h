@@ -21,7 +21,8 @@ void launch_df_vel(hipStream_t st, int mode, const DevScene* T, const Chunk* C, 
 void launch_df_err(hipStream_t st, int mode, const DevScene* T, const Chunk* C, int nchunks);
 // kind: 0 dt, 1 density (before loop), 2 density, 3 divergence (before loop), 4 divergence, 5 PCISPH
 void launch_finalize(hipStream_t st, int kind, const DevScene* T, int nscenes, int* active);
-void launch_df_positions(hipStream_t st, const DevScene* T, const Chunk* C, int nchunks);
+// also clears active[0..1] for the divergence solve
+void launch_df_positions(hipStream_t st, const DevScene* T, const Chunk* C, int nchunks, int* active);
 void launch_pci_density(hipStream_t st, const DevScene* T, const Chunk* C, int nchunks);
 void launch_pci_pacc(hipStream_t st, const DevScene* T, const Chunk* C, int nchunks, int iter);
 void launch_pci_integrate(hipStream_t st, const DevScene* T, const Chunk* C, int nchunks);

@@ -1,7 +1,7 @@
 // sphg_kernels.hip -- gfx950 kernels of the SPH solver step.
 //
 // Every kernel takes a scene table and a chunk table; block b works on
-// chunk b (256 particles or 4096 cells of one scene).  Sums of a scene are
+// chunk b (256 particles or 1024 cells of one scene).  Sums of a scene are
 // formed in a fixed order: a butterfly inside each wavefront, wave results
 // combined in index order, one partial per chunk, and the last chunk to
 // finish (an atomic arrival counter) adds the scene's partials in chunk
@@ -421,7 +421,7 @@ __global__ __launch_bounds__(kBlock) void k_keys(const DevScene* __restrict__ T,
     (void)wave_count_add(S.cellCount, cl);
 }
 
-// Per 4096-cell chunk: total count.
+// Per 1024-cell chunk: total count.
 __global__ __launch_bounds__(kBlock) void k_scan_sum(const DevScene* __restrict__ T, const Chunk* __restrict__ C) {
     const Chunk c = C[blockIdx.x];
     const DevScene& S = T[c.scene];
@@ -1295,9 +1295,13 @@ __global__ __launch_bounds__(kBlock) void k_df_err(const DevScene* __restrict__ 
 
 // updatePositions (src/DFSPHSolver.cpp:216-236) fused with the cell key and
 // histogram of the grid rebuild.
-__global__ __launch_bounds__(kBlock) void k_df_positions(const DevScene* __restrict__ T, const Chunk* __restrict__ C) {
+// Also clears the loop counters (scenes iterating, largest count) for the
+// divergence solve, which the list build's finalize starts counting.
+__global__ __launch_bounds__(kBlock) void k_df_positions(const DevScene* __restrict__ T, const Chunk* __restrict__ C,
+                                                         int* active) {
     const Chunk c = C[blockIdx.x];
     const DevScene& S = T[c.scene];
+    if (blockIdx.x == 0 && threadIdx.x < 2) active[threadIdx.x] = 0;
     const int i = c.begin + threadIdx.x;
     int cl = -1;
     if (i < S.n) {
@@ -1476,6 +1480,11 @@ __global__ __launch_bounds__(kBlock) void k_finalize(const DevScene* __restrict_
                                                            : scene_partial_sum(S.partial, S.nChunks, sh);
     if (threadIdx.x != 0) return;
     if (KIND == FIN_DT) {
+        // the density solve's loop counters start here (counted by FIN_DENS0)
+        if (sc == 0) {
+            active[0] = 0;
+            active[1] = 0;
+        }
         const double maxVel = fmax(0.1, red);
         const double ts = ss->dt;
         const double cflTs = S.cfl * 0.4 * 2.0 * S.r / maxVel;
@@ -1921,7 +1930,9 @@ void launch_finalize(hipStream_t st, int kind, const DevScene* T, int nscenes, i
         default: LAUNCH(k_finalize<FIN_PCI>, nscenes, st, T, nscenes, active); break;
     }
 }
-void launch_df_positions(hipStream_t st, const DevScene* T, const Chunk* C, int n) { if (n) LAUNCH(k_df_positions, n, st, T, C); }
+void launch_df_positions(hipStream_t st, const DevScene* T, const Chunk* C, int n, int* active) {
+    if (n) LAUNCH(k_df_positions, n, st, T, C, active);
+}
 void launch_pci_density(hipStream_t st, const DevScene* T, const Chunk* C, int n) { if (n) LAUNCH(k_pci_density, n, st, T, C); }
 void launch_pci_pacc(hipStream_t st, const DevScene* T, const Chunk* C, int n, int iter) { if (n) LAUNCH(k_pci_pacc, n, st, T, C, iter); }
 void launch_pci_integrate(hipStream_t st, const DevScene* T, const Chunk* C, int n) { if (n) LAUNCH(k_pci_integrate, n, st, T, C); }
