@@ -321,6 +321,30 @@ __device__ inline void walk27(const int* __restrict__ start, int c, int nx, int 
     }
 }
 
+// walk27 with the data of four candidates loaded before any is visited
+// (`load(j)` then `visit(j, data)`, in walk order), so a lane has four
+// independent loads in flight instead of one chain.
+template <class Load, class Visit>
+__device__ inline void walk27x4(const int* __restrict__ start, int c, int nx, int nxny, Load&& load, Visit&& visit) {
+#pragma unroll 1
+    for (int dz = -1; dz <= 1; ++dz) {
+#pragma unroll 1
+        for (int dy = -1; dy <= 1; ++dy) {
+            const int base = c + dy * nx + dz * nxny;
+            const int s = start[base - 1];
+            const int e = start[base + 2];
+            for (int j0 = s; j0 < e; j0 += 4) {
+                decltype(load(0)) d[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) d[u] = load(min(j0 + u, e - 1));
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+                    if (j0 + u < e) visit(j0 + u, d[u]);
+            }
+        }
+    }
+}
+
 // Poly6 W (src/Poly6.h:31-42) of a squared distance.
 __device__ inline double P6_d(double r2, const KCoefD& k) {
     const double d = k.h2 - r2;
@@ -1348,22 +1372,21 @@ __global__ __launch_bounds__(kBlock) void k_pci_density(const DevScene* __restri
         const int ci = S.cell[i];
         const int nx = S.nx, nxny = S.nx * S.ny;
         double ws = 0.0;
-        walk27(S.cellStart, ci, nx, nxny, [&](int j) {
-            const float4 xj = S.x[j];
-            const float4 dj = S.xs[j];
-            const double dx = (px - (double)xj.x) + (ex - (double)dj.x);
-            const double dy = (py - (double)xj.y) + (ey - (double)dj.y);
-            const double dz = (pz - (double)xj.z) + (ez - (double)dj.z);
-            const double r2 = dx * dx + dy * dy + dz * dz;
-            if (r2 <= k.hhi2) ws += W_d(sqrt(r2) * k.invh, k);
-        });
+        walk27x4(S.cellStart, ci, nx, nxny, [&](int j) { XV r; r.x = ld4(S.x, j); r.v = ld4(S.xs, j); return r; },
+                 [&](int, const XV& q) {
+                     const double dx = (px - (double)q.x.x) + (ex - (double)q.v.x);
+                     const double dy = (py - (double)q.x.y) + (ey - (double)q.v.y);
+                     const double dz = (pz - (double)q.x.z) + (ez - (double)q.v.z);
+                     const double r2 = dx * dx + dy * dy + dz * dz;
+                     if (r2 <= k.hhi2) ws += W_d(sqrt(r2) * k.invh, k);
+                 });
         double bw = 0.0;
-        walk27(S.bCellStart, ci, nx, nxny, [&](int b) {
-            const float4 y = S.xb[b];
-            const double dx = (px - (double)y.x) + ex, dy = (py - (double)y.y) + ey, dz = (pz - (double)y.z) + ez;
-            const double r2 = dx * dx + dy * dy + dz * dz;
-            if (r2 <= k.hhi2) bw += (double)y.w * W_d(sqrt(r2) * k.invh, k);
-        });
+        walk27x4(S.bCellStart, ci, nx, nxny, [&](int b) { return ld4(S.xb, b); },
+                 [&](int, const float4& y) {
+                     const double dx = (px - (double)y.x) + ex, dy = (py - (double)y.y) + ey, dz = (pz - (double)y.z) + ez;
+                     const double r2 = dx * dx + dy * dy + dz * dz;
+                     if (r2 <= k.hhi2) bw += (double)y.w * W_d(sqrt(r2) * k.invh, k);
+                 });
         const double rho0 = S.rho0D;
         const double drho = (S.mD * ws + rho0 * bw) - rho0;
         const double rho = rho0 + drho;
