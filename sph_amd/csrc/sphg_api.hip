@@ -342,7 +342,12 @@ void fill_coef(KCoef& k, double h) {
     k.co2 = (float)(h6 / 64.0);
     k.hlo2 = (float)(h * h * (1.0 - 1e-4));
     k.hhi2 = (float)(h * h * (1.0 + 1e-4));
+    k.hsk2 = 0.f;
 }
+
+// PCISPH shell width as a fraction of h: the shell list covers predicted
+// displacements up to kPciSkin h / 2 (12.5 m/s at h = 0.02, dt = 2e-4)
+constexpr double kPciSkin = 0.25;
 
 constexpr int kXcds = 8;
 
@@ -820,7 +825,7 @@ struct Plan {
     size_t npad = 0, total = 0;
     size_t oX[2], oV[2], oK[2] = {0, 0}, oKV[2] = {0, 0}, oID[2];
     size_t oA, oXK, oGB, oAUX, oXS, oPACC, oGBS, oNRM, oADH, oRHO, oALPHA, oDRHOS, oDIV, oP;
-    size_t oCELLU, oCELL, oPERM, oCC, oCS, oCCS, oNBR, oNNB, oNG, oGC, oXB, oBCS, oPART, oSS, oTMP;
+    size_t oCELLU, oCELL, oPERM, oCC, oCS, oCCS, oNBR, oNNB, oNSH, oNG, oGC, oXB, oBCS, oPART, oSS, oTMP;
     size_t oEMIT = 0, oGROUP = 0;
     size_t oBN = 0, oBSELF = 0, oBDENS = 0, oBPRES = 0, oBPR = 0, oPBN[2] = {0, 0}, oPV = 0, oPRS = 0, oPVS = 0,
            oX0S = 0, oOCC = 0;
@@ -914,6 +919,7 @@ int make_plan(const sphg_scene* s, Plan& P) {
     P.oCCS = A.reserve(sizeof(int) * (size_t)nCellChunks);
     P.oNBR = A.reserve(sizeof(uint32_t) * npad * (size_t)maxNbr);
     P.oNNB = A.reserve(sizeof(uint16_t) * npad);
+    P.oNSH = (method == PCISPH) ? A.reserve(sizeof(uint16_t) * npad) : 0;
     P.oNG = A.reserve(npad);
     P.oGC = A.reserve(sizeof(uint16_t) * (npad / kGroup));
     P.oXB = A.reserve(sizeof(float4) * std::max<size_t>(nb, 1));
@@ -1216,6 +1222,14 @@ int sphg_scene_init(sphg_scene* s) {
     D.pbOn = d.boundary_method == SPHG_BOUNDARY_PCISPH ? 1 : 0;
     D.pbNf = d.pb_normal_fct;
     D.pbTf = d.pb_tangential_fct;
+    // PCISPH predicted densities from the list and a shell list instead of
+    // the 27-cell walk (not with the PCISPH boundary method, whose collision
+    // projection also moves the predicted positions)
+    if (method == PCISPH && !D.pbOn) {
+        const double hs = h * (1.0 + kPciSkin);
+        D.kc.hsk2 = (float)(hs * hs * (1.0 + 1e-4));
+        D.pciSkinHalf2 = (0.5 * kPciSkin * h) * (0.5 * kPciSkin * h);
+    }
     // the viscosity forces' boundary terms exist for Akinci boundaries only
     // (src/XSPHViscosity.h:58, src/StandardViscosityForce.h:62, src/ArtificialViscosity.h:66)
     if (D.pbOn) D.nub = 0.f;
@@ -1256,6 +1270,7 @@ int sphg_scene_init(sphg_scene* s) {
     D.cellChunkSum = (int*)(b + P.oCCS);
     D.nbr = (uint32_t*)(b + P.oNBR);
     D.nnb = (uint16_t*)(b + P.oNNB);
+    D.nsh = P.oNSH ? (uint16_t*)(b + P.oNSH) : nullptr;
     D.ngrad = (uint8_t*)(b + P.oNG);
     D.gcnt = (uint16_t*)(b + P.oGC);
     D.xb = F4(P.oXB);

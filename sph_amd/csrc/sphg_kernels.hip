@@ -345,6 +345,13 @@ __device__ inline void walk27x4(const int* __restrict__ start, int c, int nx, in
     }
 }
 
+// PCISPH: a predicted displacement longer than half the shell's skin lets
+// two particles outside each other's list and shell come within h, so the
+// step's predicted densities take the 27-cell walk.
+__device__ inline void shell_guard(const DevScene& S, double dx, double dy, double dz) {
+    if (S.kc.hsk2 > 0.f && dx * dx + dy * dy + dz * dz > S.pciSkinHalf2) atomicOr(&S.ss->shellBad, 1u);
+}
+
 // Poly6 W (src/Poly6.h:31-42) of a squared distance.
 __device__ inline double P6_d(double r2, const KCoefD& k) {
     const double d = k.h2 - r2;
@@ -699,6 +706,12 @@ __global__ __launch_bounds__(kBlock) void k_build(const DevScene* __restrict__ T
     // scene are far below 2^32 elements)
     uint32_t* const Lg = S.nbr;
     const uint32_t lbase = lane_base(S, i);
+    // PCISPH: the shell, candidates within h + skin that are not in the
+    // list, fills the slots from the last one down; it stands in for the
+    // 27-cell walk of the predicted densities while it fits beside the list
+    const float hsk = S.kc.hsk2;
+    int ns = 0;
+    bool shellLost = false;
     int cnt = 0, ngr = 0;
     double ws = 0.0, sgx = 0.0, sgy = 0.0, sgz = 0.0, ssq = 0.0;
     // DFSPH: sum_j g_ij (v_i - v_j).(x_i - x_j) of the divergence error
@@ -706,14 +719,28 @@ __global__ __launch_bounds__(kBlock) void k_build(const DevScene* __restrict__ T
     const float4 vi = (M == DFSPH && valid) ? ld4(S.v, i) : make_float4(0.f, 0.f, 0.f, 0.f);
     const double ux = vi.x, uy = vi.y, uz = vi.z;
     // FP64 test and sums of one candidate that passed the FP32 test
+    int nc = 0;   // FP32 survivors of pass 1 (slots [0, nc) while pass 2 reads them)
+    // a survivor the FP64 test rejects goes to the shell (PCISPH)
+    auto toShell = [&](uint32_t j) {
+        if (M != PCISPH) return;
+        if (ns < maxN - nc) stg(L, (size_t)slot_of(maxN - 1 - ns), j);
+        else shellLost = true;
+        ++ns;
+    };
     auto visit = [&](uint32_t j, const float4& xj, const float4& vj) {
         const Pair p = pair_of(xx, xy, xz, xj, k);
-        if (!(p.r2 <= k.hhi2)) return;
+        if (!(p.r2 <= k.hhi2)) {
+            toShell(j);
+            return;
+        }
         const double g = G_d(p, k);
         bool gnz = (g != 0.0);
         if (p.r2 >= k.hlo2 || fabs(p.q - 0.6666666666666666) < 1e-9 || p.r2 < 1e-9) {
             const Exact e = exact_q(xi, xj, S.hD);
-            if (!(e.q <= 1.0)) return;
+            if (!(e.q <= 1.0)) {
+                toShell(j);
+                return;
+            }
             gnz = grad_nonzero64(e);
         }
         if (cnt < maxN) stg(L, (size_t)slot_of(cnt), j);
@@ -735,13 +762,15 @@ __global__ __launch_bounds__(kBlock) void k_build(const DevScene* __restrict__ T
             float3 q[4];
 #pragma unroll
             for (int u = 0; u < 4; ++u) q[u] = fetch(min(j0 + u, e0 - 1), r);
-            bool ok[4];
+            bool ok[4], sh[4];
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
                 const int j = j0 + u;
                 const float fx = xi.x - q[u].x, fy = xi.y - q[u].y, fz = xi.z - q[u].z;
+                const float d2 = fx * fx + fy * fy + fz * fz;
                 // evaluated without short-circuit: one exec-mask section
-                ok[u] = (j < e0) & (j != i) & (fx * fx + fy * fy + fz * fz <= hh);
+                ok[u] = (j < e0) & (j != i) & (d2 <= hh);
+                sh[u] = (M == PCISPH) & (j < e0) & (j != i) & (d2 > hh) & (d2 <= hsk);
             }
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
@@ -750,6 +779,13 @@ __global__ __launch_bounds__(kBlock) void k_build(const DevScene* __restrict__ T
                     // walk, which rewrites its list: the slot is clamped
                     stg(Lg, slot_of(min(nc, maxN - 1)) + lbase, (uint32_t)(j0 + u));
                     ++nc;
+                }
+                if (M == PCISPH && sh[u]) {
+                    // below the list's slots only (list writes that later
+                    // reach the shell are caught by nc + ns > maxN)
+                    if (nc + ns < maxN) stg(Lg, slot_of(maxN - 1 - ns) + lbase, (uint32_t)(j0 + u));
+                    else shellLost = true;
+                    ++ns;
                 }
             }
         }
@@ -764,7 +800,6 @@ __global__ __launch_bounds__(kBlock) void k_build(const DevScene* __restrict__ T
         return make_float3(p.x, p.y, p.z);
     };
     if (valid) {
-        int nc = 0;
 #pragma unroll
         for (int r = 0; r < 9; ++r) {
             const int off = (r % 3 - 1) * nx + (r / 3 - 1) * nxny;
@@ -772,6 +807,7 @@ __global__ __launch_bounds__(kBlock) void k_build(const DevScene* __restrict__ T
             if (r < nStaged) scanRow(r, s0, e0, nc, ldsFetch);
             else scanRow(r, s0, e0, nc, gblFetch);
         }
+        if (M == PCISPH && nc + ns > maxN) shellLost = true;
         if (nc <= maxN) {
             // 4 survivors per pass, their list entries and positions loaded
             // first; visit writes slots <= the one it reads, so entries
@@ -793,6 +829,8 @@ __global__ __launch_bounds__(kBlock) void k_build(const DevScene* __restrict__ T
                     if (q0 + u < nc) visit(jj[u], xj[u], vj[u]);
             }
         } else {
+            shellLost = true;
+            ns = maxN;   // toShell writes nothing from here on
             walk27(S.cellStart, ci, nx, nxny, [&](int j) {
                 const float4 xj = ld4(S.x, j);
                 const float fx = xi.x - xj.x, fy = xi.y - xj.y, fz = xi.z - xj.z;
@@ -838,7 +876,8 @@ __global__ __launch_bounds__(kBlock) void k_build(const DevScene* __restrict__ T
             az += sa * p.dz;
         }
     };
-    const int room = maxN - own;
+    // the boundary survivors use the free slots between the list and the shell
+    const int room = max(0, maxN - own - ((M == PCISPH && !shellLost) ? ns : 0));
     int nbc = 0;
     walk27(S.bCellStart, ci, nx, nxny, [&](int b) {
         const float4 y = ld4(S.xb, b);
@@ -868,6 +907,10 @@ __global__ __launch_bounds__(kBlock) void k_build(const DevScene* __restrict__ T
     // pad the list with the particle itself up to the group's longest list
     // (overwrites the boundary survivors' slots)
     for (int q = own; q < pad; ++q) stg(L, (size_t)slot_of(q), (uint32_t)i);
+    if (M == PCISPH) {
+        if (pad + ns > maxN) shellLost = true;   // the padding reached the shell
+        stg(S.nsh, (size_t)i, (uint16_t)(shellLost ? 0xFFFF : ns));
+    }
     if ((i & 63) == 0) stg(S.gcnt, (size_t)(i >> 6), (uint16_t)pad);
     if (cnt > maxN) atomicMax(&S.ss->overflow, (unsigned)cnt);
     stg(S.nnb, (size_t)i, (uint16_t)own);
@@ -1120,6 +1163,7 @@ __global__ __launch_bounds__(kBlock) void k_forces(const DevScene* __restrict__ 
         const double wz = vz0 + (az + (double)S.gz) * dt;
         S.xs[i] = make_float4((float)(wx * dt), (float)(wy * dt), (float)(wz * dt), 0.f);
         if (S.pbOn) S.pv[i] = make_float4((float)wx, (float)wy, (float)wz, 0.f);
+        shell_guard(S, wx * dt, wy * dt, wz * dt);
     }
 }
 
@@ -1137,6 +1181,7 @@ __global__ __launch_bounds__(kBlock) void k_step_begin(const DevScene* __restric
     if (c.begin == 0 && threadIdx.x == 0) {
         Scalars* ss = S.ss;
         ss->steps += 1u;
+        ss->shellBad = 0u;
         if (S.method == PCISPH) {
             ss->it = 0;
             const bool run = (ss->maxDensErr > S.eta || 0 < S.minIt) && 0 < S.maxIt;
@@ -1361,6 +1406,7 @@ __global__ __launch_bounds__(kBlock) void k_pci_density(const DevScene* __restri
     __shared__ double sh[4];
     const int i = c.begin + threadIdx.x;
     const KCoefD k = S.kd;
+    const bool useShell = S.kc.hsk2 > 0.f && ss->shellBad == 0u;
     double e = 0.0;
     if (i < S.n) {
         // x* = x + d with d = v* dt held separately: x*_i - x*_j =
@@ -1372,14 +1418,51 @@ __global__ __launch_bounds__(kBlock) void k_pci_density(const DevScene* __restri
         const int ci = S.cell[i];
         const int nx = S.nx, nxny = S.nx * S.ny;
         double ws = 0.0;
-        walk27x4(S.cellStart, ci, nx, nxny, [&](int j) { XV r; r.x = ld4(S.x, j); r.v = ld4(S.xs, j); return r; },
-                 [&](int, const XV& q) {
-                     const double dx = (px - (double)q.x.x) + (ex - (double)q.v.x);
-                     const double dy = (py - (double)q.x.y) + (ey - (double)q.v.y);
-                     const double dz = (pz - (double)q.x.z) + (ez - (double)q.v.z);
-                     const double r2 = dx * dx + dy * dy + dz * dz;
-                     if (r2 <= k.hhi2) ws += W_d(sqrt(r2) * k.invh, k);
-                 });
+        auto term = [&](const XV& q) {
+            const double dx = (px - (double)q.x.x) + (ex - (double)q.v.x);
+            const double dy = (py - (double)q.x.y) + (ey - (double)q.v.y);
+            const double dz = (pz - (double)q.x.z) + (ez - (double)q.v.z);
+            const double r2 = dx * dx + dy * dy + dz * dz;
+            if (r2 <= k.hhi2) ws += W_d(sqrt(r2) * k.invh, k);
+        };
+        auto xv = [&](uint32_t j) { XV r; r.x = ld4o(S.x, j); r.v = ld4o(S.xs, j); return r; };
+        const int nsi = useShell ? (int)S.nsh[i] : 0xFFFF;
+        if (nsi != 0xFFFF) {
+            // every candidate of the 27 cells that can come within h at x*
+            // is the particle itself, a list member (|x_i - x_j| <= h) or a
+            // shell member (<= h + skin): the same terms in another order
+            ws = W_d(0.0, k);
+            const uint32_t* L = list_of(S, i);
+            const int own = S.nnb[i];
+            for (int q0 = 0; q0 < own; q0 += 4) {
+                const v4u b = ld_block(L, q0 >> 2);
+                const uint32_t jj[4] = {b.x, q0 + 1 < own ? b.y : b.x, q0 + 2 < own ? b.z : b.x,
+                                        q0 + 3 < own ? b.w : b.x};
+                XV d[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) d[u] = xv(jj[u]);
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+                    if (q0 + u < own) term(d[u]);
+            }
+            // shell member s is in slot maxN - 1 - s: block maxN/4 - 1 - s/4,
+            // component 3 - s%4
+            const int maxN = S.maxNbr;
+            for (int s0 = 0; s0 < nsi; s0 += 4) {
+                const v4u b = ld_block(L, (maxN >> 2) - 1 - (s0 >> 2));
+                const uint32_t jj[4] = {b.w, s0 + 1 < nsi ? b.z : b.w, s0 + 2 < nsi ? b.y : b.w,
+                                        s0 + 3 < nsi ? b.x : b.w};
+                XV d[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) d[u] = xv(jj[u]);
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+                    if (s0 + u < nsi) term(d[u]);
+            }
+        } else {
+            walk27x4(S.cellStart, ci, nx, nxny, [&](int j) { return xv((uint32_t)j); },
+                     [&](int, const XV& q) { term(q); });
+        }
         double bw = 0.0;
         walk27x4(S.bCellStart, ci, nx, nxny, [&](int b) { return ld4(S.xb, b); },
                  [&](int, const float4& y) {
@@ -1451,6 +1534,7 @@ __global__ __launch_bounds__(kBlock) void k_pci_pacc(const DevScene* __restrict_
     const double wz = (double)v.z + (((double)a.z + az) + (double)S.gz) * dt;
     S.xs[i] = make_float4((float)(wx * dt), (float)(wy * dt), (float)(wz * dt), 0.f);
     if (S.pbOn) S.pv[i] = make_float4((float)wx, (float)wy, (float)wz, 0.f);
+    shell_guard(S, wx * dt, wy * dt, wz * dt);
 }
 
 // PCISPH integrate (src/PCISPHSolver.cpp:444-474): a += g + pacc.
