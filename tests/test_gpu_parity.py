@@ -197,6 +197,47 @@ def test_pcisph_surface_tension_adhesion(ctx):
     g.close()
 
 
+@pytest.mark.parametrize("case", ["shell", "shell_lost", "displacement_guard"])
+def test_pcisph_predicted_density_paths(ctx, case):
+    """PCISPH predicted densities take the candidates of the 27 cells of x
+    (src/PCISPHSolver.cpp:164-212).  The HIP path sums them from the list
+    plus a shell list; a particle whose shell does not fit beside its list
+    (max_neighbors 48) walks the cells, and so does a step whose predicted
+    displacements exceed half the shell's skin (20 m/s at dt 1e-3).  Every
+    path matches the oracle's walk."""
+    from dataclasses import replace
+    spec = configs.small_scene(sphg.PCISPH, n_side=10, seed=13, surface_tension_method=sphg.SURFACE_TENSION_AKINCI,
+                               surface_tension=0.05)
+    params, vel = spec.params, None
+    if case == "shell_lost":
+        params = replace(params, max_neighbors=48)
+    if case == "displacement_guard":
+        params = replace(params, time_step=1e-3, min_time_step=1e-3, max_time_step=1e-3)
+        vel = np.tile(np.array([[20.0, 0.0, 0.0]], np.float32), (spec.n, 1))
+    g = sphg.Scene(ctx, params, spec.fluid, spec.boundary, vel)
+    o = po.Oracle(params, spec.fluid, spec.boundary, velocities=vel)
+    matched = 0
+    for step in range(4):
+        o.set_state(g.get(sphg.FIELD_X), g.get(sphg.FIELD_V), None, None, params.time_step, 0)
+        g.step(1)
+        o.step(1)
+        sg, so = g.scalars(), o.scalars()
+        assert abs(sg.iterations - int(so["iterations"])) <= 1
+        if sg.iterations != int(so["iterations"]):
+            continue
+        matched += 1
+        errs = {
+            "rho": rel_elem(g.get(sphg.FIELD_RHO), o.get(po.OF_RHO)),
+            "p": rel_norm(g.get(sphg.FIELD_P), o.get(po.OF_P)),
+            "v": rel_norm(g.get(sphg.FIELD_V), o.get(po.OF_V)),
+            "x": rel_norm(g.get(sphg.FIELD_X), o.get(po.OF_X)),
+        }
+        bad = {k: v for k, v in errs.items() if not v < TOL}
+        assert not bad, (case, step, errs)
+    assert matched >= 3
+    g.close()
+
+
 def test_batch_alone_vs_mix(ctx):
     """Scenes stepped together produce exactly what they produce alone."""
     specs = [configs.small_scene(sphg.DFSPH, n_side=n, seed=s) for n, s in ((6, 1), (11, 2), (8, 3))]
