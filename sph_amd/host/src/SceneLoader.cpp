@@ -269,7 +269,7 @@ bool Simulation::importScene(const std::string& path) {
     SimulationInfo info;
     if (!SceneLoader::readConfiguration(info, path)) throw SphError("cannot read " + path);
     const SceneInfo& s = info.sceneData;
-    if (s.device != "gpu") throw SphError("Configuration.device '" + s.device + "': this build runs on the GPU only");
+    if (s.device != "gpu") throw SphError("Configuration.device '" + s.device + "': sph_amd runs on the GPU only");
     setTimeStep(s.timeStep);
     setFPS(s.fps);
     setMinTimeStep(s.minTimeStep);
