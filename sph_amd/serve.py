@@ -15,7 +15,11 @@ Continuous batching (`serve_batched`):
     (`sphg.Batch`), in quanta of at most `quantum` steps, so that requests
     that arrive meanwhile join at the next quantum;
   * a scene that has done its steps is read back, freed and its request
-    completed; the rest continue.
+    completed; the rest continue;
+  * with `parallel_methods`, the three methods' batches run their quantum at
+    the same time, each on its own context (HIP stream) driven by its own
+    host thread: a launch-bound batch of small scenes then overlaps the
+    others instead of waiting for them.
   The scene geometry of an arriving request is built on a host thread while
   the GPU steps the resident scenes.
 
@@ -124,13 +128,24 @@ def build_spec(req: Request, spacing: float = 0.02) -> SceneSpec:
 # --------------------------------------------------------------- backends
 
 class GpuBackend:
-    """The HIP solver library on one device."""
+    """The HIP solver library on one device.  With `method_contexts` each
+    solver method gets a context (stream) of its own, so that the batches of
+    different methods can run concurrently."""
 
-    def __init__(self, ctx: sphg.Context):
+    def __init__(self, ctx: sphg.Context, method_contexts: bool = False):
         self.ctx = ctx
+        self.extra = []
+        self.ctxs = {}
+        for m in METHODS:
+            if method_contexts and self.ctxs:
+                c = sphg.Context(ctx.device)
+                self.extra.append(c)
+                self.ctxs[m] = c
+            else:
+                self.ctxs[m] = ctx
 
     def create(self, spec: SceneSpec):
-        return sphg.Scene(self.ctx, spec.params, spec.fluid, spec.boundary, init=False)
+        return sphg.Scene(self.ctxs[spec.params.method], spec.params, spec.fluid, spec.boundary, init=False)
 
     def plan_bytes(self, scene) -> int:
         return scene.plan_bytes()
@@ -139,7 +154,7 @@ class GpuBackend:
         scene.init()
 
     def batch(self, scenes):
-        return sphg.Batch(self.ctx, scenes)
+        return sphg.Batch(scenes[0].ctx, scenes)
 
     def step(self, scene, k: int) -> None:
         scene.step(k)
@@ -149,6 +164,12 @@ class GpuBackend:
 
     def close(self, obj) -> None:
         obj.close()
+
+    def shutdown(self) -> None:
+        """Closes the contexts this backend created."""
+        for c in self.extra:
+            c.close()
+        self.extra = []
 
 
 @dataclass
@@ -189,8 +210,11 @@ def _spec_pool():
 
 def serve_batched(backend, requests: List[Request], budget_bytes: int, quantum: int = 25,
                   clock: Callable[[], float] = time.perf_counter, sleep: Callable[[float], None] = time.sleep,
-                  keep_results: bool = True, log: Callable[[str], None] = None) -> ServeResult:
-    """Continuous batching of `requests` (sorted by arrival) on `backend`."""
+                  keep_results: bool = True, log: Callable[[str], None] = None,
+                  parallel_methods: bool = False) -> ServeResult:
+    """Continuous batching of `requests` (sorted by arrival) on `backend`.
+    With `parallel_methods` the methods' batches run each quantum on their
+    own host threads (the backend gives each method its own stream)."""
     pending = deque(sorted(requests, key=lambda q: q.arrival))
     waiting: deque = deque()
     active: Dict[int, List[Job]] = {m: [] for m in METHODS}
@@ -199,6 +223,7 @@ def serve_batched(backend, requests: List[Request], budget_bytes: int, quantum: 
     rejected: List[int] = []
     used = 0
     pool = _spec_pool()
+    runner = ThreadPoolExecutor(max_workers=len(METHODS), thread_name_prefix="batch") if parallel_methods else None
     t0 = clock()
     try:
         while pending or waiting or any(active.values()):
@@ -238,14 +263,26 @@ def serve_batched(backend, requests: List[Request], budget_bytes: int, quantum: 
                 if pending:
                     sleep(max(0.0, pending[0].arrival - (clock() - t0)))
                 continue
+            passes = []
             for m in METHODS:
                 js = active[m]
                 if not js:
                     continue
                 if batches[m] is None:
                     batches[m] = backend.batch([j.scene for j in js])
-                k = min(quantum, min(j.remaining for j in js))
-                batches[m].run(k)
+                passes.append((m, js, min(quantum, min(j.remaining for j in js))))
+            if runner is not None and len(passes) > 1:
+                futs = [runner.submit(batches[m].run, k) for m, _, k in passes]
+                # every pass ends before any error is raised (the batches
+                # are closed on the way out)
+                errs = [f.exception() for f in futs]
+                for e in errs:
+                    if e is not None:
+                        raise e
+            else:
+                for m, _, k in passes:
+                    batches[m].run(k)
+            for m, js, k in passes:
                 for j in js:
                     j.remaining -= k
                 fin = [j for j in js if j.remaining == 0]
@@ -275,6 +312,8 @@ def serve_batched(backend, requests: List[Request], budget_bytes: int, quantum: 
             if j.scene is not None:
                 backend.close(j.scene)
         pool.shutdown(wait=True)
+        if runner is not None:
+            runner.shutdown(wait=True)
     done = [j.done for j in jobs if j.done >= 0]
     first = min(q.arrival for q in requests) if requests else 0.0
     return ServeResult("batched", jobs, (max(done) - first) if done else 0.0, rejected)
@@ -340,14 +379,16 @@ def main():
     ap.add_argument("--quantum", type=int, default=25)
     ap.add_argument("--seed", type=int, default=5)
     ap.add_argument("--check", type=int, default=4, help="requests whose batched and plain results are compared")
+    ap.add_argument("--serial-methods", action="store_true",
+                    help="step the methods' batches one after another on one stream")
     args = ap.parse_args()
 
     reqs = request_stream(args.requests, args.rate, args.nmin, args.nmax, args.smin, args.smax, args.seed)
     ctx = sphg.Context(0)
-    be = GpuBackend(ctx)
+    be = GpuBackend(ctx, method_contexts=not args.serial_methods)
     budget = int(args.budget_gb * 2 ** 30)
     log = lambda s: print(s, flush=True)   # noqa: E731
-    rb = serve_batched(be, reqs, budget, args.quantum, log=log)
+    rb = serve_batched(be, reqs, budget, args.quantum, log=log, parallel_methods=not args.serial_methods)
     rp = serve_plain(be, reqs, budget, log=log)
     sb, sp = rb.summary(), rp.summary()
     # the first `check` requests: identical results alone and in the mix
@@ -363,11 +404,13 @@ def main():
            "stream": {"requests": args.requests, "rate_per_s": args.rate, "n": [args.nmin, args.nmax],
                       "steps": [args.smin, args.smax], "methods": "WCSPH/PCISPH/DFSPH uniform",
                       "geometry": "dam-break block or sphere drop, spacing 0.02", "seed": args.seed,
-                      "budget_gb": args.budget_gb, "quantum": args.quantum},
+                      "budget_gb": args.budget_gb, "quantum": args.quantum,
+                      "methods_in_parallel": not args.serial_methods},
            "batched": sb, "plain": sp,
            "throughput_ratio": round(sb["particle_steps_per_s"] / sp["particle_steps_per_s"], 3)
            if sb["particle_steps_per_s"] and sp["particle_steps_per_s"] else None,
            "identical_results": f"{same}/{checked}"}
+    be.shutdown()
     ctx.close()
     print(json.dumps(out), flush=True)
 

@@ -387,6 +387,7 @@ class Context:
 
     def __init__(self, device: int = 0):
         self.lib = load()
+        self.device = int(device)
         h = C.c_void_p()
         _check(self.lib.sphg_ctx_create(int(device), C.byref(h)))
         self.h = h

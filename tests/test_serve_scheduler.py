@@ -29,13 +29,14 @@ class FakeBatch:
         self.be, self.scenes, self.closed = be, scenes, False
 
     def run(self, k):
-        assert not self.closed
-        assert all(s.ready and not s.closed for s in self.scenes)
-        self.be.quanta.append((len(self.scenes), k))
-        for s in self.scenes:
-            s.steps += k
-        # one launch sequence for the whole batch: fixed cost + work
-        self.be.clock.t += k * (2e-4 + 1e-9 * sum(s.n for s in self.scenes))
+        with self.be.lock:      # batches of different methods may run on threads
+            assert not self.closed
+            assert all(s.ready and not s.closed for s in self.scenes)
+            self.be.quanta.append((len(self.scenes), k))
+            for s in self.scenes:
+                s.steps += k
+            # one launch sequence for the whole batch: fixed cost + work
+            self.be.clock.t += k * (2e-4 + 1e-9 * sum(s.n for s in self.scenes))
 
     def close(self):
         self.closed = True
@@ -45,6 +46,8 @@ class FakeBackend:
     BYTES_PER_PARTICLE = 600
 
     def __init__(self, clock):
+        import threading
+        self.lock = threading.Lock()
         self.clock = clock
         self.quanta = []
         self.used = 0
@@ -126,6 +129,24 @@ def test_batched_exact_steps_budget_fifo_quantum():
     assert order == sorted(order)
     # scenes actually shared launches
     assert max(n for n, _ in be.quanta) > 1
+
+
+def test_batched_methods_in_parallel_same_steps_and_budget():
+    """The methods' batches run their quanta on threads: every request still
+    gets exactly its steps, admission stays within the budget and in arrival
+    order, and everything is freed."""
+    clk = Clock()
+    be = FakeBackend(clk)
+    reqs = _stream()
+    budget = 3 * 4096 * FakeBackend.BYTES_PER_PARTICLE
+    res = serve.serve_batched(be, reqs, budget, quantum=7, clock=clk, sleep=clk.sleep, parallel_methods=True)
+    assert not res.rejected and len(res.jobs) == len(reqs)
+    for j in res.jobs:
+        assert int(j.result[0]) == j.req.steps
+        assert j.admitted >= j.req.arrival - 1e-12
+    assert be.peak <= budget and be.used == 0
+    order = [j.req.rid for j in sorted(res.jobs, key=lambda j: j.admitted)]
+    assert order == sorted(order)
 
 
 def test_batched_rejects_what_never_fits():
