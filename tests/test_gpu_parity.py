@@ -238,6 +238,49 @@ def test_pcisph_predicted_density_paths(ctx, case):
     g.close()
 
 
+@pytest.mark.parametrize("method", [sphg.DFSPH, sphg.WCSPH])
+def test_coincident_particles_match_oracle(ctx, method):
+    """Particles at exactly the same position: each is the other's neighbor
+    at r = 0, where W counts and every gradient term is zero
+    (src/CubicSpline.h:58).  Six duplicated particles step as in the oracle.
+    (PCISPH is left out: its loop runs on the largest density error, which
+    two coincident particles keep high because they exert no pressure on
+    each other, so its iteration count runs to ~80 and is not comparable.)"""
+    spec = configs.small_scene(method, n_side=7, seed=17)
+    p = spec.params
+    fluid = np.concatenate([spec.fluid, spec.fluid[[3, 50, 101, 170, 222, 300]]]).astype(np.float32)
+    g = sphg.Scene(ctx, p, fluid, spec.boundary)
+    o = po.Oracle(p, fluid, spec.boundary)
+    matched = 0
+    for step in range(4):
+        sc = g.scalars()
+        dfsph = method == sphg.DFSPH
+        o.set_state(g.get(sphg.FIELD_X), g.get(sphg.FIELD_V), g.get(sphg.FIELD_K) if dfsph else None,
+                    g.get(sphg.FIELD_KV) if dfsph else None, sc.time_step, sc.iterations if dfsph else 0)
+        g.step(1)
+        o.step(1)
+        sg, so = g.scalars(), o.scalars()
+        assert abs(sg.iterations - int(so["iterations"])) <= 1
+        if sg.iterations != int(so["iterations"]) or (dfsph and sg.iterations_v != int(so["iterations_v"])):
+            continue
+        matched += 1
+        for f, of in ((sphg.FIELD_V, po.OF_V), (sphg.FIELD_X, po.OF_X)):
+            err = rel_norm(g.get(f), o.get(of))
+            assert err < TOL, (method, step, f, err)
+        assert np.all(np.isfinite(g.get(sphg.FIELD_V)))
+    assert matched >= 2
+    g.close()
+
+
+def test_empty_fluid_is_rejected(ctx):
+    """A scene without fluid particles and without emitters is refused at
+    initialisation with SPHG_ESTATE and a message, not run."""
+    spec = configs.small_scene(sphg.DFSPH, n_side=4)
+    with pytest.raises(sphg.SphgError) as e:
+        sphg.Scene(ctx, spec.params, np.zeros((0, 3), np.float32), spec.boundary)
+    assert e.value.code == sphg.ESTATE and "no fluid particles" in str(e.value)
+
+
 def test_batch_alone_vs_mix(ctx):
     """Scenes stepped together produce exactly what they produce alone."""
     specs = [configs.small_scene(sphg.DFSPH, n_side=n, seed=s) for n, s in ((6, 1), (11, 2), (8, 3))]
