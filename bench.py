@@ -23,7 +23,9 @@ sweeps are also launched ahead for converged scenes, which return at once:
 their time is charged to the working launches; `avg_launch_us` is the plain
 average over all launches, the figure rocprofv3 reports).  `roofline.traffic` is the HBM
 traffic per launch of that kernel from the rocprofv3 FETCH_SIZE/WRITE_SIZE
-summary in profiles/ (null when that summary is for another kernel).
+summary in profiles/ (null when that summary is for another kernel), and
+`roofline.traffic_frac` that traffic over the same launch duration against
+the peak.
 cpu_baseline times the FP64 OpenMP restatement of the reference
 (oracle/liboracle_fast.so) on the host cores, stepping from the GPU's state
 at the end of the timed window.
@@ -170,8 +172,11 @@ def main():
         n_useful = useful.get(dom, kernels[dom]["launches"])
         avg_s = kernels[dom]["total_ms"] * 1e-3 / n_useful
         achieved = bytes_launch / avg_s / 1e9
+        traffic = traffic_for(dom)
         roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_for(dom), "kernel": dom,
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": dom,
+                # the measured HBM traffic per launch over the same duration
+                "traffic_frac": round(traffic / avg_s / 1e9 / HBM_PEAK_GBS, 4) if traffic else None,
                 "bytes_per_launch": bytes_launch, "avg_launch_us": kernels[dom]["avg_us"],
                 "useful_launches": round(n_useful), "avg_useful_launch_us": round(avg_s * 1e6, 3)}
         print(f"[bench] kernels: {json.dumps(kernels)}", file=sys.stderr, flush=True)
