@@ -49,6 +49,23 @@ def test_missing_required_key_is_an_error(tmp_path):
     assert p.returncode == 1 and "etaV" in p.stderr
 
 
+def test_unknown_keys_are_ignored(tmp_path):
+    """As in the reference's nlohmann-based loader, keys it does not know are
+    ignored at every level, so a scene written for another front end (or
+    with this one's additive keys) still parses."""
+    def edit(d):
+        d["Unknown"] = {"x": 1}
+        d["Configuration"]["someFutureKey"] = [1, 2, 3]
+        d["Fluid"]["comment"] = "ignored"
+        d["Boundary"][0]["note"] = {"nested": True}
+    path = _variant(tmp_path, edit)
+    p = _run("-f", path, "--info")
+    assert p.returncode == 0, p.stderr
+    info = json.loads(p.stdout)
+    ref = json.loads(_run("-f", os.path.join(SCENES, "dam_break_dfsph_small.json"), "--info").stdout)
+    assert info == ref
+
+
 def test_wrong_type_is_an_error(tmp_path):
     path = _variant(tmp_path, lambda d: d["Configuration"].__setitem__("particleRadius", "big"))
     p = _run("-f", path, "--info")
