@@ -176,7 +176,10 @@ int sphg_ctx_graphs(sphg_ctx* ctx, int32_t enable);
  * setters (src/Simulation.cpp:207-231, 342-380). */
 int sphg_scene_create(sphg_ctx* ctx, const sphg_scene_desc* desc, sphg_scene** out);
 int sphg_scene_destroy(sphg_scene* scene);
-/* FluidModel::init(points, velocities) (src/FluidModel.cpp:37-49); vel may be NULL. */
+/* FluidModel::init(points, velocities) (src/FluidModel.cpp:37-49); vel may be NULL.
+ * A scene holds at most 2^28 particle slots (fluid plus emitter budgets),
+ * and slots x max_neighbors must stay below 2^32 (44.7M slots at the
+ * default 96): sphg_scene_init / plan_bytes refuse larger ones. */
 int sphg_scene_set_fluid(sphg_scene* scene, const float* xyz, const float* vel, uint32_t n);
 /* BoundaryModel::init / AkinciBoundaryModel::addCube... (src/BoundaryModel.cpp:3-9). */
 int sphg_scene_set_boundary(sphg_scene* scene, const float* xyz, uint32_t nb);

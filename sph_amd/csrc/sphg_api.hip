@@ -874,6 +874,13 @@ int make_plan(const sphg_scene* s, Plan& P) {
     const int nCellChunks = (int)((ncells + 1 + kCellChunk - 1) / kCellChunk);
     // a multiple of 4: the lists are padded to multiples of 4 per group
     const int maxNbr = ((d.max_neighbors ? (int)d.max_neighbors : kDefaultMaxNbr) + 3) & ~3;
+    // the kernels address particle arrays with 32-bit byte offsets and the
+    // neighbor lists with 32-bit element offsets
+    if (npad > (size_t(1) << 28))
+        return fail(SPHG_EINVAL, "more than 2^28 particle slots in one scene");
+    if ((uint64_t)npad * (uint64_t)maxNbr > 0xFFFFFFFFull)
+        return fail(SPHG_EINVAL, "particle slots x max_neighbors above 2^32 - 1 (" + std::to_string(npad) + " x " +
+                                     std::to_string(maxNbr) + "): lower max_neighbors or split the scene");
 
     int flags = 0;
     // a viscosity force exists iff its coefficient is nonzero

@@ -272,6 +272,20 @@ def test_coincident_particles_match_oracle(ctx, method):
     g.close()
 
 
+def test_list_offset_limit_is_enforced(ctx):
+    """Neighbor lists are addressed with 32-bit element offsets: a scene
+    whose slots x max_neighbors exceed 2^32 - 1 is refused when its plan is
+    made, before any allocation (here 70k slots x 65532 neighbors)."""
+    from dataclasses import replace
+    spec = configs.small_scene(sphg.DFSPH, n_side=41)
+    assert spec.n * 65532 > 2 ** 32
+    g = sphg.Scene(ctx, replace(spec.params, max_neighbors=65532), spec.fluid, spec.boundary, init=False)
+    with pytest.raises(sphg.SphgError) as e:
+        g.plan_bytes()
+    assert e.value.code == sphg.EINVAL and "max_neighbors" in str(e.value)
+    g.close()
+
+
 def test_empty_fluid_is_rejected(ctx):
     """A scene without fluid particles and without emitters is refused at
     initialisation with SPHG_ESTATE and a message, not run."""
