@@ -1118,6 +1118,8 @@ int sphg_scene_set_boundary(sphg_scene* s, const float* xyz, uint32_t nb) {
     if (s->ready) return fail(SPHG_ESTATE, "set_boundary after init");
     if (nb && s->desc.boundary_method != SPHG_BOUNDARY_AKINCI && s->desc.boundary_method != SPHG_BOUNDARY_PCISPH)
         return fail(SPHG_EINVAL, "boundary particles need boundaryMethod 1 (PCISPH) or 2 (Akinci)");
+    // boundary indices share the neighbor lists and the int cell tables
+    if (nb > (1u << 28)) return fail(SPHG_EINVAL, "more than 2^28 boundary particles in one scene");
     s->bxyz.assign(xyz, xyz + 3 * (size_t)nb);
     s->nb = nb;
     s->bnrm.clear();
