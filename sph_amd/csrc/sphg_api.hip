@@ -342,12 +342,7 @@ void fill_coef(KCoef& k, double h) {
     k.co2 = (float)(h6 / 64.0);
     k.hlo2 = (float)(h * h * (1.0 - 1e-4));
     k.hhi2 = (float)(h * h * (1.0 + 1e-4));
-    k.hsk2 = 0.f;
 }
-
-// PCISPH shell width as a fraction of h: the shell list covers predicted
-// displacements up to kPciSkin h / 2 (12.5 m/s at h = 0.02, dt = 2e-4)
-constexpr double kPciSkin = 0.25;
 
 constexpr int kXcds = 8;
 
@@ -1234,11 +1229,7 @@ int sphg_scene_init(sphg_scene* s) {
     // PCISPH predicted densities from the list and a shell list instead of
     // the 27-cell walk (not with the PCISPH boundary method, whose collision
     // projection also moves the predicted positions)
-    if (method == PCISPH && !D.pbOn) {
-        const double hs = h * (1.0 + kPciSkin);
-        D.kc.hsk2 = (float)(hs * hs * (1.0 + 1e-4));
-        D.pciSkinHalf2 = (0.5 * kPciSkin * h) * (0.5 * kPciSkin * h);
-    }
+    if (method == PCISPH && !D.pbOn) D.pciShell = 1;
     // the viscosity forces' boundary terms exist for Akinci boundaries only
     // (src/XSPHViscosity.h:58, src/StandardViscosityForce.h:62, src/ArtificialViscosity.h:66)
     if (D.pbOn) D.nub = 0.f;

@@ -53,7 +53,6 @@ struct KCoef {
     float co1;     // cohesion:          32 / (pi h^9)      src/Cohesion.h:26
     float co2;     // cohesion offset:   h^6 / 64           src/Cohesion.h:27
     float hlo2, hhi2;   // h^2 (1 -/+ 1e-4): FP32 pre-filter band of the exact test
-    float hsk2;         // PCISPH: (h + skin)^2 (1 + 1e-4), the shell list's FP32 bound (0: no shell)
 };
 
 // FP64 copies used by the neighbor sums (FP32 state, FP64 arithmetic).
@@ -75,6 +74,8 @@ struct Scalars {
     unsigned steps, sumIt, sumItV, ctr, overflow;
     unsigned nActive;           // active particles (the emitters raise it)
     unsigned shellBad;          // PCISPH: this step's shell lists cannot stand in for the 27-cell walk
+    unsigned maxD2, maxD2Prev;  // PCISPH: largest squared predicted displacement (FP32 bits) of this
+                                // step so far and of the previous step, which sizes this step's shell
 };
 
 // One emitter's constants and state (src/Emitter.h:9-55).  Its group's
@@ -122,7 +123,7 @@ struct DevScene {
     // collision projection last wrote (NaN where nothing collided)
     int pbOn, pad4;
     double pbNf, pbTf;
-    double pciSkinHalf2;        // PCISPH: (skin / 2)^2, the largest predicted displacement the shell covers
+    int pciShell, pad5;         // PCISPH: predicted densities from the list and a shell list
     uint16_t *nsh;              // PCISPH: shell entries per particle (candidates within h + skin
                                 // that are not in the list, stored from the list's last slot down)
     float4 *bn;                 // boundary normals (sorted like xb)

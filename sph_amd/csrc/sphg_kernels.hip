@@ -345,11 +345,21 @@ __device__ inline void walk27x4(const int* __restrict__ start, int c, int nx, in
     }
 }
 
-// PCISPH: a predicted displacement longer than half the shell's skin lets
-// two particles outside each other's list and shell come within h, so the
-// step's predicted densities take the 27-cell walk.
+// PCISPH shell width: 2.5 times the largest predicted displacement of the
+// previous step (so a step's displacements may grow by 25%), at least
+// 0.05 h.  The same for every kernel of a step (the previous step's maximum
+// is copied at the step's start).
+__device__ inline float pci_skin(const DevScene& S) {
+    const float md = sqrtf(__uint_as_float(S.ss->maxD2Prev));
+    return fmaxf(0.05f * S.kc.h, 2.5f * md);
+}
+// A predicted displacement longer than half the skin lets two particles
+// outside each other's list and shell come within h, so the step's
+// predicted densities take the 27-cell walk.
 __device__ inline void shell_guard(const DevScene& S, double dx, double dy, double dz) {
-    if (S.kc.hsk2 > 0.f && dx * dx + dy * dy + dz * dz > S.pciSkinHalf2) atomicOr(&S.ss->shellBad, 1u);
+    if (!S.pciShell) return;
+    const double half = 0.5 * (double)pci_skin(S);
+    if (dx * dx + dy * dy + dz * dz > half * half) atomicOr(&S.ss->shellBad, 1u);
 }
 
 // Poly6 W (src/Poly6.h:31-42) of a squared distance.
@@ -709,7 +719,11 @@ __global__ __launch_bounds__(kBlock) void k_build(const DevScene* __restrict__ T
     // PCISPH: the shell, candidates within h + skin that are not in the
     // list, fills the slots from the last one down; it stands in for the
     // 27-cell walk of the predicted densities while it fits beside the list
-    const float hsk = S.kc.hsk2;
+    float hsk = 0.f;
+    if (M == PCISPH && S.pciShell) {
+        const float hs = S.kc.h + pci_skin(S);
+        hsk = hs * hs * (1.0f + 1e-4f);
+    }
     int ns = 0;
     bool shellLost = false;
     int cnt = 0, ngr = 0;
@@ -1182,6 +1196,8 @@ __global__ __launch_bounds__(kBlock) void k_step_begin(const DevScene* __restric
         Scalars* ss = S.ss;
         ss->steps += 1u;
         ss->shellBad = 0u;
+        ss->maxD2Prev = ss->maxD2;
+        ss->maxD2 = 0u;
         if (S.method == PCISPH) {
             ss->it = 0;
             const bool run = (ss->maxDensErr > S.eta || 0 < S.minIt) && 0 < S.maxIt;
@@ -1406,13 +1422,15 @@ __global__ __launch_bounds__(kBlock) void k_pci_density(const DevScene* __restri
     __shared__ double sh[4];
     const int i = c.begin + threadIdx.x;
     const KCoefD k = S.kd;
-    const bool useShell = S.kc.hsk2 > 0.f && ss->shellBad == 0u;
+    const bool useShell = S.pciShell && ss->shellBad == 0u;
     double e = 0.0;
+    double dd = 0.0;   // squared predicted displacement, for the next step's shell width
     if (i < S.n) {
         // x* = x + d with d = v* dt held separately: x*_i - x*_j =
         // (x_i - x_j) + (d_i - d_j) keeps FP64 accuracy from FP32 storage
         const float4 xi = S.x[i];
         const float4 di = S.xs[i];
+        dd = (double)(di.x * di.x + di.y * di.y + di.z * di.z);
         const double px = xi.x, py = xi.y, pz = xi.z;
         const double ex = di.x, ey = di.y, ez = di.z;
         const int ci = S.cell[i];
@@ -1482,6 +1500,11 @@ __global__ __launch_bounds__(kBlock) void k_pci_density(const DevScene* __restri
     }
     const double bm = block_max(e, sh);
     if (threadIdx.x == 0) S.partial[c.begin / kBlock] = bm;
+    if (S.pciShell) {
+        // the max is order-free: one atomic per block on non-negative FP32 bits
+        const double dm = block_max(dd, sh);
+        if (threadIdx.x == 0) atomicMax(&S.ss->maxD2, __float_as_uint((float)dm));
+    }
 }
 
 // Pressure acceleration with the Spiky gradient at the current positions and
