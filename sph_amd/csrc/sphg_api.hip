@@ -819,7 +819,7 @@ struct Plan {
     int nChunks = 0, nCellChunks = 0, maxNbr = 0, flags = 0;
     size_t npad = 0, total = 0;
     size_t oX[2], oV[2], oK[2] = {0, 0}, oKV[2] = {0, 0}, oID[2];
-    size_t oA, oXK, oGB, oAUX, oXS, oPACC, oGBS, oNRM, oADH, oRHO, oALPHA, oDRHOS, oDIV, oP;
+    size_t oVYZ, oA, oXK, oGB, oAUX, oXS, oPACC, oGBS, oNRM, oADH, oRHO, oALPHA, oDRHOS, oDIV, oP;
     size_t oCELLU, oCELL, oPERM, oCC, oCS, oCCS, oNBR, oNNB, oNSH, oNG, oGC, oXB, oBCS, oPART, oSS, oTMP;
     size_t oEMIT = 0, oGROUP = 0;
     size_t oBN = 0, oBSELF = 0, oBDENS = 0, oBPRES = 0, oBPR = 0, oPBN[2] = {0, 0}, oPV = 0, oPRS = 0, oPVS = 0,
@@ -901,6 +901,7 @@ int make_plan(const sphg_scene* s, Plan& P) {
         for (int p = 0; p < 2; ++p) { oK[p] = A.reserve(f1); oKV[p] = A.reserve(f1); }
     P.oA = A.reserve(f4);
     P.oXK = A.reserve(f4);
+    P.oVYZ = (method == DFSPH) ? A.reserve(2 * f1) : 0;
     P.oGB = A.reserve(f4);
     P.oAUX = (method != DFSPH) ? A.reserve(f4) : 0;
     P.oXS = (method == PCISPH) ? A.reserve(f4) : 0;
@@ -1250,6 +1251,7 @@ int sphg_scene_init(sphg_scene* s) {
     auto F1 = [&](size_t o) { return (float*)(b + o); };
     D.a = F4(P.oA);
     D.xk = F4(P.oXK);
+    D.vyz = P.oVYZ ? (float2*)(b + P.oVYZ) : nullptr;
     D.gb = F4(P.oGB);
     D.aux = P.oAUX ? F4(P.oAUX) : nullptr;
     D.xs = P.oXS ? F4(P.oXS) : nullptr;

@@ -9,7 +9,8 @@
 //
 // HBM layout per scene (FP32, struct-of-arrays, particles sorted by
 // (cell, id) every step):
-//   x, v          float4 (w unused)              -- double-buffered for the sort
+//   x, v          float4 (x.w: DFSPH v_x copy; v.w unused) -- double-buffered for the sort
+//   vyz           float2 (v_y, v_z) copy (DFSPH), beside x.w for the error sweeps
 //   xk            float4 (x, y, z, s)             -- s = the scalar the next
 //                 neighbor sweep gathers (warm-start k, kappa, kappa_v, p/rho^2)
 //   gb            float4 (sum_b rho0 V_b gradW_ib, sum_b rho0 V_b W_ib)
@@ -123,6 +124,8 @@ struct DevScene {
     // collision projection last wrote (NaN where nothing collided)
     int pbOn, pad4;
     double pbNf, pbTf;
+    float2 *vyz;                // DFSPH: (v_y, v_z), written by the velocity sweeps with v_x in x.w;
+                                // the error sweeps gather x and vyz, 24 B per pair
     int pciShell, pad5;         // PCISPH: predicted densities from the list and a shell list
     uint16_t *nsh;              // PCISPH: shell entries per particle (candidates within h + skin
                                 // that are not in the list, stored from the list's last slot down)

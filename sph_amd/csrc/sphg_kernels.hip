@@ -106,6 +106,11 @@ __device__ inline float4 ld4o(const float4* p, uint32_t j) {
     const v4f t = *(gptr<const v4f>)((gptr<const char>)p + (j << 4));
     return make_float4(t.x, t.y, t.z, t.w);
 }
+typedef float v2f __attribute__((ext_vector_type(2)));
+__device__ inline float2 ld2o(const float2* p, uint32_t j) {
+    const v2f t = *(gptr<const v2f>)((gptr<const char>)p + (j << 3));
+    return make_float2(t.x, t.y);
+}
 // The xyz of element j of a float4 array: a 12-byte load.  (Measured on
 // gfx950: slightly faster for the list build's survivors, slower than the
 // 16-byte load for the solver sweeps' gathers.)
@@ -283,6 +288,11 @@ __device__ inline void sweep4(const DevScene& S, int i, Load&& load, Body&& body
 
 struct XV {
     float4 x, v;
+};
+// x (with v_x in w) and (v_y, v_z) of a neighbor: 24 B, against 32 for x and v
+struct XVP {
+    float4 xv;
+    float2 yz;
 };
 struct NbF {
     float4 x, v, n;
@@ -1315,6 +1325,10 @@ __global__ __launch_bounds__(kBlock) void k_df_vel(const DevScene* __restrict__ 
     v.y = (float)((double)v.y - dt * sy);
     v.z = (float)((double)v.z - dt * sz);
     S.v[i] = v;
+    // copies for the error sweep's gathers: v_x into the unused x.w (no
+    // kernel gathers x during this sweep), (v_y, v_z) into vyz
+    ((gptr<float>)S.x)[4 * (size_t)i + 3] = v.x;
+    ((gptr<v2f>)S.vyz)[i] = v2f{v.y, v.z};
     if (MODE == 0) S.k[i] = 0.0f;
     if (MODE == 1) S.k[i] = (float)((double)S.k[i] + (double)S.drhoS[i] * (double)S.alpha[i]);
     if (MODE == 2) S.kv[i] = (float)((double)S.kv[i] + (double)S.divErr[i] * (double)S.alpha[i]);
@@ -1350,10 +1364,11 @@ __global__ __launch_bounds__(kBlock) void k_df_err(const DevScene* __restrict__ 
         const double xx = xi.x, xy = xi.y, xz = xi.z;
         const double ux = vi.x, uy = vi.y, uz = vi.z;
         double dc = 0.0;
-        sweep4(S, i, [&](uint32_t j) { XV r; r.x = ld4o(S.x, j); r.v = ld4o(S.v, j); return r; },
-               [&](uint32_t, const XV& d) {
-                   const PairG p = pairg_of(xx, xy, xz, d.x, k);
-                   dc += p.g * ((ux - (double)d.v.x) * p.dx + (uy - (double)d.v.y) * p.dy + (uz - (double)d.v.z) * p.dz);
+        // the velocity sweep before this one wrote x.w and vyz
+        sweep4(S, i, [&](uint32_t j) { XVP r; r.xv = ld4o(S.x, j); r.yz = ld2o(S.vyz, j); return r; },
+               [&](uint32_t, const XVP& d) {
+                   const PairG p = pairg_of(xx, xy, xz, d.xv, k);
+                   dc += p.g * ((ux - (double)d.xv.w) * p.dx + (uy - (double)d.yz.x) * p.dy + (uz - (double)d.yz.y) * p.dz);
                });
         const float4 gb = S.gb[i];
         dc = S.mD * dc + (ux * (double)gb.x + uy * (double)gb.y + uz * (double)gb.z);
