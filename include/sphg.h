@@ -152,7 +152,10 @@ enum {
     SPHG_FIELD_PBNORMAL = 16,  /* float[n*3] PCISPH boundary method: the fluid normal of the last
                                   collision projection (FluidModel::n; NaN where nothing collided) */
     SPHG_FIELD_BDENSITY = 17,  /* float[nb] PCISPH boundary method: boundary densities, input order */
-    SPHG_FIELD_BPRESSURE = 18  /* float[nb] PCISPH boundary method: boundary pressures, input order */
+    SPHG_FIELD_BPRESSURE = 18, /* float[nb] PCISPH boundary method: boundary pressures, input order */
+    SPHG_FIELD_DRHO = 19,      /* float[n] rho - rho0 as stored (the density's deviation at full FP32
+                                  precision; FIELD_RHO adds rho0 and rounds again) */
+    SPHG_FIELD_DRHOSTAR = 20   /* float[n] DFSPH rho* - rho0 as stored, the error the density solve runs on */
 };
 
 const char* sphg_version(void);

@@ -18,6 +18,13 @@ OF_DIVERR, OF_K, OF_KV, OF_BVOL, OF_NORMAL, OF_PACC, OF_ACC_STEP, OF_BX = 7, 8, 
 OF_PBN, OF_BDENS, OF_BPRESS, OF_PREDR = 15, 16, 17, 18
 _VEC3 = {OF_X, OF_V, OF_A, OF_NORMAL, OF_PACC, OF_ACC_STEP, OF_PBN, OF_PREDR}
 
+# Planted mistakes of the parity oracle (enum Plant in sph_oracle.cpp), for
+# tests that show a check catches the mistake it is meant to catch.
+PLANTS = ["NONE", "KAPPA_X2", "KAPPA_DT", "KAPPA_RHO0", "K_NO_ALPHA", "SWEEP_SIGN", "KV_DT2", "DIV_LOOP_MAXV",
+          "ALPHA_NO_SUMSQ", "DT_ABS_V", "DT_NO_G", "DT_099_GE3", "WC_P_OVER_RHO", "WC_PJ_AS_PI", "WC_BOUNDARY_SYM",
+          "XSPH_NO_DT", "XSPH_RHO_I", "PCI_NO_CLAMP", "PCI_CUBIC", "PCI_P_OVER_RHO", "ST_NPLUS", "ST_K_HALF",
+          "ADH_COEF", "ADH_NO_RHO0"]
+
 
 class OracleDesc(C.Structure):
     _fields_ = [
@@ -93,6 +100,7 @@ def load(fast: bool = False) -> C.CDLL:
         "oracle_set_fluid": (C.c_int, [P, fp, fp, U]),
         "oracle_set_boundary": (C.c_int, [P, fp, U]),
         "oracle_init": (C.c_int, [P]),
+        "oracle_plant": (C.c_int, [C.c_int]),
         "oracle_set_state": (C.c_int, [P, fp, fp, fp, fp, D, U]),
         "oracle_step": (C.c_int, [P, U]),
         "oracle_add_emitter": (C.c_int, [P, C.POINTER(OracleEmitter)]),
@@ -124,6 +132,14 @@ def load(fast: bool = False) -> C.CDLL:
         f.argtypes = args
     _libs[name] = lib
     return lib
+
+
+def plant(name: str) -> None:
+    """Switch on one planted mistake ("NONE" restores the faithful oracle)
+    in the parity build, for every oracle of the process."""
+    n = load().oracle_plant(PLANTS.index(name))
+    if n != len(PLANTS):
+        raise RuntimeError(f"oracle plant table mismatch: library has {n} entries, wrapper {len(PLANTS)}")
 
 
 def _dp(a):

@@ -49,6 +49,39 @@
 
 namespace {
 
+// Planted mistakes.  oracle_plant(id) switches one deliberate error on, so
+// the tests can show that each of their checks fails on the mistake it is
+// meant to catch; 0 (the default) is the faithful restatement.
+enum Plant {
+    PL_NONE = 0,
+    PL_KAPPA_X2,        // kappa twice as large (src/DFSPHSolver.cpp:346, :531)
+    PL_KAPPA_DT,        // density kappa divided by dt instead of dt^2 (:346)
+    PL_KAPPA_RHO0,      // kappa divided by rho0 instead of rho_i
+    PL_K_NO_ALPHA,      // k += rho* - rho0, without alpha (:349, :534)
+    PL_SWEEP_SIGN,      // v += instead of v -= in the velocity sweep (:353-394)
+    PL_KV_DT2,          // kappa_v divided by dt^2 instead of dt (:531)
+    PL_DIV_LOOP_MAXV,   // divergence loop bounded by maxIterationsV (:512 uses maxIterations)
+    PL_ALPHA_NO_SUMSQ,  // alpha without the sum of squared gradients (:180-186)
+    PL_DT_ABS_V,        // |v| instead of sqrt(|v|) in the CFL rule (:833, :848)
+    PL_DT_NO_G,         // the CFL speed without (a + g) dt (:848)
+    PL_DT_099_GE3,      // the 0.99 branch after 3 or more iterations (:859 tests > 3)
+    PL_WC_P_OVER_RHO,   // p/rho instead of p/rho^2 (src/WCSPHSolver.cpp:159)
+    PL_WC_PJ_AS_PI,     // p_i/rho_i^2 for both particles of a pair (:159)
+    PL_WC_BOUNDARY_SYM, // 2 p_i/rho_i^2 at the boundary (:184)
+    PL_XSPH_NO_DT,      // XSPH not divided by dt (src/XSPHViscosity.h:55)
+    PL_XSPH_RHO_I,      // XSPH weighted by m/rho_i instead of m/rho_j (:55)
+    PL_PCI_NO_CLAMP,    // p += delta (rho - rho0) without max(0, .) (src/PCISPHSolver.cpp:262)
+    PL_PCI_CUBIC,       // cubic spline instead of Spiky gradient (:313-353)
+    PL_PCI_P_OVER_RHO,  // p/rho instead of p/rho^2 in the pressure acceleration (:325)
+    PL_ST_NPLUS,        // (n_i + n_j) in the curvature term (src/SurfaceTensionAkinci.h:90)
+    PL_ST_K_HALF,       // K_ij = rho0 / (rho_i + rho_j) (:86)
+    PL_ADH_COEF,        // adhesion scaled by 0.007/h^3.25 (src/Adhesion.h:21 computes it, :30-31 omit it)
+    PL_ADH_NO_RHO0,     // adhesion without rho0 (src/AdhesionForce.h:57)
+    PL_COUNT
+};
+int g_plant = PL_NONE;
+inline bool planted(int p) { return g_plant == p; }
+
 struct V3 {
     double x, y, z;
     V3() : x(0), y(0), z(0) {}
@@ -567,7 +600,9 @@ struct Oracle {
             V3 ai = a[i];
             for (const PointInfo& pj : c.neighbors) {
                 const uint32_t j = pj.id;
-                ai += nu * (m / rho[j]) * (v[j] - vi) * K.W(ri - x[j]) / dt;
+                const double rj = planted(PL_XSPH_RHO_I) ? rho[i] : rho[j];
+                if (planted(PL_XSPH_NO_DT)) ai += nu * (m / rj) * (v[j] - vi) * K.W(ri - x[j]);
+                else ai += nu * (m / rj) * (v[j] - vi) * K.W(ri - x[j]) / dt;
             }
             if (nub != 0.0 && !pbm) {
                 for (const PointInfo& pb : c.bneighbors) {
@@ -670,10 +705,10 @@ struct Oracle {
                 const uint32_t j = pj.id;
                 const V3 rij = ri - x[j];
                 const double rijMag = length(rij);
-                const double kij = 2.0 * rho0 / (dens_i + rho[j]);
+                const double kij = (planted(PL_ST_K_HALF) ? 1.0 : 2.0) * rho0 / (dens_i + rho[j]);
                 if (rijMag > 0.0 && rijMag <= h) {
                     V3 cohesion = -m * K.cohesion(rij) * rij / rijMag;
-                    V3 curvature = -1.0 * (ni - normal[j]);
+                    V3 curvature = -1.0 * (planted(PL_ST_NPLUS) ? ni + normal[j] : ni - normal[j]);
                     ai += st * kij * (cohesion + curvature);
                 }
             }
@@ -693,7 +728,8 @@ struct Oracle {
                 const uint32_t b = pb.id;
                 V3 rib = ri - xb[b];
                 double mod_rib = length(rib);
-                if (mod_rib > 0.0) ai -= beta * rho0 * vb[b] * K.adhesion(rib) * rib / mod_rib;
+                const double c = (planted(PL_ADH_COEF) ? K.adc : 1.0) * (planted(PL_ADH_NO_RHO0) ? 1.0 : rho0);
+                if (mod_rib > 0.0) ai -= beta * c * vb[b] * K.adhesion(rib) * rib / mod_rib;
             }
             a[i] = ai;
         }
@@ -730,10 +766,12 @@ struct Oracle {
             const V3 ri = x[i];
             const Bucket& c = cellOf(ri);
             V3 ai = a[i];
-            const double densPress_i = p[i] / (rho[i] * rho[i]);
+            const double densPress_i = planted(PL_WC_P_OVER_RHO) ? p[i] / rho[i] : p[i] / (rho[i] * rho[i]);
             for (const PointInfo& pj : c.neighbors) {
                 const uint32_t j = pj.id;
-                ai -= m * (densPress_i + p[j] / (rho[j] * rho[j])) * K.gradW(ri - x[j]);
+                double dpj = planted(PL_WC_P_OVER_RHO) ? p[j] / rho[j] : p[j] / (rho[j] * rho[j]);
+                if (planted(PL_WC_PJ_AS_PI)) dpj = densPress_i;
+                ai -= m * (densPress_i + dpj) * K.gradW(ri - x[j]);
             }
             if (pbm) {   // src/WCSPHSolver.cpp:160-172
                 const double mb = rho0 * vbPcisph();
@@ -744,7 +782,7 @@ struct Oracle {
             } else {
                 for (const PointInfo& pc : c.bneighbors) {
                     const uint32_t b = pc.id;
-                    ai -= rho0 * vb[b] * (densPress_i * 1.0) * K.gradW(ri - xb[b]);
+                    ai -= rho0 * vb[b] * (densPress_i * (planted(PL_WC_BOUNDARY_SYM) ? 2.0 : 1.0)) * K.gradW(ri - xb[b]);
                 }
             }
             a[i] = ai;
@@ -899,7 +937,7 @@ struct Oracle {
         double me = 0;
         #pragma omp parallel for reduction(max : me)
         for (long i = 0; i < n; ++i) {
-            double error = std::max(0.0, rho[i] - rho0);
+            double error = planted(PL_PCI_NO_CLAMP) ? rho[i] - rho0 : std::max(0.0, rho[i] - rho0);
             p[i] += delta * error;
             me = std::max(me, error / rho0);
         }
@@ -915,10 +953,12 @@ struct Oracle {
             const V3 ri = x[i];
             const Bucket& c = cellOf(ri);
             V3 pa;
-            const double densPress_i = p[i] / (rho[i] * rho[i]);
+            const bool pr = planted(PL_PCI_P_OVER_RHO);
+            auto grad = [&](V3 rr) { return planted(PL_PCI_CUBIC) ? K.gradW(rr) : K.spikyGrad(rr); };
+            const double densPress_i = pr ? p[i] / rho[i] : p[i] / (rho[i] * rho[i]);
             for (const PointInfo& pj : c.neighbors) {
                 const uint32_t j = pj.id;
-                pa -= m * (densPress_i + p[j] / (rho[j] * rho[j])) * K.spikyGrad(ri - x[j]);
+                pa -= m * (densPress_i + (pr ? p[j] / rho[j] : p[j] / (rho[j] * rho[j]))) * grad(ri - x[j]);
             }
             if (pbm) {   // src/PCISPHSolver.cpp:330-340
                 const double mb = rho0 * vbPcisph();
@@ -929,7 +969,7 @@ struct Oracle {
             } else {
                 for (const PointInfo& pc : c.bneighbors) {
                     const uint32_t b = pc.id;
-                    pa -= rho0 * vb[b] * (densPress_i) * K.spikyGrad(ri - xb[b]);
+                    pa -= rho0 * vb[b] * (densPress_i) * grad(ri - xb[b]);
                 }
             }
             pacc[i] = pa;
@@ -1007,7 +1047,7 @@ struct Oracle {
                 sumGrad += massGrad;
             }
             double sumGradMag = length(sumGrad);
-            double denominator = sumGradMag * sumGradMag + sumSqGrad;
+            double denominator = sumGradMag * sumGradMag + (planted(PL_ALPHA_NO_SUMSQ) ? 0.0 : sumSqGrad);
             alpha[i] = (std::fabs(denominator) > eps) ? rho[i] / denominator : 0.0;
         }
     }
@@ -1066,13 +1106,15 @@ struct Oracle {
         #pragma omp parallel for
         for (long i = 0; i < n; ++i) {
             const V3 ri = x[i];
-            const double ki = err[i] * alpha[i] / (scale * rho[i]);
-            acc[i] += err[i] * alpha[i];
+            const double kx = planted(PL_KAPPA_X2) ? 2.0 : 1.0;
+            const bool r0 = planted(PL_KAPPA_RHO0);
+            const double ki = kx * err[i] * alpha[i] / (scale * (r0 ? rho0 : rho[i]));
+            acc[i] += planted(PL_K_NO_ALPHA) ? err[i] : err[i] * alpha[i];
             const Bucket& c = cellOf(ri);
             V3 sum;
             for (const PointInfo& pj : c.neighbors) {
                 const uint32_t j = pj.id;
-                const double kj = err[j] * alpha[j] / (scale * rho[j]);
+                const double kj = kx * err[j] * alpha[j] / (scale * (r0 ? rho0 : rho[j]));
                 const double kSum = ki + kj;
                 if (std::fabs(kSum) > eps) sum += m * kSum * K.gradW(ri - x[j]);
             }
@@ -1082,7 +1124,8 @@ struct Oracle {
                     sum += rho0 * vb[b] * ki * K.gradW(ri - xb[b]);
                 }
             }
-            v[i] -= dt * sum;
+            if (planted(PL_SWEEP_SIGN)) v[i] += dt * sum;
+            else v[i] -= dt * sum;
         }
     }
     // src/DFSPHSolver.cpp:311-406.  The density error passed to the sweep is
@@ -1097,7 +1140,7 @@ struct Oracle {
         avgDensError = 0;
         while (((avgDensError > d.eta) || (iterations < d.min_iterations)) && iterations < d.max_iterations) {
             for (size_t i = 0; i < N(); ++i) err[i] = rhoStar[i] - rho0;
-            velocitySweep(err, dt * dt, k);
+            velocitySweep(err, planted(PL_KAPPA_DT) ? dt : dt * dt, k);
             predictDensities();
             ++iterations;
         }
@@ -1138,8 +1181,9 @@ struct Oracle {
         avgDivError0 = avgDivError;
         iterationsV = 0;
         avgDivError = 0;
-        while (((avgDivError > d.eta_v / dt) || (iterationsV < d.min_iterations_v)) && iterationsV < d.max_iterations) {
-            velocitySweep(divErr, dt, kv);
+        const unsigned bound = planted(PL_DIV_LOOP_MAXV) ? d.max_iterations_v : d.max_iterations;
+        while (((avgDivError > d.eta_v / dt) || (iterationsV < d.min_iterations_v)) && iterationsV < bound) {
+            velocitySweep(divErr, planted(PL_KV_DT2) ? dt * dt : dt, kv);
             computeDivergenceError();
             ++iterationsV;
         }
@@ -1179,16 +1223,16 @@ struct Oracle {
         double mv = 0.1;
         #pragma omp parallel for reduction(max : mv)
         for (long i = 0; i < n; ++i) {
-            V3 acc = a[i] + g;
+            V3 acc = planted(PL_DT_NO_G) ? a[i] : a[i] + g;
             V3 cur = v[i] + acc * dt;
-            mv = std::max(mv, std::sqrt(length(cur)));
+            mv = std::max(mv, planted(PL_DT_ABS_V) ? length(cur) : std::sqrt(length(cur)));
         }
         maxVel = mv;
         double newTs;
         double cflTs = d.cfl_factor * 0.4 * 2.0 * r / mv;
         lastCfl = cflTs;
         if (cflTs < dt) newTs = cflTs;
-        else if (iterations > 3) newTs = dt * 0.99;
+        else if (iterations > 3 || (planted(PL_DT_099_GE3) && iterations == 3)) newTs = dt * 0.99;
         else newTs = dt * 1.001;
         newTs = std::max(newTs, minTs);
         newTs = std::min(newTs, maxTs);
@@ -1346,6 +1390,14 @@ int oracle_add_emitter(void* h, const oracle_emitter* e) {
     return (int)em.group.size();
 }
 int oracle_init(void* h) { ((Oracle*)h)->init(); return 0; }
+
+// Switch on planted mistake `id` (0 = none) for every oracle of the
+// process; returns the number of plant ids, or -1 for an unknown one.
+int oracle_plant(int id) {
+    if (id < 0 || id >= PL_COUNT) return -1;
+    g_plant = id;
+    return PL_COUNT;
+}
 
 // Inject a state (FP32 arrays, promoted) for one-step parity from a GPU
 // state: positions, velocities, warm-start accumulators (may be null), the

@@ -1593,6 +1593,10 @@ int sphg_scene_get(sphg_scene* s, int32_t field, void* out) {
         case SPHG_FIELD_V: launch_unsort_f4(st, V.v, V.id, s->tmp, n, 3); bytes = 12ull * n; break;
         case SPHG_FIELD_A: launch_unsort_f4(st, V.a, V.id, s->tmp, n, 3); bytes = 12ull * n; break;
         case SPHG_FIELD_RHO: launch_unsort_f1(st, V.drho, V.id, s->tmp, n, V.rho0D); bytes = 4ull * n; break;
+        case SPHG_FIELD_DRHO: launch_unsort_f1(st, V.drho, V.id, s->tmp, n); bytes = 4ull * n; break;
+        case SPHG_FIELD_DRHOSTAR:
+            if (V.method != DFSPH) return fail(SPHG_EINVAL, "DFSPH field on another method");
+            launch_unsort_f1(st, V.drhoS, V.id, s->tmp, n); bytes = 4ull * n; break;
         case SPHG_FIELD_P:
             if (!V.p) return fail(SPHG_EINVAL, "no pressure field for this method");
             launch_unsort_f1(st, V.p, V.id, s->tmp, n); bytes = 4ull * n; break;

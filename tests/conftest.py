@@ -43,3 +43,11 @@ def rel_elem(a, b) -> float:
     a = np.asarray(a, dtype=np.float64)
     b = np.asarray(b, dtype=np.float64)
     return float(np.max(np.abs(a - b) / np.abs(b))) if b.size else 0.0
+
+
+def dev_err(g_drho, o_rho, rho0) -> float:
+    """Normwise relative error of a density deviation: the solver's stored
+    rho - rho0 (FIELD_DRHO / FIELD_DRHOSTAR) against the oracle's FP64
+    density minus rho0.  The deviation, not the density near rho0, is what
+    the pressure solvers run on."""
+    return rel_norm(g_drho, np.asarray(o_rho, dtype=np.float64) - float(rho0))

@@ -11,7 +11,7 @@ Bars:
 import numpy as np
 import pytest
 
-from conftest import rel_elem, rel_norm
+from conftest import dev_err, rel_elem, rel_norm
 from sph_amd import configs, sphg
 import oracle.pyoracle as po
 
@@ -79,6 +79,7 @@ def test_init_density_alpha_volume(ctx):
     g, o = _pair(ctx, spec)
     assert rel_elem(g.get(sphg.FIELD_BVOL), o.get(po.OF_BVOL)) < TOL
     assert rel_elem(g.get(sphg.FIELD_RHO), o.get(po.OF_RHO)) < TOL
+    assert dev_err(g.get(sphg.FIELD_DRHO), o.get(po.OF_RHO), spec.params.density0) < TOL
     a_g, a_o = g.get(sphg.FIELD_ALPHA), o.get(po.OF_ALPHA)
     assert np.array_equal(a_g == 0, a_o == 0)
     nz = a_o != 0
@@ -97,8 +98,8 @@ def _dfsph_compare(g, o, report):
     errs = {
         "v": rel_norm(g.get(sphg.FIELD_V), o.get(po.OF_V)),
         "x": rel_norm(g.get(sphg.FIELD_X), o.get(po.OF_X)),
-        "rho": rel_elem(g.get(sphg.FIELD_RHO), o.get(po.OF_RHO)),
-        "rhostar": rel_elem(g.get(sphg.FIELD_RHOSTAR), o.get(po.OF_RHOSTAR)),
+        "drho": dev_err(g.get(sphg.FIELD_DRHO), o.get(po.OF_RHO), g.params.density0),
+        "drhostar": dev_err(g.get(sphg.FIELD_DRHOSTAR), o.get(po.OF_RHOSTAR), g.params.density0),
         "alpha": rel_norm(g.get(sphg.FIELD_ALPHA), o.get(po.OF_ALPHA)),
         "k": rel_norm(g.get(sphg.FIELD_K), o.get(po.OF_K)),
         "kv": rel_norm(g.get(sphg.FIELD_KV), o.get(po.OF_KV)),
@@ -153,7 +154,7 @@ def test_wcsph_steps_identical_state(ctx):
         g.step(1)
         o.step(1)
         errs = {
-            "rho": rel_elem(g.get(sphg.FIELD_RHO), o.get(po.OF_RHO)),
+            "drho": dev_err(g.get(sphg.FIELD_DRHO), o.get(po.OF_RHO), spec.params.density0),
             "p": rel_norm(g.get(sphg.FIELD_P), o.get(po.OF_P)),
             "a": rel_norm(g.get(sphg.FIELD_A), o.get(po.OF_ACC_STEP)),
             "v": rel_norm(g.get(sphg.FIELD_V), o.get(po.OF_V)),
@@ -184,7 +185,7 @@ def test_pcisph_surface_tension_adhesion(ctx):
             continue
         matched += 1
         errs = {
-            "rho": rel_elem(g.get(sphg.FIELD_RHO), o.get(po.OF_RHO)),
+            "drho": dev_err(g.get(sphg.FIELD_DRHO), o.get(po.OF_RHO), spec.params.density0),
             "p": rel_norm(g.get(sphg.FIELD_P), o.get(po.OF_P)),
             "pacc": rel_norm(g.get(sphg.FIELD_PACC), o.get(po.OF_PACC)),
             "v": rel_norm(g.get(sphg.FIELD_V), o.get(po.OF_V)),
@@ -228,7 +229,7 @@ def test_pcisph_predicted_density_paths(ctx, case):
             continue
         matched += 1
         errs = {
-            "rho": rel_elem(g.get(sphg.FIELD_RHO), o.get(po.OF_RHO)),
+            "drho": dev_err(g.get(sphg.FIELD_DRHO), o.get(po.OF_RHO), spec.params.density0),
             "p": rel_norm(g.get(sphg.FIELD_P), o.get(po.OF_P)),
             "v": rel_norm(g.get(sphg.FIELD_V), o.get(po.OF_V)),
             "x": rel_norm(g.get(sphg.FIELD_X), o.get(po.OF_X)),
@@ -570,7 +571,7 @@ def test_pcisph_boundary_method(ctx, method):
         ok = ~np.isnan(no[:, 0])
         hits += int(ok.sum())
         errs = {"x": rel_norm(g.get(sphg.FIELD_X), o.get(po.OF_X)), "v": rel_norm(g.get(sphg.FIELD_V), o.get(po.OF_V)),
-                "rho": rel_elem(g.get(sphg.FIELD_RHO), o.get(po.OF_RHO))}
+                "drho": dev_err(g.get(sphg.FIELD_DRHO), o.get(po.OF_RHO), spec.params.density0)}
         if not dfs:
             errs["bdens"] = rel_elem(g.get(sphg.FIELD_BDENSITY), o.get(po.OF_BDENS))
             errs["bpres"] = rel_norm(g.get(sphg.FIELD_BPRESSURE), o.get(po.OF_BPRESS))
