@@ -119,7 +119,13 @@ typedef struct sphg_scene_scalars {
     double max_density_error;       /* PCISPH */
     int32_t iterations, iterations_v;
     uint32_t steps, sum_iterations, sum_iterations_v;
-    uint32_t neighbor_overflow;     /* largest neighbor count seen above capacity, or 0 */
+    uint32_t neighbor_overflow;     /* largest neighbor count above max_neighbors during the last
+                                       step / batch call that stepped this scene, or 0 */
+    /* PCISPH predicted densities: steps that read the neighbor and shell
+     * lists (particles whose shell did not fit walk their 27 cells; see
+     * SPHG_FIELD_NSH), and steps whose displacement guard sent every particle
+     * to the 27-cell walk (always, with the PCISPH boundary method) */
+    uint32_t pcisph_shell_steps, pcisph_walk_steps;
 } sphg_scene_scalars;
 
 typedef struct sphg_step_stats {
@@ -148,14 +154,17 @@ enum {
     SPHG_FIELD_PACC = 12,    /* float[n*3] PCISPH pressure acceleration */
     SPHG_FIELD_KEYS = 13,    /* int32[n*3] cell keys floor(x/h) */
     SPHG_FIELD_NNB = 14,     /* int32[n] fluid neighbors with |r| <= h, self excluded */
-    SPHG_FIELD_NGRAD = 15,   /* int32[n] of which with nonzero kernel gradient */
+    SPHG_FIELD_NGRAD = 15,   /* int32[n] of which with nonzero kernel gradient (saturates at 255; the
+                                divergence error's cutoff is at 20) */
     SPHG_FIELD_PBNORMAL = 16,  /* float[n*3] PCISPH boundary method: the fluid normal of the last
                                   collision projection (FluidModel::n; NaN where nothing collided) */
     SPHG_FIELD_BDENSITY = 17,  /* float[nb] PCISPH boundary method: boundary densities, input order */
     SPHG_FIELD_BPRESSURE = 18, /* float[nb] PCISPH boundary method: boundary pressures, input order */
     SPHG_FIELD_DRHO = 19,      /* float[n] rho - rho0 as stored (the density's deviation at full FP32
                                   precision; FIELD_RHO adds rho0 and rounds again) */
-    SPHG_FIELD_DRHOSTAR = 20   /* float[n] DFSPH rho* - rho0 as stored, the error the density solve runs on */
+    SPHG_FIELD_DRHOSTAR = 20,  /* float[n] DFSPH rho* - rho0 as stored, the error the density solve runs on */
+    SPHG_FIELD_NSH = 21        /* int32[n] PCISPH: shell-list entries of each particle at the last list
+                                  build, -1 where the shell did not fit (that particle walks its cells) */
 };
 
 const char* sphg_version(void);

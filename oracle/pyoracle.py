@@ -109,6 +109,7 @@ def load(fast: bool = False) -> C.CDLL:
         "oracle_sample_box_normals": (C.c_long, [dp, dp, D, C.c_int, dp, dp, C.c_long]),
         "oracle_round_state_f32": (C.c_int, [P]),
         "oracle_set_f32_positions": (C.c_int, [P, C.c_int]),
+        "oracle_set_eps": (C.c_int, [P, D]),
         "oracle_num_fluid": (U, [P]),
         "oracle_num_boundary": (U, [P]),
         "oracle_get": (C.c_int, [P, C.c_int, dp]),
@@ -235,6 +236,10 @@ class Oracle:
         """Round DFSPH's updated positions to FP32 before the neighbor
         search, densities and alpha factors of the step."""
         self.lib.oracle_set_f32_positions(self.h, 1 if on else 0)
+
+    def set_eps(self, eps: float = 1e-5):
+        """The threshold of the DFSPH eps branches (1e-5 in the reference)."""
+        self.lib.oracle_set_eps(self.h, float(eps))
 
     def get(self, field: int) -> np.ndarray:
         if field in (OF_BVOL, OF_BDENS, OF_BPRESS):

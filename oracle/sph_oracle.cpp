@@ -370,7 +370,7 @@ struct Oracle {
     double avgDensError = 0, avgDivError = 0, avgDensError0 = 0, avgDivError0 = 0;
     unsigned iterations = 0, iterationsV = 0, steps = 0, sumIt = 0, sumItV = 0;
     bool f32Positions = false;   // round DFSPH's updated positions to FP32 (test emulation of FP32 storage)
-    double eps = 1e-5;
+    double eps = 1e-5;   // src/DFSPHSolver.cpp:26 (oracle_set_eps moves it for the tests)
     double maxVel = 0, lastCfl = 0;
 
     // PCISPH state (src/PCISPHSolver.h:12-17)
@@ -1451,6 +1451,16 @@ int oracle_round_state_f32(void* h) {
 // that follow see the positions a solver storing them in FP32 sees.
 int oracle_set_f32_positions(void* h, int on) {
     ((Oracle*)h)->f32Positions = on != 0;
+    return 0;
+}
+
+// The threshold of the DFSPH eps branches (|k_i + k_j| > eps and |k_i| >
+// eps in the warm start and both solves, src/DFSPHSolver.cpp:26,364,368,
+// 548,552,642,656,670; the alpha denominator test at :182).  Tests move it
+// by a small relative amount to find the particles whose step depends on a
+// branch decided within a solver's FP32 resolution of the threshold.
+int oracle_set_eps(void* h, double eps) {
+    ((Oracle*)h)->eps = eps;
     return 0;
 }
 

@@ -197,6 +197,10 @@ struct sphg_scene {
     DevScene view[2];          // view[p]: buffers of parity p current
     int par = 0;
     bool ready = false;
+    // DFSPH: the last call stepped the scene, so the fields a step forms
+    // before its grid rebuild (a, normals, rho*) are in the particle order
+    // of the other half's ids (the order before the rebuild's sort)
+    bool stepped = false;
     std::vector<float> fxyz, fvel, bxyz;
     std::vector<uint32_t> border;   // sorted boundary slot -> input index
     std::vector<sphg_emitter_desc> emitters;
@@ -682,6 +686,8 @@ int run_batch(sphg_ctx* ctx, Batch& B, uint32_t nsteps, sphg_step_stats* stats) 
         HIP_TRY(hipStreamSynchronize(st));
     }
     uint32_t syncs = 0;
+    // the overflow maximum reports the lists this call's steps use
+    launch_clear_overflow(st, B.dT[0], (int)B.scenes.size());
     HIP_TRY(hipEventRecord(e0, st));
     for (uint32_t k = 0; k < nsteps; ++k) {
         int rc = batch_one_step(ctx, B, syncs);
@@ -697,7 +703,7 @@ int run_batch(sphg_ctx* ctx, Batch& B, uint32_t nsteps, sphg_step_stats* stats) 
     HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    unsigned overflow = 0;
+    unsigned overflow = 0, nover = 0;
     if (stats) {
         std::memset(stats, 0, sizeof(*stats));
         stats->gpu_ms = ms;
@@ -712,7 +718,9 @@ int run_batch(sphg_ctx* ctx, Batch& B, uint32_t nsteps, sphg_step_stats* stats) 
             stats->iterations_v += now.sumItV - before[s].sumItV;
         }
         overflow = std::max(overflow, now.overflow);
+        nover += now.overflow ? 1u : 0u;
         sphg_scene* sc = B.scenes[s];
+        sc->stepped = nsteps > 0 || sc->stepped;
         if (!sc->emitters.empty() && now.nActive != sc->n) {
             sc->n = now.nActive;
             sc->view[0].n = sc->view[1].n = (int)now.nActive;
@@ -720,8 +728,11 @@ int run_batch(sphg_ctx* ctx, Batch& B, uint32_t nsteps, sphg_step_stats* stats) 
         }
     }
     if (overflow)
-        return fail(SPHG_EOVERFLOW, "a particle had " + std::to_string(overflow) +
-                                        " neighbors, above max_neighbors; results of that step are invalid");
+        return fail(SPHG_EOVERFLOW, std::to_string(nover) + " scene(s) had a particle with up to " +
+                                        std::to_string(overflow) +
+                                        " neighbors, above max_neighbors: their results of this call are invalid "
+                                        "(sphg_scene_get_scalars names them: neighbor_overflow > 0); the other "
+                                        "scenes of the batch stepped normally");
     return SPHG_OK;
 }
 
@@ -819,7 +830,7 @@ struct Plan {
     int nChunks = 0, nCellChunks = 0, maxNbr = 0, flags = 0;
     size_t npad = 0, total = 0;
     size_t oX[2], oV[2], oK[2] = {0, 0}, oKV[2] = {0, 0}, oID[2];
-    size_t oVYZ, oA, oXK, oGB, oAUX, oXS, oPACC, oGBS, oNRM, oADH, oRHO, oALPHA, oDRHOS, oDIV, oP;
+    size_t oVYZ, oD0, oA, oXK, oGB, oAUX, oXS, oPACC, oGBS, oNRM, oADH, oRHO, oALPHA, oDRHOS, oDIV, oP;
     size_t oCELLU, oCELL, oPERM, oCC, oCS, oCCS, oNBR, oNNB, oNSH, oNG, oGC, oXB, oBCS, oPART, oSS, oTMP;
     size_t oEMIT = 0, oGROUP = 0;
     size_t oBN = 0, oBSELF = 0, oBDENS = 0, oBPRES = 0, oBPR = 0, oPBN[2] = {0, 0}, oPV = 0, oPRS = 0, oPVS = 0,
@@ -902,6 +913,7 @@ int make_plan(const sphg_scene* s, Plan& P) {
     P.oA = A.reserve(f4);
     P.oXK = A.reserve(f4);
     P.oVYZ = (method == DFSPH) ? A.reserve(2 * f1) : 0;
+    P.oD0 = (method == DFSPH) ? A.reserve(sizeof(double) * npad) : 0;
     P.oGB = A.reserve(f4);
     P.oAUX = (method != DFSPH) ? A.reserve(f4) : 0;
     P.oXS = (method == PCISPH) ? A.reserve(f4) : 0;
@@ -1252,6 +1264,7 @@ int sphg_scene_init(sphg_scene* s) {
     D.a = F4(P.oA);
     D.xk = F4(P.oXK);
     D.vyz = P.oVYZ ? (float2*)(b + P.oVYZ) : nullptr;
+    D.d0 = P.oD0 ? (double*)(b + P.oD0) : nullptr;
     D.gb = F4(P.oGB);
     D.aux = P.oAUX ? F4(P.oAUX) : nullptr;
     D.xs = P.oXS ? F4(P.oXS) : nullptr;
@@ -1452,6 +1465,7 @@ int sphg_scene_set_state(sphg_scene* s, const float* xyz, const float* vel, cons
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     ss.dt = time_step;
     ss.it = (int)last_iterations;
+    s->stepped = false;
     HIP_TRY(hipMemcpyAsync(s->ss, &ss, sizeof(Scalars), hipMemcpyHostToDevice, ctx->stream));
     if (s->desc.method == DFSPH) {
         launch_keys(ctx->stream, B.dT[0], B.dP, B.nP);
@@ -1588,15 +1602,18 @@ int sphg_scene_get(sphg_scene* s, int32_t field, void* out) {
     const DevScene& V = s->view[s->par];
     const int n = (int)s->n;
     size_t bytes = 0;
+    // DFSPH forms a, the normals and rho* before the step's grid rebuild:
+    // they are in the order of the ids before the rebuild's sort
+    const uint32_t* pre = (V.method == DFSPH && s->stepped) ? s->view[1 - s->par].id : V.id;
     switch (field) {
         case SPHG_FIELD_X: launch_unsort_f4(st, V.x, V.id, s->tmp, n, 3); bytes = 12ull * n; break;
         case SPHG_FIELD_V: launch_unsort_f4(st, V.v, V.id, s->tmp, n, 3); bytes = 12ull * n; break;
-        case SPHG_FIELD_A: launch_unsort_f4(st, V.a, V.id, s->tmp, n, 3); bytes = 12ull * n; break;
+        case SPHG_FIELD_A: launch_unsort_f4(st, V.a, pre, s->tmp, n, 3); bytes = 12ull * n; break;
         case SPHG_FIELD_RHO: launch_unsort_f1(st, V.drho, V.id, s->tmp, n, V.rho0D); bytes = 4ull * n; break;
         case SPHG_FIELD_DRHO: launch_unsort_f1(st, V.drho, V.id, s->tmp, n); bytes = 4ull * n; break;
         case SPHG_FIELD_DRHOSTAR:
             if (V.method != DFSPH) return fail(SPHG_EINVAL, "DFSPH field on another method");
-            launch_unsort_f1(st, V.drhoS, V.id, s->tmp, n); bytes = 4ull * n; break;
+            launch_unsort_f1(st, V.drhoS, pre, s->tmp, n); bytes = 4ull * n; break;
         case SPHG_FIELD_P:
             if (!V.p) return fail(SPHG_EINVAL, "no pressure field for this method");
             launch_unsort_f1(st, V.p, V.id, s->tmp, n); bytes = 4ull * n; break;
@@ -1608,13 +1625,14 @@ int sphg_scene_get(sphg_scene* s, int32_t field, void* out) {
             if (V.method != DFSPH) return fail(SPHG_EINVAL, "DFSPH field on another method");
             const float* src = field == SPHG_FIELD_ALPHA ? V.alpha : field == SPHG_FIELD_RHOSTAR ? V.drhoS
                              : field == SPHG_FIELD_DIVERR ? V.divErr : field == SPHG_FIELD_K ? V.k : V.kv;
-            launch_unsort_f1(st, src, V.id, s->tmp, n, field == SPHG_FIELD_RHOSTAR ? V.rho0D : 0.0);
+            launch_unsort_f1(st, src, field == SPHG_FIELD_RHOSTAR ? pre : V.id, s->tmp, n,
+                             field == SPHG_FIELD_RHOSTAR ? V.rho0D : 0.0);
             bytes = 4ull * n;
             break;
         }
         case SPHG_FIELD_NORMAL:
             if (!V.nrm) return fail(SPHG_EINVAL, "no surface tension in this scene");
-            launch_unsort_f4(st, V.nrm, V.id, s->tmp, n, 3); bytes = 12ull * n; break;
+            launch_unsort_f4(st, V.nrm, pre, s->tmp, n, 3); bytes = 12ull * n; break;
         case SPHG_FIELD_PACC:
             if (!V.pacc) return fail(SPHG_EINVAL, "PCISPH field on another method");
             launch_unsort_f4(st, V.pacc, V.id, s->tmp, n, 3); bytes = 12ull * n; break;
@@ -1624,6 +1642,17 @@ int sphg_scene_get(sphg_scene* s, int32_t field, void* out) {
             launch_keys3(st, s->self->dT[0], 0, n, (int*)s->tmp);
             bytes = 12ull * n;
             break;
+        }
+        case SPHG_FIELD_NSH: {
+            if (!V.nsh) return fail(SPHG_EINVAL, "shell lists belong to PCISPH scenes");
+            std::vector<uint16_t> c16(n);
+            std::vector<uint32_t> ids(n);
+            HIP_TRY(hipMemcpyAsync(ids.data(), V.id, 4ull * n, hipMemcpyDeviceToHost, st));
+            HIP_TRY(hipMemcpyAsync(c16.data(), V.nsh, 2ull * n, hipMemcpyDeviceToHost, st));
+            HIP_TRY(hipStreamSynchronize(st));
+            int32_t* o = (int32_t*)out;
+            for (int i = 0; i < n; ++i) o[ids[i]] = c16[i] == 0xFFFF ? -1 : (int32_t)c16[i];
+            return SPHG_OK;
         }
         case SPHG_FIELD_NNB:
         case SPHG_FIELD_NGRAD: {
@@ -1695,6 +1724,8 @@ int sphg_scene_get_scalars(sphg_scene* s, sphg_scene_scalars* out) {
     out->sum_iterations = v.sumIt;
     out->sum_iterations_v = v.sumItV;
     out->neighbor_overflow = v.overflow;
+    out->pcisph_shell_steps = v.shellSteps;
+    out->pcisph_walk_steps = v.walkSteps;
     return SPHG_OK;
 }
 

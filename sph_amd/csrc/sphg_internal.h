@@ -72,11 +72,16 @@ struct Scalars {
     double dt, time, maxVel, cflTs;
     double avgErr, avgErr0, avgErrV, avgErrV0, maxDensErr;
     int it, itV, done, doneV;
-    unsigned steps, sumIt, sumItV, ctr, overflow;
+    unsigned steps, sumIt, sumItV, ctr;
+    unsigned overflow;          // largest neighbor count above max_neighbors in the lists the current
+                                // step call has used (0: none)
+    unsigned listOver;          // the same for the lists built last (reset by the key pass before each build)
     unsigned nActive;           // active particles (the emitters raise it)
     unsigned shellBad;          // PCISPH: this step's shell lists cannot stand in for the 27-cell walk
     unsigned maxD2, maxD2Prev;  // PCISPH: largest squared predicted displacement (FP32 bits) of this
                                 // step so far and of the previous step, which sizes this step's shell
+    unsigned shellSteps, walkSteps;  // PCISPH: steps whose predicted densities used the shell lists /
+                                     // took the 27-cell walk for every particle
 };
 
 // One emitter's constants and state (src/Emitter.h:9-55).  Its group's
@@ -125,7 +130,10 @@ struct DevScene {
     int pbOn, pad4;
     double pbNf, pbTf;
     float2 *vyz;                // DFSPH: (v_y, v_z), written by the velocity sweeps with v_x in x.w;
-                                // the error sweeps gather x and vyz, 24 B per pair
+                                // the error sweeps gather x and vyz, 24 B per pair.  In the density
+                                // solve the two hold u, the step's velocity change so far, instead
+    double *d0;                 // DFSPH: the density solve's constant part of the predicted-density
+                                // sum, from the velocity at the step's start
     int pciShell, pad5;         // PCISPH: predicted densities from the list and a shell list
     uint16_t *nsh;              // PCISPH: shell entries per particle (candidates within h + skin
                                 // that are not in the list, stored from the list's last slot down)

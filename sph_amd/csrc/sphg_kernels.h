@@ -35,6 +35,7 @@ void launch_gather_f3(hipStream_t st, float4* dst, const float* src, const uint3
 void launch_pb_correct(hipStream_t st, int mode, const DevScene* T, const Chunk* C, int n);
 void launch_pb_predcorrect(hipStream_t st, const DevScene* T, const Chunk* C, int n);
 void launch_time_advance(hipStream_t st, const DevScene* T, int nscenes);
+void launch_clear_overflow(hipStream_t st, const DevScene* T, int nscenes);
 void launch_fill_int(hipStream_t st, int* p, int v, long n);
 void launch_unsort_f4(hipStream_t st, const float4* src, const uint32_t* id, float* dst, int n, int comps);
 void launch_unsort_f1(hipStream_t st, const float* src, const uint32_t* id, float* dst, int n, double offset = 0.0);

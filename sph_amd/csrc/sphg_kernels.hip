@@ -294,6 +294,11 @@ struct XVP {
     float4 xv;
     float2 yz;
 };
+// the density solve's first error sweep: x (with u_x in w), v0 and (u_y, u_z)
+struct XVU {
+    float4 x, v;
+    float2 yz;
+};
 struct NbF {
     float4 x, v, n;
     float d;
@@ -464,6 +469,8 @@ __global__ __launch_bounds__(kBlock) void k_keys(const DevScene* __restrict__ T,
     const Chunk c = C[blockIdx.x];
     const DevScene& S = T[c.scene];
     const int i = c.begin + threadIdx.x;
+    // the lists about to be built start without an overflow
+    if (c.begin == 0 && threadIdx.x == 0) S.ss->listOver = 0u;
     int cl = -1;
     if (i < S.n) {
         cl = cell_of(S, S.x[i]);
@@ -936,7 +943,10 @@ __global__ __launch_bounds__(kBlock) void k_build(const DevScene* __restrict__ T
         stg(S.nsh, (size_t)i, (uint16_t)(shellLost ? 0xFFFF : ns));
     }
     if ((i & 63) == 0) stg(S.gcnt, (size_t)(i >> 6), (uint16_t)pad);
-    if (cnt > maxN) atomicMax(&S.ss->overflow, (unsigned)cnt);
+    if (cnt > maxN) {
+        atomicMax(&S.ss->overflow, (unsigned)cnt);
+        atomicMax(&S.ss->listOver, (unsigned)cnt);
+    }
     stg(S.nnb, (size_t)i, (uint16_t)own);
     stg(S.ngrad, (size_t)i, (uint8_t)min(ngr, 255));
     const double m = S.mD;
@@ -1246,16 +1256,32 @@ __global__ __launch_bounds__(kBlock) void k_dt(const DevScene* __restrict__ T, c
     if (threadIdx.x == 0) S.partial[c.begin / kBlock] = bm;
 }
 
+// The density solve (predict, warm start, density iterations) keeps the
+// velocity of the step's start, v0, unchanged in v and accumulates the
+// step's change u = v - v0 in x.w and vyz, the slots its error sweeps
+// gather; the position update forms v = v0 + u.  The predicted density is
+// rho + dt (D0 + U): D0, the sum over v0, is formed once per step in FP64,
+// and U, the sum over u, in every iteration.  Velocities rounded to FP32
+// after every sweep would put their rounding (~1e-7 of |v|) into rho*, which
+// kappa = (rho* - rho0) alpha / (dt^2 rho) amplifies; u is the size of the
+// step's corrections, so its FP32 rounding is far below that.
+__device__ inline void store_u(const DevScene& S, int i, double ux, double uy, double uz) {
+    ((gptr<float>)S.x)[4 * (size_t)i + 3] = (float)ux;
+    ((gptr<v2f>)S.vyz)[i] = v2f{(float)uy, (float)uz};
+}
+
 // predictVelocities (src/DFSPHSolver.cpp:191-214) fused with the first loop
 // of densityWarmStart (:602-613): k_i <- min(k_i/rho_i, 1e-9)/dt^2, staged
-// in xk.w for the warm-start sweep.
+// in xk.w for the warm-start sweep.  The predicted velocity is kept as
+// u = (a + g) dt beside v0 (and, with the PCISPH boundary method, the change
+// of its velocity correction).
 __global__ __launch_bounds__(kBlock) void k_df_predict(const DevScene* __restrict__ T, const Chunk* __restrict__ C) {
     const Chunk c = C[blockIdx.x];
     const DevScene& S = T[c.scene];
     const int i = c.begin + threadIdx.x;
     if (i >= S.n) return;
     const double dt = S.ss->dt;
-    float4 v = S.v[i];
+    const float4 v = S.v[i];
     double ax = S.gx, ay = S.gy, az = S.gz;
     if (S.flags & F_FORCES) {
         const float4 a = S.a[i];
@@ -1263,21 +1289,23 @@ __global__ __launch_bounds__(kBlock) void k_df_predict(const DevScene* __restric
         ay = (double)a.y + ay;
         az = (double)a.z + az;
     }
-    v.x = (float)((double)v.x + ax * dt);
-    v.y = (float)((double)v.y + ay * dt);
-    v.z = (float)((double)v.z + az * dt);
+    double ux = ax * dt, uy = ay * dt, uz = az * dt;
     if (S.pbOn) {
         // correctVelocities with the normals of the last step's projection
         // (src/DFSPHSolver.cpp:51-58)
         const float4 n = S.pbn[i];
-        double ux = v.x, uy = v.y, uz = v.z;
-        if (pb_velocity(S, n.x, n.y, n.z, ux, uy, uz)) v = make_float4((float)ux, (float)uy, (float)uz, v.w);
+        double wx = (double)v.x + ux, wy = (double)v.y + uy, wz = (double)v.z + uz;
+        if (pb_velocity(S, n.x, n.y, n.z, wx, wy, wz)) {
+            ux = wx - (double)v.x;
+            uy = wy - (double)v.y;
+            uz = wz - (double)v.z;
+        }
     }
-    S.v[i] = v;
     const double rho = S.rho0D + (double)S.drho[i];
     const double kw = -fmax(-(double)S.k[i] / rho, -1.0e-9) / (dt * dt);
     const float4 x = S.x[i];
     S.xk[i] = make_float4(x.x, x.y, x.z, (float)kw);
+    store_u(S, i, ux, uy, uz);
 }
 
 // Velocity correction sweep shared by the warm start (src/DFSPHSolver.cpp:
@@ -1320,15 +1348,23 @@ __global__ __launch_bounds__(kBlock) void k_df_vel(const DevScene* __restrict__ 
         sy += cb * (double)gb.y;
         sz += cb * (double)gb.z;
     }
-    float4 v = S.v[i];
-    v.x = (float)((double)v.x - dt * sx);
-    v.y = (float)((double)v.y - dt * sy);
-    v.z = (float)((double)v.z - dt * sz);
-    S.v[i] = v;
-    // copies for the error sweep's gathers: v_x into the unused x.w (no
-    // kernel gathers x during this sweep), (v_y, v_z) into vyz
-    ((gptr<float>)S.x)[4 * (size_t)i + 3] = v.x;
-    ((gptr<v2f>)S.vyz)[i] = v2f{v.y, v.z};
+    if (MODE <= 1) {
+        // the density solve: the change u (x.w, vyz; no kernel gathers x
+        // during this sweep) takes the correction, v keeps v0
+        const float uxf = ((gptr<const float>)S.x)[4 * (size_t)i + 3];
+        const float2 uyz = ld2o(S.vyz, (uint32_t)i);
+        store_u(S, i, (double)uxf - dt * sx, (double)uyz.x - dt * sy, (double)uyz.y - dt * sz);
+    } else {
+        float4 v = S.v[i];
+        v.x = (float)((double)v.x - dt * sx);
+        v.y = (float)((double)v.y - dt * sy);
+        v.z = (float)((double)v.z - dt * sz);
+        S.v[i] = v;
+        // copies for the error sweep's gathers: v_x into the unused x.w (no
+        // kernel gathers x during this sweep), (v_y, v_z) into vyz
+        ((gptr<float>)S.x)[4 * (size_t)i + 3] = v.x;
+        ((gptr<v2f>)S.vyz)[i] = v2f{v.y, v.z};
+    }
     if (MODE == 0) S.k[i] = 0.0f;
     if (MODE == 1) S.k[i] = (float)((double)S.k[i] + (double)S.drhoS[i] * (double)S.alpha[i]);
     if (MODE == 2) S.kv[i] = (float)((double)S.kv[i] + (double)S.divErr[i] * (double)S.alpha[i]);
@@ -1360,18 +1396,50 @@ __global__ __launch_bounds__(kBlock) void k_df_err(const DevScene* __restrict__ 
     double e = 0.0;
     if (i < S.n) {
         const float4 xi = ld4(S.x, i);
-        const float4 vi = ld4(S.v, i);
         const double xx = xi.x, xy = xi.y, xz = xi.z;
-        const double ux = vi.x, uy = vi.y, uz = vi.z;
-        double dc = 0.0;
-        // the velocity sweep before this one wrote x.w and vyz
-        sweep4(S, i, [&](uint32_t j) { XVP r; r.xv = ld4o(S.x, j); r.yz = ld2o(S.vyz, j); return r; },
-               [&](uint32_t, const XVP& d) {
-                   const PairG p = pairg_of(xx, xy, xz, d.xv, k);
-                   dc += p.g * ((ux - (double)d.xv.w) * p.dx + (uy - (double)d.yz.x) * p.dy + (uz - (double)d.yz.y) * p.dz);
-               });
         const float4 gb = S.gb[i];
-        dc = S.mD * dc + (ux * (double)gb.x + uy * (double)gb.y + uz * (double)gb.z);
+        double dc;
+        if (DENS) {
+            // the density solve: x.w and vyz hold u, v holds v0 (see store_u)
+            const float2 yz = ld2o(S.vyz, (uint32_t)i);
+            const double ux = xi.w, uy = yz.x, uz = yz.y;
+            double su = 0.0;
+            if (MODE == 0) {
+                // the constant part D0 over v0, once per step
+                const float4 vi = ld4(S.v, i);
+                const double wx = vi.x, wy = vi.y, wz = vi.z;
+                double s0 = 0.0;
+                sweep4(S, i,
+                       [&](uint32_t j) { XVU r; r.x = ld4o(S.x, j); r.v = ld4o(S.v, j); r.yz = ld2o(S.vyz, j); return r; },
+                       [&](uint32_t, const XVU& d) {
+                           const PairG p = pairg_of(xx, xy, xz, d.x, k);
+                           s0 += p.g * ((wx - (double)d.v.x) * p.dx + (wy - (double)d.v.y) * p.dy + (wz - (double)d.v.z) * p.dz);
+                           su += p.g * ((ux - (double)d.x.w) * p.dx + (uy - (double)d.yz.x) * p.dy + (uz - (double)d.yz.y) * p.dz);
+                       });
+                const double d0 = S.mD * s0 + (wx * (double)gb.x + wy * (double)gb.y + wz * (double)gb.z);
+                ((gptr<double>)S.d0)[i] = d0;
+                dc = d0;
+            } else {
+                sweep4(S, i, [&](uint32_t j) { XVP r; r.xv = ld4o(S.x, j); r.yz = ld2o(S.vyz, j); return r; },
+                       [&](uint32_t, const XVP& d) {
+                           const PairG p = pairg_of(xx, xy, xz, d.xv, k);
+                           su += p.g * ((ux - (double)d.xv.w) * p.dx + (uy - (double)d.yz.x) * p.dy + (uz - (double)d.yz.y) * p.dz);
+                       });
+                dc = ((gptr<const double>)S.d0)[i];
+            }
+            dc += S.mD * su + (ux * (double)gb.x + uy * (double)gb.y + uz * (double)gb.z);
+        } else {
+            const float4 vi = ld4(S.v, i);
+            const double ux = vi.x, uy = vi.y, uz = vi.z;
+            dc = 0.0;
+            // the velocity sweep before this one wrote x.w and vyz
+            sweep4(S, i, [&](uint32_t j) { XVP r; r.xv = ld4o(S.x, j); r.yz = ld2o(S.vyz, j); return r; },
+                   [&](uint32_t, const XVP& d) {
+                       const PairG p = pairg_of(xx, xy, xz, d.xv, k);
+                       dc += p.g * ((ux - (double)d.xv.w) * p.dx + (uy - (double)d.yz.x) * p.dy + (uz - (double)d.yz.y) * p.dz);
+                   });
+            dc = S.mD * dc + (ux * (double)gb.x + uy * (double)gb.y + uz * (double)gb.z);
+        }
         const double drho = (double)S.drho[i];
         const double rho = S.rho0D + drho;
         const double al = S.alpha[i];
@@ -1402,12 +1470,19 @@ __global__ __launch_bounds__(kBlock) void k_df_positions(const DevScene* __restr
     const Chunk c = C[blockIdx.x];
     const DevScene& S = T[c.scene];
     if (blockIdx.x == 0 && threadIdx.x < 2) active[threadIdx.x] = 0;
+    // the lists about to be built start without an overflow
+    if (c.begin == 0 && threadIdx.x == 0) S.ss->listOver = 0u;
     const int i = c.begin + threadIdx.x;
     int cl = -1;
     if (i < S.n) {
         const double dt = S.ss->dt;
         float4 x = S.x[i];
-        const float4 v = S.v[i];
+        // the density solve's velocity: v0 + u (see store_u)
+        const float4 v0 = S.v[i];
+        const float2 uyz = S.vyz[i];
+        const float4 v = make_float4((float)((double)v0.x + (double)x.w), (float)((double)v0.y + (double)uyz.x),
+                                     (float)((double)v0.z + (double)uyz.y), 0.f);
+        S.v[i] = v;
         x.x = (float)((double)x.x + (double)v.x * dt);
         x.y = (float)((double)x.y + (double)v.y * dt);
         x.z = (float)((double)x.z + (double)v.z * dt);
@@ -1959,7 +2034,24 @@ __global__ __launch_bounds__(kBlock) void k_pb_predcorrect(const DevScene* __res
 // WCSPH: time += dt after the integration (src/WCSPHSolver.cpp:77).
 __global__ void k_time_advance(const DevScene* __restrict__ T, int nscenes) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s < nscenes) T[s].ss->time += T[s].ss->dt;
+    if (s >= nscenes) return;
+    Scalars* ss = T[s].ss;
+    ss->time += ss->dt;
+    // which predicted-density path the PCISPH step took
+    if (T[s].method == PCISPH) {
+        if (T[s].pciShell && ss->shellBad == 0u) ss->shellSteps += 1u;
+        else ss->walkSteps += 1u;
+    }
+}
+
+// The neighbor-overflow maximum of every scene of a batch starts each step
+// call at the overflow of the lists the call starts with -- DFSPH's first
+// sweeps use the lists of the previous call's last rebuild (or of init /
+// set_state); WCSPH and PCISPH build theirs at the start of each step -- so
+// it reports the lists of that call's steps only.
+__global__ void k_clear_overflow(const DevScene* __restrict__ T, int nscenes) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < nscenes) T[s].ss->overflow = (T[s].method == DFSPH) ? T[s].ss->listOver : 0u;
 }
 
 // ============================================================ utilities
@@ -2116,6 +2208,9 @@ void launch_emit(hipStream_t st, DevScene* Tc, DevScene* To, int nscenes, int ce
 }
 void launch_time_advance(hipStream_t st, const DevScene* T, int nscenes) {
     hipLaunchKernelGGL(k_time_advance, dim3((nscenes + 255) / 256), dim3(256), 0, st, T, nscenes);
+}
+void launch_clear_overflow(hipStream_t st, const DevScene* T, int nscenes) {
+    hipLaunchKernelGGL(k_clear_overflow, dim3((nscenes + 255) / 256), dim3(256), 0, st, T, nscenes);
 }
 void launch_fill_int(hipStream_t st, int* p, int v, long n) {
     if (n <= 0) return;

@@ -16,6 +16,11 @@ Continuous batching (`serve_batched`):
     that arrive meanwhile join at the next quantum;
   * a scene that has done its steps is read back, freed and its request
     completed; the rest continue;
+  * a step call that reports a neighbor overflow (SPHG_EOVERFLOW: some
+    particle had more neighbors than its list holds) fails only the requests
+    whose scenes overflowed: they are freed and completed with an error, the
+    batch is rebuilt without them, and the other scenes, which stepped
+    normally, continue;
   * with `parallel_methods`, the three methods' batches run their quantum at
     the same time, each on its own context (HIP stream) driven by its own
     host thread: a launch-bound batch of small scenes then overlaps the
@@ -162,6 +167,10 @@ class GpuBackend:
     def result(self, scene) -> np.ndarray:
         return scene.get(sphg.FIELD_X)
 
+    def overflowed(self, scene) -> bool:
+        """Whether the scene's last step call had a neighbor overflow."""
+        return scene.scalars().neighbor_overflow > 0
+
     def close(self, obj) -> None:
         obj.close()
 
@@ -183,6 +192,7 @@ class Job:
     admitted: float = -1.0
     done: float = -1.0
     result: Optional[np.ndarray] = None
+    error: Optional[str] = None      # set when the request failed (completed without a result)
 
 
 @dataclass
@@ -193,10 +203,11 @@ class ServeResult:
     rejected: List[int] = field(default_factory=list)
 
     def summary(self) -> dict:
-        done = [j for j in self.jobs if j.done >= 0]
+        done = [j for j in self.jobs if j.done >= 0 and j.error is None]
+        failed = [j for j in self.jobs if j.error is not None]
         lat = np.array([j.done - j.req.arrival for j in done]) if done else np.zeros(1)
         work = float(sum(j.n * j.req.steps for j in done))
-        return {"mode": self.mode, "requests": len(done), "rejected": len(self.rejected),
+        return {"mode": self.mode, "requests": len(done), "rejected": len(self.rejected), "failed": len(failed),
                 "particle_steps": work, "makespan_s": round(self.makespan, 3),
                 "particle_steps_per_s": round(work / self.makespan, 1) if self.makespan > 0 else None,
                 "latency_p50_s": round(float(np.percentile(lat, 50)), 4),
@@ -206,6 +217,23 @@ class ServeResult:
 
 def _spec_pool():
     return ThreadPoolExecutor(max_workers=2, thread_name_prefix="spec")
+
+
+def _is_overflow(e: BaseException) -> bool:
+    return isinstance(e, sphg.SphgError) and e.code == sphg.EOVERFLOW
+
+
+def _run_quantum(batch, k):
+    """Runs a batch quantum; returns the neighbor-overflow error it raised,
+    if any (the other scenes of the batch have stepped normally), and lets
+    every other error propagate."""
+    try:
+        batch.run(k)
+    except sphg.SphgError as e:
+        if not _is_overflow(e):
+            raise
+        return e
+    return None
 
 
 def serve_batched(backend, requests: List[Request], budget_bytes: int, quantum: int = 25,
@@ -271,23 +299,46 @@ def serve_batched(backend, requests: List[Request], budget_bytes: int, quantum: 
                 if batches[m] is None:
                     batches[m] = backend.batch([j.scene for j in js])
                 passes.append((m, js, min(quantum, min(j.remaining for j in js))))
+            over = {}
             if runner is not None and len(passes) > 1:
-                futs = [runner.submit(batches[m].run, k) for m, _, k in passes]
+                futs = [runner.submit(_run_quantum, batches[m], k) for m, _, k in passes]
                 # every pass ends before any error is raised (the batches
                 # are closed on the way out)
                 errs = [f.exception() for f in futs]
                 for e in errs:
                     if e is not None:
                         raise e
+                for (m, _, _), f in zip(passes, futs):
+                    if f.result() is not None:
+                        over[m] = f.result()
             else:
                 for m, _, k in passes:
-                    batches[m].run(k)
+                    e = _run_quantum(batches[m], k)
+                    if e is not None:
+                        over[m] = e
             for m, js, k in passes:
+                if m in over:
+                    bad = [j for j in js if backend.overflowed(j.scene)]
+                    if not bad:
+                        raise over[m]
+                    backend.close(batches[m])
+                    batches[m] = None
+                    for j in bad:
+                        backend.close(j.scene)
+                        j.scene = None
+                        used -= j.bytes
+                        j.done = clock() - t0
+                        j.error = "neighbor overflow: a particle had more neighbors than max_neighbors"
+                        if log:
+                            log(f"[serve] failed req {j.req.rid} {METHOD_NAMES[m]} n={j.n}: {j.error}")
+                    js = [j for j in js if j.error is None]
+                    active[m] = js
                 for j in js:
                     j.remaining -= k
                 fin = [j for j in js if j.remaining == 0]
                 if fin:
-                    backend.close(batches[m])
+                    if batches[m] is not None:
+                        backend.close(batches[m])
                     batches[m] = None
                     for j in fin:
                         res = backend.result(j.scene)
@@ -352,9 +403,16 @@ def serve_plain(backend, requests: List[Request], budget_bytes: int,
                 continue
             backend.init(scene)
             j.admitted = clock() - t0
-            backend.step(scene, q.steps)
-            res = backend.result(scene)
-            j.result = res if keep_results else None
+            try:
+                backend.step(scene, q.steps)
+            except sphg.SphgError as e:
+                if not _is_overflow(e):
+                    backend.close(scene)
+                    raise
+                j.error = "neighbor overflow: a particle had more neighbors than max_neighbors"
+            if j.error is None:
+                res = backend.result(scene)
+                j.result = res if keep_results else None
             backend.close(scene)
             j.done = clock() - t0
             if log:
@@ -365,6 +423,30 @@ def serve_plain(backend, requests: List[Request], budget_bytes: int,
     done = [j.done for j in jobs if j.done >= 0]
     first = min(q.arrival for q in requests) if requests else 0.0
     return ServeResult("plain", jobs, (max(done) - first) if done else 0.0, rejected)
+
+
+def compare(ctx, reqs: List[Request], budget_bytes: int, quantum: int = 25, check: int = 4,
+            parallel_methods: bool = True, log: Callable[[str], None] = None) -> dict:
+    """Serves `reqs` with continuous batching, then plain (one at a time),
+    on the same device; returns both summaries, the throughput ratio and how
+    many of the first `check` requests had identical results in both."""
+    be = GpuBackend(ctx, method_contexts=parallel_methods)
+    try:
+        rb = serve_batched(be, reqs, budget_bytes, quantum, log=log, parallel_methods=parallel_methods)
+        rp = serve_plain(be, reqs, budget_bytes, log=log)
+    finally:
+        be.shutdown()
+    sb, sp = rb.summary(), rp.summary()
+    same = checked = 0
+    byid_b = {j.req.rid: j for j in rb.jobs}
+    for j in rp.jobs[:check]:
+        jb = byid_b.get(j.req.rid)
+        if jb is not None and jb.result is not None and j.result is not None:
+            checked += 1
+            same += int(np.array_equal(jb.result, j.result))
+    ratio = (round(sb["particle_steps_per_s"] / sp["particle_steps_per_s"], 3)
+             if sb["particle_steps_per_s"] and sp["particle_steps_per_s"] else None)
+    return {"batched": sb, "plain": sp, "throughput_ratio": ratio, "identical_results": f"{same}/{checked}"}
 
 
 def main():
@@ -385,32 +467,16 @@ def main():
 
     reqs = request_stream(args.requests, args.rate, args.nmin, args.nmax, args.smin, args.smax, args.seed)
     ctx = sphg.Context(0)
-    be = GpuBackend(ctx, method_contexts=not args.serial_methods)
-    budget = int(args.budget_gb * 2 ** 30)
     log = lambda s: print(s, flush=True)   # noqa: E731
-    rb = serve_batched(be, reqs, budget, args.quantum, log=log, parallel_methods=not args.serial_methods)
-    rp = serve_plain(be, reqs, budget, log=log)
-    sb, sp = rb.summary(), rp.summary()
-    # the first `check` requests: identical results alone and in the mix
-    same = 0
-    checked = 0
-    byid_b = {j.req.rid: j for j in rb.jobs}
-    for j in rp.jobs[: args.check]:
-        jb = byid_b.get(j.req.rid)
-        if jb is not None and jb.result is not None and j.result is not None:
-            checked += 1
-            same += int(np.array_equal(jb.result, j.result))
+    r = compare(ctx, reqs, int(args.budget_gb * 2 ** 30), args.quantum, args.check,
+                parallel_methods=not args.serial_methods, log=log)
     out = {"metric": "served-scene p50/p99 latency and particle-steps/s, continuous batching vs plain GPU",
            "stream": {"requests": args.requests, "rate_per_s": args.rate, "n": [args.nmin, args.nmax],
                       "steps": [args.smin, args.smax], "methods": "WCSPH/PCISPH/DFSPH uniform",
                       "geometry": "dam-break block or sphere drop, spacing 0.02", "seed": args.seed,
                       "budget_gb": args.budget_gb, "quantum": args.quantum,
-                      "methods_in_parallel": not args.serial_methods},
-           "batched": sb, "plain": sp,
-           "throughput_ratio": round(sb["particle_steps_per_s"] / sp["particle_steps_per_s"], 3)
-           if sb["particle_steps_per_s"] and sp["particle_steps_per_s"] else None,
-           "identical_results": f"{same}/{checked}"}
-    be.shutdown()
+                      "methods_in_parallel": not args.serial_methods}}
+    out.update(r)
     ctx.close()
     print(json.dumps(out), flush=True)
 

@@ -26,10 +26,10 @@ FIELD_X, FIELD_V, FIELD_A, FIELD_RHO, FIELD_P = 0, 1, 2, 3, 4
 FIELD_ALPHA, FIELD_RHOSTAR, FIELD_DIVERR, FIELD_K, FIELD_KV = 5, 6, 7, 8, 9
 FIELD_BVOL, FIELD_NORMAL, FIELD_PACC, FIELD_KEYS, FIELD_NNB, FIELD_NGRAD = 10, 11, 12, 13, 14, 15
 FIELD_PBNORMAL, FIELD_BDENSITY, FIELD_BPRESSURE = 16, 17, 18
-FIELD_DRHO, FIELD_DRHOSTAR = 19, 20
+FIELD_DRHO, FIELD_DRHOSTAR, FIELD_NSH = 19, 20, 21
 
 _VEC3 = {FIELD_X, FIELD_V, FIELD_A, FIELD_NORMAL, FIELD_PACC, FIELD_PBNORMAL}
-_INT = {FIELD_KEYS, FIELD_NNB, FIELD_NGRAD}
+_INT = {FIELD_KEYS, FIELD_NNB, FIELD_NGRAD, FIELD_NSH}
 
 
 class SphgError(RuntimeError):
@@ -95,6 +95,8 @@ class SceneScalars(C.Structure):
         ("sum_iterations", C.c_uint32),
         ("sum_iterations_v", C.c_uint32),
         ("neighbor_overflow", C.c_uint32),
+        ("pcisph_shell_steps", C.c_uint32),
+        ("pcisph_walk_steps", C.c_uint32),
     ]
 
 

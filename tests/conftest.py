@@ -45,9 +45,18 @@ def rel_elem(a, b) -> float:
     return float(np.max(np.abs(a - b) / np.abs(b))) if b.size else 0.0
 
 
-def dev_err(g_drho, o_rho, rho0) -> float:
+def dev_err(g_drho, o_rho, rho0, floor: float = 0.0) -> float:
     """Normwise relative error of a density deviation: the solver's stored
     rho - rho0 (FIELD_DRHO / FIELD_DRHOSTAR) against the oracle's FP64
-    density minus rho0.  The deviation, not the density near rho0, is what
-    the pressure solvers run on."""
-    return rel_norm(g_drho, np.asarray(o_rho, dtype=np.float64) - float(rho0))
+    density minus rho0, max|a - b| / max(max|b|, floor).  The deviation, not
+    the density near rho0, is what the pressure solvers run on.  `floor`
+    (the density solve's own tolerance eta rho0 where one applies) keeps a
+    block at rest, whose deviations are all near zero, from dividing by
+    nothing: 1e-5 of eta rho0 = 1e-4 kg/m^3 at eta = 1e-4 is 1e-9 of rho0."""
+    a = np.asarray(g_drho, dtype=np.float64)
+    b = np.asarray(o_rho, dtype=np.float64) - float(rho0)
+    if not b.size:
+        return 0.0
+    den = max(float(np.abs(b).max()), float(floor))
+    num = float(np.abs(a - b).max())
+    return num / den if den > 0 else num

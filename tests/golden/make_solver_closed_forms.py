@@ -219,7 +219,7 @@ def dfsph_pair(d, U, dt):
     c = D("0.5")
     x = [(c, c, c), (c + d, c, c)]
     v = [(U, D(1) / 8, D(0)), (-U, D(0), D(1) / 4)]
-    far = [(D(50), D(50), D(50))]
+    far = [(D(1), D(1), D(1))]
     vb = akinci_volumes(far)
     rho = densities(x, far, vb)
     alpha = alphas(x, far, vb, rho)
@@ -280,7 +280,7 @@ def dfsph_cluster(dt):
     v = [mul(-rate, sub(p, (c, c, c))) for p in x0]
     assert rate * dt == D(1) / 64
     x = [add(x0[i], mul(dt, v[i])) for i in range(len(x0))]
-    far = [(D(50), D(50), D(50))]
+    far = [(D(1), D(1), D(1))]
     vb = akinci_volumes(far)
     rho = densities(x, far, vb)
     alpha = alphas(x, far, vb, rho)
@@ -431,7 +431,7 @@ def surface_tension(gamma, dt):
     # particle 1 at q = 0.375 from particle 0 (inner branch), particle 2 at
     # q = 0.75 (outer branch); 1 and 2 are 0.0527 apart (q = 0.84)
     x = shifted([("0", "0", "0"), ("3/128", "0", "0"), ("0", "3/64", "0")])
-    far = [(D(50), D(50), D(50))]
+    far = [(D(1), D(1), D(1))]
     vb = akinci_volumes(far)
     rho = densities(x, far, vb)
     n = len(x)

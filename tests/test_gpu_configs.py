@@ -24,8 +24,20 @@ rebuilds the grid, densities and alpha), and steps both once from there:
 Fields are held to 1e-5 relative, normwise (max|gpu - ref| / max|ref|),
 the divergence error to 1e-4 (its floor is set by FP32 velocities between
 the sweeps); density deviations rho - rho0 and rho* - rho0 are compared as
-stored.  Iteration counts agree within one; fields are compared on a step
-whose counts agree (up to three consecutive steps are tried).
+stored, against max(max|ref|, eta rho0).  Iteration counts agree within
+one; fields are compared on a step whose counts agree (up to three
+consecutive steps are tried).  The DFSPH oracle rounds its updated positions
+to FP32 as the device stores them; the error against the FP64 run of the
+same step is logged beside.
+
+The reference's DFSPH eps branches (|k_i + k_j| > 1e-5) are hard
+thresholds: a particle whose rho* - rho0 puts a pair sum within the FP32
+state's resolution of 1e-5 may take the other branch on the device, which
+moves it and its neighbors by ~1e-4 m/s.  Such particles are found by
+stepping the oracle with the threshold moved by -1e-4 and +1e-4 (relative);
+they are counted (~2,100 of the 1M particles in one step, with the
+neighbors their flips move; at most 0.5% is allowed) and left out of the
+field comparison.
 """
 import os
 import time
@@ -47,26 +59,85 @@ def _log(msg):
     print(f"[configs] {msg}", flush=True)
 
 
-def _inject(g, o):
+def _state(g):
     sc = g.scalars()
-    o.set_state(g.get(sphg.FIELD_X), g.get(sphg.FIELD_V),
-                g.get(sphg.FIELD_K) if g.params.method == sphg.DFSPH else None,
-                g.get(sphg.FIELD_KV) if g.params.method == sphg.DFSPH else None,
-                sc.time_step, sc.iterations)
+    dfs = g.params.method == sphg.DFSPH
+    return (g.get(sphg.FIELD_X), g.get(sphg.FIELD_V), g.get(sphg.FIELD_K) if dfs else None,
+            g.get(sphg.FIELD_KV) if dfs else None, sc.time_step, sc.iterations)
 
 
-def _dfsph_fields(g, o):
-    rho0 = g.params.density0
-    return {
-        "v": rel_norm(g.get(sphg.FIELD_V), o.get(po.OF_V)),
-        "x": rel_norm(g.get(sphg.FIELD_X), o.get(po.OF_X)),
-        "drho": dev_err(g.get(sphg.FIELD_DRHO), o.get(po.OF_RHO), rho0),
-        "drhostar": dev_err(g.get(sphg.FIELD_DRHOSTAR), o.get(po.OF_RHOSTAR), rho0),
-        "alpha": rel_norm(g.get(sphg.FIELD_ALPHA), o.get(po.OF_ALPHA)),
-        "k": rel_norm(g.get(sphg.FIELD_K), o.get(po.OF_K)),
-        "kv": rel_norm(g.get(sphg.FIELD_KV), o.get(po.OF_KV)),
-        "diverr": rel_norm(g.get(sphg.FIELD_DIVERR), o.get(po.OF_DIVERR)),
-    }
+def _inject(g, o, state=None):
+    """Hands the device's state (or a saved one) to the oracle.  A DFSPH
+    oracle rounds its updated positions to FP32 (oracle_set_f32_positions),
+    as the device stores them, so the grid rebuild, densities, alpha and the
+    divergence solve of the step see the same positions on both sides."""
+    o.set_state(*(state or _state(g)))
+    if g.params.method == sphg.DFSPH:
+        o.set_f32_positions(True)
+
+
+def _floor(p):
+    """Deviation scale below which density errors are measured against the
+    solver's tolerance eta rho0 (a scene near rest has deviations near 0)."""
+    return p.eta * p.density0
+
+
+_DF = {"v": (sphg.FIELD_V, po.OF_V), "x": (sphg.FIELD_X, po.OF_X), "drho": (sphg.FIELD_DRHO, po.OF_RHO),
+       "drhostar": (sphg.FIELD_DRHOSTAR, po.OF_RHOSTAR), "alpha": (sphg.FIELD_ALPHA, po.OF_ALPHA),
+       "k": (sphg.FIELD_K, po.OF_K), "kv": (sphg.FIELD_KV, po.OF_KV), "diverr": (sphg.FIELD_DIVERR, po.OF_DIVERR)}
+
+
+def _oracle_fields(o, rho0):
+    out = {k: o.get(f) for k, (_, f) in _DF.items()}
+    out["drho"] = out["drho"] - rho0
+    out["drhostar"] = out["drhostar"] - rho0
+    return out
+
+
+def _per(a):
+    a = np.abs(np.asarray(a, dtype=np.float64))
+    return a.max(1) if a.ndim > 1 else a
+
+
+def _eps_sensitive(o, start, ref, rho0, rel=1e-4):
+    """Particles whose step depends on a DFSPH eps branch (|k_i + k_j| >
+    1e-5, src/DFSPHSolver.cpp:26,364,642) decided within `rel` of its
+    threshold: the oracle steps again from `start` with the threshold moved
+    by -rel and by +rel, and a particle is sensitive where either run moves
+    one of its fields by more than 1e-7 of the field's largest value.  An
+    FP32 state resolves rho* - rho0 to ~1e-9 kg/m^3, so near the threshold
+    its decision may go the other way than the FP64 run's; such a flip moves
+    the velocities of the particle and its neighbors by ~1e-4 m/s."""
+    sens = None
+    for f in (1.0 - rel, 1.0 + rel):
+        o.set_state(*start)
+        o.set_f32_positions(True)
+        o.set_eps(1e-5 * f)
+        o.step(1)
+        alt = _oracle_fields(o, rho0)
+        for k, r in ref.items():
+            m = _per(alt[k] - r) > 1e-7 * max(float(np.abs(r).max()), 1e-30)
+            sens = m if sens is None else (sens | m)
+    o.set_eps(1e-5)
+    return sens
+
+
+def _dfsph_fields(g, o, mask=None, ref=None):
+    """Normwise errors of the DFSPH fields; particles in `mask` are left out
+    (normalised by the whole field's largest value all the same)."""
+    rho0, fl = g.params.density0, _floor(g.params)
+    ref = ref or _oracle_fields(o, rho0)
+    keep = slice(None) if mask is None else ~mask
+    out = {}
+    for k, (gf, _) in _DF.items():
+        a = np.asarray(g.get(gf), dtype=np.float64)
+        b = ref[k]
+        den = float(np.abs(b).max())
+        if k in ("drho", "drhostar"):
+            den = max(den, fl)
+        num = float(_per(a - b)[keep].max()) if a.size else 0.0
+        out[k] = num / den if den > 0 else num
+    return out
 
 
 def _check(errs, tols=None):
@@ -105,7 +176,8 @@ def _grid_parity(g, o, spec, sample=10000, dense=512, seed=5):
         assert set(got.tolist()) == set(cand[keep].tolist()), f"neighbors of particle {i}"
         ql, rk = q[keep], rl[keep]
         nz = (rk > 1e-5) & np.where(ql <= 0.5, ql * (3.0 * ql - 2.0) != 0.0, (1.0 - ql) * (1.0 - ql) != 0.0)
-        assert ngrad[i] == int(nz.sum()), f"nonzero-gradient count of particle {i}"
+        # the stored count saturates at 255 (it feeds the cutoff at 20)
+        assert ngrad[i] == min(int(nz.sum()), 255), f"nonzero-gradient count of particle {i}"
     return len(pick), int(counts.max())
 
 
@@ -116,13 +188,15 @@ def test_dfsph_dam_1m_from_mid_collapse(ctx):
     g.step(150)
     _log(f"1M: 150 device steps, t = {g.scalars().time:.4f} s, it_d {g.scalars().iterations} ({time.time() - t0:.1f} s)")
     o = po.Oracle(spec.params, spec.fluid, spec.boundary)
-    _inject(g, o)
+    start = _state(g)
+    _inject(g, o, start)
     npick, cmax = _grid_parity(g, o, spec)
     _log(f"1M: keys, buckets and {npick} neighbor sets exact (longest list {cmax}) ({time.time() - t0:.1f} s)")
     matched = None
     for attempt in range(3):
         if attempt:
-            _inject(g, o)
+            start = _state(g)
+            _inject(g, o, start)
         g.step(1)
         o.step(1)
         sg, so = g.scalars(), o.scalars()
@@ -130,13 +204,33 @@ def test_dfsph_dam_1m_from_mid_collapse(ctx):
         assert abs(its[0] - its[1]) <= 1 and abs(its[2] - its[3]) <= 1, its
         assert abs(sg.time_step - so["time_step"]) <= 1e-6 * so["time_step"]
         if its[0] == its[1] and its[2] == its[3]:
-            matched = _dfsph_fields(g, o)
-            _log(f"1M: step matched, iterations {its}, errors "
-                 f"{ {k: f'{v:.1e}' for k, v in matched.items()} } ({time.time() - t0:.1f} s)")
+            ref = _oracle_fields(o, spec.params.density0)
+            sens = _eps_sensitive(o, start, ref, spec.params.density0)
+            matched = _dfsph_fields(g, o, sens, ref)
+            whole = _dfsph_fields(g, o, None, ref)
+            _log(f"1M: step matched, iterations {its}; {int(sens.sum())} particles at an eps branch threshold; "
+                 f"errors elsewhere { {k: f'{v:.1e}' for k, v in matched.items()} }, over all particles "
+                 f"{ {k: f'{v:.1e}' for k, v in whole.items()} } ({time.time() - t0:.1f} s)")
             break
         _log(f"1M: iteration counts {its} differ by one; next step")
     assert matched is not None
+    assert sens.sum() <= 5e-3 * spec.n, int(sens.sum())
     _check(matched, {"diverr": TOL_DIVERR})
+    # the same step in FP64 throughout (positions not rounded): the error of
+    # the FP32 state against an FP64 run, stated; particles whose count of
+    # nonzero-gradient neighbors sits at the divergence cutoff of 20 can
+    # switch sides on a one-ulp position difference, so this is not held to
+    # the parity bar
+    o.set_state(*start)
+    o.set_f32_positions(False)
+    o.step(1)
+    so = o.scalars()
+    if (int(so["iterations"]), int(so["iterations_v"])) == (its[0], its[2]):
+        f64 = _dfsph_fields(g, o, sens)
+        _log(f"1M: against the FP64 run of the same step: { {k: f'{v:.1e}' for k, v in f64.items()} }")
+        assert f64["v"] < 1e-3 and f64["x"] < 1e-5, f64
+    else:
+        _log(f"1M: the FP64 run took {int(so['iterations'])}/{int(so['iterations_v'])} iterations")
     o.close()
     g.close()
 
@@ -162,7 +256,7 @@ def test_pcisph_spheres_full_size(ctx):
         assert fa <= TOL and fn <= TOL, (fa, fn)
         if gi == oi:
             matched = {"a": fa, "normal": fn,
-                       "drho": dev_err(g.get(sphg.FIELD_DRHO), o.get(po.OF_RHO), rho0),
+                       "drho": dev_err(g.get(sphg.FIELD_DRHO), o.get(po.OF_RHO), rho0, _floor(spec.params)),
                        "p": rel_norm(g.get(sphg.FIELD_P), o.get(po.OF_P)),
                        "pacc": rel_norm(g.get(sphg.FIELD_PACC), o.get(po.OF_PACC)),
                        "v": rel_norm(g.get(sphg.FIELD_V), o.get(po.OF_V)),
@@ -186,7 +280,7 @@ def test_wcsph_dam_32k_full_size(ctx):
         _inject(g, o)
         g.step(1)
         o.step(1)
-        errs = {"drho": dev_err(g.get(sphg.FIELD_DRHO), o.get(po.OF_RHO), rho0),
+        errs = {"drho": dev_err(g.get(sphg.FIELD_DRHO), o.get(po.OF_RHO), rho0, _floor(spec.params)),
                 "p": rel_norm(g.get(sphg.FIELD_P), o.get(po.OF_P)),
                 "a": rel_norm(g.get(sphg.FIELD_A), o.get(po.OF_ACC_STEP)),
                 "v": rel_norm(g.get(sphg.FIELD_V), o.get(po.OF_V)),
@@ -213,7 +307,7 @@ def test_dense_lattice_build_reads_past_its_stage(ctx):
     npick, cmax = _grid_parity(g, o, dspec, sample=4096, dense=256)
     assert cmax > 200, cmax
     rho0 = params.density0
-    assert dev_err(g.get(sphg.FIELD_DRHO), o.get(po.OF_RHO), rho0) <= TOL
+    assert dev_err(g.get(sphg.FIELD_DRHO), o.get(po.OF_RHO), rho0, _floor(params)) <= TOL
     assert rel_norm(g.get(sphg.FIELD_ALPHA), o.get(po.OF_ALPHA)) <= TOL
     _log(f"dense lattice: {spec.n} particles, longest list {cmax}, r {r}")
     o.close()
@@ -234,8 +328,9 @@ def test_batched_256_scenes(ctx):
     b = sphg.Batch(ctx, scenes)
     compared = 0
     for step in range(5):
+        starts = {k: _state(scenes[k]) for k in watch}
         for k, o in watch.items():
-            _inject(scenes[k], o)
+            _inject(scenes[k], o, starts[k])
         b.run(1)
         for k, o in watch.items():
             o.step(1)
@@ -243,7 +338,10 @@ def test_batched_256_scenes(ctx):
             its = (sg.iterations, int(so["iterations"]), sg.iterations_v, int(so["iterations_v"]))
             assert abs(its[0] - its[1]) <= 1 and abs(its[2] - its[3]) <= 1, (k, its)
             if its[0] == its[1] and its[2] == its[3]:
-                _check(_dfsph_fields(scenes[k], o), {"diverr": TOL_DIVERR})
+                rho0 = specs[k].params.density0
+                ref = _oracle_fields(o, rho0)
+                sens = _eps_sensitive(o, starts[k], ref, rho0)
+                _check(_dfsph_fields(scenes[k], o, sens, ref), {"diverr": TOL_DIVERR})
                 compared += 1
     assert compared >= 8, compared
     b.close()

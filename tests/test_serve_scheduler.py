@@ -2,7 +2,7 @@
 stands in for the device (the GPU path runs in test_gpu_serve.py)."""
 import numpy as np
 
-from sph_amd import serve
+from sph_amd import serve, sphg
 
 
 class Clock:
@@ -22,6 +22,7 @@ class FakeScene:
         self.steps = 0
         self.ready = False
         self.closed = False
+        self.overflow = False       # its neighbor lists overflow on every step
 
 
 class FakeBatch:
@@ -37,6 +38,9 @@ class FakeBatch:
                 s.steps += k
             # one launch sequence for the whole batch: fixed cost + work
             self.be.clock.t += k * (2e-4 + 1e-9 * sum(s.n for s in self.scenes))
+        if any(s.overflow for s in self.scenes):
+            # as the library: every scene stepped, the call reports the overflow
+            raise sphg.SphgError(sphg.EOVERFLOW, "neighbor overflow")
 
     def close(self):
         self.closed = True
@@ -75,6 +79,9 @@ class FakeBackend:
 
     def result(self, s):
         return np.array([s.steps], dtype=np.int64)
+
+    def overflowed(self, s):
+        return s.overflow
 
     def close(self, obj):
         if isinstance(obj, FakeScene) and obj.ready and not obj.closed:
@@ -184,3 +191,54 @@ def test_batching_beats_plain_on_launch_bound_stream():
     sb, sp = rb.summary(), rp.summary()
     assert sb["particle_steps_per_s"] > 2 * sp["particle_steps_per_s"]
     assert sb["latency_p50_s"] < sp["latency_p50_s"]
+
+
+def test_neighbor_overflow_fails_only_its_request():
+    """A scene whose neighbor lists overflow fails its own request; the other
+    resident scenes of its batch and of the other methods finish with their
+    exact step counts, and the memory comes back."""
+    for parallel in (False, True):
+        clk = Clock()
+        be = FakeBackend(clk)
+        reqs = _stream(16, rate=200.0, seed=11)
+        bad = reqs[3].rid
+        create = be.create
+
+        def create_marked(spec, _c=create):
+            sc = _c(spec)
+            sc.overflow = spec.name.startswith(f"req{bad}_")
+            return sc
+        be.create = create_marked
+        r = serve.serve_batched(be, reqs, budget_bytes=10 ** 12, quantum=7, clock=clk, sleep=clk.sleep,
+                                parallel_methods=parallel)
+        byid = {j.req.rid: j for j in r.jobs}
+        assert byid[bad].error and byid[bad].result is None
+        for q in reqs:
+            if q.rid != bad:
+                j = byid[q.rid]
+                assert j.error is None and int(j.result[0]) == q.steps, (q.rid, j.error)
+        s = r.summary()
+        assert s["failed"] == 1 and s["requests"] == len(reqs) - 1
+        assert be.used == 0
+
+    clk = Clock()
+    be = FakeBackend(clk)
+    reqs = _stream(6, rate=200.0, seed=12)
+    bad = reqs[2].rid
+    create = be.create
+
+    def create_marked2(spec, _c=create):
+        sc = _c(spec)
+        sc.overflow = spec.name.startswith(f"req{bad}_")
+        return sc
+    be.create = create_marked2
+
+    def step(sc, k, _s=be.step):
+        _s(sc, k)
+        if sc.overflow:
+            raise sphg.SphgError(sphg.EOVERFLOW, "neighbor overflow")
+    be.step = step
+    r = serve.serve_plain(be, reqs, budget_bytes=10 ** 12, clock=clk, sleep=clk.sleep)
+    byid = {j.req.rid: j for j in r.jobs}
+    assert byid[bad].error and all(byid[q.rid].error is None for q in reqs if q.rid != bad)
+    assert r.summary()["failed"] == 1
