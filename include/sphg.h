@@ -265,6 +265,21 @@ int sphg_scene_info(sphg_scene* scene, uint32_t* n, uint32_t* nb, uint32_t* ncel
  * src/Simulation.cpp:84-90). */
 int sphg_batch_step(sphg_ctx* ctx, sphg_scene* const* scenes, uint32_t nscenes, uint32_t nsteps,
                     sphg_step_stats* stats);
+/* sphg_batch_step without any device allocation, for callers that cannot
+ * allocate (code under HIP graph capture, serving loops that forbid
+ * hipMalloc).  No counterpart in the reference.  The batch's tables -- two
+ * scene tables of 928 bytes per scene, 8 bytes per 256 fluid slots and 8
+ * bytes per 1024 grid cells, each rounded up to 256 bytes -- are written to
+ * a static device array of SPHG_NOALLOC_SCRATCH_BYTES that each device holds
+ * until the process ends; a batch that needs more is refused with
+ * SPHG_EINVAL.  Every call on a device shares that array: calls that
+ * overlap on one device (from two contexts or threads) race.  The scenes
+ * themselves were allocated by sphg_scene_init; gpu_ms is the host clock
+ * around the steps (no HIP events are created), and kernel timing must be
+ * off.  Results are those of sphg_batch_step, bit for bit. */
+#define SPHG_NOALLOC_SCRATCH_BYTES (8u << 20)
+int sphg_batch_step_noalloc(sphg_ctx* ctx, sphg_scene* const* scenes, uint32_t nscenes, uint32_t nsteps,
+                            sphg_step_stats* stats);
 /* The same with the batch kept between calls (its chunk tables and its
  * iteration-count prediction), for a serving loop that steps a changing set
  * of resident scenes in quanta: create when the set changes, run per

@@ -18,10 +18,20 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
 using namespace sphg;
+
+// Device memory of the allocation-free batch step (sphg_batch_step_noalloc):
+// the batch's two scene tables and its particle and cell chunk tables live
+// here instead of in memory the call allocates.  Each device that the
+// library runs on holds one copy until the process ends; host code uses its
+// address from hipGetSymbolAddress (scratch_address), never this name.
+static __device__ __align__(256) unsigned char scratch[SPHG_NOALLOC_SCRATCH_BYTES];
+// the table sizes sphg.h states for it
+static_assert(sizeof(sphg::DevScene) == 928 && sizeof(sphg::Chunk) == 8, "update the sizes in sphg.h");
 
 namespace {
 
@@ -136,6 +146,7 @@ struct Scope {
 
 struct sphg_ctx {
     int device = 0;
+    unsigned char* scratch = nullptr;   // this device's address of `scratch`
     hipStream_t stream = nullptr;
     hipStream_t copyStream = nullptr;   // snapshot read-backs (default priority)
     bool graphs = false;                // replay captured step segments (persistent batches)
@@ -173,6 +184,9 @@ struct Batch {
     // live across calls
     bool graphs = false;
     DevScene* tab0 = nullptr;
+    // the tables are the library's allocations (false: they point into the
+    // allocation-free step's scratch array, which is not freed)
+    bool ownTables = true;
     // launch schedule (any choice gives the same results): loop iterations
     // launched past the previous step's count before the first read-back,
     // and segment replay (-1: the context's setting)
@@ -182,6 +196,7 @@ struct Batch {
         for (auto& a : seg)
             for (auto& e : a)
                 if (e) (void)hipGraphExecDestroy(e);
+        if (!ownTables) return;
         if (dT[0]) (void)hipFree(dT[0]);
         if (dT[1]) (void)hipFree(dT[1]);
         if (dP) (void)hipFree(dP);
@@ -674,11 +689,15 @@ int batch_one_step(sphg_ctx* ctx, Batch& B, uint32_t& syncs) {
     return SPHG_OK;
 }
 
-int run_batch(sphg_ctx* ctx, Batch& B, uint32_t nsteps, sphg_step_stats* stats) {
+// `events`: time the steps with HIP events (false: with the host clock
+// around the steps and a stream synchronisation, creating nothing).
+int run_batch(sphg_ctx* ctx, Batch& B, uint32_t nsteps, sphg_step_stats* stats, bool events = true) {
     hipStream_t st = ctx->stream;
-    hipEvent_t e0, e1;
-    HIP_TRY(hipEventCreate(&e0));
-    HIP_TRY(hipEventCreate(&e1));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (events) {
+        HIP_TRY(hipEventCreate(&e0));
+        HIP_TRY(hipEventCreate(&e1));
+    }
     std::vector<Scalars> before(B.scenes.size());
     if (stats) {
         for (size_t s = 0; s < B.scenes.size(); ++s)
@@ -688,21 +707,27 @@ int run_batch(sphg_ctx* ctx, Batch& B, uint32_t nsteps, sphg_step_stats* stats) 
     uint32_t syncs = 0;
     // the overflow maximum reports the lists this call's steps use
     launch_clear_overflow(st, B.dT[0], (int)B.scenes.size());
-    HIP_TRY(hipEventRecord(e0, st));
+    const auto h0 = std::chrono::steady_clock::now();
+    if (events) HIP_TRY(hipEventRecord(e0, st));
     for (uint32_t k = 0; k < nsteps; ++k) {
         int rc = batch_one_step(ctx, B, syncs);
         if (rc) {
-            (void)hipEventDestroy(e0);
-            (void)hipEventDestroy(e1);
+            if (e0) (void)hipEventDestroy(e0);
+            if (e1) (void)hipEventDestroy(e1);
             return rc;
         }
     }
-    HIP_TRY(hipEventRecord(e1, st));
-    HIP_TRY(hipEventSynchronize(e1));
     float ms = 0.f;
-    HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
+    if (events) {
+        HIP_TRY(hipEventRecord(e1, st));
+        HIP_TRY(hipEventSynchronize(e1));
+        HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+    } else {
+        HIP_TRY(hipStreamSynchronize(st));
+        ms = (float)std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - h0).count();
+    }
     unsigned overflow = 0, nover = 0;
     if (stats) {
         std::memset(stats, 0, sizeof(*stats));
@@ -992,6 +1017,28 @@ int check_batch(sphg_ctx* ctx, sphg_scene* const* scenes, uint32_t nscenes) {
     return SPHG_OK;
 }
 
+// This device's address of `scratch`, looked up once per device (with the
+// device current) and kept.
+std::mutex g_scratch_mu;
+std::map<int, unsigned char*> g_scratch_addr;
+
+int scratch_address(int device, unsigned char** out) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    auto it = g_scratch_addr.find(device);
+    if (it != g_scratch_addr.end()) {
+        *out = it->second;
+        return SPHG_OK;
+    }
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+        return fail(SPHG_EHIP, std::string("address of the allocation-free step's scratch array: ") +
+                                   hipGetErrorString(e));
+    g_scratch_addr[device] = (unsigned char*)p;
+    *out = (unsigned char*)p;
+    return SPHG_OK;
+}
+
 }  // namespace
 
 // ================================================================ C API
@@ -1021,6 +1068,11 @@ int sphg_ctx_create(int32_t device, sphg_ctx** out) {
     if (e != hipSuccess) {
         delete c;
         return fail(SPHG_EHIP, std::string("context set-up: ") + hipGetErrorString(e));
+    }
+    const int rc = scratch_address(device, &c->scratch);
+    if (rc) {
+        delete c;
+        return rc;
     }
     *out = c;
     return SPHG_OK;
@@ -1515,6 +1567,50 @@ int sphg_batch_step(sphg_ctx* ctx, sphg_scene* const* scenes, uint32_t nscenes, 
     rc = upload_batch_tables(ctx, B);
     if (rc) return rc;
     rc = run_batch(ctx, B, nsteps, stats);
+    if (rc) return rc;
+    HIP_TRY(hipGetLastError());
+    return SPHG_OK;
+}
+
+int sphg_batch_step_noalloc(sphg_ctx* ctx, sphg_scene* const* scenes, uint32_t nscenes, uint32_t nsteps,
+                            sphg_step_stats* stats) {
+    if (!ctx || (!scenes && nscenes)) return fail(SPHG_EINVAL, "NULL argument");
+    if (nscenes == 0) return SPHG_OK;
+    HIP_TRY(hipSetDevice(ctx->device));
+    int rc = check_batch(ctx, scenes, nscenes);
+    if (rc) return rc;
+    if (ctx->prof.on)
+        return fail(SPHG_ESTATE, "kernel timing records HIP events; switch it off for the allocation-free step");
+    if (!ctx->scratch) return fail(SPHG_ESTATE, "no scratch address for this context's device");
+    // the tables' layout in the scratch array: two scene tables, then the
+    // particle-chunk and cell-chunk tables, each at a 256-byte boundary
+    size_t nP = 0, nC = 0;
+    for (uint32_t i = 0; i < nscenes; ++i) {
+        const DevScene& V = scenes[i]->view[scenes[i]->par];
+        nP += (size_t)V.nChunks;
+        nC += (size_t)V.nCellChunks;
+    }
+    auto up = [](size_t b) { return (b + 255) & ~size_t(255); };
+    const size_t tab = up(sizeof(DevScene) * nscenes);
+    const size_t offP = 2 * tab, offC = offP + up(sizeof(Chunk) * std::max<size_t>(nP, 1));
+    const size_t need = offC + up(sizeof(Chunk) * std::max<size_t>(nC, 1));
+    if (need > (size_t)SPHG_NOALLOC_SCRATCH_BYTES)
+        return fail(SPHG_EINVAL, "the batch needs " + std::to_string(need) + " bytes of tables (" +
+                                     std::to_string(nscenes) + " scenes, " + std::to_string(nP) + " particle chunks, " +
+                                     std::to_string(nC) + " cell chunks), more than the " +
+                                     std::to_string((size_t)SPHG_NOALLOC_SCRATCH_BYTES) +
+                                     " bytes the allocation-free step holds; use sphg_batch_step");
+    Batch B;
+    B.ownTables = false;
+    B.scenes.assign(scenes, scenes + nscenes);
+    B.dT[0] = (DevScene*)(ctx->scratch);
+    B.dT[1] = (DevScene*)(ctx->scratch + tab);
+    B.dP = (Chunk*)(ctx->scratch + offP);
+    B.dC = (Chunk*)(ctx->scratch + offC);
+    B.tab0 = B.dT[0];
+    rc = upload_batch_tables(ctx, B);
+    if (rc) return rc;
+    rc = run_batch(ctx, B, nsteps, stats, false);
     if (rc) return rc;
     HIP_TRY(hipGetLastError());
     return SPHG_OK;

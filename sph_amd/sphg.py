@@ -268,6 +268,7 @@ def load(path: str = LIB_PATH) -> C.CDLL:
         "sphg_scene_get_neighbors": (C.c_int, [P, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), U32]),
         "sphg_scene_info": (C.c_int, [P, C.POINTER(U32), C.POINTER(U32), C.POINTER(U32), C.POINTER(U64)]),
         "sphg_batch_step": (C.c_int, [P, C.POINTER(P), U32, U32, C.POINTER(StepStats)]),
+        "sphg_batch_step_noalloc": (C.c_int, [P, C.POINTER(P), U32, U32, C.POINTER(StepStats)]),
         "sphg_batch_create": (C.c_int, [P, C.POINTER(P), U32, C.POINTER(P)]),
         "sphg_batch_run": (C.c_int, [P, U32, C.POINTER(StepStats)]),
         "sphg_batch_destroy": (C.c_int, [P]),
@@ -430,6 +431,14 @@ class Context:
         arr = (C.c_void_p * len(scenes))(*[s.h for s in scenes])
         st = StepStats()
         _check(self.lib.sphg_batch_step(self.h, arr, len(scenes), int(nsteps), C.byref(st)))
+        return st
+
+    def batch_step_noalloc(self, scenes: Sequence["Scene"], nsteps: int = 1) -> StepStats:
+        """batch_step with the batch's tables in the library's static device
+        array: no device memory is allocated (sphg_batch_step_noalloc)."""
+        arr = (C.c_void_p * len(scenes))(*[s.h for s in scenes])
+        st = StepStats()
+        _check(self.lib.sphg_batch_step_noalloc(self.h, arr, len(scenes), int(nsteps), C.byref(st)))
         return st
 
     def kernel_timing(self, enable: bool, only: Optional[Sequence[str]] = None) -> None:
