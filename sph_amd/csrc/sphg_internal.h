@@ -82,6 +82,9 @@ struct Scalars {
                                 // step so far and of the previous step, which sizes this step's shell
     unsigned shellSteps, walkSteps;  // PCISPH: steps whose predicted densities used the shell lists /
                                      // took the 27-cell walk for every particle
+    unsigned d0Valid;           // DFSPH: d0 holds the divergence of the current velocities (written by
+                                // the list build and the divergence error sweeps; cleared by the cube
+                                // clamp, which changes the velocities after them)
 };
 
 // One emitter's constants and state (src/Emitter.h:9-55).  Its group's
@@ -132,8 +135,9 @@ struct DevScene {
     float2 *vyz;                // DFSPH: (v_y, v_z), written by the velocity sweeps with v_x in x.w;
                                 // the error sweeps gather x and vyz, 24 B per pair.  In the density
                                 // solve the two hold u, the step's velocity change so far, instead
-    double *d0;                 // DFSPH: the density solve's constant part of the predicted-density
-                                // sum, from the velocity at the step's start
+    double *d0;                 // DFSPH: sum_j m (v_i - v_j).gradW_ij + v_i.gb_i of the current
+                                // velocities: the density solve's constant part of the predicted
+                                // density sum, from the velocity at the step's start
     int pciShell, pad5;         // PCISPH: predicted densities from the list and a shell list
     uint16_t *nsh;              // PCISPH: shell entries per particle (candidates within h + skin
                                 // that are not in the list, stored from the list's last slot down)

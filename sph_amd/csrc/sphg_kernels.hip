@@ -968,6 +968,9 @@ __global__ __launch_bounds__(kBlock) void k_build(const DevScene* __restrict__ T
         const double dc = m * dv + (ux * (double)gbf.x + uy * (double)gbf.y + uz * (double)gbf.z);
         const double de = (ngr < 20) ? 0.0 : fmax(dc, 0.0);
         const double rhof = rho0 + (double)(float)drho;
+        // the divergence of the velocities: the next step's D0 unless a
+        // divergence iteration replaces it
+        ((gptr<double>)S.d0)[i] = dc;
         stg(S.divErr, (size_t)i, (float)de);
         st4(S.xk, i, make_float4(xi.x, xi.y, xi.z, (float)(de * (double)alpha / (S.ss->dt * rhof))));
         e = de / rho0;
@@ -988,6 +991,7 @@ __global__ __launch_bounds__(kBlock) void k_build(const DevScene* __restrict__ T
         __shared__ double sh[4];
         const double bs = block_sum(e, sh);
         if (threadIdx.x == 0) S.partial[c.begin / kBlock] = bs;
+        if (c.begin == 0 && threadIdx.x == 0) S.ss->d0Valid = 1u;
     }
 }
 
@@ -1404,8 +1408,9 @@ __global__ __launch_bounds__(kBlock) void k_df_err(const DevScene* __restrict__ 
             const float2 yz = ld2o(S.vyz, (uint32_t)i);
             const double ux = xi.w, uy = yz.x, uz = yz.y;
             double su = 0.0;
-            if (MODE == 0) {
-                // the constant part D0 over v0, once per step
+            if (MODE == 0 && !ss->d0Valid) {
+                // the constant part D0 over v0, once per step (unless the
+                // previous divergence sweep left it: see d0Valid)
                 const float4 vi = ld4(S.v, i);
                 const double wx = vi.x, wy = vi.y, wz = vi.z;
                 double s0 = 0.0;
@@ -1419,6 +1424,8 @@ __global__ __launch_bounds__(kBlock) void k_df_err(const DevScene* __restrict__ 
                 const double d0 = S.mD * s0 + (wx * (double)gb.x + wy * (double)gb.y + wz * (double)gb.z);
                 ((gptr<double>)S.d0)[i] = d0;
                 dc = d0;
+                // (d0Valid stays 0 until the next divergence sweep: other
+                // blocks of this launch test it)
             } else {
                 sweep4(S, i, [&](uint32_t j) { XVP r; r.xv = ld4o(S.x, j); r.yz = ld2o(S.vyz, j); return r; },
                        [&](uint32_t, const XVP& d) {
@@ -1439,6 +1446,8 @@ __global__ __launch_bounds__(kBlock) void k_df_err(const DevScene* __restrict__ 
                        dc += p.g * ((ux - (double)d.xv.w) * p.dx + (uy - (double)d.yz.x) * p.dy + (uz - (double)d.yz.y) * p.dz);
                    });
             dc = S.mD * dc + (ux * (double)gb.x + uy * (double)gb.y + uz * (double)gb.z);
+            // the divergence of the current velocities: the next step's D0
+            ((gptr<double>)S.d0)[i] = dc;
         }
         const double drho = (double)S.drho[i];
         const double rho = S.rho0D + drho;
@@ -1780,7 +1789,10 @@ __global__ __launch_bounds__(kBlock) void k_cube(const DevScene* __restrict__ T,
     const Chunk c = C[blockIdx.x];
     const DevScene& S = T[c.scene];
     const int i = c.begin + threadIdx.x;
-    if (!S.cubeOn || i >= S.n) return;
+    if (!S.cubeOn) return;
+    // the clamp changes velocities after the divergence sweeps formed d0
+    if (c.begin == 0 && threadIdx.x == 0 && S.method == DFSPH) S.ss->d0Valid = 0u;
+    if (i >= S.n) return;
     const float4 xf = S.x[i], vf = S.v[i];
     float p[3] = {xf.x, xf.y, xf.z};
     double u[3] = {vf.x, vf.y, vf.z};
