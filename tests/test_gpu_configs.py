@@ -22,8 +22,7 @@ rebuilds the grid, densities and alpha), and steps both once from there:
     and two scenes against the oracle at every step.
 
 Fields are held to 1e-5 relative, normwise (max|gpu - ref| / max|ref|),
-the divergence error to 1e-4 (its floor is set by FP32 velocities between
-the sweeps); density deviations rho - rho0 and rho* - rho0 are compared as
+the divergence error to 5e-5 (see TOL_DIVERR); density deviations rho - rho0 and rho* - rho0 are compared as
 stored, against max(max|ref|, eta rho0).  Iteration counts agree within
 one; fields are compared on a step whose counts agree (up to three
 consecutive steps are tried).  The DFSPH oracle rounds its updated positions
@@ -52,7 +51,13 @@ import oracle.pyoracle as po
 pytestmark = pytest.mark.gpu
 
 TOL = 1e-5
-TOL_DIVERR = 1e-4
+# The divergence error read back after the divergence solve is a cancelling
+# sum over velocities that the device stores in FP32 once the density solve
+# ends (the oracle keeps them in FP64): at 1M particles, speeds up to 3 m/s,
+# the rounding (6e-8 relative) leaves ~2e-5 of its largest value (measured
+# 2.3e-5).  kappa_v, which the solver takes from it, and the velocities are
+# within 1e-5.
+TOL_DIVERR = 5e-5
 
 
 def _log(msg):
