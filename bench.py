@@ -281,10 +281,10 @@ def main():
             if max(q.n_target for q in reqs) >= 1_000_000:
                 break
             seed += 1
-        serve = srv.compare(ctx, reqs, 200 * 2 ** 30, quantum=25, check=4, parallel_methods=True)
+        serve = srv.compare(ctx, reqs, 200 * 2 ** 30, quantum=25, check=4, parallel=True, split_n=srv.SPLIT_N)
         serve["stream"] = (f"24 requests, Poisson 30/s, WCSPH/PCISPH/DFSPH uniform, N log-uniform 2,048-2,000,000 "
                            f"(largest {max(q.n_target for q in reqs)}), 100-300 steps, seed {seed}, 200 GB budget, "
-                           "quantum 25, methods on their own streams")
+                           f"quantum 25, lanes by method and size (split at {srv.SPLIT_N}) on their own streams")
         log(f"serve: ratio {serve['throughput_ratio']}, p99 {serve['batched']['latency_p99_s']} s")
 
     ctx.close()

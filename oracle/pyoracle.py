@@ -110,6 +110,7 @@ def load(fast: bool = False) -> C.CDLL:
         "oracle_round_state_f32": (C.c_int, [P]),
         "oracle_set_f32_positions": (C.c_int, [P, C.c_int]),
         "oracle_set_eps": (C.c_int, [P, D]),
+        "oracle_divergence_error": (C.c_int, [P, fp]),
         "oracle_num_fluid": (U, [P]),
         "oracle_num_boundary": (U, [P]),
         "oracle_get": (C.c_int, [P, C.c_int, dp]),
@@ -240,6 +241,15 @@ class Oracle:
     def set_eps(self, eps: float = 1e-5):
         """The threshold of the DFSPH eps branches (1e-5 in the reference)."""
         self.lib.oracle_set_eps(self.h, float(eps))
+
+    def divergence_error(self, velocities) -> np.ndarray:
+        """The divergence error (src/DFSPHSolver.cpp:408-492) of the current
+        positions and lists with `velocities` (n x 3) in place of the
+        oracle's own; also left in OF_DIVERR."""
+        v = np.ascontiguousarray(velocities, dtype=np.float32).reshape(-1, 3)
+        assert v.shape[0] == self.n
+        self.lib.oracle_divergence_error(self.h, _fp(v))
+        return self.get(OF_DIVERR)
 
     def get(self, field: int) -> np.ndarray:
         if field in (OF_BVOL, OF_BDENS, OF_BPRESS):

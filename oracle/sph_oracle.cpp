@@ -1459,6 +1459,22 @@ int oracle_set_f32_positions(void* h, int on) {
 // 548,552,642,656,670; the alpha denominator test at :182).  Tests move it
 // by a small relative amount to find the particles whose step depends on a
 // branch decided within a solver's FP32 resolution of the threshold.
+// computeDivergenceError (src/DFSPHSolver.cpp:408-492) of the current
+// positions, neighbor lists and densities with the velocities `vel` (3n
+// floats) in place of the oracle's own; the result replaces the divergence
+// error field, the oracle's velocities are kept.  The divergence error is a
+// cancelling sum of velocity differences, so a solver's value is checked
+// both against the FP64 run's and against this, the same function of the
+// velocities the solver holds.
+int oracle_divergence_error(void* h, const float* vel) {
+    Oracle* o = (Oracle*)h;
+    const std::vector<V3> keep = o->v;
+    for (size_t i = 0; i < o->N(); ++i) o->v[i] = V3(vel[3 * i], vel[3 * i + 1], vel[3 * i + 2]);
+    o->computeDivergenceError();
+    o->v = keep;
+    return 0;
+}
+
 int oracle_set_eps(void* h, double eps) {
     ((Oracle*)h)->eps = eps;
     return 0;

@@ -34,10 +34,24 @@
 
 namespace sphg {
 
-constexpr int kBlock = 256;          // threads per block = particles per chunk
-constexpr int kGroup = 64;           // particles per neighbor-list group (one wavefront)
-constexpr int kCellChunk = 1024;     // cells per scan chunk (256 threads x 4: ~300 blocks for a 1M-particle grid)
+// The block size and the scan chunk are compile-time choices of the layout
+// (the chunk tables, the lists' groups and the reduction partials are built
+// on them); a build may pick others with -DSPHG_BLOCK=128|256|512 and
+// -DSPHG_CELL_CHUNK=<multiple of the block, up to 16 blocks>.
+#ifndef SPHG_BLOCK
+#define SPHG_BLOCK 256
+#endif
+#ifndef SPHG_CELL_CHUNK
+#define SPHG_CELL_CHUNK 1024
+#endif
+constexpr int kBlock = SPHG_BLOCK;            // threads per block = particles per chunk
+constexpr int kWaves = kBlock / 64;           // wavefronts per block
+constexpr int kGroup = 64;                    // particles per neighbor-list group (one wavefront)
+constexpr int kCellChunk = SPHG_CELL_CHUNK;   // cells per scan chunk (1024: ~300 blocks for a 1M-particle grid)
 constexpr int kCellPerThread = kCellChunk / kBlock;
+static_assert(kBlock == 128 || kBlock == 256 || kBlock == 512, "block of 2, 4 or 8 wavefronts");
+static_assert(kCellChunk % kBlock == 0 && kCellPerThread >= 1 && kCellPerThread <= 16,
+              "a scan chunk is 1 to 16 cells per thread");
 
 enum Method { WCSPH = 0, PCISPH = 1, DFSPH = 2 };
 enum Flags { F_XSPH = 1, F_ST = 2, F_ADH = 4, F_VSTD = 8, F_VART = 16 };

@@ -1,7 +1,7 @@
 // sphg_kernels.hip -- gfx950 kernels of the SPH solver step.
 //
 // Every kernel takes a scene table and a chunk table; block b works on
-// chunk b (256 particles or 1024 cells of one scene).  Sums of a scene are
+// chunk b (kBlock = 256 particles or kCellChunk = 1024 cells of one scene).  Sums of a scene are
 // formed in a fixed order: a butterfly inside each wavefront, wave results
 // combined in index order, one partial per chunk, and the last chunk to
 // finish (an atomic arrival counter) adds the scene's partials in chunk
@@ -28,13 +28,16 @@ __device__ inline double wave_max(double v) {
     for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
     return v;
 }
-// Result valid in every thread.  sh: 4 doubles of shared memory.
+// Result valid in every thread; the wave results are added in wave order.
+// sh: kWaves doubles of shared memory.
 __device__ inline double block_sum(double v, double* sh) {
     v = wave_sum(v);
     const int w = threadIdx.x >> 6;
     if ((threadIdx.x & 63) == 0) sh[w] = v;
     __syncthreads();
-    double r = ((sh[0] + sh[1]) + sh[2]) + sh[3];
+    double r = sh[0];
+#pragma unroll
+    for (int k = 1; k < kWaves; ++k) r += sh[k];
     __syncthreads();
     return r;
 }
@@ -43,13 +46,15 @@ __device__ inline double block_max(double v, double* sh) {
     const int w = threadIdx.x >> 6;
     if ((threadIdx.x & 63) == 0) sh[w] = v;
     __syncthreads();
-    double r = fmax(fmax(sh[0], sh[1]), fmax(sh[2], sh[3]));
+    double r = sh[0];
+#pragma unroll
+    for (int k = 1; k < kWaves; ++k) r = fmax(r, sh[k]);
     __syncthreads();
     return r;
 }
 
 // Sum (or max) of one scene's chunk partials in chunk order.
-// Each thread combines partials t, t + 256, t + 512, ... in that order; the
+// Each thread combines partials t, t + kBlock, t + 2 kBlock, ... in that order; the
 // loads of 8 of them are issued before the first is used, so the loop waits
 // on memory once per 8 partials instead of once per partial.
 constexpr int kPartU = 8;
@@ -479,11 +484,11 @@ __global__ __launch_bounds__(kBlock) void k_keys(const DevScene* __restrict__ T,
     (void)wave_count_add(S.cellCount, cl);
 }
 
-// Per 1024-cell chunk: total count.
+// Per cell chunk: total count.
 __global__ __launch_bounds__(kBlock) void k_scan_sum(const DevScene* __restrict__ T, const Chunk* __restrict__ C) {
     const Chunk c = C[blockIdx.x];
     const DevScene& S = T[c.scene];
-    __shared__ double sh[4];
+    __shared__ double sh[kWaves];
     int acc = 0;
     const int c0 = c.begin + threadIdx.x * kCellPerThread;
 #pragma unroll
@@ -505,13 +510,13 @@ __device__ inline int wave_incl_scan(int v) {
     return v;
 }
 
-// One block per scene: exclusive scan of the chunk totals, in place, 256
+// One block per scene: exclusive scan of the chunk totals, in place, kBlock
 // totals per pass with the running total carried between passes.
 __global__ __launch_bounds__(kBlock) void k_scan_offsets(const DevScene* __restrict__ T, int nscenes) {
     const int s = blockIdx.x;
     if (s >= nscenes) return;
     const DevScene& S = T[s];
-    __shared__ int wsum[kBlock / 64];
+    __shared__ int wsum[kWaves];
     __shared__ int carry;
     if (threadIdx.x == 0) carry = 0;
     __syncthreads();
@@ -536,7 +541,7 @@ __global__ __launch_bounds__(kBlock) void k_scan_offsets(const DevScene* __restr
 __global__ __launch_bounds__(kBlock) void k_scan_final(const DevScene* __restrict__ T, const Chunk* __restrict__ C) {
     const Chunk c = C[blockIdx.x];
     const DevScene& S = T[c.scene];
-    __shared__ int wsum[4];
+    __shared__ int wsum[kWaves];
     const int c0 = c.begin + threadIdx.x * kCellPerThread;
     int vals[kCellPerThread];
     int local = 0;
@@ -988,7 +993,7 @@ __global__ __launch_bounds__(kBlock) void k_build(const DevScene* __restrict__ T
     }
     }
     if (M == DFSPH) {
-        __shared__ double sh[4];
+        __shared__ double sh[kWaves];
         const double bs = block_sum(e, sh);
         if (threadIdx.x == 0) S.partial[c.begin / kBlock] = bs;
         if (c.begin == 0 && threadIdx.x == 0) S.ss->d0Valid = 1u;
@@ -1240,7 +1245,7 @@ __global__ __launch_bounds__(kBlock) void k_dt(const DevScene* __restrict__ T, c
     const Chunk c = C[blockIdx.x];
     const DevScene& S = T[c.scene];
     const Scalars* ss = S.ss;
-    __shared__ double sh[4];
+    __shared__ double sh[kWaves];
     const int i = c.begin + threadIdx.x;
     const double dt = ss->dt;
     double val = 0.0;
@@ -1393,7 +1398,7 @@ __global__ __launch_bounds__(kBlock) void k_df_err(const DevScene* __restrict__ 
     constexpr bool DENS = (MODE <= 1);
     if (MODE == 1 && ss->done) return;
     if (MODE == 3 && ss->doneV) return;
-    __shared__ double sh[4];
+    __shared__ double sh[kWaves];
     const int i = c.begin + threadIdx.x;
     const KCoefD k = S.kd;
     const double dt = ss->dt;
@@ -1518,7 +1523,7 @@ __global__ __launch_bounds__(kBlock) void k_pci_density(const DevScene* __restri
     const DevScene& S = T[c.scene];
     const Scalars* ss = S.ss;
     if (ss->done) return;
-    __shared__ double sh[4];
+    __shared__ double sh[kWaves];
     const int i = c.begin + threadIdx.x;
     const KCoefD k = S.kd;
     const bool useShell = S.pciShell && ss->shellBad == 0u;
@@ -1704,7 +1709,7 @@ __global__ __launch_bounds__(kBlock) void k_finalize(const DevScene* __restrict_
     Scalars* ss = S.ss;
     if ((KIND == FIN_DENS || KIND == FIN_PCI) && ss->done) return;
     if (KIND == FIN_DIV && ss->doneV) return;
-    __shared__ double sh[4];
+    __shared__ double sh[kWaves];
     const double red = (KIND == FIN_DT || KIND == FIN_PCI) ? scene_partial_max(S.partial, S.nChunks, sh)
                                                            : scene_partial_sum(S.partial, S.nChunks, sh);
     if (threadIdx.x != 0) return;

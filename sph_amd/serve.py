@@ -10,10 +10,12 @@ Continuous batching (`serve_batched`):
     admitted, first come first served, when the device bytes of its scene
     (known before allocation, `Scene.plan_bytes`) fit the memory budget
     beside the scenes already resident;
-  * resident scenes stay on the device; the scenes of one method are stepped
-    together, one launch per solver phase covering all of them
-    (`sphg.Batch`), in quanta of at most `quantum` steps, so that requests
-    that arrive meanwhile join at the next quantum;
+  * resident scenes stay on the device, grouped in lanes: one per solver
+    method, split by size (`split_n`) so that small scenes do not wait on
+    the step time of giant ones; a lane's scenes are stepped together, one
+    launch per solver phase covering all of them (`sphg.Batch`), in quanta
+    of at most `quantum` steps, so that requests that arrive meanwhile join
+    at the lane's next quantum;
   * a scene that has done its steps is read back, freed and its request
     completed; the rest continue;
   * a step call that reports a neighbor overflow (SPHG_EOVERFLOW: some
@@ -21,10 +23,11 @@ Continuous batching (`serve_batched`):
     whose scenes overflowed: they are freed and completed with an error, the
     batch is rebuilt without them, and the other scenes, which stepped
     normally, continue;
-  * with `parallel_methods`, the three methods' batches run their quantum at
-    the same time, each on its own context (HIP stream) driven by its own
-    host thread: a launch-bound batch of small scenes then overlaps the
-    others instead of waiting for them.
+  * with `parallel`, the lanes run their quanta at the same time and
+    independently, each on its own context (HIP stream) driven by its own
+    host thread: a launch-bound lane of small scenes then overlaps the
+    others instead of waiting for them, and a lane whose quantum ends is
+    refilled at once.
   The scene geometry of an arriving request is built on a host thread while
   the GPU steps the resident scenes.
 
@@ -41,10 +44,9 @@ from __future__ import annotations
 import argparse
 import json
 import math
-import threading
 import time
 from collections import deque
-from concurrent.futures import ThreadPoolExecutor
+from concurrent.futures import FIRST_COMPLETED, Future, ThreadPoolExecutor, wait
 from dataclasses import dataclass, field
 from typing import Callable, Dict, List, Optional
 
@@ -132,25 +134,45 @@ def build_spec(req: Request, spacing: float = 0.02) -> SceneSpec:
 
 # --------------------------------------------------------------- backends
 
-class GpuBackend:
-    """The HIP solver library on one device.  With `method_contexts` each
-    solver method gets a context (stream) of its own, so that the batches of
-    different methods can run concurrently."""
+SMALL, LARGE = 0, 1
+SPLIT_N = 262144     # default size class boundary, fluid particles
 
-    def __init__(self, ctx: sphg.Context, method_contexts: bool = False):
+
+def lane_of(method: int, n: int, split_n: int = 0):
+    """The batch a resident scene steps in: its method and, when `split_n` is
+    positive, its size class (LARGE from `split_n` fluid particles up).  One
+    batch step lasts as long as its biggest scene's step, so a small scene
+    that shares a batch with a 2M-particle one advances at the big scene's
+    pace; in a lane of its own it advances at its own."""
+    return (method, LARGE if split_n and n >= split_n else SMALL)
+
+
+class GpuBackend:
+    """The HIP solver library on one device.  With `lane_contexts` each lane
+    (solver method and size class, `lane_of`) gets a context (HIP stream) of
+    its own, so that the batches of different lanes can run concurrently."""
+
+    def __init__(self, ctx: sphg.Context, lane_contexts: bool = False):
         self.ctx = ctx
+        self.lane_contexts = lane_contexts
         self.extra = []
         self.ctxs = {}
-        for m in METHODS:
-            if method_contexts and self.ctxs:
-                c = sphg.Context(ctx.device)
-                self.extra.append(c)
-                self.ctxs[m] = c
-            else:
-                self.ctxs[m] = ctx
 
-    def create(self, spec: SceneSpec):
-        return sphg.Scene(self.ctxs[spec.params.method], spec.params, spec.fluid, spec.boundary, init=False)
+    def _context(self, lane):
+        if lane is None or not self.lane_contexts:
+            return self.ctx
+        c = self.ctxs.get(lane)
+        if c is None:
+            if self.ctxs:
+                c = sphg.Context(self.ctx.device)
+                self.extra.append(c)
+            else:
+                c = self.ctx
+            self.ctxs[lane] = c
+        return c
+
+    def create(self, spec: SceneSpec, lane=None):
+        return sphg.Scene(self._context(lane), spec.params, spec.fluid, spec.boundary, init=False)
 
     def plan_bytes(self, scene) -> int:
         return scene.plan_bytes()
@@ -179,6 +201,7 @@ class GpuBackend:
         for c in self.extra:
             c.close()
         self.extra = []
+        self.ctxs = {}
 
 
 @dataclass
@@ -186,6 +209,7 @@ class Job:
     req: Request
     spec_future: object = None
     scene: object = None
+    lane: tuple = None
     bytes: int = 0
     n: int = 0
     remaining: int = 0
@@ -236,127 +260,167 @@ def _run_quantum(batch, k):
     return None
 
 
+@dataclass
+class _Lane:
+    jobs: List[Job] = field(default_factory=list)   # resident scenes, admission order
+    batch: object = None                            # sphg.Batch over `jobs`, or None
+    stale: bool = False                             # `jobs` changed since `batch` was built
+    flight: object = None                           # (future, jobs, k) of the running quantum
+
+
 def serve_batched(backend, requests: List[Request], budget_bytes: int, quantum: int = 25,
                   clock: Callable[[], float] = time.perf_counter, sleep: Callable[[float], None] = time.sleep,
                   keep_results: bool = True, log: Callable[[str], None] = None,
-                  parallel_methods: bool = False) -> ServeResult:
+                  parallel: bool = False, split_n: int = 0) -> ServeResult:
     """Continuous batching of `requests` (sorted by arrival) on `backend`.
-    With `parallel_methods` the methods' batches run each quantum on their
-    own host threads (the backend gives each method its own stream)."""
+
+    Resident scenes are grouped in lanes (`lane_of`: method, and size class
+    when `split_n` > 0); each lane's scenes step together as one batch.
+    With `parallel` every lane runs its quanta on a host thread of its own
+    (the backend gives each lane its own stream) and independently of the
+    others: when one lane's quantum ends its finished scenes are completed,
+    arrivals admitted and its next quantum started while the other lanes run
+    on.  Without it the lanes take turns on the calling thread."""
     pending = deque(sorted(requests, key=lambda q: q.arrival))
     waiting: deque = deque()
-    active: Dict[int, List[Job]] = {m: [] for m in METHODS}
-    batches: Dict[int, object] = {m: None for m in METHODS}
+    lanes: Dict[tuple, _Lane] = {}
     jobs: List[Job] = []
     rejected: List[int] = []
     used = 0
     pool = _spec_pool()
-    runner = ThreadPoolExecutor(max_workers=len(METHODS), thread_name_prefix="batch") if parallel_methods else None
+    runner = ThreadPoolExecutor(max_workers=2 * len(METHODS), thread_name_prefix="lane") if parallel else None
     t0 = clock()
+
+    def admit(block: bool):
+        """Admits waiting requests in arrival order while they fit.  Unless
+        `block`, it stops at a request whose geometry is still being built
+        (the lanes keep running meanwhile)."""
+        nonlocal used
+        while waiting:
+            j = waiting[0]
+            if j.scene is None:
+                if not block and not j.spec_future.done():
+                    break
+                spec = j.spec_future.result()
+                j.n = spec.n
+                j.lane = lane_of(j.req.method, spec.n, split_n)
+                j.scene = backend.create(spec, j.lane)
+                j.bytes = backend.plan_bytes(j.scene)
+            if j.bytes > budget_bytes:
+                backend.close(j.scene)
+                j.scene = None
+                rejected.append(j.req.rid)
+                waiting.popleft()
+                continue
+            if used + j.bytes > budget_bytes:
+                break
+            backend.init(j.scene)
+            used += j.bytes
+            j.remaining = j.req.steps
+            j.admitted = clock() - t0
+            ln = lanes.setdefault(j.lane, _Lane())
+            ln.jobs.append(j)
+            ln.stale = True
+            waiting.popleft()
+
+    def launch(ln: _Lane):
+        if ln.batch is None or ln.stale:
+            if ln.batch is not None:
+                backend.close(ln.batch)
+            ln.batch = backend.batch([j.scene for j in ln.jobs])
+            ln.stale = False
+        js = list(ln.jobs)
+        k = min(quantum, min(j.remaining for j in js))
+        if runner is not None:
+            fut = runner.submit(_run_quantum, ln.batch, k)
+        else:
+            fut = Future()
+            try:
+                fut.set_result(_run_quantum(ln.batch, k))
+            except BaseException as e:  # noqa: BLE001 - re-raised by land()
+                fut.set_exception(e)
+        ln.flight = (fut, js, k)
+
+    def land(ln: _Lane):
+        """The lane's quantum has ended: account its steps, fail overflowed
+        scenes, complete finished ones."""
+        nonlocal used
+        fut, js, k = ln.flight
+        ln.flight = None
+        over = fut.result()            # raises what the quantum raised, other than an overflow
+        if over is not None:
+            bad = [j for j in js if backend.overflowed(j.scene)]
+            if not bad:
+                raise over
+            backend.close(ln.batch)
+            ln.batch = None
+            for j in bad:
+                ln.jobs.remove(j)
+                backend.close(j.scene)
+                j.scene = None
+                used -= j.bytes
+                j.done = clock() - t0
+                j.error = "neighbor overflow: a particle had more neighbors than max_neighbors"
+                if log:
+                    log(f"[serve] failed req {j.req.rid} {METHOD_NAMES[j.req.method]} n={j.n}: {j.error}")
+            js = [j for j in js if j.error is None]
+        for j in js:
+            j.remaining -= k
+        fin = [j for j in js if j.remaining == 0]
+        if fin:
+            if ln.batch is not None:
+                backend.close(ln.batch)
+            ln.batch = None
+            for j in fin:
+                res = backend.result(j.scene)
+                j.result = res if keep_results else None
+                backend.close(j.scene)
+                j.scene = None
+                used -= j.bytes
+                j.done = clock() - t0
+                ln.jobs.remove(j)
+                if log:
+                    log(f"[serve] done req {j.req.rid} {METHOD_NAMES[j.req.method]} n={j.n} "
+                        f"steps={j.req.steps} latency {j.done - j.req.arrival:.3f}s")
+
     try:
-        while pending or waiting or any(active.values()):
+        while pending or waiting or any(ln.jobs for ln in lanes.values()):
             now = clock() - t0
             while pending and pending[0].arrival <= now:
                 j = Job(pending.popleft())
                 j.spec_future = pool.submit(build_spec, j.req)
                 waiting.append(j)
                 jobs.append(j)
-            # admission, first come first served, by device bytes
-            while waiting:
-                j = waiting[0]
-                if j.scene is None:
-                    spec = j.spec_future.result()
-                    j.scene = backend.create(spec)
-                    j.n = spec.n
-                    j.bytes = backend.plan_bytes(j.scene)
-                if j.bytes > budget_bytes:
-                    backend.close(j.scene)
-                    j.scene = None
-                    rejected.append(j.req.rid)
-                    waiting.popleft()
-                    continue
-                if used + j.bytes > budget_bytes:
-                    break
-                backend.init(j.scene)
-                used += j.bytes
-                j.remaining = j.req.steps
-                j.admitted = clock() - t0
-                m = j.req.method
-                if batches[m] is not None:
-                    backend.close(batches[m])
-                    batches[m] = None
-                active[m].append(j)
-                waiting.popleft()
-            if not any(active.values()):
-                if pending:
+            # with nothing resident there is nothing to overlap: wait for the
+            # head request's geometry
+            admit(block=not any(ln.jobs for ln in lanes.values()))
+            for ln in lanes.values():
+                if ln.jobs and ln.flight is None:
+                    launch(ln)
+                    if runner is None:
+                        land(ln)
+            flying = {ln.flight[0]: ln for ln in lanes.values() if ln.flight is not None}
+            if not flying:
+                if not any(ln.jobs for ln in lanes.values()) and pending and not waiting:
                     sleep(max(0.0, pending[0].arrival - (clock() - t0)))
                 continue
-            passes = []
-            for m in METHODS:
-                js = active[m]
-                if not js:
-                    continue
-                if batches[m] is None:
-                    batches[m] = backend.batch([j.scene for j in js])
-                passes.append((m, js, min(quantum, min(j.remaining for j in js))))
-            over = {}
-            if runner is not None and len(passes) > 1:
-                futs = [runner.submit(_run_quantum, batches[m], k) for m, _, k in passes]
-                # every pass ends before any error is raised (the batches
-                # are closed on the way out)
-                errs = [f.exception() for f in futs]
-                for e in errs:
-                    if e is not None:
-                        raise e
-                for (m, _, _), f in zip(passes, futs):
-                    if f.result() is not None:
-                        over[m] = f.result()
-            else:
-                for m, _, k in passes:
-                    e = _run_quantum(batches[m], k)
-                    if e is not None:
-                        over[m] = e
-            for m, js, k in passes:
-                if m in over:
-                    bad = [j for j in js if backend.overflowed(j.scene)]
-                    if not bad:
-                        raise over[m]
-                    backend.close(batches[m])
-                    batches[m] = None
-                    for j in bad:
-                        backend.close(j.scene)
-                        j.scene = None
-                        used -= j.bytes
-                        j.done = clock() - t0
-                        j.error = "neighbor overflow: a particle had more neighbors than max_neighbors"
-                        if log:
-                            log(f"[serve] failed req {j.req.rid} {METHOD_NAMES[m]} n={j.n}: {j.error}")
-                    js = [j for j in js if j.error is None]
-                    active[m] = js
-                for j in js:
-                    j.remaining -= k
-                fin = [j for j in js if j.remaining == 0]
-                if fin:
-                    if batches[m] is not None:
-                        backend.close(batches[m])
-                    batches[m] = None
-                    for j in fin:
-                        res = backend.result(j.scene)
-                        j.result = res if keep_results else None
-                        backend.close(j.scene)
-                        j.scene = None
-                        used -= j.bytes
-                        j.done = clock() - t0
-                        if log:
-                            log(f"[serve] done req {j.req.rid} {METHOD_NAMES[m]} n={j.n} steps={j.req.steps} "
-                                f"latency {j.done - j.req.arrival:.3f}s")
-                    active[m] = [j for j in js if j.remaining > 0]
+            timeout = max(0.0, pending[0].arrival - (clock() - t0)) if pending else None
+            # a head request whose geometry is being built wakes the loop too
+            head = [waiting[0].spec_future] if waiting and waiting[0].scene is None else []
+            done, _ = wait(list(flying) + head, timeout=timeout, return_when=FIRST_COMPLETED)
+            for f in done:
+                if f in flying:
+                    land(flying[f])
     finally:
-        for m in METHODS:
-            if batches[m] is not None:
-                backend.close(batches[m])
-        for js in active.values():
-            for j in js:
+        if runner is not None:
+            # every running quantum ends before its batch is closed
+            for ln in lanes.values():
+                if ln.flight is not None:
+                    wait([ln.flight[0]])
+        for ln in lanes.values():
+            if ln.batch is not None:
+                backend.close(ln.batch)
+            for j in ln.jobs:
                 if j.scene is not None:
                     backend.close(j.scene)
         for j in waiting:
@@ -426,13 +490,13 @@ def serve_plain(backend, requests: List[Request], budget_bytes: int,
 
 
 def compare(ctx, reqs: List[Request], budget_bytes: int, quantum: int = 25, check: int = 4,
-            parallel_methods: bool = True, log: Callable[[str], None] = None) -> dict:
+            parallel: bool = True, split_n: int = SPLIT_N, log: Callable[[str], None] = None) -> dict:
     """Serves `reqs` with continuous batching, then plain (one at a time),
     on the same device; returns both summaries, the throughput ratio and how
     many of the first `check` requests had identical results in both."""
-    be = GpuBackend(ctx, method_contexts=parallel_methods)
+    be = GpuBackend(ctx, lane_contexts=parallel)
     try:
-        rb = serve_batched(be, reqs, budget_bytes, quantum, log=log, parallel_methods=parallel_methods)
+        rb = serve_batched(be, reqs, budget_bytes, quantum, log=log, parallel=parallel, split_n=split_n)
         rp = serve_plain(be, reqs, budget_bytes, log=log)
     finally:
         be.shutdown()
@@ -446,7 +510,34 @@ def compare(ctx, reqs: List[Request], budget_bytes: int, quantum: int = 25, chec
             same += int(np.array_equal(jb.result, j.result))
     ratio = (round(sb["particle_steps_per_s"] / sp["particle_steps_per_s"], 3)
              if sb["particle_steps_per_s"] and sp["particle_steps_per_s"] else None)
-    return {"batched": sb, "plain": sp, "throughput_ratio": ratio, "identical_results": f"{same}/{checked}"}
+    out = {"batched": sb, "plain": sp, "throughput_ratio": ratio, "identical_results": f"{same}/{checked}"}
+    bound = throughput_bound(rp)
+    if bound:
+        out["throughput_bound"] = bound
+    return out
+
+
+def throughput_bound(plain: ServeResult) -> Optional[dict]:
+    """The largest batched/plain throughput ratio the stream allows, from
+    the plain run: batching cannot step a scene faster than the device's
+    saturated particle-step rate, taken per method as the fastest alone rate
+    (particle-steps per second from admission to result) of any request of
+    that method.  With every request at that rate the stream would take
+    sum(work_i / rate[method_i]); the plain makespan over that is the bound.
+    A stream whose work sits in scenes already large enough to saturate the
+    device alone has a bound near 1."""
+    done = [j for j in plain.jobs if j.done >= 0 and j.error is None and j.done > j.admitted]
+    if not done or plain.makespan <= 0:
+        return None
+    best = {}
+    for j in done:
+        r = j.n * j.req.steps / (j.done - j.admitted)
+        best[j.req.method] = max(best.get(j.req.method, 0.0), r)
+    ideal = sum(j.n * j.req.steps / best[j.req.method] for j in done)
+    big = sum(j.n * j.req.steps for j in done if j.n >= 262144) / sum(j.n * j.req.steps for j in done)
+    return {"ratio": round(plain.makespan / ideal, 3),
+            "alone_rate_per_method": {METHOD_NAMES[m]: round(r, 1) for m, r in sorted(best.items())},
+            "work_share_n_ge_262144": round(big, 3)}
 
 
 def main():
@@ -461,21 +552,24 @@ def main():
     ap.add_argument("--quantum", type=int, default=25)
     ap.add_argument("--seed", type=int, default=5)
     ap.add_argument("--check", type=int, default=4, help="requests whose batched and plain results are compared")
-    ap.add_argument("--serial-methods", action="store_true",
-                    help="step the methods' batches one after another on one stream")
+    ap.add_argument("--serial", action="store_true",
+                    help="step the lanes' batches one after another on one stream")
+    ap.add_argument("--split-n", type=int, default=SPLIT_N,
+                    help="fluid size from which a scene steps in its method's large-scene lane (0: one lane "
+                         "per method)")
     args = ap.parse_args()
 
     reqs = request_stream(args.requests, args.rate, args.nmin, args.nmax, args.smin, args.smax, args.seed)
     ctx = sphg.Context(0)
     log = lambda s: print(s, flush=True)   # noqa: E731
     r = compare(ctx, reqs, int(args.budget_gb * 2 ** 30), args.quantum, args.check,
-                parallel_methods=not args.serial_methods, log=log)
+                parallel=not args.serial, split_n=args.split_n, log=log)
     out = {"metric": "served-scene p50/p99 latency and particle-steps/s, continuous batching vs plain GPU",
            "stream": {"requests": args.requests, "rate_per_s": args.rate, "n": [args.nmin, args.nmax],
                       "steps": [args.smin, args.smax], "methods": "WCSPH/PCISPH/DFSPH uniform",
                       "geometry": "dam-break block or sphere drop, spacing 0.02", "seed": args.seed,
                       "budget_gb": args.budget_gb, "quantum": args.quantum,
-                      "methods_in_parallel": not args.serial_methods}}
+                      "lanes_in_parallel": not args.serial, "split_n": args.split_n}}
     out.update(r)
     ctx.close()
     print(json.dumps(out), flush=True)

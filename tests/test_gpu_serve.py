@@ -49,9 +49,9 @@ def test_served_results_same_alone_and_in_mix(ctx):
         assert np.array_equal(j.result, jb.result), j.req
         methods.add(j.req.method)
     assert len(methods) >= 2
-    # the methods' batches on their own streams, run concurrently: the same results
-    bp = serve.GpuBackend(ctx, method_contexts=True)
-    rc = serve.serve_batched(bp, reqs, 8 << 30, quantum=5, parallel_methods=True)
+    # the lanes (method and size class) on their own streams, run concurrently: the same results
+    bp = serve.GpuBackend(ctx, lane_contexts=True)
+    rc = serve.serve_batched(bp, reqs, 8 << 30, quantum=5, parallel=True, split_n=4096)
     assert not rc.rejected
     byid_c = {j.req.rid: j for j in rc.jobs}
     for j in rp.jobs:

@@ -777,3 +777,47 @@ def test_warm_start_eps_branch_linearity_and_clamp():
     rel = np.abs(c1 - below).max() / np.abs(c1).max()
     assert 0.005 < rel < 0.02
     o.close()
+
+
+def _cubic_grad_np(d, h):
+    """Cubic spline gradient (src/CubicSpline.h) for difference vectors d
+    (..., 3): 48/(pi h^3) q (3q - 2) r/(|r| h) for q <= 1/2, -48/(pi h^3)
+    (1 - q)^2 r/(|r| h) for q <= 1, 0 for |r| <= 1e-5 and q > 1."""
+    rl = np.linalg.norm(d, axis=-1)
+    q = rl / h
+    l = 48.0 / (math.pi * h ** 3)
+    f = np.where(q <= 0.5, l * q * (3.0 * q - 2.0), -l * (1.0 - q) ** 2)
+    f = np.where((rl > 1e-5) & (q <= 1.0), f, 0.0)
+    return (f / np.where(rl > 0, rl * h, 1.0))[..., None] * d
+
+
+def test_divergence_error_of_given_velocities_brute_force():
+    """oracle_divergence_error: computeDivergenceError (src/DFSPHSolver.cpp:
+    408-492) with given velocities, against a brute-force sum over all fluid
+    and boundary pairs: sum_j m (v_i - v_j).gradW_ij + sum_b rho0 V_b
+    v_i.gradW_ib, zero below 20 fluid neighbors of nonzero gradient, else
+    clamped at 0.  The oracle's own velocities are kept."""
+    spec = configs.small_scene(sphg.DFSPH, n_side=8)
+    p = spec.params
+    o = po.Oracle(p, spec.fluid, spec.boundary)
+    o.step(2)
+    v_own = o.get(po.OF_V)
+    x = o.get(po.OF_X)
+    xb = np.asarray(spec.boundary, dtype=np.float64)
+    vb = o.get(po.OF_BVOL)
+    rng = np.random.default_rng(17)
+    vel = (rng.standard_normal((spec.n, 3)) * 0.3).astype(np.float32)
+    got = o.divergence_error(vel)
+    assert np.array_equal(o.get(po.OF_V), v_own)
+    h = p.support_radius
+    m = p.density0 * (2.0 * p.particle_radius) ** 3
+    v = vel.astype(np.float64)
+    g = _cubic_grad_np(x[:, None, :] - x[None, :, :], h)                       # (n, n, 3)
+    de = m * np.einsum("ijk,ijk->i", v[:, None, :] - v[None, :, :], g)
+    gb = _cubic_grad_np(x[:, None, :] - xb[None, :, :], h)                     # (n, nb, 3)
+    de += p.density0 * np.einsum("ijk,ik->i", (vb[None, :, None] * gb), v)
+    cnt = (np.linalg.norm(g, axis=-1) > 0).sum(1)
+    exp = np.where(cnt < 20, 0.0, np.maximum(de, 0.0))
+    assert (cnt < 20).any() and (cnt >= 20).any() and (exp > 0).any()
+    assert np.abs(got - exp).max() <= 1e-10 * np.abs(exp).max()
+    o.close()

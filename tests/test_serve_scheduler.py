@@ -1,8 +1,28 @@
 """Serving scheduler logic on the CPU: a fake backend with a virtual clock
 stands in for the device (the GPU path runs in test_gpu_serve.py)."""
 import numpy as np
+import pytest
 
 from sph_amd import serve, sphg
+
+
+class _SyncPool:
+    """Builds each request's geometry at submission: with the virtual clock
+    a background build would finish at an arbitrary virtual time."""
+
+    def submit(self, fn, *args):
+        from concurrent.futures import Future
+        f = Future()
+        f.set_result(fn(*args))
+        return f
+
+    def shutdown(self, wait=True):
+        pass
+
+
+@pytest.fixture(autouse=True)
+def _sync_specs(monkeypatch):
+    monkeypatch.setattr(serve, "_spec_pool", _SyncPool)
 
 
 class Clock:
@@ -17,8 +37,10 @@ class Clock:
 
 
 class FakeScene:
-    def __init__(self, spec):
+    def __init__(self, spec, lane=None):
         self.n = spec.n
+        self.method = spec.params.method
+        self.lane = lane
         self.steps = 0
         self.ready = False
         self.closed = False
@@ -28,6 +50,7 @@ class FakeScene:
 class FakeBatch:
     def __init__(self, be, scenes):
         self.be, self.scenes, self.closed = be, scenes, False
+        be.batches.append(sorted({(s.method, s.lane) for s in scenes}))
 
     def run(self, k):
         with self.be.lock:      # batches of different methods may run on threads
@@ -54,12 +77,13 @@ class FakeBackend:
         self.lock = threading.Lock()
         self.clock = clock
         self.quanta = []
+        self.batches = []       # the (method, lane) pairs of each batch built
         self.used = 0
         self.peak = 0
         self.admitted = []
 
-    def create(self, spec):
-        return FakeScene(spec)
+    def create(self, spec, lane=None):
+        return FakeScene(spec, lane)
 
     def plan_bytes(self, s):
         return s.n * self.BYTES_PER_PARTICLE
@@ -138,22 +162,42 @@ def test_batched_exact_steps_budget_fifo_quantum():
     assert max(n for n, _ in be.quanta) > 1
 
 
-def test_batched_methods_in_parallel_same_steps_and_budget():
-    """The methods' batches run their quanta on threads: every request still
-    gets exactly its steps, admission stays within the budget and in arrival
-    order, and everything is freed."""
-    clk = Clock()
-    be = FakeBackend(clk)
-    reqs = _stream()
-    budget = 3 * 4096 * FakeBackend.BYTES_PER_PARTICLE
-    res = serve.serve_batched(be, reqs, budget, quantum=7, clock=clk, sleep=clk.sleep, parallel_methods=True)
-    assert not res.rejected and len(res.jobs) == len(reqs)
-    for j in res.jobs:
-        assert int(j.result[0]) == j.req.steps
-        assert j.admitted >= j.req.arrival - 1e-12
-    assert be.peak <= budget and be.used == 0
-    order = [j.req.rid for j in sorted(res.jobs, key=lambda j: j.admitted)]
-    assert order == sorted(order)
+def test_batched_lanes_in_parallel_same_steps_and_budget():
+    """The lanes' batches run their quanta on threads, each lane refilled as
+    soon as its own quantum ends: every request still gets exactly its
+    steps, admission stays within the budget and in arrival order, and
+    everything is freed."""
+    for split in (0, 2048):
+        clk = Clock()
+        be = FakeBackend(clk)
+        reqs = _stream()
+        budget = 3 * 4096 * FakeBackend.BYTES_PER_PARTICLE
+        res = serve.serve_batched(be, reqs, budget, quantum=7, clock=clk, sleep=clk.sleep, parallel=True,
+                                  split_n=split)
+        assert not res.rejected and len(res.jobs) == len(reqs)
+        for j in res.jobs:
+            assert int(j.result[0]) == j.req.steps
+            assert j.admitted >= j.req.arrival - 1e-12
+        assert be.peak <= budget and be.used == 0
+        order = [j.req.rid for j in sorted(res.jobs, key=lambda j: j.admitted)]
+        assert order == sorted(order)
+
+
+def test_lanes_split_by_method_and_size():
+    """With a size boundary every batch holds scenes of one method and one
+    size class, and both classes occur; without one, one lane per method."""
+    reqs = _stream(30, rate=2000.0)
+    for split, parallel in ((2048, False), (2048, True), (0, False)):
+        clk = Clock()
+        be = FakeBackend(clk)
+        res = serve.serve_batched(be, reqs, 1 << 40, quantum=5, clock=clk, sleep=clk.sleep, parallel=parallel,
+                                  split_n=split)
+        assert all(int(j.result[0]) == j.req.steps for j in res.jobs)
+        assert all(len(b) == 1 for b in be.batches)
+        for j in res.jobs:
+            assert j.lane == serve.lane_of(j.req.method, j.n, split)
+        classes = {b[0][1][1] for b in be.batches}
+        assert classes == ({serve.SMALL, serve.LARGE} if split else {serve.SMALL})
 
 
 def test_batched_rejects_what_never_fits():
@@ -204,13 +248,13 @@ def test_neighbor_overflow_fails_only_its_request():
         bad = reqs[3].rid
         create = be.create
 
-        def create_marked(spec, _c=create):
-            sc = _c(spec)
+        def create_marked(spec, lane=None, _c=create):
+            sc = _c(spec, lane)
             sc.overflow = spec.name.startswith(f"req{bad}_")
             return sc
         be.create = create_marked
         r = serve.serve_batched(be, reqs, budget_bytes=10 ** 12, quantum=7, clock=clk, sleep=clk.sleep,
-                                parallel_methods=parallel)
+                                parallel=parallel, split_n=2048 if parallel else 0)
         byid = {j.req.rid: j for j in r.jobs}
         assert byid[bad].error and byid[bad].result is None
         for q in reqs:
@@ -227,8 +271,8 @@ def test_neighbor_overflow_fails_only_its_request():
     bad = reqs[2].rid
     create = be.create
 
-    def create_marked2(spec, _c=create):
-        sc = _c(spec)
+    def create_marked2(spec, lane=None, _c=create):
+        sc = _c(spec, lane)
         sc.overflow = spec.name.startswith(f"req{bad}_")
         return sc
     be.create = create_marked2
@@ -242,3 +286,85 @@ def test_neighbor_overflow_fails_only_its_request():
     byid = {j.req.rid: j for j in r.jobs}
     assert byid[bad].error and all(byid[q.rid].error is None for q in reqs if q.rid != bad)
     assert r.summary()["failed"] == 1
+
+
+def test_throughput_bound_from_plain_run():
+    """The bound divides the plain makespan by the stream's time at each
+    method's fastest alone rate: with a cost model whose per-particle rate
+    rises with n, the bound lies above 1 and below the rate spread."""
+    clk = Clock()
+    be = FakeBackend(clk)
+    reqs = _stream(12, rate=2000.0)
+    rp = serve.serve_plain(be, reqs, 1 << 40, clock=clk, sleep=clk.sleep)
+    b = serve.throughput_bound(rp)
+    assert b is not None and set(b["alone_rate_per_method"]) <= {"WCSPH", "PCISPH", "DFSPH"}
+    # fake step cost 2e-4 + 1e-9 n per step: the rate n / (2e-4 + 1e-9 n) grows with n
+    ns = [j.n for j in rp.jobs]
+    spread = (max(ns) / (2e-4 + 1e-9 * max(ns))) / (min(ns) / (2e-4 + 1e-9 * min(ns)))
+    assert 1.0 < b["ratio"] <= spread * 1.001
+    assert b["work_share_n_ge_262144"] == 0.0
+
+
+def test_admission_does_not_wait_for_geometry_while_scenes_run(monkeypatch):
+    """A request whose geometry is still being built (here: ready 0.05 s of
+    virtual time after its submission) does not hold up the resident
+    scenes: their quanta go on, and the request is admitted at the first
+    turn of the loop after its geometry is ready.  With nothing resident
+    the loop waits for it."""
+    from concurrent.futures import Future
+    clk = Clock()
+    delay = 0.05
+
+    class LateFuture(Future):
+        def result(self, timeout=None):
+            if not self.done():          # waiting for the build: the clock moves on to it
+                clk.t = max(clk.t, self.ready)
+                self.set_result(self.fn(*self.args))
+            return super().result()
+
+    class DelayedPool:
+        pending = []
+
+        def submit(self, fn, *args):
+            f = LateFuture()
+            f.fn, f.args, f.ready = fn, args, clk.t + delay
+            DelayedPool.pending.append(f)
+            return f
+
+        def shutdown(self, wait=True):
+            pass
+
+    def tick():
+        for f in DelayedPool.pending:
+            if not f.done() and clk.t >= f.ready:
+                f.set_result(f.fn(*f.args))
+
+    monkeypatch.setattr(serve, "_spec_pool", DelayedPool)
+    be = FakeBackend(clk)
+    run = FakeBatch.run
+
+    starts = []
+
+    def run_and_tick(self, k):
+        starts.append(clk.t)
+        run(self, k)
+        tick()
+    monkeypatch.setattr(FakeBatch, "run", run_and_tick)
+    reqs = serve.request_stream(6, 200.0, nmin=512, nmax=4096, smin=200, smax=300, seed=3)
+    res = serve.serve_batched(be, reqs, 1 << 40, quantum=5, clock=clk, sleep=clk.sleep)
+    byid = {j.req.rid: j for j in res.jobs}
+    ready = {f.args[0].rid: f.ready for f in DelayedPool.pending}
+    for q in reqs:
+        j = byid[q.rid]
+        assert int(j.result[0]) == q.steps
+        assert j.admitted >= ready[q.rid] - 1e-12              # not before its geometry is ready
+        assert ready[q.rid] >= q.arrival + delay - 1e-12       # built from its arrival on
+    # the first request ran alone while the others' geometry was built, and
+    # each later one joined within a quantum of its geometry being ready
+    assert be.quanta[0][0] == 1
+    for q in reqs[1:]:
+        assert byid[q.rid].admitted - ready[q.rid] < 0.01, q.rid
+    # quanta ran while the second request's geometry was being built (a
+    # blocking admission would have stopped the clock's other work there)
+    r1 = ready[reqs[1].rid]
+    assert any(r1 - delay < t < r1 for t in starts)
