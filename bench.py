@@ -36,7 +36,9 @@ density iterations both solvers needed on those steps.
 4,096-262,144) for 100 steps as one batch and one at a time.  `serve`
 serves a 24-request Poisson stream (WCSPH/PCISPH/DFSPH, N log-uniform in
 2,048-2,000,000 with one request above 1M, 100-300 steps) with continuous
-batching and one request at a time, with p50/p99 completion latency.
+batching and one request at a time, with p50/p99 completion latency and the
+throughput ratio the stream allows at best; `serve_burst` does the same for
+64 requests of 2,048-262,144 particles arriving at once.
 """
 import argparse
 import json
@@ -268,7 +270,7 @@ def main():
         log(f"cpu baseline {cpu['value']:.3g} particle-steps/s on {threads} threads")
 
     # ---------------------------------------------- batching and serving
-    batch = serve = None
+    batch = serve = burst = None
     if not args.no_extras:
         from sph_amd import bench_batch, serve as srv
         bspecs = configs.batched_dfsph(64)
@@ -286,6 +288,12 @@ def main():
                            f"(largest {max(q.n_target for q in reqs)}), 100-300 steps, seed {seed}, 200 GB budget, "
                            f"quantum 25, lanes by method and size (split at {srv.SPLIT_N}) on their own streams")
         log(f"serve: ratio {serve['throughput_ratio']}, p99 {serve['batched']['latency_p99_s']} s")
+        reqs = srv.request_stream(64, 1000.0, 2048, 262144, 100, 300, 5)
+        burst = srv.compare(ctx, reqs, 200 * 2 ** 30, quantum=25, check=4, parallel=True, split_n=srv.SPLIT_N)
+        burst["stream"] = ("64 requests, Poisson 1000/s (a burst), WCSPH/PCISPH/DFSPH uniform, N log-uniform "
+                           "2,048-262,144, 100-300 steps, seed 5, 200 GB budget, quantum 25, lanes by method and "
+                           f"size (split at {srv.SPLIT_N}) on their own streams")
+        log(f"serve burst: ratio {burst['throughput_ratio']}, p99 {burst['batched']['latency_p99_s']} s")
 
     ctx.close()
     out = {
@@ -321,6 +329,8 @@ def main():
         out["batch"] = batch
     if serve:
         out["serve"] = serve
+    if burst:
+        out["serve_burst"] = burst
     if kernels:
         out["kernels"] = kernels
     if warm:
