@@ -595,17 +595,23 @@ __global__ __launch_bounds__(kBlock) void k_rank_reorder(const DevScene* __restr
     const int cl = S.cellU[i];
     const int s = S.cellStart[cl], e = S.cellStart[cl + 1];
     const uint32_t me = S.id[i];
+    // the particle's data is loaded before the rank loop and before any
+    // store (a wait for a load also waits for the stores issued before it)
+    const float4 xo = S.x[i], vo = S.v[i];
+    const bool df = S.method == DFSPH;
+    const float ko = df ? S.k[i] : 0.f, kvo = df ? S.kv[i] : 0.f;
+    const float4 no = (df && S.pbOn) ? S.pbn[i] : make_float4(0.f, 0.f, 0.f, 0.f);
     int rank = 0;
     for (int q = s; q < e; ++q) rank += (S.perm[q] < me) ? 1 : 0;
     const int d = s + rank;
-    S.x2[d] = S.x[i];
-    S.v2[d] = S.v[i];
+    S.x2[d] = xo;
+    S.v2[d] = vo;
     S.id2[d] = me;
     S.cell[d] = cl;
-    if (S.method == DFSPH) {
-        S.k2[d] = S.k[i];
-        S.kv2[d] = S.kv[i];
-        if (S.pbOn) S.pbn2[d] = S.pbn[i];
+    if (df) {
+        S.k2[d] = ko;
+        S.kv2[d] = kvo;
+        if (S.pbOn) S.pbn2[d] = no;
     }
     if (S.pbOn) S.everOcc[cl] = 1;
     S.cellCount[cl] = 0;
@@ -763,7 +769,9 @@ __global__ __launch_bounds__(kBlock) void k_build(const DevScene* __restrict__ T
         else shellLost = true;
         ++ns;
     };
-    auto visit = [&](uint32_t j, const float4& xj, const float4& vj) {
+    // pos: the slot the candidate was read from (-1: from the walk); an
+    // entry is written only where the compaction moves it
+    auto visit = [&](uint32_t j, const float4& xj, const float4& vj, int pos) {
         const Pair p = pair_of(xx, xy, xz, xj, k);
         if (!(p.r2 <= k.hhi2)) {
             toShell(j);
@@ -779,7 +787,7 @@ __global__ __launch_bounds__(kBlock) void k_build(const DevScene* __restrict__ T
             }
             gnz = grad_nonzero64(e);
         }
-        if (cnt < maxN) stg(L, (size_t)slot_of(cnt), j);
+        if (cnt < maxN && cnt != pos) stg(L, (size_t)slot_of(cnt), j);
         ++cnt;
         ngr += gnz ? 1 : 0;
         ws += W_d(p.q, k);
@@ -836,12 +844,19 @@ __global__ __launch_bounds__(kBlock) void k_build(const DevScene* __restrict__ T
         return make_float3(p.x, p.y, p.z);
     };
     if (valid) {
+        // the 9 rows' ranges are loaded before pass 1's first list store:
+        // a wait for a load also waits for every store issued before it
+        int rs[9], re[9];
 #pragma unroll
         for (int r = 0; r < 9; ++r) {
             const int off = (r % 3 - 1) * nx + (r / 3 - 1) * nxny;
-            const int s0 = ldg(S.cellStart, (size_t)(ci - 1 + off)), e0 = ldg(S.cellStart, (size_t)(ci + 2 + off));
-            if (r < nStaged) scanRow(r, s0, e0, nc, ldsFetch);
-            else scanRow(r, s0, e0, nc, gblFetch);
+            rs[r] = ldg(S.cellStart, (size_t)(ci - 1 + off));
+            re[r] = ldg(S.cellStart, (size_t)(ci + 2 + off));
+        }
+#pragma unroll
+        for (int r = 0; r < 9; ++r) {
+            if (r < nStaged) scanRow(r, rs[r], re[r], nc, ldsFetch);
+            else scanRow(r, rs[r], re[r], nc, gblFetch);
         }
         if (M == PCISPH && nc + ns > maxN) shellLost = true;
         if (nc <= maxN) {
@@ -862,7 +877,7 @@ __global__ __launch_bounds__(kBlock) void k_build(const DevScene* __restrict__ T
                 }
 #pragma unroll
                 for (int u = 0; u < 4; ++u)
-                    if (q0 + u < nc) visit(jj[u], xj[u], vj[u]);
+                    if (q0 + u < nc) visit(jj[u], xj[u], vj[u], q0 + u);
             }
         } else {
             shellLost = true;
@@ -871,7 +886,7 @@ __global__ __launch_bounds__(kBlock) void k_build(const DevScene* __restrict__ T
                 const float4 xj = ld4(S.x, j);
                 const float fx = xi.x - xj.x, fy = xi.y - xj.y, fz = xi.z - xj.z;
                 if (!(fx * fx + fy * fy + fz * fz <= hh) || j == i) return;
-                visit((uint32_t)j, xj, vel((uint32_t)j));
+                visit((uint32_t)j, xj, vel((uint32_t)j), -1);
             });
         }
     }
@@ -915,14 +930,33 @@ __global__ __launch_bounds__(kBlock) void k_build(const DevScene* __restrict__ T
     // the boundary survivors use the free slots between the list and the shell
     const int room = max(0, maxN - own - ((M == PCISPH && !shellLost) ? ns : 0));
     int nbc = 0;
-    walk27(S.bCellStart, ci, nx, nxny, [&](int b) {
-        const float4 y = ld4(S.xb, b);
-        const float fx = xi.x - y.x, fy = xi.y - y.y, fz = xi.z - y.z;
-        if (fx * fx + fy * fy + fz * fz <= hh) {
-            if (nbc < room) stg(Lg, slot_of(own + nbc) + lbase, (uint32_t)b);
-            ++nbc;
+    {
+        // the 9 rows' ranges first, then 4 candidates' loads at a time: a
+        // wait for a load also waits for the survivor stores before it
+        int bs[9], be[9];
+#pragma unroll
+        for (int r = 0; r < 9; ++r) {
+            const int off = (r % 3 - 1) * nx + (r / 3 - 1) * nxny;
+            bs[r] = ldg(S.bCellStart, (size_t)(ci - 1 + off));
+            be[r] = ldg(S.bCellStart, (size_t)(ci + 2 + off));
         }
-    });
+#pragma unroll 1
+        for (int r = 0; r < 9; ++r) {
+            for (int b0 = bs[r]; b0 < be[r]; b0 += 4) {
+                float4 y[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) y[u] = ld4(S.xb, min(b0 + u, be[r] - 1));
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const float fx = xi.x - y[u].x, fy = xi.y - y[u].y, fz = xi.z - y[u].z;
+                    if ((b0 + u < be[r]) & (fx * fx + fy * fy + fz * fz <= hh)) {
+                        if (nbc < room) stg(Lg, slot_of(own + nbc) + lbase, (uint32_t)(b0 + u));
+                        ++nbc;
+                    }
+                }
+            }
+        }
+    }
     if (nbc <= room) {
         for (int q0 = 0; q0 < nbc; q0 += 4) {
             float4 yb[4];
@@ -1350,9 +1384,23 @@ __global__ __launch_bounds__(kBlock) void k_df_vel(const DevScene* __restrict__ 
                sy += g * p.dy;
                sz += g * p.dz;
            });
+    // every load of the epilogue is issued before its first store: a wait
+    // for a load also waits for the stores issued before it
+    const float4 gb = S.gb[i];
+    float kold = 0.f, kerr = 0.f, kal = 0.f;
+    if (MODE == 1) { kold = S.k[i]; kerr = S.drhoS[i]; kal = S.alpha[i]; }
+    if (MODE == 2) { kold = S.kv[i]; kerr = S.divErr[i]; kal = S.alpha[i]; }
+    float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+    float uxf = 0.f;
+    float2 uyz = make_float2(0.f, 0.f);
+    if (MODE <= 1) {
+        uxf = ((gptr<const float>)S.x)[4 * (size_t)i + 3];
+        uyz = ld2o(S.vyz, (uint32_t)i);
+    } else {
+        v = S.v[i];
+    }
     const double cb = (MODE == 0) ? 2.0 * si : si;
     if (fabs(cb) > eps) {
-        const float4 gb = S.gb[i];
         sx += cb * (double)gb.x;
         sy += cb * (double)gb.y;
         sz += cb * (double)gb.z;
@@ -1360,11 +1408,8 @@ __global__ __launch_bounds__(kBlock) void k_df_vel(const DevScene* __restrict__ 
     if (MODE <= 1) {
         // the density solve: the change u (x.w, vyz; no kernel gathers x
         // during this sweep) takes the correction, v keeps v0
-        const float uxf = ((gptr<const float>)S.x)[4 * (size_t)i + 3];
-        const float2 uyz = ld2o(S.vyz, (uint32_t)i);
         store_u(S, i, (double)uxf - dt * sx, (double)uyz.x - dt * sy, (double)uyz.y - dt * sz);
     } else {
-        float4 v = S.v[i];
         v.x = (float)((double)v.x - dt * sx);
         v.y = (float)((double)v.y - dt * sy);
         v.z = (float)((double)v.z - dt * sz);
@@ -1375,8 +1420,8 @@ __global__ __launch_bounds__(kBlock) void k_df_vel(const DevScene* __restrict__ 
         ((gptr<v2f>)S.vyz)[i] = v2f{v.y, v.z};
     }
     if (MODE == 0) S.k[i] = 0.0f;
-    if (MODE == 1) S.k[i] = (float)((double)S.k[i] + (double)S.drhoS[i] * (double)S.alpha[i]);
-    if (MODE == 2) S.kv[i] = (float)((double)S.kv[i] + (double)S.divErr[i] * (double)S.alpha[i]);
+    if (MODE == 1) S.k[i] = (float)((double)kold + (double)kerr * (double)kal);
+    if (MODE == 2) S.kv[i] = (float)((double)kold + (double)kerr * (double)kal);
 }
 
 // Density or divergence error sweep with the scene reduction and the loop
@@ -1407,6 +1452,10 @@ __global__ __launch_bounds__(kBlock) void k_df_err(const DevScene* __restrict__ 
         const float4 xi = ld4(S.x, i);
         const double xx = xi.x, xy = xi.y, xz = xi.z;
         const float4 gb = S.gb[i];
+        // loaded before any store (a wait for a load also waits for the
+        // stores issued before it)
+        const float drhof = S.drho[i], alf = S.alpha[i];
+        const int ngr = DENS ? 20 : (int)S.ngrad[i];
         double dc;
         if (DENS) {
             // the density solve: x.w and vyz hold u, v holds v0 (see store_u)
@@ -1454,9 +1503,9 @@ __global__ __launch_bounds__(kBlock) void k_df_err(const DevScene* __restrict__ 
             // the divergence of the current velocities: the next step's D0
             ((gptr<double>)S.d0)[i] = dc;
         }
-        const double drho = (double)S.drho[i];
+        const double drho = (double)drhof;
         const double rho = S.rho0D + drho;
-        const double al = S.alpha[i];
+        const double al = alf;
         double s;
         if (DENS) {
             const double drs = fmax(drho + dc * dt, 0.0);
@@ -1464,7 +1513,7 @@ __global__ __launch_bounds__(kBlock) void k_df_err(const DevScene* __restrict__ 
             s = drs * al / (dt * dt * rho);
             e = drs / S.rho0D;
         } else {
-            const double de = (S.ngrad[i] < 20) ? 0.0 : fmax(dc, 0.0);
+            const double de = (ngr < 20) ? 0.0 : fmax(dc, 0.0);
             S.divErr[i] = (float)de;
             s = de * al / (dt * rho);
             e = de / S.rho0D;
