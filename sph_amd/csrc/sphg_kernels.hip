@@ -1,11 +1,12 @@
 // sphg_kernels.hip -- gfx950 kernels of the SPH solver step.
 //
 // Every kernel takes a scene table and a chunk table; block b works on
-// chunk b (kBlock = 256 particles or kCellChunk = 1024 cells of one scene).  Sums of a scene are
-// formed in a fixed order: a butterfly inside each wavefront, wave results
-// combined in index order, one partial per chunk, and the last chunk to
-// finish (an atomic arrival counter) adds the scene's partials in chunk
-// order.  Results therefore do not depend on scheduling.
+// chunk b (kBlock = 256 particles or kCellChunk = 1024 cells of one
+// scene).  Sums of a scene are formed in a fixed order: a butterfly inside
+// each wavefront, wave results combined in index order, one partial per
+// chunk, and then the partials added in chunk order by k_finalize (one
+// block per scene, launched after the sweep).  Results therefore do not
+// depend on scheduling.
 //
 // Reference algorithm: josemanuel28v/SPH src/SPHSolver.cpp,
 // src/DFSPHSolver.cpp, src/WCSPHSolver.cpp, src/PCISPHSolver.cpp, the kernel
