@@ -57,3 +57,30 @@ def test_served_results_same_alone_and_in_mix(ctx):
     for j in rp.jobs:
         assert np.array_equal(j.result, byid_c[j.req.rid].result), j.req
     bp.shutdown()
+
+
+def test_served_stream_with_a_million_particle_request(ctx):
+    """A stream shaped like the configured one -- WCSPH/PCISPH/DFSPH, N from
+    2k up to a request of ~1.2M particles -- served with lanes by method and
+    size on their own streams, and plain: every result is the same alone and
+    in the mix, bit for bit, and the large request ran in its own lane."""
+    reqs = serve.request_stream(6, rate=400.0, nmin=2048, nmax=20000, smin=10, smax=25, seed=31)
+    big = serve.Request(rid=99, arrival=0.002, method=sphg.DFSPH, n_target=1_200_000, steps=12,
+                        geometry="block", seed=12345)
+    reqs = sorted(reqs + [big], key=lambda q: q.arrival)
+    be = serve.GpuBackend(ctx, lane_contexts=True)
+    try:
+        rb = serve.serve_batched(be, reqs, 64 << 30, quantum=5, parallel=True, split_n=serve.SPLIT_N)
+    finally:
+        be.shutdown()
+    rp = serve.serve_plain(serve.GpuBackend(ctx), reqs, 64 << 30)
+    assert not rb.rejected and not rp.rejected
+    byid = {j.req.rid: j for j in rb.jobs}
+    for j in rp.jobs:
+        jb = byid[j.req.rid]
+        assert j.result is not None and jb.result is not None, j.req
+        assert np.array_equal(j.result, jb.result), j.req
+    jb = byid[99]
+    assert jb.n >= 1_000_000 and jb.lane == serve.lane_of(sphg.DFSPH, jb.n, serve.SPLIT_N)
+    assert jb.lane[1] == serve.LARGE
+    assert {j.req.method for j in rp.jobs} == set(serve.METHODS)
