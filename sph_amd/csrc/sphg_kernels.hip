@@ -863,11 +863,13 @@ __global__ __launch_bounds__(kBlock) void k_build(const DevScene* __restrict__ T
         if (nc <= maxN) {
             // 4 survivors per pass, their list entries and positions loaded
             // first; visit writes slots <= the one it reads, so entries
-            // already loaded are not overwritten before use
+            // already loaded are not overwritten before use, and the next
+            // block of entries is requested before this pass's data is used
+            v4u bn = ld_block(L, 0);
             for (int q0 = 0; q0 < nc; q0 += 4) {
                 // one block of 4 slots; those past the last survivor hold
                 // stale values and are replaced by the first (a valid index)
-                const v4u blk = ld_block(L, q0 >> 2);
+                const v4u blk = bn;
                 const uint32_t jj[4] = {blk.x, q0 + 1 < nc ? blk.y : blk.x, q0 + 2 < nc ? blk.z : blk.x,
                                         q0 + 3 < nc ? blk.w : blk.x};
                 float4 xj[4], vj[4];
@@ -876,6 +878,8 @@ __global__ __launch_bounds__(kBlock) void k_build(const DevScene* __restrict__ T
                     xj[u] = ld3o(S.x, jj[u]);
                     vj[u] = vel(jj[u]);
                 }
+                bn = ld_block(L, (q0 + 4 < nc ? q0 + 4 : q0) >> 2);
+                asm volatile("" ::: "memory");
 #pragma unroll
                 for (int u = 0; u < 4; ++u)
                     if (q0 + u < nc) visit(jj[u], xj[u], vj[u], q0 + u);
