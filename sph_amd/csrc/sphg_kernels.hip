@@ -726,15 +726,27 @@ __global__ __launch_bounds__(kBlock) void k_build(const DevScene* __restrict__ T
             if (base[r + 1] <= kWaveStage) nStaged = r + 1;
     }
     {
+        // 4 loads per lane in flight before the first is written to LDS
         const int tot = base[nStaged];
-        for (int t = lane; t < tot; t += 64) {
-            int f = first[0], b0 = 0;
+        for (int t0 = lane; t0 < tot; t0 += 256) {
+            float4 p[4];
 #pragma unroll
-            for (int q = 1; q < 9; ++q)
-                if (t >= base[q]) { f = first[q]; b0 = base[q]; }
-            const float4 p = ld4(S.x, (size_t)(f + (t - b0)));
-            sxy[w][t] = make_float2(p.x, p.y);
-            sz[w][t] = p.z;
+            for (int u = 0; u < 4; ++u) {
+                const int t = t0 + 64 * u;
+                int f = first[0], b0 = 0;
+#pragma unroll
+                for (int q = 1; q < 9; ++q)
+                    if (t >= base[q]) { f = first[q]; b0 = base[q]; }
+                p[u] = (t < tot) ? ld4(S.x, (size_t)(f + (t - b0))) : make_float4(0.f, 0.f, 0.f, 0.f);
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int t = t0 + 64 * u;
+                if (t < tot) {
+                    sxy[w][t] = make_float2(p[u].x, p[u].y);
+                    sz[w][t] = p[u].z;
+                }
+            }
         }
     }
     __syncthreads();
