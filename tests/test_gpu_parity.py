@@ -980,31 +980,48 @@ def test_inflow_only_scene(ctx, method):
 def test_long_run_aggregate_statistics(ctx, method, steps):
     """Long runs: the GPU and the oracle run freely from the same state (the
     trajectories part ways at the FP32 storage rounding) through the dam's
-    collapse.  The time averages of kinetic and potential energy and of the
-    mean density agree within 5e-3 (measured 5e-5 to 1.6e-3; the oracle's
-    OpenMP reductions follow the host's thread count); the per-step
-    series differ by less than 2e-2, 5e-3 and 5e-3 on average (they drift
-    apart in phase as the trajectories part); the solver iteration totals
-    agree within 3%."""
+    collapse.  Compared:
+      * time averages of kinetic and potential energy and of the mean
+        density deviation rho - rho0 (the free surface's missing neighbors
+        make it ~-55 kg/m^3 here): within 5e-3, 5e-3 and 2e-2;
+      * the solver's own density error, time-averaged (DFSPH: the average
+        density error its loop ends on; PCISPH: the largest one): within
+        1e-2 and 5e-2 (an FP64 oracle run against one rounding its state to
+        FP32 every step differs by 1.5e-4 and 1.1e-2);
+      * the per-step series, on average: within 2e-2 (kinetic energy), 5e-3
+        (potential) and 5e-2 (deviation); they drift apart in phase as the
+        trajectories part;
+      * the solver iteration totals: within 3%."""
     spec = configs.small_scene(method, n_side=12)
     g, o = _pair(ctx, spec)
-    m = spec.params.density0 * (2 * spec.params.particle_radius) ** 3
+    rho0 = spec.params.density0
+    m = rho0 * (2 * spec.params.particle_radius) ** 3
+    errkey = {sphg.DFSPH: "avg_density_error", sphg.PCISPH: "max_density_error"}.get(method)
     G, O = [], []
     for _ in range(steps):
         g.step(1)
         o.step(1)
-        for S, x, v, rho, it in ((G, g.get(sphg.FIELD_X), g.get(sphg.FIELD_V), g.get(sphg.FIELD_RHO), g.scalars().iterations),
-                                 (O, o.get(po.OF_X), o.get(po.OF_V), o.get(po.OF_RHO), int(o.scalars()["iterations"]))):
+        sg, so = g.scalars(), o.scalars()
+        for S, x, v, dev, it, err in (
+                (G, g.get(sphg.FIELD_X), g.get(sphg.FIELD_V), g.get(sphg.FIELD_DRHO), sg.iterations,
+                 getattr(sg, errkey) if errkey else 0.0),
+                (O, o.get(po.OF_X), o.get(po.OF_V), o.get(po.OF_RHO) - rho0, int(so["iterations"]),
+                 so[errkey] if errkey else 0.0)):
             v = v.astype(np.float64)
-            S.append((0.5 * m * (v * v).sum(), m * 9.81 * x[:, 2].astype(np.float64).sum(), rho.mean(), it))
+            S.append((0.5 * m * (v * v).sum(), m * 9.81 * x[:, 2].astype(np.float64).sum(),
+                      np.asarray(dev, dtype=np.float64).mean(), it, err))
     G, O = np.array(G), np.array(O)
     rel = lambda a, b: float(np.abs(a - b).mean() / np.abs(b).mean())
     avg = lambda a, b: float(abs(a.mean() - b.mean()) / abs(b.mean()))
-    series = {"ekin": rel(G[:, 0], O[:, 0]), "epot": rel(G[:, 1], O[:, 1]), "rho": rel(G[:, 2], O[:, 2])}
-    means = {"ekin": avg(G[:, 0], O[:, 0]), "epot": avg(G[:, 1], O[:, 1]), "rho": avg(G[:, 2], O[:, 2])}
+    series = {"ekin": rel(G[:, 0], O[:, 0]), "epot": rel(G[:, 1], O[:, 1]), "dev": rel(G[:, 2], O[:, 2])}
+    means = {"ekin": avg(G[:, 0], O[:, 0]), "epot": avg(G[:, 1], O[:, 1]), "dev": avg(G[:, 2], O[:, 2])}
+    if errkey:
+        means["err"] = avg(G[:, 4], O[:, 4])
     print("long run: series", series, "time averages", means, "iterations", G[:, 3].sum(), O[:, 3].sum())
-    assert all(v < 5e-3 for v in means.values()), means
-    assert series["ekin"] < 2e-2 and series["epot"] < 5e-3 and series["rho"] < 5e-3, series
+    assert means["ekin"] < 5e-3 and means["epot"] < 5e-3 and means["dev"] < 2e-2, means
+    if errkey:
+        assert means["err"] < (1e-2 if method == sphg.DFSPH else 5e-2), means
+    assert series["ekin"] < 2e-2 and series["epot"] < 5e-3 and series["dev"] < 5e-2, series
     assert abs(G[:, 3].sum() - O[:, 3].sum()) <= 0.03 * max(O[:, 3].sum(), 1)
     # the dam collapsed: kinetic energy rose well above zero and potential fell
     assert G[:, 0].max() > 0.05 * abs(G[0, 1]) and G[-1, 1] < G[0, 1]
