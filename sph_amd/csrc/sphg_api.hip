@@ -108,15 +108,27 @@ struct Profiler {
     }
 };
 
+// Device memory of scenes, batch tables and staging is stream-ordered
+// (hipMallocAsync / hipFreeAsync on the owning context's stream): a plain
+// hipMalloc or hipFree waits for the work queued on every stream of the
+// device, so admitting or retiring a scene would stall the other contexts'
+// scenes (measured: 2-6 ms per call while another context steps a 1M scene).
+// Freed blocks stay in the device's default pool (release threshold raised
+// at context creation) for the next allocation.
+hipError_t dev_alloc(void** p, size_t bytes, hipStream_t st) { return hipMallocAsync(p, bytes, st); }
+void dev_free(void* p, hipStream_t st) {
+    if (p) (void)hipFreeAsync(p, st);
+}
+
 // A temporary device allocation freed on every return path.
 struct DevTemp {
     void* p = nullptr;
-    DevTemp() = default;
+    hipStream_t st;
+    explicit DevTemp(hipStream_t s) : st(s) {}
     DevTemp(const DevTemp&) = delete;
     DevTemp& operator=(const DevTemp&) = delete;
-    ~DevTemp() {
-        if (p) (void)hipFree(p);
-    }
+    hipError_t alloc(size_t bytes) { return dev_alloc(&p, bytes, st); }
+    ~DevTemp() { dev_free(p, st); }
     float* f() const { return (float*)p; }
 };
 
@@ -192,15 +204,16 @@ struct Batch {
     // and segment replay (-1: the context's setting)
     int slackD = 1, slackV = 0, replay = -1;
     hipGraphExec_t seg[3][2] = {};
+    hipStream_t allocStream = nullptr;  // the stream the tables were allocated on
     ~Batch() {
         for (auto& a : seg)
             for (auto& e : a)
                 if (e) (void)hipGraphExecDestroy(e);
         if (!ownTables) return;
-        if (dT[0]) (void)hipFree(dT[0]);
-        if (dT[1]) (void)hipFree(dT[1]);
-        if (dP) (void)hipFree(dP);
-        if (dC) (void)hipFree(dC);
+        dev_free(dT[0], allocStream);
+        dev_free(dT[1], allocStream);
+        dev_free(dP, allocStream);
+        dev_free(dC, allocStream);
     }
 };
 
@@ -223,8 +236,9 @@ struct sphg_scene {
     uint32_t n = 0, nb = 0;         // n: active particles
     uint32_t cap = 0;               // particle slots: fluid + every emitter's budget
     double dt0 = 0, dtMin = 0, dtMax = 0;
-    char* mem = nullptr;
+    char* mem = nullptr;            // stream-ordered on ctx->stream
     size_t memBytes = 0;
+    hipStream_t memStream = nullptr;
     float* tmp = nullptr;      // 4 floats per particle of staging
     Scalars* ss = nullptr;
     Batch* self = nullptr;
@@ -257,7 +271,7 @@ sphg_scene::~sphg_scene() {
         sn->scene = nullptr;
     }
     delete self;
-    if (mem) (void)hipFree(mem);
+    dev_free(mem, memStream);
 }
 
 namespace {
@@ -415,13 +429,15 @@ int upload_batch_tables(sphg_ctx* ctx, Batch& B) {
     P = xcd_order(P);
     Cc = xcd_order(Cc);
     if (!B.dT[0]) {
-        HIP_TRY(hipMalloc(&B.dT[0], sizeof(DevScene) * ns));
-        HIP_TRY(hipMalloc(&B.dT[1], sizeof(DevScene) * ns));
+        B.allocStream = ctx->stream;
+        HIP_TRY(dev_alloc((void**)&B.dT[0], sizeof(DevScene) * ns, ctx->stream));
+        HIP_TRY(dev_alloc((void**)&B.dT[1], sizeof(DevScene) * ns, ctx->stream));
         B.tab0 = B.dT[0];
     }
     if (!B.dP) {
-        HIP_TRY(hipMalloc(&B.dP, sizeof(Chunk) * std::max<size_t>(P.size(), 1)));
-        HIP_TRY(hipMalloc(&B.dC, sizeof(Chunk) * std::max<size_t>(Cc.size(), 1)));
+        B.allocStream = ctx->stream;
+        HIP_TRY(dev_alloc((void**)&B.dP, sizeof(Chunk) * std::max<size_t>(P.size(), 1), ctx->stream));
+        HIP_TRY(dev_alloc((void**)&B.dC, sizeof(Chunk) * std::max<size_t>(Cc.size(), 1), ctx->stream));
     }
     HIP_TRY(hipMemcpyAsync(B.dT[0], t0.data(), sizeof(DevScene) * ns, hipMemcpyHostToDevice, ctx->stream));
     HIP_TRY(hipMemcpyAsync(B.dT[1], t1.data(), sizeof(DevScene) * ns, hipMemcpyHostToDevice, ctx->stream));
@@ -1063,6 +1079,15 @@ int sphg_ctx_create(int32_t device, sphg_ctx** out) {
     c->device = device;
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->copyStream, hipStreamNonBlocking);
+    if (e == hipSuccess) {
+        // freed scene memory stays in the device's default pool for the next
+        // scene instead of going back to the driver at the next synchronisation
+        hipMemPool_t pool = nullptr;
+        if (hipDeviceGetDefaultMemPool(&pool, device) == hipSuccess && pool) {
+            uint64_t keep = UINT64_MAX;
+            (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+        }
+    }
     if (e == hipSuccess) e = hipMalloc(&c->dActive, 2 * sizeof(int));
     if (e == hipSuccess) e = hipHostMalloc(&c->hActive, 2 * sizeof(int), hipHostMallocDefault);
     if (e != hipSuccess) {
@@ -1206,8 +1231,8 @@ int sphg_scene_set_pb_normals(sphg_scene* s, const float* nrm) {
     HIP_TRY(hipSetDevice(ctx->device));
     const size_t n = s->n;
     if (!n) return SPHG_OK;
-    DevTemp d;
-    HIP_TRY(hipMalloc(&d.p, sizeof(float) * 3 * n));
+    DevTemp d(ctx->stream);
+    HIP_TRY(d.alloc(sizeof(float) * 3 * n));
     HIP_TRY(hipMemcpyAsync(d.p, nrm, sizeof(float) * 3 * n, hipMemcpyHostToDevice, ctx->stream));
     const DevScene& V = s->view[s->par];
     launch_gather_f3(ctx->stream, V.pbn, d.f(), V.id, (int)n);
@@ -1241,7 +1266,8 @@ int sphg_scene_init(sphg_scene* s) {
     const long ncells = P.ncells;
     const uint32_t n = s->n, nb = s->nb;
     const int nChunks = P.nChunks, nCellChunks = P.nCellChunks, maxNbr = P.maxNbr, flags = P.flags;
-    hipError_t he = hipMalloc(&s->mem, P.total);
+    hipError_t he = dev_alloc((void**)&s->mem, P.total, ctx->stream);
+    s->memStream = ctx->stream;
     if (he != hipSuccess) return fail(SPHG_ENOMEM, "device allocation of " + std::to_string(P.total) + " bytes failed");
     s->memBytes = P.total;
     char* b = s->mem;
@@ -1465,8 +1491,8 @@ int sphg_scene_init(sphg_scene* s) {
 
     // fluid upload (staging in tmp is too small for xyz+vel: copy through a scratch allocation)
     if (n) {
-        DevTemp stage;
-        HIP_TRY(hipMalloc(&stage.p, sizeof(float) * 6 * (size_t)n));
+        DevTemp stage(ctx->stream);
+        HIP_TRY(stage.alloc(sizeof(float) * 6 * (size_t)n));
         float* dstage = stage.f();
         HIP_TRY(hipMemcpyAsync(dstage, s->fxyz.data(), sizeof(float) * 3 * n, hipMemcpyHostToDevice, ctx->stream));
         HIP_TRY(hipMemcpyAsync(dstage + 3 * (size_t)n, s->fvel.data(), sizeof(float) * 3 * n, hipMemcpyHostToDevice,
@@ -1500,8 +1526,8 @@ int sphg_scene_set_state(sphg_scene* s, const float* xyz, const float* vel, cons
     sphg_ctx* ctx = s->ctx;
     HIP_TRY(hipSetDevice(ctx->device));
     const size_t n = s->n;
-    DevTemp stage;
-    HIP_TRY(hipMalloc(&stage.p, sizeof(float) * 8 * n));
+    DevTemp stage(ctx->stream);
+    HIP_TRY(stage.alloc(sizeof(float) * 8 * n));
     float* dstage = stage.f();
     HIP_TRY(hipMemcpyAsync(dstage, xyz, sizeof(float) * 3 * n, hipMemcpyHostToDevice, ctx->stream));
     HIP_TRY(hipMemcpyAsync(dstage + 3 * n, vel, sizeof(float) * 3 * n, hipMemcpyHostToDevice, ctx->stream));

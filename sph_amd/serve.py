@@ -9,7 +9,9 @@ Continuous batching (`serve_batched`):
   * requests arrive on a wall clock (Poisson process); a waiting request is
     admitted, first come first served, when the device bytes of its scene
     (known before allocation, `Scene.plan_bytes`) fit the memory budget
-    beside the scenes already resident;
+    beside the scenes already resident; its scene is initialised on its
+    lane's context at the lane's next launch, so that a context is never
+    used by two host threads at once;
   * resident scenes stay on the device, grouped in lanes: one per solver
     method, split by size (`split_n`) so that small scenes do not wait on
     the step time of giant ones; a lane's scenes are stepped together, one
@@ -214,6 +216,7 @@ class Job:
     n: int = 0
     remaining: int = 0
     admitted: float = -1.0
+    inited: bool = False
     done: float = -1.0
     result: Optional[np.ndarray] = None
     error: Optional[str] = None      # set when the request failed (completed without a result)
@@ -314,7 +317,9 @@ def serve_batched(backend, requests: List[Request], budget_bytes: int, quantum: 
                 continue
             if used + j.bytes > budget_bytes:
                 break
-            backend.init(j.scene)
+            # its device memory is reserved now; the scene is initialised
+            # (allocated and uploaded) on its lane's context at the lane's
+            # next launch, while no quantum of that lane is running
             used += j.bytes
             j.remaining = j.req.steps
             j.admitted = clock() - t0
@@ -324,6 +329,10 @@ def serve_batched(backend, requests: List[Request], budget_bytes: int, quantum: 
             waiting.popleft()
 
     def launch(ln: _Lane):
+        for j in ln.jobs:
+            if not j.inited:
+                backend.init(j.scene)
+                j.inited = True
         if ln.batch is None or ln.stale:
             if ln.batch is not None:
                 backend.close(ln.batch)

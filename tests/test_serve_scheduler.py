@@ -368,3 +368,48 @@ def test_admission_does_not_wait_for_geometry_while_scenes_run(monkeypatch):
     # blocking admission would have stopped the clock's other work there)
     r1 = ready[reqs[1].rid]
     assert any(r1 - delay < t < r1 for t in starts)
+
+
+def test_scene_initialised_only_while_its_lane_is_idle():
+    """A scene is initialised on its lane's context, so that initialisation
+    never overlaps a running quantum of the same lane (a context is used by
+    one host thread at a time); the lanes' quanta run on threads and take a
+    little real time here, so an admission during a quantum would show."""
+    import threading
+    import time as _time
+    clk = Clock()
+    be = FakeBackend(clk)
+    running = set()
+    guard = threading.Lock()
+    run = FakeBatch.run
+    lanes_seen = []
+
+    def run_slowly(self, k):
+        lanes = {s.lane for s in self.scenes}
+        with guard:
+            running.update(lanes)
+        try:
+            _time.sleep(0.002)
+            run(self, k)
+        finally:
+            with guard:
+                running.difference_update(lanes)
+
+    init = be.init
+
+    def init_checked(s):
+        with guard:
+            assert s.lane not in running, s.lane
+            lanes_seen.append(s.lane)
+        init(s)
+    be.init = init_checked
+    FakeBatch.run = run_slowly
+    try:
+        reqs = _stream(20, rate=300.0, seed=13)
+        res = serve.serve_batched(be, reqs, 1 << 40, quantum=3, clock=clk, sleep=clk.sleep, parallel=True,
+                                  split_n=2048)
+    finally:
+        FakeBatch.run = run
+    assert all(int(j.result[0]) == j.req.steps for j in res.jobs)
+    assert len(lanes_seen) == len(reqs) and len(set(lanes_seen)) > 2
+    assert be.used == 0
