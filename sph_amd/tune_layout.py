@@ -40,6 +40,7 @@ WORKLOADS = {
     "dfsph_1m": (10, 200),
     "batch64": (5, 30),
     "wcsph_32k": (20, 400),
+    "dfsph_small": (5, 40),     # a 1,728-particle dam break (a quick check of identical results)
 }
 
 
@@ -77,7 +78,8 @@ def _run_child(lib: str, workload: str, out_x: str) -> dict:
         for s in scenes:
             s.close()
     else:
-        spec = configs.dfsph_dam_1m() if workload == "dfsph_1m" else configs.wcsph_dam_32k()
+        spec = {"dfsph_1m": configs.dfsph_dam_1m, "wcsph_32k": configs.wcsph_dam_32k,
+                "dfsph_small": lambda: configs.small_scene(sphg.DFSPH, n_side=12)}[workload]()
         s = sphg.Scene(ctx, spec.params, spec.fluid, spec.boundary)
         s.step(warm)
         ctx.synchronize()
