@@ -1,8 +1,8 @@
 // sph_oracle.cpp -- FP64 CPU restatement of the reference SPH solver path.
 //
-// TEST INFRASTRUCTURE ONLY.  This file is the checker for the HIP solver in
-// sph_amd/csrc: only tests/, __graft_entry__.smoke() and bench.py's CPU
-// baseline leg may load it.  The product path never calls it.
+// TEST INFRASTRUCTURE ONLY.  This file is the oracle the HIP solver in
+// sph_amd/csrc is tested against: only tests/, __graft_entry__.smoke() and
+// bench.py's CPU baseline leg may load it.  The product path never calls it.
 //
 // It restates, in double precision and with the reference's own data
 // structures (an unordered_map cell hash whose buckets hold per-cell

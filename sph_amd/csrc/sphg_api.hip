@@ -1081,7 +1081,7 @@ int sphg_ctx_create(int32_t device, sphg_ctx** out) {
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->copyStream, hipStreamNonBlocking);
     if (e == hipSuccess) {
         // freed scene memory stays in the device's default pool for the next
-        // scene instead of going back to the driver at the next synchronisation
+        // scene instead of being released at the next synchronisation
         hipMemPool_t pool = nullptr;
         if (hipDeviceGetDefaultMemPool(&pool, device) == hipSuccess && pool) {
             uint64_t keep = UINT64_MAX;
