@@ -327,6 +327,13 @@ int sphg_ctx_kernel_time(sphg_ctx* ctx, const char* name, uint64_t* launches, do
  * named: each event pair costs the stream a few microseconds, so a
  * throughput measurement times only the kernel it needs. */
 int sphg_ctx_kernel_timing_only(sphg_ctx* ctx, const char* const* names, uint32_t count);
+/* While timing is on, record only every `stride`-th launch of each kernel
+ * (from the first): a sample of the launches, at 1/stride of the event
+ * pairs' cost to the stream; launch counts and summed milliseconds then
+ * cover the sampled launches.  Enabling timing resets the stride to 1.  The
+ * reference's TimeManager counters (src/TimeManager.h:81-94) have no
+ * sampling. */
+int sphg_ctx_kernel_timing_stride(sphg_ctx* ctx, uint32_t stride);
 
 /* Scene geometry (set-up path, FP64 then rounded to float):
  * AkinciBoundaryModel::sampleCube (src/AkinciBoundayModel.cpp:100-134);

@@ -66,6 +66,8 @@ const char* kKindNames[K_NKINDS] = {
 struct Profiler {
     bool on = false;
     uint64_t mask = ~0ull;      // kinds recorded while on
+    uint32_t stride = 1;        // record every stride-th launch of each kind
+    uint64_t seen[K_NKINDS] = {};   // launches of each kind since enabling
     struct Rec {
         int kind;
         hipEvent_t a, b;
@@ -138,7 +140,8 @@ struct Scope {
     int kind;
     hipEvent_t a{}, b{};
     bool rec;
-    Scope(Profiler* p_, hipStream_t s, int k) : p(p_), st(s), kind(k), rec(p_->on && ((p_->mask >> k) & 1)) {
+    Scope(Profiler* p_, hipStream_t s, int k)
+        : p(p_), st(s), kind(k), rec(p_->on && ((p_->mask >> k) & 1) && (p_->seen[k]++ % p_->stride == 0)) {
         if (rec) {
             a = p->get();
             (void)hipEventRecord(a, st);
@@ -1984,12 +1987,20 @@ int sphg_ctx_kernel_timing(sphg_ctx* ctx, int32_t enable) {
     ctx->prof.collect();
     ctx->prof.on = enable != 0;
     ctx->prof.mask = ~0ull;
+    ctx->prof.stride = 1;
     if (enable) {
         for (int k = 0; k < K_NKINDS; ++k) {
             ctx->prof.count[k] = 0;
             ctx->prof.ms[k] = 0.0;
+            ctx->prof.seen[k] = 0;
         }
     }
+    return SPHG_OK;
+}
+
+int sphg_ctx_kernel_timing_stride(sphg_ctx* ctx, uint32_t stride) {
+    if (!ctx || stride == 0) return fail(SPHG_EINVAL, "NULL ctx or zero stride");
+    ctx->prof.stride = stride;
     return SPHG_OK;
 }
 

@@ -278,6 +278,7 @@ def load(path: str = LIB_PATH) -> C.CDLL:
         "sphg_ctx_kernel_timing": (C.c_int, [P, I32]),
         "sphg_ctx_kernel_time": (C.c_int, [P, C.c_char_p, C.POINTER(U64), dp]),
         "sphg_ctx_kernel_timing_only": (C.c_int, [P, C.POINTER(C.c_char_p), U32]),
+        "sphg_ctx_kernel_timing_stride": (C.c_int, [P, U32]),
         "sphg_sample_box": (I64, [dp, dp, D, fp, I64]),
         "sphg_sample_sphere": (I64, [dp, D, D, fp, I64]),
         "sphg_fluid_block": (I64, [dp, dp, D, D, U64, fp, I64]),
@@ -449,6 +450,11 @@ class Context:
             _check(self.lib.sphg_ctx_kernel_timing_only(self.h, names, len(only)))
         else:
             _check(self.lib.sphg_ctx_kernel_timing(self.h, 1 if enable else 0))
+
+    def kernel_timing_stride(self, stride: int) -> None:
+        """While timing, record every `stride`-th launch of each kernel only
+        (a sample; enabling timing resets it to 1)."""
+        _check(self.lib.sphg_ctx_kernel_timing_stride(self.h, int(stride)))
 
     def kernel_time(self, name: str):
         n = C.c_uint64()

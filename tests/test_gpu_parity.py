@@ -915,8 +915,8 @@ def test_warm_start_eps_branch_linearity_and_clamp_on_device(ctx):
 def test_kernel_timing_counts_and_selection(ctx):
     """Per-kernel event timing counts every launch of each kind (one build,
     one dt per DFSPH step; at least one density error sweep more than
-    velocity sweeps), records only the kinds named when asked to, and
-    changes no result."""
+    velocity sweeps), records only the kinds named when asked to, records
+    every n-th launch when given a stride, and changes no result."""
     spec = configs.small_scene(sphg.DFSPH, n_side=10, seed=7)
     a = sphg.Scene(ctx, spec.params, spec.fluid, spec.boundary)
     b = sphg.Scene(ctx, spec.params, spec.fluid, spec.boundary)
@@ -934,6 +934,18 @@ def test_kernel_timing_counts_and_selection(ctx):
         with pytest.raises(sphg.SphgError):
             ctx.kernel_timing(True, only=["no_such_kernel"])
         assert np.array_equal(a.get(sphg.FIELD_X), b.get(sphg.FIELD_X))
+        # a sample: every 3rd launch of each kind, from the first
+        c = sphg.Scene(ctx, spec.params, spec.fluid, spec.boundary)
+        ctx.kernel_timing(True)
+        ctx.kernel_timing_stride(3)
+        c.step(6)
+        ctx.kernel_timing(False)
+        assert ctx.kernel_time("build")[0] == 2 and ctx.kernel_time("dt")[0] == 2
+        assert ctx.kernel_time("df_err_density")[0] == (ne + 2) // 3
+        assert np.array_equal(a.get(sphg.FIELD_X), c.get(sphg.FIELD_X))
+        c.close()
+        with pytest.raises(sphg.SphgError):
+            ctx.kernel_timing_stride(0)
     finally:
         ctx.kernel_timing(False)
         a.close()
