@@ -15,9 +15,20 @@
 #include "sphg_internal.h"
 #include "sphg_kernels.h"
 
+#include <type_traits>
+
 namespace sphg {
 
 // ------------------------------------------------------------ utilities
+
+// f(integral_constant<int, I>) for I = 0 .. N-1, unrolled at compile time
+template <int N, int I = 0, class F>
+__device__ inline void static_for(F&& f) {
+    if constexpr (I < N) {
+        f(std::integral_constant<int, I>{});
+        static_for<N, I + 1>(f);
+    }
+}
 
 __device__ inline double wave_sum(double v) {
 #pragma unroll
@@ -130,9 +141,12 @@ template <class T> __device__ inline void stg(T* p, uint32_t i, T v) {
     *(gptr<T>)((gptr<char>)p + (size_t)(i * (uint32_t)sizeof(T))) = v;
 }
 
-// 1/sqrt(r2), zero for r2 <= 1e-30 (coincident points contribute no gradient)
+// 1/sqrt(r2), zero for r2 <= 1e-30 (coincident points contribute no gradient).
+// The seed is the hardware FP32 reciprocal square root: every r2 it is kept
+// for is a normal float (1e-30 > 2^-126), where rsqrtf is that same
+// instruction without its subnormal rescaling.
 __device__ inline double rsqrt_d(double r2) {
-    const double y = (double)rsqrtf((float)r2);
+    const double y = (double)__builtin_amdgcn_rsqf((float)r2);
     const double z = y * (1.5 - 0.5 * r2 * y * y);
     return (r2 > 1e-30) ? z : 0.0;
 }
@@ -677,13 +691,17 @@ __global__ __launch_bounds__(kBlock) void k_bvolume_pack(const DevScene* __restr
 #define SPHG_BUILD_STAGE 1
 #endif
 constexpr int kWaveStage = SPHG_BUILD_STAGE ? 816 : 1;   // candidates per wavefront: 4 x 9.8 KB per block
+// Pass 1 reads a staged row 4 candidates at a time from one LDS address (the
+// 4 reads take immediate offsets); the last pass of a lane's row may read up
+// to 3 entries past the staged ones, so the stage has 3 more.
+constexpr int kWaveStagePad = kWaveStage + 3;
 
 template <int M>
 __global__ __launch_bounds__(kBlock) void k_build(const DevScene* __restrict__ T, const Chunk* __restrict__ C) {
     const Chunk c = C[blockIdx.x];
     const DevScene& S = T[c.scene];
-    __shared__ float2 sxy[kBlock / 64][kWaveStage];
-    __shared__ float sz[kBlock / 64][kWaveStage];
+    __shared__ float2 sxy[kBlock / 64][kWaveStagePad];
+    __shared__ float sz[kBlock / 64][kWaveStagePad];
     const int i = c.begin + threadIdx.x;
     const bool valid = i < S.n;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -753,10 +771,27 @@ __global__ __launch_bounds__(kBlock) void k_build(const DevScene* __restrict__ T
 
     uint32_t* L = list_of_w(S, valid ? i : (i & ~63));
     const int maxN = S.maxNbr;
-    // the same slot as uniform base + 32-bit element offset (lists of a
-    // scene are far below 2^32 elements)
-    uint32_t* const Lg = S.nbr;
-    const uint32_t lbase = lane_base(S, i);
+    // Pass 1 stores at byte offsets from the wavefront's group list (a
+    // uniform base, so each store takes the base + 32-bit offset form; a
+    // group's list is maxN KB).  The cursor is the offset of the lane's
+    // next slot with the lane field (bits 4..9) held at all ones: + 4 then
+    // carries from the last slot of a 4-slot block straight into the next
+    // block (bits 10 up), and the store address is the cursor with the lane
+    // field put back.  Its bits 2..3 and 10 up count the survivors.
+    char* const Lw = (char*)(S.nbr + (size_t)__builtin_amdgcn_readfirstlane(i >> 6) * (size_t)maxN * kGroup);
+    constexpr uint32_t kLaneBits = 0x3F0u;
+    const uint32_t lane16 = (uint32_t)lane << 4;
+    uint32_t cur = kLaneBits;
+    const uint32_t curMax = kLaneBits | (slot_of(maxN - 1) << 2);
+    // past the last slot the particle takes the one-pass walk, which
+    // rewrites its list: the store is clamped to the last slot
+    auto put = [&](uint32_t j) {
+        const uint32_t a = (min(cur, curMax) & ~kLaneBits) | lane16;
+        cur = (cur + 4u) | kLaneBits;
+        *(gptr<uint32_t>)((gptr<char>)Lw + a) = j;
+    };
+    auto putSlot = [&](int s, uint32_t j) { *(gptr<uint32_t>)((gptr<char>)Lw + ((slot_of(s) << 2) | lane16)) = j; };
+    auto curCount = [&]() { return (int)(((cur >> 10) << 2) | ((cur >> 2) & 3u)); };
     // PCISPH: the shell, candidates within h + skin that are not in the
     // list, fills the slots from the last one down; it stands in for the
     // 27-cell walk of the predicted densities while it fits beside the list
@@ -832,25 +867,60 @@ __global__ __launch_bounds__(kBlock) void k_build(const DevScene* __restrict__ T
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
                 if (ok[u]) {
-                    // past the last slot the particle takes the one-pass
-                    // walk, which rewrites its list: the slot is clamped
-                    stg(Lg, slot_of(min(nc, maxN - 1)) + lbase, (uint32_t)(j0 + u));
-                    ++nc;
+                    put((uint32_t)(j0 + u));
+                    if (M == PCISPH) ++nc;
                 }
                 if (M == PCISPH && sh[u]) {
                     // below the list's slots only (list writes that later
                     // reach the shell are caught by nc + ns > maxN)
-                    if (nc + ns < maxN) stg(Lg, slot_of(maxN - 1 - ns) + lbase, (uint32_t)(j0 + u));
+                    if (nc + ns < maxN) putSlot(maxN - 1 - ns, (uint32_t)(j0 + u));
                     else shellLost = true;
                     ++ns;
                 }
             }
         }
     };
-    auto ldsFetch = [&](int j, int r) {
-        const int t = j + (base[r] - first[r]);
-        const float2 a = sxy[w][t];
-        return make_float3(a.x, a.y, sz[w][t]);
+    // pass 1 over a staged row: candidate j sits at stage entry
+    // j + base[r] - first[r], and one LDS address per row advances by 4
+    // entries a pass, so the candidates need no per-read index arithmetic.
+    // Entries past the lane's row end (at most 3, inside the padded stage)
+    // are read and fail the u < e0 - j0 test.  Particle i itself can only
+    // lie in row 4 (cells ci-1..ci+1): the other rows are offset by at
+    // least nx >= 3 cells, so only that row tests j != i.
+    auto scanRowLds = [&](auto rc, int s0, int e0, int& nc) {
+        constexpr int r = decltype(rc)::value;
+        const int t0 = s0 + (base[r] - first[r]);
+        const float2* pxy = &sxy[w][t0];
+        const float* pz = &sz[w][t0];
+        for (int j0 = s0; j0 < e0; j0 += 4, pxy += 4, pz += 4) {
+            float d2[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const float2 a = pxy[u];
+                const float fx = xi.x - a.x, fy = xi.y - a.y, fz = xi.z - pz[u];
+                d2[u] = fx * fx + fy * fy + fz * fz;
+            }
+            const int rem = e0 - j0;
+            bool ok[4], sh[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const bool in = (u == 0 || u < rem) & (r != 4 || j0 + u != i);
+                ok[u] = in & (d2[u] <= hh);
+                sh[u] = (M == PCISPH) & in & (d2[u] > hh) & (d2[u] <= hsk);
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                if (ok[u]) {
+                    put((uint32_t)(j0 + u));
+                    if (M == PCISPH) ++nc;
+                }
+                if (M == PCISPH && sh[u]) {
+                    if (nc + ns < maxN) putSlot(maxN - 1 - ns, (uint32_t)(j0 + u));
+                    else shellLost = true;
+                    ++ns;
+                }
+            }
+        }
     };
     auto gblFetch = [&](int j, int) {
         const float4 p = ld4(S.x, (size_t)j);
@@ -866,11 +936,12 @@ __global__ __launch_bounds__(kBlock) void k_build(const DevScene* __restrict__ T
             rs[r] = ldg(S.cellStart, (size_t)(ci - 1 + off));
             re[r] = ldg(S.cellStart, (size_t)(ci + 2 + off));
         }
-#pragma unroll
-        for (int r = 0; r < 9; ++r) {
-            if (r < nStaged) scanRow(r, rs[r], re[r], nc, ldsFetch);
+        static_for<9>([&](auto rc) {
+            constexpr int r = decltype(rc)::value;
+            if (r < nStaged) scanRowLds(rc, rs[r], re[r], nc);
             else scanRow(r, rs[r], re[r], nc, gblFetch);
-        }
+        });
+        nc = curCount();
         if (M == PCISPH && nc + ns > maxN) shellLost = true;
         if (nc <= maxN) {
             // 4 survivors per pass, their list entries and positions loaded
@@ -967,7 +1038,7 @@ __global__ __launch_bounds__(kBlock) void k_build(const DevScene* __restrict__ T
                 for (int u = 0; u < 4; ++u) {
                     const float fx = xi.x - y[u].x, fy = xi.y - y[u].y, fz = xi.z - y[u].z;
                     if ((b0 + u < be[r]) & (fx * fx + fy * fy + fz * fz <= hh)) {
-                        if (nbc < room) stg(Lg, slot_of(own + nbc) + lbase, (uint32_t)(b0 + u));
+                        if (nbc < room) putSlot(own + nbc, (uint32_t)(b0 + u));
                         ++nbc;
                     }
                 }
