@@ -30,6 +30,49 @@ __device__ inline void static_for(F&& f) {
     }
 }
 
+// Global-memory access.  The arrays come out of the scene table, so the
+// compiler cannot prove their address space; these casts make it emit
+// global (not flat) loads and stores.
+typedef float v4f __attribute__((ext_vector_type(4)));
+template <class T> using gptr = __attribute__((address_space(1))) T*;
+__device__ inline float4 ld4(const float4* p, size_t i) {
+    const v4f t = ((gptr<const v4f>)p)[i];
+    return make_float4(t.x, t.y, t.z, t.w);
+}
+__device__ inline void st4(float4* p, size_t i, float4 v) {
+    v4f t;
+    t.x = v.x; t.y = v.y; t.z = v.z; t.w = v.w;
+    ((gptr<v4f>)p)[i] = t;
+}
+template <class T> __device__ inline T ldg(const T* p, size_t i) { return ((gptr<const T>)p)[i]; }
+// p as a global-address-space pointer, for p[i] and p->field of scalar types
+template <class T> __device__ inline gptr<T> G(T* p) { return (gptr<T>)p; }
+// Element j of a float4 array of fewer than 2^28 elements: the 32-bit byte
+// offset lets the load take the uniform-base + 32-bit-offset address form
+// (no 64-bit address arithmetic per element).
+__device__ inline float4 ld4o(const float4* p, uint32_t j) {
+    const v4f t = *(gptr<const v4f>)((gptr<const char>)p + (j << 4));
+    return make_float4(t.x, t.y, t.z, t.w);
+}
+typedef float v2f __attribute__((ext_vector_type(2)));
+__device__ inline float2 ld2o(const float2* p, uint32_t j) {
+    const v2f t = *(gptr<const v2f>)((gptr<const char>)p + (j << 3));
+    return make_float2(t.x, t.y);
+}
+// The xyz of element j of a float4 array: a 12-byte load.  (Measured on
+// gfx950: slightly faster for the list build's survivors, slower than the
+// 16-byte load for the solver sweeps' gathers.)
+typedef float v3f __attribute__((ext_vector_type(3)));
+__device__ inline float4 ld3o(const float4* p, uint32_t j) {
+    const v3f t = *(gptr<const v3f>)((gptr<const char>)p + (j << 4));
+    return make_float4(t.x, t.y, t.z, 0.0f);
+}
+template <class T> __device__ inline void stg(T* p, size_t i, T v) { ((gptr<T>)p)[i] = v; }
+template <class T> __device__ inline void stg(T* p, uint32_t i, T v) {
+    *(gptr<T>)((gptr<char>)p + (size_t)(i * (uint32_t)sizeof(T))) = v;
+}
+
+
 __device__ inline double wave_sum(double v) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
@@ -67,15 +110,16 @@ __device__ inline double block_max(double v, double* sh) {
 
 // Sum (or max) of one scene's chunk partials in chunk order.
 // Each thread combines partials t, t + kBlock, t + 2 kBlock, ... in that order; the
-// loads of 8 of them are issued before the first is used, so the loop waits
-// on memory once per 8 partials instead of once per partial.
-constexpr int kPartU = 8;
+// loads of 4096 / kBlock of them are issued before the first is used, so
+// the loop waits on memory once for up to 4096 partials (a 1M-particle
+// scene's chunks) instead of once per partial.
+constexpr int kPartU = 4096 / kBlock;
 __device__ inline double scene_partial_sum(const double* partial, int n, double* sh) {
     double acc = 0.0;
     for (int b = threadIdx.x; b < n; b += kBlock * kPartU) {
         double v[kPartU];
 #pragma unroll
-        for (int u = 0; u < kPartU; ++u) v[u] = (b + u * kBlock < n) ? partial[b + u * kBlock] : 0.0;
+        for (int u = 0; u < kPartU; ++u) v[u] = (b + u * kBlock < n) ? ldg(partial, (size_t)(b + u * kBlock)) : 0.0;
 #pragma unroll
         for (int u = 0; u < kPartU; ++u) acc += v[u];
     }
@@ -86,7 +130,7 @@ __device__ inline double scene_partial_max(const double* partial, int n, double*
     for (int b = threadIdx.x; b < n; b += kBlock * kPartU) {
         double v[kPartU];
 #pragma unroll
-        for (int u = 0; u < kPartU; ++u) v[u] = (b + u * kBlock < n) ? partial[b + u * kBlock] : -1e300;
+        for (int u = 0; u < kPartU; ++u) v[u] = (b + u * kBlock < n) ? ldg(partial, (size_t)(b + u * kBlock)) : -1e300;
 #pragma unroll
         for (int u = 0; u < kPartU; ++u) acc = fmax(acc, v[u]);
     }
@@ -100,46 +144,6 @@ __device__ inline double scene_partial_max(const double* partial, int n, double*
 // lose the small differences the solvers run on (rho* - rho0 of a density
 // near 1000).  1/|r| is an FP32 rsqrt refined by one FP64 Newton step
 // (relative error ~1e-14), so no FP64 transcendental is issued.
-
-// Global-memory access.  The arrays come out of the scene table, so the
-// compiler cannot prove their address space; these casts make it emit
-// global (not flat) loads and stores.
-typedef float v4f __attribute__((ext_vector_type(4)));
-template <class T> using gptr = __attribute__((address_space(1))) T*;
-__device__ inline float4 ld4(const float4* p, size_t i) {
-    const v4f t = ((gptr<const v4f>)p)[i];
-    return make_float4(t.x, t.y, t.z, t.w);
-}
-__device__ inline void st4(float4* p, size_t i, float4 v) {
-    v4f t;
-    t.x = v.x; t.y = v.y; t.z = v.z; t.w = v.w;
-    ((gptr<v4f>)p)[i] = t;
-}
-template <class T> __device__ inline T ldg(const T* p, size_t i) { return ((gptr<const T>)p)[i]; }
-// Element j of a float4 array of fewer than 2^28 elements: the 32-bit byte
-// offset lets the load take the uniform-base + 32-bit-offset address form
-// (no 64-bit address arithmetic per element).
-__device__ inline float4 ld4o(const float4* p, uint32_t j) {
-    const v4f t = *(gptr<const v4f>)((gptr<const char>)p + (j << 4));
-    return make_float4(t.x, t.y, t.z, t.w);
-}
-typedef float v2f __attribute__((ext_vector_type(2)));
-__device__ inline float2 ld2o(const float2* p, uint32_t j) {
-    const v2f t = *(gptr<const v2f>)((gptr<const char>)p + (j << 3));
-    return make_float2(t.x, t.y);
-}
-// The xyz of element j of a float4 array: a 12-byte load.  (Measured on
-// gfx950: slightly faster for the list build's survivors, slower than the
-// 16-byte load for the solver sweeps' gathers.)
-typedef float v3f __attribute__((ext_vector_type(3)));
-__device__ inline float4 ld3o(const float4* p, uint32_t j) {
-    const v3f t = *(gptr<const v3f>)((gptr<const char>)p + (j << 4));
-    return make_float4(t.x, t.y, t.z, 0.0f);
-}
-template <class T> __device__ inline void stg(T* p, size_t i, T v) { ((gptr<T>)p)[i] = v; }
-template <class T> __device__ inline void stg(T* p, uint32_t i, T v) {
-    *(gptr<T>)((gptr<char>)p + (size_t)(i * (uint32_t)sizeof(T))) = v;
-}
 
 // 1/sqrt(r2), zero for r2 <= 1e-30 (coincident points contribute no gradient).
 // The seed is the hardware FP32 reciprocal square root: every r2 it is kept
@@ -480,7 +484,8 @@ __device__ inline int wave_count_add(int* counter, int key) {
     const uint64_t later = heads & ~upto;
     const int end = later ? __ffsll((long long)later) - 1 : 64;             // one past the run
     int base = 0;
-    if (head && key >= 0) base = atomicAdd(&counter[key], end - lane);
+    if (head && key >= 0)
+        base = __hip_atomic_fetch_add(G(counter) + key, end - lane, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     base = __shfl(base, first);
     return base + (lane - first);
 }
@@ -490,11 +495,11 @@ __global__ __launch_bounds__(kBlock) void k_keys(const DevScene* __restrict__ T,
     const DevScene& S = T[c.scene];
     const int i = c.begin + threadIdx.x;
     // the lists about to be built start without an overflow
-    if (c.begin == 0 && threadIdx.x == 0) S.ss->listOver = 0u;
+    if (c.begin == 0 && threadIdx.x == 0) G(S.ss)->listOver = 0u;
     int cl = -1;
     if (i < S.n) {
-        cl = cell_of(S, S.x[i]);
-        S.cellU[i] = cl;
+        cl = cell_of(S, ld4(S.x, (size_t)i));
+        G(S.cellU)[i] = cl;
     }
     (void)wave_count_add(S.cellCount, cl);
 }
@@ -509,10 +514,10 @@ __global__ __launch_bounds__(kBlock) void k_scan_sum(const DevScene* __restrict_
 #pragma unroll
     for (int u = 0; u < kCellPerThread; ++u) {
         const int cc = c0 + u;
-        if (cc < S.ncells) acc += S.cellCount[cc];
+        if (cc < S.ncells) acc += ldg(S.cellCount, (size_t)cc);
     }
     const double tot = block_sum((double)acc, sh);
-    if (threadIdx.x == 0) S.cellChunkSum[c.begin / kCellChunk] = (int)tot;
+    if (threadIdx.x == 0) G(S.cellChunkSum)[c.begin / kCellChunk] = (int)tot;
 }
 
 __device__ inline int wave_incl_scan(int v) {
@@ -538,14 +543,14 @@ __global__ __launch_bounds__(kBlock) void k_scan_offsets(const DevScene* __restr
     const int n = S.nCellChunks;
     for (int b = 0; b < n; b += kBlock) {
         const int t = b + threadIdx.x;
-        const int v = (t < n) ? S.cellChunkSum[t] : 0;
+        const int v = (t < n) ? ldg(S.cellChunkSum, (size_t)t) : 0;
         const int incl = wave_incl_scan(v);
         const int w = threadIdx.x >> 6;
         if ((threadIdx.x & 63) == 63) wsum[w] = incl;
         __syncthreads();
         int off = carry;
         for (int q = 0; q < w; ++q) off += wsum[q];
-        if (t < n) S.cellChunkSum[t] = off + incl - v;
+        if (t < n) G(S.cellChunkSum)[t] = off + incl - v;
         __syncthreads();
         if (threadIdx.x == kBlock - 1) carry = off + incl;
         __syncthreads();
@@ -563,7 +568,7 @@ __global__ __launch_bounds__(kBlock) void k_scan_final(const DevScene* __restric
 #pragma unroll
     for (int u = 0; u < kCellPerThread; ++u) {
         const int cc = c0 + u;
-        vals[u] = (cc < S.ncells) ? S.cellCount[cc] : 0;
+        vals[u] = (cc < S.ncells) ? ldg(S.cellCount, (size_t)cc) : 0;
         local += vals[u];
     }
     const int incl = wave_incl_scan(local);
@@ -572,18 +577,18 @@ __global__ __launch_bounds__(kBlock) void k_scan_final(const DevScene* __restric
     __syncthreads();
     int woff = 0;
     for (int q = 0; q < w; ++q) woff += wsum[q];
-    int run = S.cellChunkSum[c.begin / kCellChunk] + woff + incl - local;
+    int run = ldg(S.cellChunkSum, (size_t)(c.begin / kCellChunk)) + woff + incl - local;
 #pragma unroll
     for (int u = 0; u < kCellPerThread; ++u) {
         const int cc = c0 + u;
         if (cc < S.ncells) {
-            S.cellStart[cc] = run;
-            S.cellCount[cc] = 0;
+            G(S.cellStart)[cc] = run;
+            G(S.cellCount)[cc] = 0;
         }
         run += vals[u];
     }
     // the chunk table covers ncells + 1 entries: the last one is the end
-    if (c0 <= S.ncells && S.ncells < c0 + kCellPerThread) S.cellStart[S.ncells] = S.n;
+    if (c0 <= S.ncells && S.ncells < c0 + kCellPerThread) G(S.cellStart)[S.ncells] = S.n;
 }
 
 // Scatter the particle ids into their cells' slots (arrival order).
@@ -591,9 +596,9 @@ __global__ __launch_bounds__(kBlock) void k_scatter(const DevScene* __restrict__
     const Chunk c = C[blockIdx.x];
     const DevScene& S = T[c.scene];
     const int i = c.begin + threadIdx.x;
-    const int cl = (i < S.n) ? S.cellU[i] : -1;
+    const int cl = (i < S.n) ? ldg(S.cellU, (size_t)i) : -1;
     const int r = wave_count_add(S.cellCount, cl);
-    if (i < S.n) S.perm[S.cellStart[cl] + r] = S.id[i];
+    if (i < S.n) G(S.perm)[ldg(S.cellStart, (size_t)cl) + r] = ldg(S.id, (size_t)i);
 }
 
 // Final slot of each particle = cell start + number of particles of its
@@ -607,29 +612,37 @@ __global__ __launch_bounds__(kBlock) void k_rank_reorder(const DevScene* __restr
     const DevScene& S = T[c.scene];
     const int i = c.begin + threadIdx.x;
     if (i >= S.n) return;
-    const int cl = S.cellU[i];
-    const int s = S.cellStart[cl], e = S.cellStart[cl + 1];
-    const uint32_t me = S.id[i];
+    const int cl = ldg(S.cellU, (size_t)i);
+    const int s = ldg(S.cellStart, (size_t)cl), e = ldg(S.cellStart, (size_t)cl + 1);
+    const uint32_t me = ldg(S.id, (size_t)i);
     // the particle's data is loaded before the rank loop and before any
     // store (a wait for a load also waits for the stores issued before it)
-    const float4 xo = S.x[i], vo = S.v[i];
+    const float4 xo = ld4(S.x, (size_t)i), vo = ld4(S.v, (size_t)i);
     const bool df = S.method == DFSPH;
-    const float ko = df ? S.k[i] : 0.f, kvo = df ? S.kv[i] : 0.f;
-    const float4 no = (df && S.pbOn) ? S.pbn[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const float ko = df ? ldg(S.k, (size_t)i) : 0.f, kvo = df ? ldg(S.kv, (size_t)i) : 0.f;
+    const float4 no = (df && S.pbOn) ? ld4(S.pbn, (size_t)i) : make_float4(0.f, 0.f, 0.f, 0.f);
+    // rank within the cell: the ids of its first 8 members are loaded
+    // together (a fluid cell holds ~8; the cell holds this particle, so
+    // e > s and the clamped index stays in it), the rest one at a time
+    uint32_t q8[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) q8[u] = ldg(S.perm, (size_t)min(s + u, e - 1));
     int rank = 0;
-    for (int q = s; q < e; ++q) rank += (S.perm[q] < me) ? 1 : 0;
+#pragma unroll
+    for (int u = 0; u < 8; ++u) rank += ((s + u < e) & (q8[u] < me)) ? 1 : 0;
+    for (int q = s + 8; q < e; ++q) rank += (ldg(S.perm, (size_t)q) < me) ? 1 : 0;
     const int d = s + rank;
-    S.x2[d] = xo;
-    S.v2[d] = vo;
-    S.id2[d] = me;
-    S.cell[d] = cl;
+    st4(S.x2, (size_t)d, xo);
+    st4(S.v2, (size_t)d, vo);
+    stg(S.id2, (size_t)d, me);
+    stg(S.cell, (size_t)d, cl);
     if (df) {
-        S.k2[d] = ko;
-        S.kv2[d] = kvo;
-        if (S.pbOn) S.pbn2[d] = no;
+        stg(S.k2, (size_t)d, ko);
+        stg(S.kv2, (size_t)d, kvo);
+        if (S.pbOn) st4(S.pbn2, (size_t)d, no);
     }
-    if (S.pbOn) S.everOcc[cl] = 1;
-    S.cellCount[cl] = 0;
+    if (S.pbOn) stg(S.everOcc, (size_t)cl, (uint8_t)1);
+    stg(S.cellCount, (size_t)cl, 0);
 }
 
 // =========================================================== boundary
@@ -1366,16 +1379,16 @@ __global__ __launch_bounds__(kBlock) void k_step_begin(const DevScene* __restric
 __global__ __launch_bounds__(kBlock) void k_dt(const DevScene* __restrict__ T, const Chunk* __restrict__ C) {
     const Chunk c = C[blockIdx.x];
     const DevScene& S = T[c.scene];
-    const Scalars* ss = S.ss;
+    const gptr<const Scalars> ss = G((const Scalars*)S.ss);
     __shared__ double sh[kWaves];
     const int i = c.begin + threadIdx.x;
     const double dt = ss->dt;
     double val = 0.0;
     if (i < S.n) {
-        const float4 v = S.v[i];
+        const float4 v = ld4(S.v, (size_t)i);
         double ax = S.gx, ay = S.gy, az = S.gz;
         if (S.flags & F_FORCES) {
-            const float4 a = S.a[i];
+            const float4 a = ld4(S.a, (size_t)i);
             ax = (double)a.x + ax;
             ay = (double)a.y + ay;
             az = (double)a.z + az;
@@ -1384,7 +1397,7 @@ __global__ __launch_bounds__(kBlock) void k_dt(const DevScene* __restrict__ T, c
         val = sqrt(sqrt(cx * cx + cy * cy + cz * cz));
     }
     const double bm = block_max(val, sh);
-    if (threadIdx.x == 0) S.partial[c.begin / kBlock] = bm;
+    if (threadIdx.x == 0) G(S.partial)[c.begin / kBlock] = bm;
 }
 
 // The density solve (predict, warm start, density iterations) keeps the
@@ -1411,11 +1424,11 @@ __global__ __launch_bounds__(kBlock) void k_df_predict(const DevScene* __restric
     const DevScene& S = T[c.scene];
     const int i = c.begin + threadIdx.x;
     if (i >= S.n) return;
-    const double dt = S.ss->dt;
-    const float4 v = S.v[i];
+    const double dt = G(S.ss)->dt;
+    const float4 v = ld4(S.v, (size_t)i);
     double ax = S.gx, ay = S.gy, az = S.gz;
     if (S.flags & F_FORCES) {
-        const float4 a = S.a[i];
+        const float4 a = ld4(S.a, (size_t)i);
         ax = (double)a.x + ax;
         ay = (double)a.y + ay;
         az = (double)a.z + az;
@@ -1424,7 +1437,7 @@ __global__ __launch_bounds__(kBlock) void k_df_predict(const DevScene* __restric
     if (S.pbOn) {
         // correctVelocities with the normals of the last step's projection
         // (src/DFSPHSolver.cpp:51-58)
-        const float4 n = S.pbn[i];
+        const float4 n = ld4(S.pbn, (size_t)i);
         double wx = (double)v.x + ux, wy = (double)v.y + uy, wz = (double)v.z + uz;
         if (pb_velocity(S, n.x, n.y, n.z, wx, wy, wz)) {
             ux = wx - (double)v.x;
@@ -1432,10 +1445,10 @@ __global__ __launch_bounds__(kBlock) void k_df_predict(const DevScene* __restric
             uz = wz - (double)v.z;
         }
     }
-    const double rho = S.rho0D + (double)S.drho[i];
-    const double kw = -fmax(-(double)S.k[i] / rho, -1.0e-9) / (dt * dt);
-    const float4 x = S.x[i];
-    S.xk[i] = make_float4(x.x, x.y, x.z, (float)kw);
+    const double rho = S.rho0D + (double)ldg(S.drho, (size_t)i);
+    const double kw = -fmax(-(double)ldg(S.k, (size_t)i) / rho, -1.0e-9) / (dt * dt);
+    const float4 x = ld4(S.x, (size_t)i);
+    st4(S.xk, (size_t)i, make_float4(x.x, x.y, x.z, (float)kw));
     store_u(S, i, ux, uy, uz);
 }
 
@@ -1622,25 +1635,25 @@ __global__ __launch_bounds__(kBlock) void k_df_positions(const DevScene* __restr
     const DevScene& S = T[c.scene];
     if (blockIdx.x == 0 && threadIdx.x < 2) active[threadIdx.x] = 0;
     // the lists about to be built start without an overflow
-    if (c.begin == 0 && threadIdx.x == 0) S.ss->listOver = 0u;
+    if (c.begin == 0 && threadIdx.x == 0) G(S.ss)->listOver = 0u;
     const int i = c.begin + threadIdx.x;
     int cl = -1;
     if (i < S.n) {
-        const double dt = S.ss->dt;
-        float4 x = S.x[i];
+        const double dt = G(S.ss)->dt;
+        float4 x = ld4(S.x, (size_t)i);
         // the density solve's velocity: v0 + u (see store_u)
-        const float4 v0 = S.v[i];
-        const float2 uyz = S.vyz[i];
+        const float4 v0 = ld4(S.v, (size_t)i);
+        const float2 uyz = ld2o(S.vyz, (uint32_t)i);
         const float4 v = make_float4((float)((double)v0.x + (double)x.w), (float)((double)v0.y + (double)uyz.x),
                                      (float)((double)v0.z + (double)uyz.y), 0.f);
-        S.v[i] = v;
+        st4(S.v, (size_t)i, v);
         x.x = (float)((double)x.x + (double)v.x * dt);
         x.y = (float)((double)x.y + (double)v.y * dt);
         x.z = (float)((double)x.z + (double)v.z * dt);
-        S.x[i] = x;
+        st4(S.x, (size_t)i, x);
         if (!S.pbOn) {
             cl = cell_of(S, x);
-            S.cellU[i] = cl;
+            G(S.cellU)[i] = cl;
         }
     }
     // with the PCISPH boundary method the projection moves the particle
@@ -1843,7 +1856,7 @@ __global__ __launch_bounds__(kBlock) void k_finalize(const DevScene* __restrict_
     const int sc = blockIdx.x;
     if (sc >= nscenes) return;
     const DevScene& S = T[sc];
-    Scalars* ss = S.ss;
+    const gptr<Scalars> ss = G(S.ss);
     if ((KIND == FIN_DENS || KIND == FIN_PCI) && ss->done) return;
     if (KIND == FIN_DIV && ss->doneV) return;
     __shared__ double sh[kWaves];
