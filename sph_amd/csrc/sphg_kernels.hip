@@ -840,7 +840,10 @@ __global__ __launch_bounds__(kBlock) void k_build(const DevScene* __restrict__ T
         }
         const double g = G_d(p, k);
         bool gnz = (g != 0.0);
-        if (p.r2 >= k.hlo2 || fabs(p.q - 0.6666666666666666) < 1e-9 || p.r2 < 1e-9) {
+        // the reference's gradient (src/CubicSpline.h:55-73) vanishes only at
+        // |r| <= 1e-5 and q = 1; near those two, and near q = 1 for the list
+        // membership, the test is repeated operation-exact in FP64
+        if (p.r2 >= k.hlo2 || p.r2 < 1e-9) {
             const Exact e = exact_q(xi, xj, S.hD);
             if (!(e.q <= 1.0)) {
                 toShell(j);
