@@ -170,21 +170,25 @@ __device__ inline Pair pair_of(double xi, double yi, double zi, float4 b, const 
     p.q = p.rl * k.invh;
     return p;
 }
-// Cubic spline W (src/CubicSpline.h:29-48).
-__device__ inline double W_d(double q, const KCoefD& k) {
+// Cubic spline W (src/CubicSpline.h:29-48); W_in for q <= 1.
+__device__ inline double W_in(double q, const KCoefD& k) {
     const double om = 1.0 - q;
     const double w1 = k.ck * ((6.0 * q - 6.0) * q * q + 1.0);
     const double w2 = 2.0 * k.ck * om * om * om;
-    return sel(q <= 1.0, sel(q <= 0.5, w1, w2), 0.0);
+    return sel(q <= 0.5, w1, w2);
 }
+__device__ inline double W_d(double q, const KCoefD& k) { return sel(q <= 1.0, W_in(q, k), 0.0); }
 // Cubic spline gradient factor: gradW(r) = g r, zero for |r| <= 1e-5 or
 // q > 1 (src/CubicSpline.h:55-73).  For q <= 1/2 the reference's
-// l q (3q-2) / (|r| h) equals l (3q-2) / h^2.
-__device__ inline double G_d(const Pair& p, const KCoefD& k) {
+// l q (3q-2) / (|r| h) equals l (3q-2) / h^2.  G_in for |r| > 1e-5, q <= 1.
+__device__ inline double G_in(const Pair& p, const KCoefD& k) {
     const double om = 1.0 - p.q;
     const double g1 = k.clh2 * (3.0 * p.q - 2.0);
     const double g2 = -k.clh * om * om * p.inv;
-    return sel(p.rl > 1e-5 && p.q <= 1.0, sel(p.q <= 0.5, g1, g2), 0.0);
+    return sel(p.q <= 0.5, g1, g2);
+}
+__device__ inline double G_d(const Pair& p, const KCoefD& k) {
+    return sel(p.rl > 1e-5 && p.q <= 1.0, G_in(p, k), 0.0);
 }
 // Cubic gradient factor of a neighbor-list member for the solver sweeps.
 // List members have q <= 1 by construction and the padding entries are the
@@ -838,11 +842,13 @@ __global__ __launch_bounds__(kBlock) void k_build(const DevScene* __restrict__ T
             toShell(j);
             return;
         }
-        const double g = G_d(p, k);
-        bool gnz = (g != 0.0);
-        // the reference's gradient (src/CubicSpline.h:55-73) vanishes only at
+        // 1e-9 <= r2 < hlo2: |r| > 1e-5 and q < 1, where W and the gradient
+        // take their inner forms and the gradient is nonzero.  The
+        // reference's gradient (src/CubicSpline.h:55-73) vanishes only at
         // |r| <= 1e-5 and q = 1; near those two, and near q = 1 for the list
-        // membership, the test is repeated operation-exact in FP64
+        // membership, the test is repeated operation-exact in FP64.
+        double g = G_in(p, k), wq = W_in(p.q, k);
+        bool gnz = true;
         if (p.r2 >= k.hlo2 || p.r2 < 1e-9) {
             const Exact e = exact_q(xi, xj, S.hD);
             if (!(e.q <= 1.0)) {
@@ -850,11 +856,13 @@ __global__ __launch_bounds__(kBlock) void k_build(const DevScene* __restrict__ T
                 return;
             }
             gnz = grad_nonzero64(e);
+            g = G_d(p, k);
+            wq = W_d(p.q, k);
         }
         if (cnt < maxN && cnt != pos) stg(L, (size_t)slot_of(cnt), j);
         ++cnt;
         ngr += gnz ? 1 : 0;
-        ws += W_d(p.q, k);
+        ws += wq;
         sgx += g * p.dx;
         sgy += g * p.dy;
         sgz += g * p.dz;

@@ -324,6 +324,33 @@ def test_list_offset_limit_is_enforced(ctx):
     g.close()
 
 
+def test_lists_past_2_pow_30_entries_match_small_lists(ctx):
+    """Lists of more than 2^30 entries (4 GiB), below the 2^32 - 1 limit: a
+    22k-particle scene with max_neighbors 65532 has 5.8 GB of lists, and
+    every 64-particle group after the first 257 starts past entry 2^30,
+    where a 32-bit byte offset from the list base would wrap.  The lists hold the same entries
+    whatever max_neighbors is, so the scene steps bit-identically with the
+    default max_neighbors."""
+    from dataclasses import replace
+    spec = configs.small_scene(sphg.DFSPH, n_side=28)
+    big = 65532
+    npad = -(-spec.n // 256) * 256
+    assert (npad // 64 - 1) * big * 64 > 2 ** 30 and npad * big < 2 ** 32
+    out = []
+    for mx in (0, big):
+        g = sphg.Scene(ctx, replace(spec.params, max_neighbors=mx), spec.fluid, spec.boundary)
+        g.step(3)
+        ctx.synchronize()
+        sc = g.scalars()
+        out.append((g.get(sphg.FIELD_X), g.get(sphg.FIELD_V), sc.sum_iterations, sc.sum_iterations_v,
+                    sc.neighbor_overflow))
+        g.close()
+    (x0, v0, it0, itv0, ov0), (x1, v1, it1, itv1, ov1) = out
+    assert ov0 == 0 and ov1 == 0
+    assert (it0, itv0) == (it1, itv1)
+    assert np.array_equal(x0, x1) and np.array_equal(v0, v1)
+
+
 def test_empty_fluid_is_rejected(ctx):
     """A scene without fluid particles and without emitters is refused at
     initialisation with SPHG_ESTATE and a message, not run."""
