@@ -13,7 +13,12 @@
  * calling thread's last failure.  Arrays are plain host pointers, borrowed
  * for the duration of the call and copied before it returns; xyz arrays hold
  * n*3 floats in particle-id order.  A context is used by one host thread at
- * a time.  All device memory is owned by the library.
+ * a time, with one exception: sphg_scene_create, the set_fluid /
+ * set_boundary / set_boundary_normals / add_emitter calls and
+ * sphg_scene_plan_bytes on a scene that has not been initialised, and
+ * sphg_scene_destroy of such a scene, touch host memory only and may run on
+ * another thread while the context runs.  All device memory is owned by the
+ * library.
  */
 #ifndef SPHG_H
 #define SPHG_H

@@ -19,6 +19,7 @@
 #include <cstring>
 #include <map>
 #include <mutex>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -168,6 +169,10 @@ struct sphg_ctx {
     int* dActive = nullptr;   // [0] scenes still iterating, [1] largest iteration count reached
     int* hActive = nullptr;   // pinned copy
     Profiler prof;
+    // the context's scenes; creating and destroying a scene that no running
+    // call uses may happen on another thread than the one running the
+    // context, so the list has a lock of its own
+    std::mutex scenesMu;
     std::vector<sphg_scene*> scenes;
 };
 
@@ -1111,11 +1116,14 @@ int sphg_ctx_destroy(sphg_ctx* ctx) {
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
     if (ctx->copyStream) (void)hipStreamSynchronize(ctx->copyStream);
-    for (auto* s : ctx->scenes) {
-        s->ctx = nullptr;
-        delete s;
+    {
+        std::lock_guard<std::mutex> lk(ctx->scenesMu);
+        for (auto* s : ctx->scenes) {
+            s->ctx = nullptr;
+            delete s;
+        }
+        ctx->scenes.clear();
     }
-    ctx->scenes.clear();
     ctx->prof.collect();
     if (ctx->dActive) (void)hipFree(ctx->dActive);
     if (ctx->hActive) (void)hipHostFree(ctx->hActive);
@@ -1144,7 +1152,10 @@ int sphg_scene_create(sphg_ctx* ctx, const sphg_scene_desc* desc, sphg_scene** o
     sphg_scene* s = new sphg_scene();
     s->ctx = ctx;
     s->desc = *desc;
-    ctx->scenes.push_back(s);
+    {
+        std::lock_guard<std::mutex> lk(ctx->scenesMu);
+        ctx->scenes.push_back(s);
+    }
     *out = s;
     return SPHG_OK;
 }
@@ -1152,8 +1163,12 @@ int sphg_scene_create(sphg_ctx* ctx, const sphg_scene_desc* desc, sphg_scene** o
 int sphg_scene_destroy(sphg_scene* s) {
     if (!s) return SPHG_OK;
     if (s->ctx) {
-        (void)hipSetDevice(s->ctx->device);
-        (void)hipStreamSynchronize(s->ctx->stream);
+        // a scene that was never given device memory has no work to wait for
+        if (s->ready || s->mem) {
+            (void)hipSetDevice(s->ctx->device);
+            (void)hipStreamSynchronize(s->ctx->stream);
+        }
+        std::lock_guard<std::mutex> lk(s->ctx->scenesMu);
         auto& v = s->ctx->scenes;
         v.erase(std::remove(v.begin(), v.end(), s), v.end());
     }

@@ -152,7 +152,10 @@ def lane_of(method: int, n: int, split_n: int = 0):
 class GpuBackend:
     """The HIP solver library on one device.  With `lane_contexts` each lane
     (solver method and size class, `lane_of`) gets a context (HIP stream) of
-    its own, so that the batches of different lanes can run concurrently."""
+    its own, so that the batches of different lanes can run concurrently.
+    The scheduler then creates and plans a lane's new scenes (host-side calls
+    the library allows beside a running context) while the lane's quantum
+    runs, and initialises, batches and closes them only while it is idle."""
 
     def __init__(self, ctx: sphg.Context, lane_contexts: bool = False):
         self.ctx = ctx
@@ -284,6 +287,9 @@ def serve_batched(backend, requests: List[Request], budget_bytes: int, quantum: 
     others: when one lane's quantum ends its finished scenes are completed,
     arrivals admitted and its next quantum started while the other lanes run
     on.  Without it the lanes take turns on the calling thread."""
+    if parallel and not getattr(backend, "lane_contexts", True):
+        raise ValueError("parallel lanes need a backend with a context per lane (GpuBackend(lane_contexts=True)): "
+                         "a context is driven by one host thread at a time")
     pending = deque(sorted(requests, key=lambda q: q.arrival))
     waiting: deque = deque()
     lanes: Dict[tuple, _Lane] = {}

@@ -34,6 +34,49 @@ def test_plan_bytes_matches_allocation(ctx):
     s.close()
 
 
+def test_scenes_prepared_on_another_thread_while_the_context_runs(ctx):
+    """Creating, planning and destroying scenes that are not initialised
+    touch host memory only and may run on another thread while the context
+    steps (the admission of a serving lane does this): the stepped scene
+    ends bit-identical to the same scene stepped with nothing beside it."""
+    import threading
+    spec = configs.small_scene(sphg.DFSPH, n_side=12)
+    other = configs.small_scene(sphg.WCSPH, n_side=8)
+
+    def run():
+        g = sphg.Scene(ctx, spec.params, spec.fluid, spec.boundary)
+        g.step(300)
+        ctx.synchronize()
+        x, v = g.get(sphg.FIELD_X), g.get(sphg.FIELD_V)
+        g.close()
+        return x, v
+
+    x0, v0 = run()
+    stop = threading.Event()
+    made = []
+    errors = []
+
+    def prepare():
+        try:
+            while not stop.is_set():
+                s = sphg.Scene(ctx, other.params, other.fluid, other.boundary, init=False)
+                made.append(s.plan_bytes())
+                s.close()
+        except Exception as e:  # noqa: BLE001 - reported below
+            errors.append(e)
+
+    t = threading.Thread(target=prepare)
+    t.start()
+    try:
+        x1, v1 = run()
+    finally:
+        stop.set()
+        t.join()
+    assert not errors, errors
+    assert len(made) >= 3 and len(set(made)) == 1
+    assert np.array_equal(x0, x1) and np.array_equal(v0, v1)
+
+
 def test_served_results_same_alone_and_in_mix(ctx):
     reqs = serve.request_stream(6, rate=500.0, nmin=2048, nmax=6000, smin=12, smax=30, seed=21)
     be = serve.GpuBackend(ctx)

@@ -395,14 +395,27 @@ def test_scene_initialised_only_while_its_lane_is_idle():
             with guard:
                 running.difference_update(lanes)
 
-    init = be.init
+    init, batch, close = be.init, be.batch, be.close
 
     def init_checked(s):
         with guard:
             assert s.lane not in running, s.lane
             lanes_seen.append(s.lane)
         init(s)
-    be.init = init_checked
+
+    def batch_checked(scenes):
+        with guard:
+            assert not {s.lane for s in scenes} & running
+        return batch(scenes)
+
+    def close_checked(obj):
+        # an initialised scene or a batch is closed only while its lane is idle
+        lanes = {s.lane for s in obj.scenes} if isinstance(obj, FakeBatch) else (
+            {obj.lane} if obj.ready else set())
+        with guard:
+            assert not lanes & running, lanes
+        close(obj)
+    be.init, be.batch, be.close = init_checked, batch_checked, close_checked
     FakeBatch.run = run_slowly
     try:
         reqs = _stream(20, rate=300.0, seed=13)
@@ -413,3 +426,15 @@ def test_scene_initialised_only_while_its_lane_is_idle():
     assert all(int(j.result[0]) == j.req.steps for j in res.jobs)
     assert len(lanes_seen) == len(reqs) and len(set(lanes_seen)) > 2
     assert be.used == 0
+
+
+def test_parallel_lanes_need_a_context_per_lane():
+    """Lanes that run on threads of their own need a context each: a backend
+    that drives every lane through one context is refused."""
+    clk = Clock()
+    be = FakeBackend(clk)
+    be.lane_contexts = False
+    with pytest.raises(ValueError, match="context per lane"):
+        serve.serve_batched(be, _stream(4), 1 << 40, clock=clk, sleep=clk.sleep, parallel=True)
+    res = serve.serve_batched(be, _stream(4), 1 << 40, clock=clk, sleep=clk.sleep, parallel=False)
+    assert all(int(j.result[0]) == j.req.steps for j in res.jobs)
