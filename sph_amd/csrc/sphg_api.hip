@@ -1099,12 +1099,13 @@ int sphg_ctx_create(int32_t device, sphg_ctx** out) {
     if (e == hipSuccess) e = hipMalloc(&c->dActive, 2 * sizeof(int));
     if (e == hipSuccess) e = hipHostMalloc(&c->hActive, 2 * sizeof(int), hipHostMallocDefault);
     if (e != hipSuccess) {
-        delete c;
-        return fail(SPHG_EHIP, std::string("context set-up: ") + hipGetErrorString(e));
+        const std::string msg = std::string("context set-up: ") + hipGetErrorString(e);
+        (void)sphg_ctx_destroy(c);   // releases the streams and buffers made so far
+        return fail(SPHG_EHIP, msg);
     }
     const int rc = scratch_address(device, &c->scratch);
     if (rc) {
-        delete c;
+        (void)sphg_ctx_destroy(c);
         return rc;
     }
     *out = c;
@@ -1114,7 +1115,7 @@ int sphg_ctx_create(int32_t device, sphg_ctx** out) {
 int sphg_ctx_destroy(sphg_ctx* ctx) {
     if (!ctx) return SPHG_OK;
     (void)hipSetDevice(ctx->device);
-    (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     if (ctx->copyStream) (void)hipStreamSynchronize(ctx->copyStream);
     {
         std::lock_guard<std::mutex> lk(ctx->scenesMu);
