@@ -27,6 +27,11 @@ their time is charged to the working launches).  `roofline.traffic` is the
 HBM traffic per launch of that kernel from the rocprofv3 FETCH_SIZE /
 WRITE_SIZE summary in profiles/ (null when that summary has no figure for
 the kernel), measured over the window `roofline.traffic_window` names.
+`roofline.valu` is the same kernel's VALU ceiling: its VALU instructions per
+launch from the rocprofv3 SQ_INSTS_VALU summary in profiles/, the time they
+take to issue at MI355X's VALU rate (1024 SIMDs x 2.4 GHz, 4 cycles per
+wave64 instruction: 78.6 TFLOP/s of FP64 FMA) and that time over this run's
+working launch; `busy_measured` is SQ_ACTIVE_INST_VALU's own figure.
 
 `cpu_baseline` times the FP64 OpenMP restatement of the reference
 (oracle/liboracle_fast.so) on the host cores, one step from each of three
@@ -53,6 +58,8 @@ sys.path.insert(0, ROOT)
 TIMING_STRIDE = 4        # timed steps: one launch in 4 of the dominant kernel carries an event pair
 HBM_PEAK_GBS = 8000.0
 PMC_SUMMARY = os.path.join(ROOT, "profiles", "dfsph_1m_pmc.json")
+SQ_SUMMARY = os.path.join(ROOT, "profiles", "dfsph_1m_sq_counters.json")
+VALU_WAVE_INSTR_PER_S = 1024 * 2.4e9 / 4   # 1024 SIMDs at 2.4 GHz, a wave64 instruction every 4 cycles
 CONFIG_STEPS = 2000
 
 # Algorithmic bytes per fluid particle (FP32) of each kernel, and whether the
@@ -89,12 +96,27 @@ def balg_step(n, nb, ncells, it_d, it_v):
     return (264 + 88 * (it_d + it_v)) * n + 16 * nb * sweeps + 4 * ncells * sweeps + 12 * ncells
 
 
-def pmc_summary():
+def pmc_summary(path=PMC_SUMMARY):
     try:
-        with open(PMC_SUMMARY) as f:
+        with open(path) as f:
             return json.load(f)
     except (OSError, ValueError):
         return {}
+
+
+def valu_ceiling(kernel, avg_s):
+    """VALU issue time of one launch of `kernel` (profiles/ SQ summary) and
+    its fraction of the measured working launch `avg_s`; None without a
+    figure for the kernel."""
+    sq = pmc_summary(SQ_SUMMARY)
+    k = sq.get("kernels", {}).get(kernel)
+    if not k:
+        return None
+    instr = k["valu_per_wavefront"] * k["wavefronts"]
+    issue_s = instr / VALU_WAVE_INSTR_PER_S
+    return {"instr_per_launch": round(instr), "issue_us": round(issue_s * 1e6, 2),
+            "peak_wave_instr_per_s": VALU_WAVE_INSTR_PER_S, "frac": round(issue_s / avg_s, 4),
+            "busy_measured": k.get("valu_busy"), "window": sq.get("workload")}
 
 
 def timed_steps(scene, ctx, steps, capture=(), state_of=None):
@@ -227,7 +249,8 @@ def main():
                 # run's launch duration
                 "traffic_frac": round(traffic / avg_s / 1e9 / HBM_PEAK_GBS, 4) if traffic else None,
                 "bytes_per_launch": bytes_launch, "avg_launch_us": kernels[dom]["avg_us"],
-                "useful_launches": round(n_useful), "avg_useful_launch_us": round(avg_s * 1e6, 3)}
+                "useful_launches": round(n_useful), "avg_useful_launch_us": round(avg_s * 1e6, 3),
+                "valu": valu_ceiling(dom, avg_s)}
         log(f"kernels: {json.dumps(kernels)}")
     scene.close()
 

@@ -131,3 +131,16 @@ def test_desc_validation_rejects_unsupported():
     h = C.c_void_p()
     # a NULL context is rejected before the description is looked at
     assert lib.sphg_scene_create(None, C.byref(d), C.byref(h)) == -1
+
+
+def test_bench_valu_ceiling_from_profile():
+    """bench.py's roofline.valu: the dominant kernel's VALU instructions per
+    launch from the committed SQ counter summary, issued at 1024 SIMDs x
+    2.4 GHz / 4 cycles, as a fraction of a given launch time."""
+    import bench
+    v = bench.valu_ceiling("df_err_density", 100e-6)
+    assert v is not None and v["instr_per_launch"] > 0
+    assert abs(v["issue_us"] - v["instr_per_launch"] * 4 / (1024 * 2.4e9) * 1e6) < 0.01
+    assert abs(v["frac"] - v["issue_us"] / 100.0) < 1e-3
+    assert 0 < v["busy_measured"] < 1
+    assert bench.valu_ceiling("no_such_kernel", 1e-6) is None
