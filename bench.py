@@ -44,7 +44,9 @@ serves a 24-request Poisson stream (WCSPH/PCISPH/DFSPH, N log-uniform in
 2,048-2,000,000 with one request above 1M, 100-300 steps) with continuous
 batching and one request at a time, with p50/p99 completion latency and the
 throughput ratio the stream allows at best; `serve_burst` does the same for
-64 requests of 2,048-262,144 particles arriving at once.
+64 requests of 2,048-262,144 particles arriving at once, and `serve_loaded`
+for 100 requests of 2,048-2,000,000 particles arriving faster than the
+device can serve them.
 """
 import argparse
 import json
@@ -301,7 +303,7 @@ def main():
         log(f"cpu baseline {cpu['value']:.3g} particle-steps/s on {threads} threads")
 
     # ---------------------------------------------- batching and serving
-    batch = serve = burst = None
+    batch = serve = burst = loaded = None
     if not args.no_extras:
         from sph_amd import bench_batch, serve as srv
         bspecs = configs.batched_dfsph(64)
@@ -317,14 +319,23 @@ def main():
         serve = srv.compare(ctx, reqs, 200 * 2 ** 30, quantum=25, check=4, parallel=True, split_n=srv.SPLIT_N)
         serve["stream"] = (f"24 requests, Poisson 30/s, WCSPH/PCISPH/DFSPH uniform, N log-uniform 2,048-2,000,000 "
                            f"(largest {max(q.n_target for q in reqs)}), 100-300 steps, seed {seed}, 200 GB budget, "
-                           f"quantum 25, lanes by method and size (split at {srv.SPLIT_N}) on their own streams")
+                           f"quantum 25, lanes by method and size (split at {srv.SPLIT_N}): each small lane on a "
+                           f"stream of its own, the large lanes on one stream, the oldest request's lane first, "
+                           f"stepping its oldest scenes up to {srv.LARGE_CAP} fluid particles")
         log(f"serve: ratio {serve['throughput_ratio']}, p99 {serve['batched']['latency_p99_s']} s")
         reqs = srv.request_stream(64, 1000.0, 2048, 262144, 100, 300, 5)
         burst = srv.compare(ctx, reqs, 200 * 2 ** 30, quantum=25, check=4, parallel=True, split_n=srv.SPLIT_N)
         burst["stream"] = ("64 requests, Poisson 1000/s (a burst), WCSPH/PCISPH/DFSPH uniform, N log-uniform "
                            "2,048-262,144, 100-300 steps, seed 5, 200 GB budget, quantum 25, lanes by method and "
-                           f"size (split at {srv.SPLIT_N}) on their own streams")
+                           f"size (split at {srv.SPLIT_N}) as in `serve`")
         log(f"serve burst: ratio {burst['throughput_ratio']}, p99 {burst['batched']['latency_p99_s']} s")
+        reqs = srv.request_stream(100, 30.0, 2048, 2_000_000, 100, 300, 11)
+        loaded = srv.compare(ctx, reqs, 200 * 2 ** 30, quantum=25, check=4, parallel=True, split_n=srv.SPLIT_N)
+        loaded["stream"] = ("100 requests, Poisson 30/s (a queue builds up: ~3.3 s of arrivals for ~9 s of work), "
+                            "WCSPH/PCISPH/DFSPH uniform, N log-uniform 2,048-2,000,000, 100-300 steps, seed 11, "
+                            "200 GB budget, quantum 25, lanes as in `serve`")
+        log(f"serve loaded: ratio {loaded['throughput_ratio']}, p50 {loaded['batched']['latency_p50_s']} s, "
+            f"p99 {loaded['batched']['latency_p99_s']} s")
 
     ctx.close()
     out = {
@@ -362,6 +373,8 @@ def main():
         out["serve"] = serve
     if burst:
         out["serve_burst"] = burst
+    if loaded:
+        out["serve_loaded"] = loaded
     if kernels:
         out["kernels"] = kernels
     if warm:

@@ -164,7 +164,9 @@ struct sphg_ctx {
     int device = 0;
     unsigned char* scratch = nullptr;   // this device's address of `scratch`
     hipStream_t stream = nullptr;
-    hipStream_t copyStream = nullptr;   // snapshot read-backs (default priority)
+    // snapshot read-backs (default priority); made with the context's first
+    // snapshot, so a context that takes none holds one hardware queue
+    hipStream_t copyStream = nullptr;
     bool graphs = false;                // replay captured step segments (persistent batches)
     int* dActive = nullptr;   // [0] scenes still iterating, [1] largest iteration count reached
     int* hActive = nullptr;   // pinned copy
@@ -1086,7 +1088,6 @@ int sphg_ctx_create(int32_t device, sphg_ctx** out) {
     sphg_ctx* c = new sphg_ctx();
     c->device = device;
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
-    if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->copyStream, hipStreamNonBlocking);
     if (e == hipSuccess) {
         // freed scene memory stays in the device's default pool for the next
         // scene instead of being released at the next synchronisation
@@ -1910,6 +1911,7 @@ int sphg_snapshot_create(sphg_scene* s, sphg_snapshot** out) {
     if (!s || !s->ctx || !out) return fail(SPHG_EINVAL, "NULL argument");
     if (!s->ready) return fail(SPHG_ESTATE, "snapshot before init");
     HIP_TRY(hipSetDevice(s->ctx->device));
+    if (!s->ctx->copyStream) HIP_TRY(hipStreamCreateWithFlags(&s->ctx->copyStream, hipStreamNonBlocking));
     sphg_snapshot* sn = new sphg_snapshot();
     sn->scene = s;
     sn->capBytes = sizeof(float) * 3 * (size_t)std::max<uint32_t>(s->cap, 1);

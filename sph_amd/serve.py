@@ -138,6 +138,7 @@ def build_spec(req: Request, spacing: float = 0.02) -> SceneSpec:
 
 SMALL, LARGE = 0, 1
 SPLIT_N = 262144     # default size class boundary, fluid particles
+LARGE_CAP = 1_000_000   # default: a large-scene lane steps its oldest scenes up to 1M fluid particles
 
 
 def lane_of(method: int, n: int, split_n: int = 0):
@@ -149,10 +150,21 @@ def lane_of(method: int, n: int, split_n: int = 0):
     return (method, LARGE if split_n and n >= split_n else SMALL)
 
 
+def group_of(lane, split_n: int = 0):
+    """The context (one HIP stream) a lane's batches run on.  With size
+    classes, the large lanes of all methods share one and take turns (a
+    large scene fills the device on its own), and every small lane has its
+    own: four streams for three methods, one per hardware queue of the
+    process's default four.  Streams beyond that share hardware queues, and
+    a small scene's short kernels then wait in order behind a large scene's
+    on its queue.  Without size classes every lane has its own."""
+    return (None, LARGE) if split_n and lane[1] == LARGE else lane
+
+
 class GpuBackend:
     """The HIP solver library on one device.  With `lane_contexts` each lane
-    (solver method and size class, `lane_of`) gets a context (HIP stream) of
-    its own, so that the batches of different lanes can run concurrently.
+    group (`group_of`) gets a context (HIP stream) of its own, so that the
+    batches of different groups can run concurrently.
     The scheduler then creates and plans a lane's new scenes (host-side calls
     the library allows beside a running context) while the lane's quantum
     runs, and initialises, batches and closes them only while it is idle."""
@@ -215,6 +227,7 @@ class Job:
     spec_future: object = None
     scene: object = None
     lane: tuple = None
+    group: tuple = None
     bytes: int = 0
     n: int = 0
     remaining: int = 0
@@ -268,25 +281,30 @@ def _run_quantum(batch, k):
 
 @dataclass
 class _Lane:
+    group: tuple = None                             # its context (`group_of`)
     jobs: List[Job] = field(default_factory=list)   # resident scenes, admission order
-    batch: object = None                            # sphg.Batch over `jobs`, or None
-    stale: bool = False                             # `jobs` changed since `batch` was built
+    batch: object = None                            # sphg.Batch over `stepping`, or None
+    stepping: List[Job] = field(default_factory=list)   # the jobs `batch` steps
     flight: object = None                           # (future, jobs, k) of the running quantum
+    turn: int = 0                                   # launches so far of all lanes when this one last launched
 
 
 def serve_batched(backend, requests: List[Request], budget_bytes: int, quantum: int = 25,
                   clock: Callable[[], float] = time.perf_counter, sleep: Callable[[float], None] = time.sleep,
                   keep_results: bool = True, log: Callable[[str], None] = None,
-                  parallel: bool = False, split_n: int = 0) -> ServeResult:
+                  parallel: bool = False, split_n: int = 0, large_cap: int = 0) -> ServeResult:
     """Continuous batching of `requests` (sorted by arrival) on `backend`.
 
     Resident scenes are grouped in lanes (`lane_of`: method, and size class
     when `split_n` > 0); each lane's scenes step together as one batch.
-    With `parallel` every lane runs its quanta on a host thread of its own
-    (the backend gives each lane its own stream) and independently of the
-    others: when one lane's quantum ends its finished scenes are completed,
-    arrivals admitted and its next quantum started while the other lanes run
-    on.  Without it the lanes take turns on the calling thread."""
+    Lanes run on the context of their group (`group_of`); the lanes of one
+    group take turns.  With `parallel` every group runs its quanta on a host
+    thread of its own (the backend gives each group its own stream) and
+    independently of the others: when a quantum ends, its lane's finished
+    scenes are completed, arrivals admitted and the group's next quantum
+    started while the other groups run on.  Without it the lanes take turns
+    on the calling thread.  With `large_cap` a large lane steps only its
+    oldest scenes, up to `large_cap` fluid particles (at least one)."""
     if parallel and not getattr(backend, "lane_contexts", True):
         raise ValueError("parallel lanes need a backend with a context per lane (GpuBackend(lane_contexts=True)): "
                          "a context is driven by one host thread at a time")
@@ -296,6 +314,7 @@ def serve_batched(backend, requests: List[Request], budget_bytes: int, quantum: 
     jobs: List[Job] = []
     rejected: List[int] = []
     used = 0
+    launches = 0
     pool = _spec_pool()
     runner = ThreadPoolExecutor(max_workers=2 * len(METHODS), thread_name_prefix="lane") if parallel else None
     t0 = clock()
@@ -313,7 +332,8 @@ def serve_batched(backend, requests: List[Request], budget_bytes: int, quantum: 
                 spec = j.spec_future.result()
                 j.n = spec.n
                 j.lane = lane_of(j.req.method, spec.n, split_n)
-                j.scene = backend.create(spec, j.lane)
+                j.group = group_of(j.lane, split_n)
+                j.scene = backend.create(spec, j.group)
                 j.bytes = backend.plan_bytes(j.scene)
             if j.bytes > budget_bytes:
                 backend.close(j.scene)
@@ -329,22 +349,32 @@ def serve_batched(backend, requests: List[Request], budget_bytes: int, quantum: 
             used += j.bytes
             j.remaining = j.req.steps
             j.admitted = clock() - t0
-            ln = lanes.setdefault(j.lane, _Lane())
+            ln = lanes.setdefault(j.lane, _Lane(group=j.group))
             ln.jobs.append(j)
-            ln.stale = True
             waiting.popleft()
 
     def launch(ln: _Lane):
-        for j in ln.jobs:
+        # a large lane steps its oldest scenes up to `large_cap` fluid
+        # particles (a large scene fills the device on its own: batching more
+        # delays the first, and their lists outgrow the Infinity Cache); the
+        # others wait, resident, for their turn
+        js = list(ln.jobs)
+        if large_cap and ln.jobs[0].lane[1] == LARGE:
+            # oldest first, up to large_cap fluid particles (at least one scene)
+            tot, k = 0, 0
+            while k < len(js) and (k == 0 or tot + js[k].n <= large_cap):
+                tot += js[k].n
+                k += 1
+            js = js[:k]
+        for j in js:
             if not j.inited:
                 backend.init(j.scene)
                 j.inited = True
-        if ln.batch is None or ln.stale:
+        if ln.batch is None or [id(j) for j in js] != [id(j) for j in ln.stepping]:
             if ln.batch is not None:
                 backend.close(ln.batch)
-            ln.batch = backend.batch([j.scene for j in ln.jobs])
-            ln.stale = False
-        js = list(ln.jobs)
+            ln.batch = backend.batch([j.scene for j in js])
+            ln.stepping = js
         k = min(quantum, min(j.remaining for j in js))
         if runner is not None:
             fut = runner.submit(_run_quantum, ln.batch, k)
@@ -409,11 +439,23 @@ def serve_batched(backend, requests: List[Request], budget_bytes: int, quantum: 
             # with nothing resident there is nothing to overlap: wait for the
             # head request's geometry
             admit(block=not any(ln.jobs for ln in lanes.values()))
-            for ln in lanes.values():
-                if ln.jobs and ln.flight is None:
+            # one quantum at a time per group; of a group's lanes, the one
+            # holding the oldest request goes first (large scenes fill the
+            # device on their own, so running their work in arrival order
+            # finishes each sooner than taking turns would, at the same
+            # throughput), then the one that launched longest ago
+            busy = {ln.group for ln in lanes.values() if ln.flight is not None}
+            order = sorted((ln for ln in lanes.values() if ln.jobs),
+                           key=lambda x: (min(j.req.arrival for j in x.jobs), x.turn))
+            for ln in order:
+                if ln.jobs and ln.flight is None and ln.group not in busy:
+                    launches += 1
+                    ln.turn = launches
                     launch(ln)
                     if runner is None:
                         land(ln)
+                    else:
+                        busy.add(ln.group)
             flying = {ln.flight[0]: ln for ln in lanes.values() if ln.flight is not None}
             if not flying:
                 if not any(ln.jobs for ln in lanes.values()) and pending and not waiting:
@@ -505,13 +547,15 @@ def serve_plain(backend, requests: List[Request], budget_bytes: int,
 
 
 def compare(ctx, reqs: List[Request], budget_bytes: int, quantum: int = 25, check: int = 4,
-            parallel: bool = True, split_n: int = SPLIT_N, log: Callable[[str], None] = None) -> dict:
+            parallel: bool = True, split_n: int = SPLIT_N, log: Callable[[str], None] = None,
+            large_cap: int = LARGE_CAP) -> dict:
     """Serves `reqs` with continuous batching, then plain (one at a time),
     on the same device; returns both summaries, the throughput ratio and how
     many of the first `check` requests had identical results in both."""
     be = GpuBackend(ctx, lane_contexts=parallel)
     try:
-        rb = serve_batched(be, reqs, budget_bytes, quantum, log=log, parallel=parallel, split_n=split_n)
+        rb = serve_batched(be, reqs, budget_bytes, quantum, log=log, parallel=parallel, split_n=split_n,
+                           large_cap=large_cap)
         rp = serve_plain(be, reqs, budget_bytes, log=log)
     finally:
         be.shutdown()
@@ -569,6 +613,9 @@ def main():
     ap.add_argument("--check", type=int, default=4, help="requests whose batched and plain results are compared")
     ap.add_argument("--serial", action="store_true",
                     help="step the lanes' batches one after another on one stream")
+    ap.add_argument("--large-cap", type=int, default=LARGE_CAP,
+                    help="fluid particles a large-scene lane steps at once, oldest scenes first, at least one "
+                         "(0: all it holds)")
     ap.add_argument("--split-n", type=int, default=SPLIT_N,
                     help="fluid size from which a scene steps in its method's large-scene lane (0: one lane "
                          "per method)")
@@ -578,13 +625,14 @@ def main():
     ctx = sphg.Context(0)
     log = lambda s: print(s, flush=True)   # noqa: E731
     r = compare(ctx, reqs, int(args.budget_gb * 2 ** 30), args.quantum, args.check,
-                parallel=not args.serial, split_n=args.split_n, log=log)
+                parallel=not args.serial, split_n=args.split_n, log=log, large_cap=args.large_cap)
     out = {"metric": "served-scene p50/p99 latency and particle-steps/s, continuous batching vs plain GPU",
            "stream": {"requests": args.requests, "rate_per_s": args.rate, "n": [args.nmin, args.nmax],
                       "steps": [args.smin, args.smax], "methods": "WCSPH/PCISPH/DFSPH uniform",
                       "geometry": "dam-break block or sphere drop, spacing 0.02", "seed": args.seed,
                       "budget_gb": args.budget_gb, "quantum": args.quantum,
-                      "lanes_in_parallel": not args.serial, "split_n": args.split_n}}
+                      "lanes_in_parallel": not args.serial, "split_n": args.split_n,
+                      "large_cap": args.large_cap}}
     out.update(r)
     ctx.close()
     print(json.dumps(out), flush=True)

@@ -51,6 +51,8 @@ class FakeBatch:
     def __init__(self, be, scenes):
         self.be, self.scenes, self.closed = be, scenes, False
         be.batches.append(sorted({(s.method, s.lane) for s in scenes}))
+        be.built.append([(s.method, s.lane) for s in scenes])
+        be.built_n.append([(s.lane[1], s.n) for s in scenes])
 
     def run(self, k):
         with self.be.lock:      # batches of different methods may run on threads
@@ -78,6 +80,8 @@ class FakeBackend:
         self.clock = clock
         self.quanta = []
         self.batches = []       # the (method, lane) pairs of each batch built
+        self.built = []         # the same per scene of each batch built
+        self.built_n = []       # (size class, fluid particles) per scene of each batch built
         self.used = 0
         self.peak = 0
         self.admitted = []
@@ -371,10 +375,12 @@ def test_admission_does_not_wait_for_geometry_while_scenes_run(monkeypatch):
 
 
 def test_scene_initialised_only_while_its_lane_is_idle():
-    """A scene is initialised on its lane's context, so that initialisation
-    never overlaps a running quantum of the same lane (a context is used by
-    one host thread at a time); the lanes' quanta run on threads and take a
-    little real time here, so an admission during a quantum would show."""
+    """A scene is initialised on its lane group's context, and batches are
+    built and closed there, never while a quantum of the same group runs (a
+    context is used by one host thread at a time); a group runs one quantum
+    at a time, the large lanes of all methods share a group and every small
+    lane has its own.  The quanta run on threads and take a little real time
+    here, so an overlap would show."""
     import threading
     import time as _time
     clk = Clock()
@@ -383,11 +389,15 @@ def test_scene_initialised_only_while_its_lane_is_idle():
     guard = threading.Lock()
     run = FakeBatch.run
     lanes_seen = []
+    groups_seen = set()
 
     def run_slowly(self, k):
         lanes = {s.lane for s in self.scenes}
         with guard:
+            # one quantum at a time per context
+            assert not lanes & running, (lanes, running)
             running.update(lanes)
+            groups_seen.update((s.method, s.lane) for s in self.scenes)
         try:
             _time.sleep(0.002)
             run(self, k)
@@ -426,6 +436,13 @@ def test_scene_initialised_only_while_its_lane_is_idle():
     assert all(int(j.result[0]) == j.req.steps for j in res.jobs)
     assert len(lanes_seen) == len(reqs) and len(set(lanes_seen)) > 2
     assert be.used == 0
+    # the scenes' contexts: the large lanes of all methods share one, each
+    # small lane has its own (at most 4 for 3 methods)
+    large = {m for m, g in groups_seen if g == (None, serve.LARGE)}
+    assert len(large) > 1
+    assert len({g for _, g in groups_seen}) <= 4
+    for j in res.jobs:
+        assert j.group == serve.group_of(j.lane, 2048)
 
 
 def test_parallel_lanes_need_a_context_per_lane():
@@ -438,3 +455,47 @@ def test_parallel_lanes_need_a_context_per_lane():
         serve.serve_batched(be, _stream(4), 1 << 40, clock=clk, sleep=clk.sleep, parallel=True)
     res = serve.serve_batched(be, _stream(4), 1 << 40, clock=clk, sleep=clk.sleep, parallel=False)
     assert all(int(j.result[0]) == j.req.steps for j in res.jobs)
+
+
+def test_large_lanes_run_oldest_request_first():
+    """The large lanes of all methods share a context; of them, the lane
+    holding the oldest request runs until that request is done, so large
+    requests complete in arrival order, whatever their step counts (taking
+    turns would finish the shortest first)."""
+    clk = Clock()
+    be = FakeBackend(clk)
+    steps = (300, 200, 100)
+    reqs = [serve.Request(i, 0.001 * i, m, 4096, s, "block", 10_000 + i)
+            for i, (m, s) in enumerate(zip((sphg.DFSPH, sphg.PCISPH, sphg.WCSPH), steps))]
+    res = serve.serve_batched(be, reqs, 1 << 40, quantum=10, clock=clk, sleep=clk.sleep, parallel=True,
+                              split_n=2048)
+    assert all(int(j.result[0]) == j.req.steps for j in res.jobs)
+    assert all(j.group == (None, serve.LARGE) for j in res.jobs)
+    done = [j.req.rid for j in sorted(res.jobs, key=lambda j: j.done)]
+    assert done == [0, 1, 2]
+
+
+def test_large_lane_steps_its_oldest_scenes_only():
+    """With large_cap a large lane's batches hold its oldest scenes up to
+    that many fluid particles (at least one scene); the others wait resident
+    (their memory is reserved) and every request still gets exactly its
+    steps; small lanes are not capped."""
+    for cap in (1, 6000):
+        clk = Clock()
+        be = FakeBackend(clk)
+        reqs = _stream(30, rate=2000.0)
+        res = serve.serve_batched(be, reqs, 1 << 40, quantum=5, clock=clk, sleep=clk.sleep, parallel=True,
+                                  split_n=2048, large_cap=cap)
+        assert all(int(j.result[0]) == j.req.steps for j in res.jobs)
+        large = [b for b in be.built_n if b[0][0] == serve.LARGE]
+        small = [b for b in be.built_n if b[0][0] == serve.SMALL]
+        assert large and small
+        for b in large:
+            ns = [n for _, n in b]
+            assert len(ns) == 1 or sum(ns) <= cap
+        if cap == 1:
+            assert max(len(b) for b in large) == 1
+        else:
+            assert max(len(b) for b in large) > 1
+        assert max(len(b) for b in small) > 1
+        assert be.used == 0
