@@ -9,9 +9,10 @@ Continuous batching (`serve_batched`):
   * requests arrive on a wall clock (Poisson process); a waiting request is
     admitted, first come first served, when the device bytes of its scene
     (known before allocation, `Scene.plan_bytes`) fit the memory budget
-    beside the scenes already resident; its scene is initialised on its
-    lane's context at the lane's next launch, so that a context is never
-    used by two host threads at once;
+    beside the scenes already resident; its scene is created and planned
+    on its lane group's context at once (host-side calls the library allows
+    beside a running context) and initialised there at the group's next
+    launch, while the group is idle;
   * resident scenes stay on the device, grouped in lanes: one per solver
     method, split by size (`split_n`) so that small scenes do not wait on
     the step time of giant ones; a lane's scenes are stepped together, one
@@ -25,11 +26,17 @@ Continuous batching (`serve_batched`):
     whose scenes overflowed: they are freed and completed with an error, the
     batch is rebuilt without them, and the other scenes, which stepped
     normally, continue;
-  * with `parallel`, the lanes run their quanta at the same time and
-    independently, each on its own context (HIP stream) driven by its own
-    host thread: a launch-bound lane of small scenes then overlaps the
-    others instead of waiting for them, and a lane whose quantum ends is
-    refilled at once.
+  * lanes run on the context (HIP stream) of their group (`group_of`):
+    each small lane has its own, and the large lanes share one, four streams
+    for the process's four hardware queues; a group runs one quantum at a
+    time, the lane with the oldest request first, and a large lane steps its
+    oldest scenes up to `large_cap` fluid particles (a large scene fills the
+    device alone: stepping large requests in arrival order finishes each
+    sooner, at the same throughput);
+  * with `parallel`, the groups run their quanta at the same time and
+    independently, each driven by its own host thread: a launch-bound lane
+    of small scenes then overlaps the large scenes instead of waiting for
+    them, and a group whose quantum ends is refilled at once.
   The scene geometry of an arriving request is built on a host thread while
   the GPU steps the resident scenes.
 
