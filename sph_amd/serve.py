@@ -313,8 +313,8 @@ def serve_batched(backend, requests: List[Request], budget_bytes: int, quantum: 
     on the calling thread.  With `large_cap` a large lane steps only its
     oldest scenes, up to `large_cap` fluid particles (at least one)."""
     if parallel and not getattr(backend, "lane_contexts", True):
-        raise ValueError("parallel lanes need a backend with a context per lane (GpuBackend(lane_contexts=True)): "
-                         "a context is driven by one host thread at a time")
+        raise ValueError("parallel lanes need a backend with a context per lane group "
+                         "(GpuBackend(lane_contexts=True)): a context is driven by one host thread at a time")
     pending = deque(sorted(requests, key=lambda q: q.arrival))
     waiting: deque = deque()
     lanes: Dict[tuple, _Lane] = {}
@@ -351,8 +351,8 @@ def serve_batched(backend, requests: List[Request], budget_bytes: int, quantum: 
             if used + j.bytes > budget_bytes:
                 break
             # its device memory is reserved now; the scene is initialised
-            # (allocated and uploaded) on its lane's context at the lane's
-            # next launch, while no quantum of that lane is running
+            # (allocated and uploaded) on its group's context when its lane
+            # next steps it, while no quantum of that group is running
             used += j.bytes
             j.remaining = j.req.steps
             j.admitted = clock() - t0
