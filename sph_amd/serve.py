@@ -9,10 +9,10 @@ Continuous batching (`serve_batched`):
   * requests arrive on a wall clock (Poisson process); a waiting request is
     admitted, first come first served, when the device bytes of its scene
     (known before allocation, `Scene.plan_bytes`) fit the memory budget
-    beside the scenes already resident; its scene is created and planned
-    on its lane group's context at once (host-side calls the library allows
-    beside a running context) and initialised there at the group's next
-    launch, while the group is idle;
+    beside the scenes already resident; its scene is built, created and
+    planned on a geometry thread on its lane group's context (host-side
+    calls the library allows beside a running context), and initialised
+    there by the group's own thread before the quantum that first steps it;
   * resident scenes stay on the device, grouped in lanes: one per solver
     method, split by size (`split_n`) so that small scenes do not wait on
     the step time of giant ones; a lane's scenes are stepped together, one
@@ -37,8 +37,8 @@ Continuous batching (`serve_batched`):
     independently, each driven by its own host thread: a launch-bound lane
     of small scenes then overlaps the large scenes instead of waiting for
     them, and a group whose quantum ends is refilled at once.
-  The scene geometry of an arriving request is built on a host thread while
-  the GPU steps the resident scenes.
+  The scheduler thread only admits, launches and retires: the scene set-up
+  and read-back calls run on the geometry threads and the groups' threads.
 
 Plain serving (`serve_plain`): the same requests one at a time in arrival
 order, each stepped alone from creation to read-back.
@@ -53,6 +53,7 @@ from __future__ import annotations
 import argparse
 import json
 import math
+import threading
 import time
 from collections import deque
 from concurrent.futures import FIRST_COMPLETED, Future, ThreadPoolExecutor, wait
@@ -181,19 +182,21 @@ class GpuBackend:
         self.lane_contexts = lane_contexts
         self.extra = []
         self.ctxs = {}
+        self.lock = threading.Lock()    # scenes are created on the geometry threads
 
     def _context(self, lane):
         if lane is None or not self.lane_contexts:
             return self.ctx
-        c = self.ctxs.get(lane)
-        if c is None:
-            if self.ctxs:
-                c = sphg.Context(self.ctx.device)
-                self.extra.append(c)
-            else:
-                c = self.ctx
-            self.ctxs[lane] = c
-        return c
+        with self.lock:
+            c = self.ctxs.get(lane)
+            if c is None:
+                if self.ctxs:
+                    c = sphg.Context(self.ctx.device)
+                    self.extra.append(c)
+                else:
+                    c = self.ctx
+                self.ctxs[lane] = c
+            return c
 
     def create(self, spec: SceneSpec, lane=None):
         return sphg.Scene(self._context(lane), spec.params, spec.fluid, spec.boundary, init=False)
@@ -286,6 +289,28 @@ def _run_quantum(batch, k):
     return None
 
 
+def _prepare_and_run(backend, fresh, old_batch, scenes, batch, k, finishing=()):
+    """One quantum of a lane on its group's thread: initialises the scenes
+    stepping for the first time, builds the batch when its scenes changed
+    (closing the one it replaces), runs the quantum and reads back the
+    scenes it finishes.  Returns the batch, the overflow error
+    `_run_quantum` reports and the read-backs (None for a scene that
+    overflowed)."""
+    if batch is None and old_batch is not None:
+        backend.close(old_batch)        # first: no other reference to it is kept
+    for sc in fresh:
+        backend.init(sc)
+    if batch is None:
+        batch = backend.batch(scenes)
+    try:
+        over = _run_quantum(batch, k)
+    except BaseException:
+        backend.close(batch)            # a batch is closed before its scenes
+        raise
+    results = [None if over is not None and backend.overflowed(sc) else backend.result(sc) for sc in finishing]
+    return batch, over, results
+
+
 @dataclass
 class _Lane:
     group: tuple = None                             # its context (`group_of`)
@@ -326,9 +351,23 @@ def serve_batched(backend, requests: List[Request], budget_bytes: int, quantum: 
     runner = ThreadPoolExecutor(max_workers=2 * len(METHODS), thread_name_prefix="lane") if parallel else None
     t0 = clock()
 
+    def prepare(req):
+        """On a geometry thread: the request's scene, created and planned on
+        its lane group's context (host-side calls, allowed beside the
+        context's running quanta)."""
+        spec = build_spec(req)
+        lane = lane_of(req.method, spec.n, split_n)
+        group = group_of(lane, split_n)
+        scene = backend.create(spec, group)
+        try:
+            return spec.n, lane, group, scene, backend.plan_bytes(scene)
+        except BaseException:
+            backend.close(scene)
+            raise
+
     def admit(block: bool):
         """Admits waiting requests in arrival order while they fit.  Unless
-        `block`, it stops at a request whose geometry is still being built
+        `block`, it stops at a request whose scene is still being prepared
         (the lanes keep running meanwhile)."""
         nonlocal used
         while waiting:
@@ -336,12 +375,7 @@ def serve_batched(backend, requests: List[Request], budget_bytes: int, quantum: 
             if j.scene is None:
                 if not block and not j.spec_future.done():
                     break
-                spec = j.spec_future.result()
-                j.n = spec.n
-                j.lane = lane_of(j.req.method, spec.n, split_n)
-                j.group = group_of(j.lane, split_n)
-                j.scene = backend.create(spec, j.group)
-                j.bytes = backend.plan_bytes(j.scene)
+                j.n, j.lane, j.group, j.scene, j.bytes = j.spec_future.result()
             if j.bytes > budget_bytes:
                 backend.close(j.scene)
                 j.scene = None
@@ -373,22 +407,23 @@ def serve_batched(backend, requests: List[Request], budget_bytes: int, quantum: 
                 tot += js[k].n
                 k += 1
             js = js[:k]
-        for j in js:
-            if not j.inited:
-                backend.init(j.scene)
-                j.inited = True
-        if ln.batch is None or [id(j) for j in js] != [id(j) for j in ln.stepping]:
-            if ln.batch is not None:
-                backend.close(ln.batch)
-            ln.batch = backend.batch([j.scene for j in js])
-            ln.stepping = js
+        # the group's thread initialises the new scenes and rebuilds the
+        # batch before the quantum, so this thread goes on serving the other
+        # groups meanwhile
+        fresh = [j for j in js if not j.inited]
+        for j in fresh:
+            j.inited = True
+        keep = ln.batch is not None and [id(j) for j in js] == [id(j) for j in ln.stepping]
         k = min(quantum, min(j.remaining for j in js))
+        args = (backend, [j.scene for j in fresh], None if keep else ln.batch, [j.scene for j in js],
+                ln.batch if keep else None, k, [j.scene for j in js if j.remaining == k])
+        ln.batch, ln.stepping = None, js    # the quantum's batch comes back with it
         if runner is not None:
-            fut = runner.submit(_run_quantum, ln.batch, k)
+            fut = runner.submit(_prepare_and_run, *args)
         else:
             fut = Future()
             try:
-                fut.set_result(_run_quantum(ln.batch, k))
+                fut.set_result(_prepare_and_run(*args))
             except BaseException as e:  # noqa: BLE001 - re-raised by land()
                 fut.set_exception(e)
         ln.flight = (fut, js, k)
@@ -399,7 +434,9 @@ def serve_batched(backend, requests: List[Request], budget_bytes: int, quantum: 
         nonlocal used
         fut, js, k = ln.flight
         ln.flight = None
-        over = fut.result()            # raises what the quantum raised, other than an overflow
+        # raises what the quantum raised, other than an overflow
+        ln.batch, over, results = fut.result()
+        read = dict(zip([id(j) for j in js if j.remaining == k], results))
         if over is not None:
             bad = [j for j in js if backend.overflowed(j.scene)]
             if not bad:
@@ -424,7 +461,7 @@ def serve_batched(backend, requests: List[Request], budget_bytes: int, quantum: 
                 backend.close(ln.batch)
             ln.batch = None
             for j in fin:
-                res = backend.result(j.scene)
+                res = read[id(j)]
                 j.result = res if keep_results else None
                 backend.close(j.scene)
                 j.scene = None
@@ -440,7 +477,7 @@ def serve_batched(backend, requests: List[Request], budget_bytes: int, quantum: 
             now = clock() - t0
             while pending and pending[0].arrival <= now:
                 j = Job(pending.popleft())
-                j.spec_future = pool.submit(build_spec, j.req)
+                j.spec_future = pool.submit(prepare, j.req)
                 waiting.append(j)
                 jobs.append(j)
             # with nothing resident there is nothing to overlap: wait for the
@@ -477,20 +514,27 @@ def serve_batched(backend, requests: List[Request], budget_bytes: int, quantum: 
                     land(flying[f])
     finally:
         if runner is not None:
-            # every running quantum ends before its batch is closed
+            # every running quantum ends before its batch is closed; a
+            # quantum that completed hands its batch back for closing
             for ln in lanes.values():
                 if ln.flight is not None:
-                    wait([ln.flight[0]])
+                    fut = ln.flight[0]
+                    wait([fut])
+                    if fut.exception() is None:
+                        ln.batch = fut.result()[0]
         for ln in lanes.values():
             if ln.batch is not None:
                 backend.close(ln.batch)
             for j in ln.jobs:
                 if j.scene is not None:
                     backend.close(j.scene)
+        # scenes prepared for requests that were never admitted
+        pool.shutdown(wait=True)
         for j in waiting:
+            if j.scene is None and j.spec_future.done() and j.spec_future.exception() is None:
+                j.scene = j.spec_future.result()[3]
             if j.scene is not None:
                 backend.close(j.scene)
-        pool.shutdown(wait=True)
         if runner is not None:
             runner.shutdown(wait=True)
     done = [j.done for j in jobs if j.done >= 0]

@@ -499,3 +499,37 @@ def test_large_lane_steps_its_oldest_scenes_only():
             assert max(len(b) for b in large) > 1
         assert max(len(b) for b in small) > 1
         assert be.used == 0
+
+
+def test_failed_quantum_aborts_and_frees_everything():
+    """A quantum that fails with an error other than a neighbor overflow
+    ends serving with that error; every batch is closed before its scenes,
+    and every scene is closed."""
+    for parallel in (False, True):
+        clk = Clock()
+        be = FakeBackend(clk)
+        built = []
+        batch = be.batch
+
+        def batch_recorded(scenes):
+            b = batch(scenes)
+            built.append(b)
+            return b
+        be.batch = batch_recorded
+        run = FakeBatch.run
+        calls = [0]
+
+        def failing(self, k):
+            calls[0] += 1
+            if calls[0] == 6:
+                raise sphg.SphgError(sphg.EHIP, "device error")
+            run(self, k)
+        FakeBatch.run = failing
+        try:
+            with pytest.raises(sphg.SphgError, match="device error"):
+                serve.serve_batched(be, _stream(20, rate=500.0), 1 << 40, quantum=5, clock=clk, sleep=clk.sleep,
+                                    parallel=parallel, split_n=2048, large_cap=3000)
+        finally:
+            FakeBatch.run = run
+        assert built and all(b.closed for b in built)
+        assert be.used == 0
