@@ -652,9 +652,8 @@ int batch_one_step(sphg_ctx* ctx, Batch& B, uint32_t& syncs) {
         const DevScene* T = B.dT[0];
         const DevScene* T2 = B.dT[1];
         rc = run_segment(ctx, B, 0, [&] {
-            launch_fill_int(st, ctx->dActive, 0, 2);
-            { Scope s(P, st, K_STEPBEGIN); launch_step_begin(st, T, B.dP, B.nP, ctx->dActive); }
-            { Scope s(P, st, K_KEYS); launch_keys(st, T, B.dP, B.nP); }
+            // the keys kernel also starts the step (WCSPH has no loop to count)
+            { Scope s(P, st, K_KEYS); launch_keys(st, T, B.dP, B.nP, ctx->dActive); }
             grid_sort(ctx, B, T);
             { Scope s(P, st, K_BUILD); launch_build(st, WCSPH, T2, B.dP, B.nP); }
             // PCISPH boundary method: boundary densities and pressures
@@ -668,7 +667,8 @@ int batch_one_step(sphg_ctx* ctx, Batch& B, uint32_t& syncs) {
             if (anyCube) { Scope s(P, st, K_CUBE); launch_cube(st, T2, B.dP, B.nP); }
             if (anyPb) { Scope s(P, st, K_PB); launch_pb_correct(st, 1, T2, B.dP, B.nP); }
             if (anyEmit) { Scope s(P, st, K_EMIT); launch_emit(st, B.dT[1], B.dT[0], ns, 0); }
-            { Scope s(P, st, K_TIME); launch_time_advance(st, T2, ns); }
+            // the forces advanced the time of scenes with none of these
+            if (anyCube || anyPb || anyEmit) { Scope s(P, st, K_TIME); launch_time_advance(st, T2, ns); }
         });
         if (rc) return rc;
         flip(B);
@@ -677,8 +677,9 @@ int batch_one_step(sphg_ctx* ctx, Batch& B, uint32_t& syncs) {
         const DevScene* T2 = B.dT[1];
         rc = run_segment(ctx, B, 0, [&] {
             launch_fill_int(st, ctx->dActive, 0, 2);
-            { Scope s(P, st, K_STEPBEGIN); launch_step_begin(st, T, B.dP, B.nP, ctx->dActive); }
-            { Scope s(P, st, K_KEYS); launch_keys(st, T, B.dP, B.nP); }
+            // the keys kernel also starts the step and counts the scenes
+            // whose pressure loop runs
+            { Scope s(P, st, K_KEYS); launch_keys(st, T, B.dP, B.nP, ctx->dActive); }
             grid_sort(ctx, B, T);
             { Scope s(P, st, K_BUILD); launch_build(st, PCISPH, T2, B.dP, B.nP); }
             if (anySt) { Scope s(P, st, K_NORMALS); launch_normals(st, T2, B.dP, B.nP); }
@@ -707,7 +708,8 @@ int batch_one_step(sphg_ctx* ctx, Batch& B, uint32_t& syncs) {
             if (anyCube) { Scope s(P, st, K_CUBE); launch_cube(st, T2, B.dP, B.nP); }
             if (anyPb) { Scope s(P, st, K_PB); launch_pb_correct(st, 2, T2, B.dP, B.nP); }
             if (anyEmit) { Scope s(P, st, K_EMIT); launch_emit(st, B.dT[1], B.dT[0], ns, 0); }
-            { Scope s(P, st, K_TIME); launch_time_advance(st, T2, ns); }
+            // the integration advanced the time of scenes with none of these
+            if (anyCube || anyPb || anyEmit) { Scope s(P, st, K_TIME); launch_time_advance(st, T2, ns); }
         });
         if (rc) return rc;
         flip(B);

@@ -6,7 +6,9 @@
 
 namespace sphg {
 
-void launch_keys(hipStream_t st, const DevScene* T, const Chunk* C, int nchunks);
+// begin_active: also the start of a WCSPH or PCISPH step, counting the
+// scenes whose pressure loop runs in begin_active[0] (null: keys only)
+void launch_keys(hipStream_t st, const DevScene* T, const Chunk* C, int nchunks, int* begin_active = nullptr);
 void launch_scan(hipStream_t st, const DevScene* T, const Chunk* CC, int ncellchunks, int nscenes);
 void launch_scatter(hipStream_t st, const DevScene* T, const Chunk* C, int nchunks);
 void launch_rank_reorder(hipStream_t st, const DevScene* T, const Chunk* C, int nchunks);
@@ -14,7 +16,6 @@ void launch_bvolume(hipStream_t st, const DevScene* T, int scene, int nb, float*
 void launch_build(hipStream_t st, int method, const DevScene* T, const Chunk* C, int nchunks);
 void launch_normals(hipStream_t st, const DevScene* T, const Chunk* C, int nchunks);
 void launch_forces(hipStream_t st, int method, const DevScene* T, const Chunk* C, int nchunks);
-void launch_step_begin(hipStream_t st, const DevScene* T, const Chunk* C, int nchunks, int* active);
 void launch_dt(hipStream_t st, const DevScene* T, const Chunk* C, int nchunks);
 void launch_df_predict(hipStream_t st, const DevScene* T, const Chunk* C, int nchunks);
 void launch_df_vel(hipStream_t st, int mode, const DevScene* T, const Chunk* C, int nchunks);

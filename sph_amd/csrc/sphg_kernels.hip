@@ -494,10 +494,39 @@ __device__ inline int wave_count_add(int* counter, int key) {
     return base + (lane - first);
 }
 
-__global__ __launch_bounds__(kBlock) void k_keys(const DevScene* __restrict__ T, const Chunk* __restrict__ C) {
+// The start of a WCSPH or PCISPH step: the scene's step counters, and for
+// PCISPH zero pressures and pressure accelerations and the pressure loop's
+// start (src/PCISPHSolver.cpp:21-50); `active` counts the scenes whose loop
+// runs.
+__device__ inline void step_begin(const DevScene& S, const Chunk& c, int i, int* active) {
+    if (S.method == PCISPH && i < S.n) {
+        S.p[i] = 0.0f;
+        S.pacc[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+    if (c.begin == 0 && threadIdx.x == 0) {
+        Scalars* ss = S.ss;
+        ss->steps += 1u;
+        ss->shellBad = 0u;
+        ss->maxD2Prev = ss->maxD2;
+        ss->maxD2 = 0u;
+        if (S.method == PCISPH) {
+            ss->it = 0;
+            const bool run = (ss->maxDensErr > S.eta || 0 < S.minIt) && 0 < S.maxIt;
+            ss->done = run ? 0 : 1;
+            if (run) atomicAdd(active, 1);
+        }
+    }
+}
+
+// Cell key of each particle and its count in its cell; with BEGIN also the
+// start of a WCSPH or PCISPH step (one launch fewer per step).
+template <bool BEGIN>
+__global__ __launch_bounds__(kBlock) void k_keys(const DevScene* __restrict__ T, const Chunk* __restrict__ C,
+                                                 int* active) {
     const Chunk c = C[blockIdx.x];
     const DevScene& S = T[c.scene];
     const int i = c.begin + threadIdx.x;
+    if (BEGIN) step_begin(S, c, i, active);
     // the lists about to be built start without an overflow
     if (c.begin == 0 && threadIdx.x == 0) G(S.ss)->listOver = 0u;
     int cl = -1;
@@ -1172,6 +1201,23 @@ __global__ __launch_bounds__(kBlock) void k_normals(const DevScene* __restrict__
     st4(S.nrm, i, make_float4((float)(nx * h), (float)(ny * h), (float)(nz * h), ldg(S.drho, i)));
 }
 
+// The end of a WCSPH or PCISPH step: time += dt (src/WCSPHSolver.cpp:77,
+// src/PCISPHSolver.cpp:80), and which predicted-density path a PCISPH step
+// took.
+__device__ inline void advance_time(const DevScene& S) {
+    Scalars* ss = S.ss;
+    ss->time += ss->dt;
+    if (S.method == PCISPH) {
+        if (S.pciShell && ss->shellBad == 0u) ss->shellSteps += 1u;
+        else ss->walkSteps += 1u;
+    }
+}
+// Whether the step's last kernel (the WCSPH forces, the PCISPH integration)
+// advances the time, one launch fewer per step: not when the cube clamp, the
+// PCISPH boundary projection or an emitter runs after it (the emitters read
+// the step's start time); k_time_advance does it then.
+__device__ inline bool time_in_last_kernel(const DevScene& S) { return !(S.cubeOn || S.pbOn || S.nEmit > 0); }
+
 // Non-pressure accelerations (and, for WCSPH, the pressure force and the
 // explicit integration).
 //   pressure  src/WCSPHSolver.cpp:152-186
@@ -1197,6 +1243,7 @@ __global__ __launch_bounds__(kBlock) void k_forces(const DevScene* __restrict__ 
     const Chunk c = C[blockIdx.x];
     const DevScene& S = T[c.scene];
     const int i = c.begin + threadIdx.x;
+    if ((M == WCSPH) && c.begin == 0 && threadIdx.x == 0 && time_in_last_kernel(S)) advance_time(S);
     if (i >= S.n) return;
     const KCoefD k = S.kd;
     const double dt = S.ss->dt;
@@ -1359,28 +1406,6 @@ __global__ __launch_bounds__(kBlock) void k_forces(const DevScene* __restrict__ 
 // WCSPH/PCISPH step-time bookkeeping: steps++ (one thread per scene) and,
 // for PCISPH, the entry test of the pressure loop (src/PCISPHSolver.cpp:365)
 // with p = pacc = 0 (:83-120).
-__global__ __launch_bounds__(kBlock) void k_step_begin(const DevScene* __restrict__ T, const Chunk* __restrict__ C, int* active) {
-    const Chunk c = C[blockIdx.x];
-    const DevScene& S = T[c.scene];
-    const int i = c.begin + threadIdx.x;
-    if (S.method == PCISPH && i < S.n) {
-        S.p[i] = 0.0f;
-        S.pacc[i] = make_float4(0.f, 0.f, 0.f, 0.f);
-    }
-    if (c.begin == 0 && threadIdx.x == 0) {
-        Scalars* ss = S.ss;
-        ss->steps += 1u;
-        ss->shellBad = 0u;
-        ss->maxD2Prev = ss->maxD2;
-        ss->maxD2 = 0u;
-        if (S.method == PCISPH) {
-            ss->it = 0;
-            const bool run = (ss->maxDensErr > S.eta || 0 < S.minIt) && 0 < S.maxIt;
-            ss->done = run ? 0 : 1;
-            if (run) atomicAdd(active, 1);
-        }
-    }
-}
 
 // ================================================================ DFSPH
 
@@ -1830,6 +1855,7 @@ __global__ __launch_bounds__(kBlock) void k_pci_integrate(const DevScene* __rest
     const Chunk c = C[blockIdx.x];
     const DevScene& S = T[c.scene];
     const int i = c.begin + threadIdx.x;
+    if (c.begin == 0 && threadIdx.x == 0 && time_in_last_kernel(S)) advance_time(S);
     if (i >= S.n) return;
     const double dt = S.ss->dt;
     const float4 a = S.a[i], pa = S.pacc[i];
@@ -2212,14 +2238,8 @@ __global__ __launch_bounds__(kBlock) void k_pb_predcorrect(const DevScene* __res
 // WCSPH: time += dt after the integration (src/WCSPHSolver.cpp:77).
 __global__ void k_time_advance(const DevScene* __restrict__ T, int nscenes) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= nscenes) return;
-    Scalars* ss = T[s].ss;
-    ss->time += ss->dt;
-    // which predicted-density path the PCISPH step took
-    if (T[s].method == PCISPH) {
-        if (T[s].pciShell && ss->shellBad == 0u) ss->shellSteps += 1u;
-        else ss->walkSteps += 1u;
-    }
+    if (s >= nscenes || time_in_last_kernel(T[s])) return;
+    advance_time(T[s]);
 }
 
 // The neighbor-overflow maximum of every scene of a batch starts each step
@@ -2290,7 +2310,11 @@ __global__ void k_load_state(const DevScene* __restrict__ T, int scene, const fl
 
 #define LAUNCH(kern, grid, stream, ...) hipLaunchKernelGGL(kern, dim3(grid), dim3(kBlock), 0, stream, __VA_ARGS__)
 
-void launch_keys(hipStream_t st, const DevScene* T, const Chunk* C, int n) { if (n) LAUNCH(k_keys, n, st, T, C); }
+void launch_keys(hipStream_t st, const DevScene* T, const Chunk* C, int n, int* begin_active) {
+    if (!n) return;
+    if (begin_active) LAUNCH(k_keys<true>, n, st, T, C, begin_active);
+    else LAUNCH(k_keys<false>, n, st, T, C, nullptr);
+}
 void launch_scan(hipStream_t st, const DevScene* T, const Chunk* CC, int ncc, int nscenes) {
     if (!ncc) return;
     LAUNCH(k_scan_sum, ncc, st, T, CC);
@@ -2318,7 +2342,6 @@ void launch_forces(hipStream_t st, int method, const DevScene* T, const Chunk* C
     else if (method == WCSPH) LAUNCH(k_forces<WCSPH>, n, st, T, C);
     else LAUNCH(k_forces<PCISPH>, n, st, T, C);
 }
-void launch_step_begin(hipStream_t st, const DevScene* T, const Chunk* C, int n, int* active) { if (n) LAUNCH(k_step_begin, n, st, T, C, active); }
 void launch_dt(hipStream_t st, const DevScene* T, const Chunk* C, int n) { if (n) LAUNCH(k_dt, n, st, T, C); }
 void launch_df_predict(hipStream_t st, const DevScene* T, const Chunk* C, int n) { if (n) LAUNCH(k_df_predict, n, st, T, C); }
 void launch_df_vel(hipStream_t st, int mode, const DevScene* T, const Chunk* C, int n) {
