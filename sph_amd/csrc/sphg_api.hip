@@ -196,6 +196,7 @@ struct Batch {
     Chunk* dP = nullptr;
     Chunk* dC = nullptr;
     int nP = 0, nC = 0;
+    int maxCellChunks = 0;     // largest cell-chunk count of one scene
     int maxIt = 0;
     int minIt = 0, minItV = 0;   // smallest minimum iteration counts in the batch
     int lastIt = 0, lastItV = 0; // iterations the loops needed in the previous step
@@ -420,6 +421,7 @@ int upload_batch_tables(sphg_ctx* ctx, Batch& B) {
     B.maxIt = 0;
     B.minIt = 1 << 30;
     B.minItV = 1 << 30;
+    int maxCC = 0;
     for (int s = 0; s < ns; ++s) {
         sphg_scene* sc = B.scenes[s];
         t0[s] = sc->view[sc->par];
@@ -428,6 +430,7 @@ int upload_batch_tables(sphg_ctx* ctx, Batch& B) {
         B.nAt[s] = sc->n;
         for (int c = 0; c < t0[s].nChunks; ++c) P.push_back(Chunk{s, c * kBlock});
         for (int c = 0; c < t0[s].nCellChunks; ++c) Cc.push_back(Chunk{s, c * kCellChunk});
+        maxCC = std::max(maxCC, t0[s].nCellChunks);
         B.maxIt = std::max(B.maxIt, std::max(t0[s].maxIt, std::max(t0[s].maxItV, 0)));
         B.minIt = std::min(B.minIt, std::min(t0[s].minIt, t0[s].maxIt));
         B.minItV = std::min(B.minItV, std::min(t0[s].minItV, t0[s].maxIt));
@@ -453,7 +456,7 @@ int upload_batch_tables(sphg_ctx* ctx, Batch& B) {
     HIP_TRY(hipMemcpyAsync(B.dT[1], t1.data(), sizeof(DevScene) * ns, hipMemcpyHostToDevice, ctx->stream));
     HIP_TRY(hipMemcpyAsync(B.dP, P.data(), sizeof(Chunk) * P.size(), hipMemcpyHostToDevice, ctx->stream));
     HIP_TRY(hipMemcpyAsync(B.dC, Cc.data(), sizeof(Chunk) * Cc.size(), hipMemcpyHostToDevice, ctx->stream));
-    if (B.nP != (int)P.size() || B.nC != (int)Cc.size()) {
+    if (B.nP != (int)P.size() || B.nC != (int)Cc.size() || B.maxCellChunks != maxCC) {
         // the captured segments hold the launch sizes
         for (auto& a : B.seg)
             for (auto& e : a)
@@ -461,6 +464,7 @@ int upload_batch_tables(sphg_ctx* ctx, Batch& B) {
     }
     B.nP = (int)P.size();
     B.nC = (int)Cc.size();
+    B.maxCellChunks = maxCC;
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     return SPHG_OK;
 }
@@ -472,7 +476,7 @@ void grid_sort(sphg_ctx* ctx, Batch& B, const DevScene* T) {
     Profiler* P = &ctx->prof;
     hipStream_t st = ctx->stream;
     const int ns = (int)B.scenes.size();
-    { Scope s(P, st, K_SCAN); launch_scan(st, T, B.dC, B.nC, ns); }
+    { Scope s(P, st, K_SCAN); launch_scan(st, T, B.dC, B.nC, ns, B.maxCellChunks); }
     { Scope s(P, st, K_SCATTER); launch_scatter(st, T, B.dP, B.nP); }
     { Scope s(P, st, K_REORDER); launch_rank_reorder(st, T, B.dP, B.nP); }
 }

@@ -9,7 +9,11 @@ namespace sphg {
 // begin_active: also the start of a WCSPH or PCISPH step, counting the
 // scenes whose pressure loop runs in begin_active[0] (null: keys only)
 void launch_keys(hipStream_t st, const DevScene* T, const Chunk* C, int nchunks, int* begin_active = nullptr);
-void launch_scan(hipStream_t st, const DevScene* T, const Chunk* CC, int ncellchunks, int nscenes);
+// maxChunks: the largest cell-chunk count of one scene; up to
+// kScanFusedChunks each final-scan block sums the totals before its chunk
+// itself (one launch fewer), above it a per-scene scan of the totals runs
+constexpr int kScanFusedChunks = 1024;
+void launch_scan(hipStream_t st, const DevScene* T, const Chunk* CC, int ncellchunks, int nscenes, int maxChunks);
 void launch_scatter(hipStream_t st, const DevScene* T, const Chunk* C, int nchunks);
 void launch_rank_reorder(hipStream_t st, const DevScene* T, const Chunk* C, int nchunks);
 void launch_bvolume(hipStream_t st, const DevScene* T, int scene, int nb, float* tmp);

@@ -591,10 +591,31 @@ __global__ __launch_bounds__(kBlock) void k_scan_offsets(const DevScene* __restr
 }
 
 // Final scan: cellStart[c] = exclusive prefix of counts; counts reset.
+// FUSED: the chunk totals are still the raw ones (no k_scan_offsets ran),
+// and the block adds up those before its chunk itself; integer sums, so
+// either way gives the same offsets.
+template <bool FUSED>
 __global__ __launch_bounds__(kBlock) void k_scan_final(const DevScene* __restrict__ T, const Chunk* __restrict__ C) {
     const Chunk c = C[blockIdx.x];
     const DevScene& S = T[c.scene];
     __shared__ int wsum[kWaves];
+    __shared__ int base;
+    const int chunk = c.begin / kCellChunk;
+    if (FUSED) {
+        int acc = 0;
+        for (int t = threadIdx.x; t < chunk; t += kBlock) acc += ldg(S.cellChunkSum, (size_t)t);
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) acc += __shfl_xor(acc, o);
+        if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = acc;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            int b = 0;
+#pragma unroll
+            for (int q = 0; q < kWaves; ++q) b += wsum[q];
+            base = b;
+        }
+        __syncthreads();
+    }
     const int c0 = c.begin + threadIdx.x * kCellPerThread;
     int vals[kCellPerThread];
     int local = 0;
@@ -610,7 +631,7 @@ __global__ __launch_bounds__(kBlock) void k_scan_final(const DevScene* __restric
     __syncthreads();
     int woff = 0;
     for (int q = 0; q < w; ++q) woff += wsum[q];
-    int run = ldg(S.cellChunkSum, (size_t)(c.begin / kCellChunk)) + woff + incl - local;
+    int run = (FUSED ? base : ldg(S.cellChunkSum, (size_t)chunk)) + woff + incl - local;
 #pragma unroll
     for (int u = 0; u < kCellPerThread; ++u) {
         const int cc = c0 + u;
@@ -2315,11 +2336,15 @@ void launch_keys(hipStream_t st, const DevScene* T, const Chunk* C, int n, int* 
     if (begin_active) LAUNCH(k_keys<true>, n, st, T, C, begin_active);
     else LAUNCH(k_keys<false>, n, st, T, C, nullptr);
 }
-void launch_scan(hipStream_t st, const DevScene* T, const Chunk* CC, int ncc, int nscenes) {
+void launch_scan(hipStream_t st, const DevScene* T, const Chunk* CC, int ncc, int nscenes, int maxChunks) {
     if (!ncc) return;
     LAUNCH(k_scan_sum, ncc, st, T, CC);
+    if (maxChunks <= kScanFusedChunks) {
+        LAUNCH(k_scan_final<true>, ncc, st, T, CC);
+        return;
+    }
     LAUNCH(k_scan_offsets, nscenes, st, T, nscenes);
-    LAUNCH(k_scan_final, ncc, st, T, CC);
+    LAUNCH(k_scan_final<false>, ncc, st, T, CC);
 }
 void launch_scatter(hipStream_t st, const DevScene* T, const Chunk* C, int n) { if (n) LAUNCH(k_scatter, n, st, T, C); }
 void launch_rank_reorder(hipStream_t st, const DevScene* T, const Chunk* C, int n) { if (n) LAUNCH(k_rank_reorder, n, st, T, C); }

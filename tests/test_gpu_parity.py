@@ -351,6 +351,33 @@ def test_lists_past_2_pow_30_entries_match_small_lists(ctx):
     assert np.array_equal(x0, x1) and np.array_equal(v0, v1)
 
 
+def test_grid_past_fused_scan_limit_steps_like_small_grid(ctx):
+    """The cell scan runs in two launches while every scene has at most
+    1024 scan chunks (each final-scan block adds up the chunk totals before
+    its own) and in three past that (a per-scene scan of the totals).  The
+    same dam break in a domain stretched along z, the slowest axis of the
+    cell index, keeps every cell's index and only adds empty cells: 2M+
+    cells take the three-launch scan and must step bit-identically."""
+    from dataclasses import replace
+    spec = configs.small_scene(sphg.DFSPH, n_side=12)
+    pts = np.concatenate([spec.fluid, spec.boundary]).astype(np.float64)
+    lo, hi = pts.min(axis=0), pts.max(axis=0)
+    tall = replace(spec.params, domain_min=tuple(lo), domain_max=(hi[0], hi[1], hi[2] + 1000.0))
+    out = []
+    for params in (spec.params, tall):
+        g = sphg.Scene(ctx, params, spec.fluid, spec.boundary)
+        out.append(g.info()["ncells"])
+        g.step(5)
+        ctx.synchronize()
+        sc = g.scalars()
+        out.append((g.get(sphg.FIELD_X), g.get(sphg.FIELD_V), sc.sum_iterations, sc.sum_iterations_v, sc.time))
+        g.close()
+    nc0, (x0, v0, *s0), nc1, (x1, v1, *s1) = out
+    assert nc0 < 1024 * 1024 < nc1 // 2
+    assert s0 == s1
+    assert np.array_equal(x0, x1) and np.array_equal(v0, v1)
+
+
 def test_empty_fluid_is_rejected(ctx):
     """A scene without fluid particles and without emitters is refused at
     initialisation with SPHG_ESTATE and a message, not run."""
