@@ -55,12 +55,12 @@ constexpr int kDefaultMaxNbr = 96;
 // ----------------------------------------------------------- profiling
 
 enum Kind {
-    K_KEYS, K_SCAN, K_SCATTER, K_FINALIZE, K_REORDER, K_BUILD, K_NORMALS, K_FORCES, K_STEPBEGIN, K_DT,
+    K_KEYS, K_SCAN, K_SCATTER, K_FINALIZE, K_REORDER, K_BUILD, K_NORMALS, K_FORCES, K_DT,
     K_DFPREDICT, K_DFVEL_WARM, K_DFVEL_DENS, K_DFVEL_DIV, K_DFERR_DENS, K_DFERR_DIV, K_DFPOS, K_PCIDENS,
     K_PCIPACC, K_PCIINT, K_TIME, K_CUBE, K_EMIT, K_PB, K_NKINDS
 };
 const char* kKindNames[K_NKINDS] = {
-    "keys", "scan", "scatter", "finalize", "rank_reorder", "build", "normals", "forces", "step_begin", "dt",
+    "keys", "scan", "scatter", "finalize", "rank_reorder", "build", "normals", "forces", "dt",
     "df_predict", "df_vel_warm", "df_vel_density", "df_vel_divergence", "df_err_density", "df_err_divergence",
     "df_positions", "pci_density", "pci_pacc", "pci_integrate", "time_advance", "cube", "emit", "pcisph_boundary"};
 
