@@ -376,6 +376,23 @@ def test_grid_past_fused_scan_limit_steps_like_small_grid(ctx):
     assert nc0 < 1024 * 1024 < nc1 // 2
     assert s0 == s1
     assert np.array_equal(x0, x1) and np.array_equal(v0, v1)
+    # batched with the stretched scene, the small one takes the three-launch
+    # scan (per-scene offsets over two scenes) and steps as it does alone
+    other = configs.small_scene(sphg.DFSPH, n_side=9, seed=3)
+    tall2 = replace(other.params, domain_min=tuple(np.concatenate([other.fluid, other.boundary]).min(axis=0)
+                                                   .astype(np.float64)),
+                    domain_max=tuple(np.concatenate([other.fluid, other.boundary]).max(axis=0).astype(np.float64)
+                                     + np.array([0.0, 0.0, 1000.0])))
+    small = sphg.Scene(ctx, spec.params, spec.fluid, spec.boundary)
+    big = sphg.Scene(ctx, tall2, other.fluid, other.boundary)
+    assert big.info()["ncells"] > 1024 * 1024
+    ctx.batch_step([small, big], 5)
+    ctx.synchronize()
+    sc = small.scalars()
+    assert (sc.sum_iterations, sc.sum_iterations_v, sc.time) == tuple(s0)
+    assert np.array_equal(small.get(sphg.FIELD_X), x0) and np.array_equal(small.get(sphg.FIELD_V), v0)
+    small.close()
+    big.close()
 
 
 def test_empty_fluid_is_rejected(ctx):
