@@ -335,6 +335,9 @@ public:
     ~GpuSolver() override;
     void init() override;
     void step() override;
+    // k steps in one library call, leaving the host state as k calls of
+    // step() do (the caller knows that no frame falls due among them)
+    void stepMany(unsigned int k);
     void resizeData() override {}
     // Refresh FluidModel's host mirror (positions, velocities, densities,
     // pressures) from the device, in particle-id order.
@@ -421,6 +424,8 @@ public:
         intervals[n] = std::chrono::duration<double>(std::chrono::steady_clock::now() - starts[n]).count();
     }
     Real getInterval(const std::string& n) { return intervals[n]; }
+    // the time at which hasToSave turns true
+    Real nextSaveTime() const { return lastTimeSave + 1.0 / fps; }
     // src/TimeManager.h:101-111
     bool hasToSave() {
         if (time - lastTimeSave >= 1.0 / fps) {

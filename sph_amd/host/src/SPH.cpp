@@ -355,10 +355,12 @@ void GpuSolver::init() {
     syncToHost();
 }
 
-void GpuSolver::step() {
+void GpuSolver::step() { stepMany(1); }
+
+void GpuSolver::stepMany(unsigned int k) {
     if (!scene) throw SphError("step before init");
     sphg_step_stats st{};
-    check(sphg_scene_step(scene, 1, &st), "sphg_scene_step");
+    check(sphg_scene_step(scene, k, &st), "sphg_scene_step");
     gpuMs += st.gpu_ms;
     check(sphg_scene_get_scalars(scene, &last), "scalars");
     steps = last.steps;
@@ -494,9 +496,28 @@ bool Simulation::step() {
 
 void Simulation::run(unsigned int maxSteps) {
     unsigned int k = 0;
+    auto* gs = dynamic_cast<GpuSolver*>(solver.get());
     while (getTime() >= getStartTime() && getTime() <= getEndTime()) {
-        step();
-        if (maxSteps && ++k >= maxSteps) break;
+        // No step advances the time by more than dtB (the fixed step of
+        // WCSPH and PCISPH; DFSPH's CFL step is clamped to the maximum), so
+        // the first floor(room / dtB) - 1 steps end at least dtB before the
+        // next frame and the end time: the per-step checks of step() would
+        // find nothing to do after any of them, and they go to the device
+        // in one call
+        unsigned int run = 1;
+        if (gs) {
+            const Real dtB = std::max((Real)gs->scalars().time_step, getMaxTimeStep());
+            const Real room = std::min(tm.nextSaveTime(), getEndTime()) - getTime();
+            if (dtB > 0 && room > 0) {
+                const double fit = std::floor(room / dtB) - 1.0;
+                if (fit >= 2.0) run = (unsigned int)std::min(fit, 1.0e6);
+            }
+            if (maxSteps) run = std::min(run, maxSteps - k);
+        }
+        if (run > 1) gs->stepMany(run);
+        else step();
+        k += run;
+        if (maxSteps && k >= maxSteps) break;
     }
     if (auto* gs = dynamic_cast<GpuSolver*>(solver.get())) gs->flushFrames();
 }

@@ -150,24 +150,32 @@ def test_emitter_scene_json(ctx, tmp_path):
     assert np.any(va[spec.n:, 2] < -1.2)
 
 
-def test_frames_cadence_and_content(ctx, tmp_path):
+@pytest.mark.parametrize("name,steps,min_frames", [("dam_break_dfsph_small.json", 60, 3),
+                                                   ("dam_break_wcsph_32k.json", 100, 2)])
+def test_frames_cadence_and_content(ctx, tmp_path, name, steps, min_frames):
     """Frames of `sphg_run -o true` (SceneLoader::writeFluid at
     TimeManager::hasToSave, src/SceneLoader.cpp:204-268,
     src/TimeManager.h:101-111), written by the asynchronous path: each
     frame holds, bit for bit, the C-ABI state at the frame's time, and the
-    frames follow the cadence rule replayed on the C-ABI time sequence."""
+    frames follow the cadence rule replayed on the C-ABI time sequence.  The
+    WCSPH scene's fixed step lets Simulation::run hand the steps between
+    frames to the device in runs (38 of every 40 here), and the replay steps
+    one at a time."""
     import shutil
     scene = tmp_path / "dam.json"
-    shutil.copy(os.path.join(SCENES, "dam_break_dfsph_small.json"), scene)
-    steps = 60
+    shutil.copy(os.path.join(SCENES, name), scene)
     p = subprocess.run([RUN, "-f", str(scene), "-o", "true", "--steps", str(steps)], capture_output=True, text=True,
                        timeout=110)
     assert p.returncode == 0, p.stderr
+    assert json.loads(p.stdout.strip().splitlines()[-1])["steps"] == steps
     frames = sorted(tmp_path.glob("dam_*.json"))
     # replay: time after each step, and hasToSave (lastTimeSave = frame/fps;
     # the first frame is 1, src/TimeManager.h:31)
     fps = 50.0
-    spec = configs.small_scene(sphg.DFSPH, n_side=8, spacing=0.05, jitter=0.01, seed=7)
+    if "wcsph" in name:
+        spec = configs.wcsph_dam_32k()
+    else:
+        spec = configs.small_scene(sphg.DFSPH, n_side=8, spacing=0.05, jitter=0.01, seed=7)
     s = sphg.Scene(ctx, spec.params, spec.fluid, spec.boundary)
     last, frame, expect = 0.0, 1, []
     for k in range(steps):
@@ -178,7 +186,7 @@ def test_frames_cadence_and_content(ctx, tmp_path):
             expect.append((frame, t, s.get(sphg.FIELD_X), s.get(sphg.FIELD_V)))
             frame += 1
     s.close()
-    assert len(expect) >= 3 and [f.name for f in frames] == [f"dam_{k:04d}.json" for k, *_ in expect]
+    assert len(expect) >= min_frames and [f.name for f in frames] == [f"dam_{k:04d}.json" for k, *_ in expect]
     for f, (k, t, x, v) in zip(frames, expect):
         with open(f) as fh:
             st = json.load(fh)
