@@ -498,21 +498,27 @@ void Simulation::run(unsigned int maxSteps) {
     unsigned int k = 0;
     auto* gs = dynamic_cast<GpuSolver*>(solver.get());
     while (getTime() >= getStartTime() && getTime() <= getEndTime()) {
-        // No step advances the time by more than dtB (the fixed step of
-        // WCSPH and PCISPH; DFSPH's CFL step is clamped to the maximum), so
-        // the first floor(room / dtB) - 1 steps end at least dtB before the
-        // next frame and the end time: the per-step checks of step() would
-        // find nothing to do after any of them, and they go to the device
-        // in one call
+        // Steps that end at least one step before the next frame time and
+        // the end time leave the per-step checks of step() nothing to do, and
+        // go to the device in one call.  Their time steps are bounded from
+        // the current one: WCSPH's and PCISPH's is fixed, and DFSPH's CFL
+        // rule (src/DFSPHSolver.cpp:827-870) grows it by at most 1.001 per
+        // step before clamping it to [minTimeStep, maxTimeStep].
         unsigned int run = 1;
         if (gs) {
-            const Real dtB = std::max((Real)gs->scalars().time_step, getMaxTimeStep());
-            const Real room = std::min(tm.nextSaveTime(), getEndTime()) - getTime();
-            if (dtB > 0 && room > 0) {
-                const double fit = std::floor(room / dtB) - 1.0;
-                if (fit >= 2.0) run = (unsigned int)std::min(fit, 1.0e6);
+            const unsigned int cap = maxSteps ? maxSteps - k : 1000000u;
+            const bool cfl = gs->getMethod() == SPHG_DFSPH;
+            const Real tEnd = std::min(tm.nextSaveTime(), getEndTime());
+            Real t = getTime(), dt = gs->scalars().time_step;
+            unsigned int fit = 0;
+            while (fit < cap && dt > 0) {
+                const Real next = cfl ? std::min(getMaxTimeStep(), std::max(getMinTimeStep(), dt * 1.0011)) : dt;
+                if (!(next > 0) || !(t + 2 * next < tEnd)) break;
+                t += next;
+                dt = next;
+                ++fit;
             }
-            if (maxSteps) run = std::min(run, maxSteps - k);
+            if (fit >= 2) run = fit;
         }
         if (run > 1) gs->stepMany(run);
         else step();
